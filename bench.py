@@ -1,0 +1,231 @@
+"""Benchmark of the MoreFit likelihood hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the mass-fit density
+f*Gauss(mu, sigma) + (1-f)*Exp(lambda) on m in [5.0, 5.6], N = 10,000,000 events generated
+on the device by seeded accept-reject, one dataset.  A step is one -2lnL + gradient
+evaluation over all N events at the fit's start values: parameter upload, FP64 hoisted
+terms, the fused per-event kernel with its deterministic reduction, totals back to the host
+(a replayed hipGraph).  `value` is evaluations per second of the FP32-compensated path;
+the FP64 path, accuracy against the FP64 oracle, the kernel's HBM roofline fraction, the
+batched toy study (4096 toys, Poisson(5000) events) against the plain per-fit FP64 path and
+the CPU baseline (the oracle, single thread) are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-toys] [--no-cpu]
+Progress goes to stderr; stdout carries exactly one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_EVENTS = 10_000_000
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-toys", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--toys", type=int, default=4096)
+    ap.add_argument("--plain-toys", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
+              "(the likelihood path is measured on one MI355X; see DESIGN.md for how it shards).", file=sys.stderr)
+        sys.exit(2)
+
+    import morefit_amd as M
+    from oracle import oracle as O
+
+    ctx = M.Context(device=0)
+    name, cus = ctx.device_name()
+    log("device", name, cus, "CUs")
+    n = N_EVENTS
+    start = M.MASS_START.copy()
+
+    # ---------------------------------------------------------------- data (on device)
+    m64 = ctx.model(M.MASS, fp32=False)
+    b64 = m64.batch([n])
+    t0 = time.perf_counter()
+    b64.generate(M.MASS_TRUTH, [229387429])
+    data = b64.download(0)
+    log(f"generated {n} events on device in {time.perf_counter() - t0:.2f} s")
+    m32 = ctx.model(M.MASS, fp32=True)
+    b32 = m32.batch([n])
+    b32.upload(0, data)
+
+    # ---------------------------------------------------------------- timed steps (FP32)
+    for _ in range(args.warmup):
+        b32.eval(M.EVAL_GRAD, start)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r32 = b32.eval(M.EVAL_GRAD, start)
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = args.steps / dt
+    log(f"fp32 -2lnL+grad: {value:.1f} evals/s ({ms_per_step:.4f} ms/eval)")
+
+    # kernel-only device time (HIP events on the library's stream)
+    k32_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
+    k32_lh_ms = b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps))
+    bytes32 = n * 4
+    achieved = bytes32 / (k32_ms * 1e-3) / 1e9
+    log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s; lh kernel {k32_lh_ms * 1e3:.2f} us")
+
+    # ---------------------------------------------------------------- FP64 path
+    for _ in range(max(5, args.warmup // 2)):
+        b64.eval(M.EVAL_GRAD, start)
+    t0 = time.perf_counter()
+    steps64 = max(20, args.steps // 2)
+    for _ in range(steps64):
+        r64 = b64.eval(M.EVAL_GRAD, start)
+    v64 = steps64 / (time.perf_counter() - t0)
+    k64_ms = b64.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
+    log(f"fp64 -2lnL+grad: {v64:.1f} evals/s, kernel {k64_ms * 1e3:.2f} us")
+
+    # ---------------------------------------------------------------- accuracy vs oracle
+    t0 = time.perf_counter()
+    ref = O.nll_sums(O.MASS, data, start, mode=1, nthreads=0)
+    stored = (data - 5.3).astype(np.float32).astype(np.float64) + 5.3
+    ref_st = O.nll_sums(O.MASS, stored, start, mode=1, nthreads=0)
+    absref = O.nll_sums(O.MASS, data, start, mode=11, nthreads=8)
+    log(f"oracle sums in {time.perf_counter() - t0:.1f} s")
+    acc = {
+        "fp64_rel_err_m2lnL": float(abs(r64[0, 0] - ref[0]) / abs(ref[0])),
+        "fp64_max_grad_err_scaled": float(np.max(np.abs(r64[0, 1:] - ref[1:]) / absref[1:])),
+        "fp32_rel_err_m2lnL": float(abs(r32[0, 0] - ref[0]) / abs(ref[0])),
+        "fp32_rel_err_m2lnL_same_stored_data": float(abs(r32[0, 0] - ref_st[0]) / abs(ref_st[0])),
+        "fp32_max_grad_err_scaled": float(np.max(np.abs(r32[0, 1:] - ref_st[1:]) / absref[1:])),
+    }
+    # fitted parameters: FP32-compensated vs FP64 on the same data, in units of sigma
+    t0 = time.perf_counter()
+    v32f, e32f, o32, st32 = b32.fit(start)
+    v64f, e64f, o64, st64 = b64.fit(start)
+    acc["fit_fp32_vs_fp64_max_dtheta_over_sigma"] = float(np.max(np.abs(v32f[0] - v64f[0]) / e64f[0]))
+    acc["fit_status"] = [o32[0].status, o64[0].status]
+    acc["fit_fp32_ms"] = st32.wall_ms
+    acc["fit_fp64_ms"] = st64.wall_ms
+    log(f"fits fp32/fp64 in {time.perf_counter() - t0:.1f} s: dtheta/sigma {acc['fit_fp32_vs_fp64_max_dtheta_over_sigma']:.2e}")
+
+    # ---------------------------------------------------------------- HBM ceilings
+    copy_gbs = ctx.copy_ceiling(1 << 31, iters=5)
+    stream = None
+    if not args.no_stream:
+        ns = 500_000_000
+        ms = ctx.model(M.MASS, fp32=True)
+        bs = ms.batch([ns])
+        bs.generate(M.MASS_TRUTH, [7])
+        ks = bs.time_kernel(M.EVAL_GRAD, iters=20)
+        kls = bs.time_kernel(M.EVAL_LH, iters=20)
+        stream = {"events": ns, "bytes": ns * 4, "grad_kernel_ms": ks, "grad_GBps": ns * 4 / (ks * 1e-3) / 1e9,
+                  "lh_kernel_ms": kls, "lh_GBps": ns * 4 / (kls * 1e-3) / 1e9}
+        stream["grad_frac"] = stream["grad_GBps"] / HBM_PEAK_GBS
+        log(f"streaming 2 GB: grad {stream['grad_GBps']:.0f} GB/s, lh {stream['lh_GBps']:.0f} GB/s, copy ceiling {copy_gbs:.0f} GB/s")
+        bs.close()
+        ms.close()
+
+    # ---------------------------------------------------------------- toy study
+    toys = None
+    if not args.no_toys:
+        rs = np.random.default_rng(229387429)
+        sizes = rs.poisson(5000, args.toys)
+        mt = ctx.model(M.MASS, fp32=True)
+        bt = mt.batch(sizes)
+        t0 = time.perf_counter()
+        bt.generate(M.MASS_TRUTH, list(range(1, args.toys + 1)))
+        tgen = time.perf_counter() - t0
+        starts = np.tile(M.MASS_START, (args.toys, 1))
+        t0 = time.perf_counter()
+        tv, te, to, tst = bt.fit(starts)
+        tfit = time.perf_counter() - t0
+        ok = np.array([o.status == 0 for o in to])
+        pulls = (tv[ok] - M.MASS_TRUTH) / te[ok]
+        batched_rate = args.toys / tfit
+        # plain per-fit FP64 GPU path on a subset (same toys, one fit at a time, one launch per call)
+        tplain = 0.0
+        for i in range(args.plain_toys):
+            d = bt.download(i)
+            t1 = time.perf_counter()
+            ctx.fit_plain(M.MASS, d, M.MASS_START, fp32=False)
+            tplain += time.perf_counter() - t1
+        plain_rate = args.plain_toys / tplain
+        toys = {"toys": args.toys, "mean_events": float(sizes.mean()), "generate_s": tgen, "fit_s": tfit,
+                "toy_fits_per_s": batched_rate, "plain_fp64_toy_fits_per_s": plain_rate,
+                "speedup_vs_plain": batched_rate / plain_rate, "converged": int(ok.sum()),
+                "pull_mean": pulls.mean(axis=0).round(4).tolist(), "pull_width": pulls.std(axis=0).round(4).tolist(),
+                "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
+                "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms,
+                "launch_shape": {"block": 256, "events_per_thread": 16}}
+        log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain ({batched_rate / plain_rate:.1f}x)")
+        bt.close()
+        mt.close()
+
+    # ---------------------------------------------------------------- CPU baseline (oracle)
+    cpu = None
+    if not args.no_cpu:
+        sample = 2_000_000
+        d = data[:, :sample]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            O.nll_sums(O.MASS, d, start, mode=1, nthreads=1)
+            reps += 1
+        el = time.perf_counter() - t0
+        per_eval_full = el / reps * (n / sample)
+        cpu = {"value": 1.0 / per_eval_full, "unit": "evals/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} evaluations of -2lnL+grad over the first {sample} of the {n} events "
+                         f"(FP64, per-output Kahan, compute_llvm.hh semantics), scaled to N={n}"}
+        log(f"cpu baseline: {cpu['value']:.3f} evals/s (1 thread)")
+
+    out = {
+        "metric": "-2lnL+grad evals/s; toy fits/s vs plain GPU path; p50/p99 fit latency",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded accept-reject sample of the density, generated on the device",
+        "config": {"workload": "mass fit f*Gauss+(1-f)*Exp on [5.0,5.6], N=1e7 events, one -2lnL+gradient evaluation per step",
+                   "events": n, "precision": "FP32 per-event arithmetic, compensated per-lane sums, FP64 totals",
+                   "block": 256, "events_per_thread": 16},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "mf_eval_grad (fp32)", "kernel_ms": k32_ms, "algorithmic_bytes": bytes32},
+        "cpu_baseline": cpu,
+        "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9},
+        "lh_kernel_ms_fp32": k32_lh_ms,
+        "accuracy": acc,
+        "copy_ceiling_GBps": copy_gbs,
+        "streaming": stream,
+        "toys": toys,
+        "device": name,
+    }
+    for x in (b32, b64):
+        x.close()
+    m32.close()
+    m64.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,108 @@
+/*
+ * morefit_capi.h -- host-level C ABI of libmorefit_capi.so: the configuration densities of
+ * MoreFit's likelihood path compiled for the MI355X backend, resident datasets, fused
+ * -2lnL/gradient/Hessian evaluation, single fits, batched toy fits and the fit service.
+ *
+ * This is the binding surface a non-C++ caller (the Python tests and bench.py via ctypes)
+ * uses; C++ callers use the morefit headers directly (morefit_amd/include/morefit.hh).
+ * Reference interfaces each entry point replaces:
+ *   mfh_model_create     PDF construction + fitter::make_kernels (benchmarking/morefit_massfit.cc:48-62,
+ *                        benchmarking/morefit_kstarmumu.cc:41-55, fitter.hh:321-658)
+ *   mfh_batch_*          EventVector upload + ComputeBlock CopyTo.. / SubmitKernel / CopyFromKahanBuffer
+ *                        (fitter.hh:704-757, 1054-1326)
+ *   mfh_batch_generate   generator::generate on the accelerator (generator.hh:71-175)
+ *   mfh_fit_plain        fitter::fit with the drop-in HIPBlock (FP64 or FP32), one launch per call
+ *   mfh_fit_cpu          fitter::fit with the host-compiled CPU backend (reference LLVM semantics)
+ *   mfh_batch_fit        toystudy::toy's fit loop (toystudy.hh:55-111) as one batched study
+ *   mfh_serve_run        no counterpart: continuous-batching fit service
+ * Status: 0 on success, else nonzero with mfh_last_error().
+ */
+#ifndef MOREFIT_CAPI_H
+#define MOREFIT_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFH_MASS 0   /* f Gauss(mu, sigma) + (1-f) Exp(lambda) on [lo, hi]; theta = mu, sigma, f, lambda */
+#define MFH_KSTAR 1  /* P-wave B->K*ll angular density; theta = FL, S3, S4, S5, AFB, S7, S8, S9 */
+
+typedef struct mfh_context mfh_context;
+typedef struct mfh_model mfh_model;
+typedef struct mfh_batch mfh_batch;
+
+typedef struct mfh_options {
+  int device;
+  int block_size;          /* evaluation work-group size (multiple of 64) */
+  int events_per_thread;   /* chunk = block_size * events_per_thread */
+  int grid;                /* 0 = one work-group per chunk, else persistent grid */
+  int fast_math;           /* FP32 hardware exp/log/rcp */
+  int accumulate;          /* FP32: 1 compensated FP32 per lane, 0 FP64 per event */
+  int centre;              /* FP32: observables stored as offsets from the range centre */
+} mfh_options;
+
+typedef struct mfh_fit_out {
+  double fval;
+  double edm;
+  int status;
+  int ncalls;
+  int iterations;
+  int hesse_ok;
+} mfh_fit_out;
+
+typedef struct mfh_batch_stats {
+  int ticks;
+  int launches_lh, launches_grad, launches_hess;
+  double wall_ms, gpu_wait_ms, host_ms;
+} mfh_batch_stats;
+
+const char* mfh_last_error(void);
+void mfh_default_options(mfh_options* o);
+
+int mfh_open(const mfh_options* opts, mfh_context** out);
+int mfh_close(mfh_context* ctx);
+int mfh_device_name(mfh_context* ctx, char* name, int len, int* compute_units);
+int mfh_copy_ceiling(mfh_context* ctx, double bytes, int iters, double* gbps);
+
+/* model: `range` = [lo, hi] per observable (NULL: the configuration ranges) */
+int mfh_model_create(mfh_context* ctx, int kind, int fp32, const double* range, int with_hessian, mfh_model** out);
+int mfh_model_destroy(mfh_model* m);
+/* info[0..7] = ntheta, nhoist, nobs, nout_lh, nout_grad, nout_hess, statements, graph_nodes;
+ * ms[0..1] = codegen ms, hipRTC compile ms (0 when the code object came from the cache) */
+int mfh_model_info(mfh_model* m, int* info, double* ms);
+int mfh_model_source(mfh_model* m, char* buf, int len);
+/* generated source without a device (warms the on-disk code-object cache) */
+int mfh_model_precompile(int kind, int fp32, int with_hessian, const mfh_options* opts, double* compile_ms);
+
+int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out);
+int mfh_batch_destroy(mfh_batch* b);
+int mfh_batch_upload(mfh_batch* b, int seg, const double* cols, long long col_stride);
+int mfh_batch_download(mfh_batch* b, int seg, double* cols);
+int mfh_batch_generate(mfh_batch* b, const double* theta_gen, const unsigned long long* seeds, int events_per_stream,
+                       unsigned long long* proposals);
+/* kind 1: value, 2: value+gradient, 3: value+gradient+Hessian (packed i>=j).  theta: nseg x ntheta.
+ * out: nseg x nout(kind) raw sums of log p and its derivatives (not scaled by -2). */
+int mfh_batch_eval(mfh_batch* b, int kind, const double* theta, int use_graph, double* out);
+/* mean device time of the evaluation kernel alone (HIP events around `iters` launches) */
+int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, double* ms_per_launch);
+/* mean wall time of the full per-call sequence (upload, prep, kernel, totals, sync) */
+int mfh_batch_time_eval(mfh_batch* b, int kind, int iters, int use_graph, double* us_per_eval);
+long long mfh_batch_bytes(mfh_batch* b);
+/* fits of the listed segments (NULL = all) from start values (nfit x ntheta) */
+int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, int use_graph, double* values, double* errors,
+                  mfh_fit_out* outs, mfh_batch_stats* stats);
+
+/* one fit through the drop-in HIPBlock (reference fitter flow); data SoA nobs x n */
+int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, double* values,
+                  double* errors, mfh_fit_out* out, double* ms);
+/* one fit through the host-compiled CPU backend (no device needed) */
+int mfh_fit_cpu(int kind, const double* data, long long n, const double* start, int nthreads, double* values, double* errors,
+                mfh_fit_out* out, double* ms);
+/* -2lnL, gradient (-2 scale) and Hessian through the CPU backend at theta (no device needed) */
+int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,188 @@
+/*
+ * morefit_hip.h -- C ABI of libmorefit_hip.so, the MI355X (gfx950) compute backend of the
+ * MoreFit likelihood hot path.
+ *
+ * This is the layer the reference's ComputeBlock implementations talk to their device
+ * through (reference compute.hh:110-149; OpenCL calls in compute_opencl.hh:166-995).
+ * Plain C: opaque handles, plain pointers and sizes, int status (0 = MF_OK) plus
+ * mf_last_error().  Device memory is owned by the library (mf_malloc/mf_free), host memory
+ * by the caller.  One device handle owns one HIP stream (default priority); everything
+ * enqueued through a handle is ordered on that stream.  A handle is not thread-safe.
+ *
+ * Kernels are generated per density by the C++ host (graph.hh emit_kernel) and compiled
+ * here with hipRTC for gfx950; mf_module_load prepends the kernel runtime (the structs
+ * below, the fused per-event accumulation, the wave64/LDS block reduction and the
+ * deterministic per-segment final reduction) to the generated source.
+ *
+ * Entry points and the reference interface each one replaces:
+ *   mf_device_open/close            OpenCLBackend ctor: platform/device/context/queue (compute_opencl.hh:30-206)
+ *   mf_malloc/mf_free               OpenCLBlock::Setup*Buffer / dtor (compute_opencl.hh:258-350)
+ *   mf_memcpy_h2d                   CopyToInputBuffer / CopyToParameterBuffer / CopyToSeedBuffer (compute_opencl.hh:351-400)
+ *   mf_memcpy_d2h                   CopyFromOutputBuffer / CopyFromKahanBuffer (compute_opencl.hh:401-424, 977-986)
+ *   mf_module_load                  MakeComputeKernel / MakeGenerateKernel build step (compute_opencl.hh:425-804)
+ *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
+ *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
+ *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
+ *   mf_sync                         Finish (clFinish)
+ *   mf_graph_*                      no counterpart (per-iteration launch sequence as a hipGraph)
+ *   mf_timer_*, mf_bench_copy       no counterpart (measurement)
+ */
+#ifndef MOREFIT_HIP_H
+#define MOREFIT_HIP_H
+
+#ifndef MF_KERNEL_SIDE
+#include <stddef.h>
+#include <stdint.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- status codes ---------------- */
+#define MF_OK 0
+#define MF_ERR_INVALID 1     /* bad argument */
+#define MF_ERR_HIP 2         /* HIP runtime error */
+#define MF_ERR_COMPILE 3     /* hipRTC compilation failed (log in mf_last_error) */
+#define MF_ERR_NOMEM 4       /* allocation failed or over budget */
+#define MF_ERR_NOKERNEL 5    /* kernel not found in module */
+#define MF_ERR_INJECTED 6    /* failure injected by mf_debug_fail_next */
+
+/* ---------------- structures shared with the device (same layout on both sides) ----------------
+ * A segment is one dataset: the single fit's events, one toy, or one served request.
+ * Its observables are columns of `stride` elements starting at data_off.  The segment's
+ * kernel parameters (floating parameters, then hoisted terms, in the reference's
+ * parameter-buffer order, fitter.hh:1063-1072) start at params[param_off].  Its events
+ * are split into chunks [begin, end) listed contiguously at chunks[chunk_begin ..
+ * chunk_end); each chunk's sums go to partials[chunk * nout ...] and the segment's totals
+ * (FP64) to results[out_off ...].  mode 0 skips the segment in a launch. */
+typedef struct mf_segment {
+  long long data_off;
+  long long stride;
+  int n;
+  int param_off;
+  int chunk_begin;
+  int chunk_end;
+  int out_off;
+  int mode;
+} mf_segment;
+
+typedef struct mf_chunk {
+  int seg;
+  int begin;
+  int end;
+  int pad;
+} mf_chunk;
+
+/* argument block of every generated evaluation kernel (mf_eval_<kind>) */
+typedef struct mf_eval_args {
+  const void* data;           /* T* SoA event columns */
+  const mf_segment* segs;
+  const mf_chunk* chunks;
+  int nchunks;
+  int nseg;
+  const void* params;         /* T* kernel parameters */
+  double* partials;           /* [nchunks][nout] */
+  double* results;            /* [..][nout] per segment at out_off */
+  unsigned int* counters;     /* [nseg] arrival counters, left at zero after every launch */
+  void* store;                /* T* per-event outputs (store kernels only) or NULL */
+  long long store_stride;     /* column stride of `store` */
+  int flags;
+  int pad;
+} mf_eval_args;
+
+/* argument block of the generated generator kernel (mf_generate):
+ * stream k owns events [begin, end) of segment seg and the xoshiro128++ state
+ * states[4k..4k+3], advanced in place. */
+typedef struct mf_gen_stream {
+  int seg;
+  int begin;
+  int end;
+  int pad;
+} mf_gen_stream;
+
+typedef struct mf_gen_args {
+  void* data;                 /* T* SoA output columns (segment layout as above) */
+  const mf_segment* segs;
+  const mf_gen_stream* streams;
+  int nstreams;
+  int pad;
+  unsigned int* states;
+  const void* params;         /* T* kernel parameters of the likelihood kernel */
+  const double* maxprob;      /* per segment */
+  unsigned long long* proposals; /* per segment proposal counters (may be NULL) */
+} mf_gen_args;
+
+#ifndef MF_KERNEL_SIDE
+
+typedef struct mf_device mf_device;
+typedef struct mf_module mf_module;
+typedef struct mf_graph mf_graph;
+
+typedef struct mf_device_props {
+  char name[64];
+  char arch[32];
+  int compute_units;
+  int wavefront_size;
+  size_t total_mem;
+  size_t free_mem;
+  int clock_khz;
+  int l2_bytes;
+} mf_device_props;
+
+const char* mf_last_error(void);
+
+/* device */
+int mf_device_open(int ordinal, mf_device** out);
+int mf_device_close(mf_device* dev);
+int mf_device_query(mf_device* dev, mf_device_props* props);
+
+/* memory (device memory is library-owned; host memory from mf_host_alloc is pinned) */
+int mf_malloc(mf_device* dev, size_t bytes, void** dptr);
+int mf_free(mf_device* dev, void* dptr);
+int mf_host_alloc(mf_device* dev, size_t bytes, void** hptr);
+int mf_host_free(mf_device* dev, void* hptr);
+int mf_memcpy_h2d(mf_device* dev, void* dst, const void* src, size_t bytes);   /* async, stream-ordered */
+int mf_memcpy_d2h(mf_device* dev, void* dst, const void* src, size_t bytes);   /* async, stream-ordered */
+int mf_memcpy_d2d(mf_device* dev, void* dst, const void* src, size_t bytes);
+int mf_fill_zero(mf_device* dev, void* dptr, size_t bytes);                    /* kernel fill, graph-safe */
+int mf_sync(mf_device* dev);
+
+/* modules: generated source (+ the kernel runtime) compiled by hipRTC for gfx950 and cached
+ * in memory and on disk by a hash of source and options */
+int mf_module_load(mf_device* dev, const char* generated_src, const char* options, mf_module** out);
+int mf_module_compile_only(const char* generated_src, const char* options, size_t* code_bytes);
+int mf_module_unload(mf_module* mod);
+int mf_module_has_kernel(mf_module* mod, const char* kernel);
+double mf_module_compile_ms(mf_module* mod);  /* 0 when served from a cache */
+
+/* launches (on the device stream) */
+int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
+int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg);
+int mf_launch_generate(mf_module* mod, const mf_gen_args* args, int grid, int block);
+
+/* hipGraph capture of everything enqueued on the device stream between begin and end */
+int mf_graph_begin(mf_device* dev);
+int mf_graph_end(mf_device* dev, mf_graph** out);
+int mf_graph_launch(mf_graph* g);
+int mf_graph_destroy(mf_graph* g);
+
+/* timing with HIP events on the device stream */
+int mf_timer_begin(mf_device* dev);
+int mf_timer_end(mf_device* dev, float* ms);
+
+/* HBM copy ceiling: a streaming read+write kernel over `bytes`, best of `iters` (GB/s of read+write) */
+int mf_bench_copy(mf_device* dev, size_t bytes, int iters, double* gbps);
+
+/* failure injection for host-side error-path tests: the next `count` calls of the named
+ * entry point ("mf_malloc", "mf_module_load", "mf_launch_eval", ...) fail with MF_ERR_INJECTED
+ * before touching the device. */
+int mf_debug_fail_next(const char* entry_point, int count);
+
+#endif /* MF_KERNEL_SIDE */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

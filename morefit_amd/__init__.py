@@ -1,0 +1,290 @@
+"""morefit_amd -- Python bindings (ctypes) of the MI355X MoreFit likelihood backend.
+
+The product is native: libmorefit_hip.so (device boundary, include/morefit_hip.h) and
+libmorefit_capi.so (host C ABI over the morefit C++ headers, include/morefit_capi.h).  This
+module only loads them and converts numpy arrays; it has no compute path of its own and
+raises if the native libraries are missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+MASS, KSTAR = 0, 1
+EVAL_LH, EVAL_GRAD, EVAL_HESS = 1, 2, 3
+NTHETA = {MASS: 4, KSTAR: 8}
+NOBS = {MASS: 1, KSTAR: 3}
+THETA_NAMES = {MASS: ["mu", "sigma", "f", "lambda"], KSTAR: ["FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"]}
+MASS_TRUTH = np.array([5.28, 0.02, 0.3, -1.5])
+KSTAR_TRUTH = np.array([0.7, 0.0, -0.1, -0.2, 0.1, 0.0, 0.0, 0.0])
+TRUTH = {MASS: MASS_TRUTH, KSTAR: KSTAR_TRUTH}
+# start values: +10 % of the window width for mu, +10 % of the value otherwise, zeros stay zero
+MASS_START = np.array([5.28 + 0.06, 0.022, 0.33, -1.65])
+KSTAR_START = np.array([0.77, 0.0, -0.11, -0.22, 0.11, 0.0, 0.0, 0.0])
+START = {MASS: MASS_START, KSTAR: KSTAR_START}
+
+
+class MorefitError(RuntimeError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("block_size", ctypes.c_int), ("events_per_thread", ctypes.c_int),
+                ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int)]
+
+
+class FitOut(ctypes.Structure):
+    _fields_ = [("fval", ctypes.c_double), ("edm", ctypes.c_double), ("status", ctypes.c_int), ("ncalls", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("hesse_ok", ctypes.c_int)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_int), ("launches_lh", ctypes.c_int), ("launches_grad", ctypes.c_int),
+                ("launches_hess", ctypes.c_int), ("wall_ms", ctypes.c_double), ("gpu_wait_ms", ctypes.c_double),
+                ("host_ms", ctypes.c_double)]
+
+
+_hip = None
+_capi = None
+
+# every symbol include/morefit_hip.h and include/morefit_capi.h declare
+HIP_SYMBOLS = [
+    "mf_last_error", "mf_device_open", "mf_device_close", "mf_device_query", "mf_malloc", "mf_free", "mf_host_alloc",
+    "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_module_load",
+    "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
+    "mf_launch_prep", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
+    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next",
+]
+CAPI_SYMBOLS = [
+    "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
+    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile",
+    "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
+    "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
+    "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu",
+]
+
+
+def libs():
+    """Load (once) and return (libmorefit_hip, libmorefit_capi); raise if not built."""
+    global _hip, _capi
+    if _capi is None:
+        hp = os.path.join(LIBDIR, "libmorefit_hip.so")
+        cp = os.path.join(LIBDIR, "libmorefit_capi.so")
+        if not (os.path.exists(hp) and os.path.exists(cp)):
+            raise MorefitError("native libraries not built: run __graft_entry__.build() (make -C morefit_amd/csrc)")
+        _hip = ctypes.CDLL(hp, mode=ctypes.RTLD_GLOBAL)
+        L = ctypes.CDLL(cp)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.mfh_last_error.restype = ctypes.c_char_p
+        L.mfh_open.argtypes = [ctypes.POINTER(Options), pp]
+        L.mfh_close.argtypes = [vp]
+        L.mfh_device_name.argtypes = [vp, ctypes.c_char_p, ip, ctypes.POINTER(ctypes.c_int)]
+        L.mfh_copy_ceiling.argtypes = [vp, ctypes.c_double, ip, dp]
+        L.mfh_model_create.argtypes = [vp, ip, ip, dp, ip, pp]
+        L.mfh_model_destroy.argtypes = [vp]
+        L.mfh_model_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
+        L.mfh_model_source.argtypes = [vp, ctypes.c_char_p, ip]
+        L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
+        L.mfh_batch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, pp]
+        L.mfh_batch_destroy.argtypes = [vp]
+        L.mfh_batch_upload.argtypes = [vp, ip, dp, ctypes.c_longlong]
+        L.mfh_batch_download.argtypes = [vp, ip, dp]
+        L.mfh_batch_generate.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_ulonglong), ip, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.mfh_batch_eval.argtypes = [vp, ip, dp, ip, dp]
+        L.mfh_batch_time_kernel.argtypes = [vp, ip, ip, dp]
+        L.mfh_batch_time_eval.argtypes = [vp, ip, ip, ip, dp]
+        L.mfh_batch_bytes.argtypes = [vp]
+        L.mfh_batch_bytes.restype = ctypes.c_longlong
+        L.mfh_batch_fit.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ip, dp, ip, dp, dp, ctypes.POINTER(FitOut), ctypes.POINTER(BatchStats)]
+        L.mfh_fit_plain.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, dp, dp, ctypes.POINTER(FitOut), dp]
+        L.mfh_fit_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
+        L.mfh_eval_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
+        _capi = L
+    return _hip, _capi
+
+
+def _check(rc):
+    if rc != 0:
+        raise MorefitError(libs()[1].mfh_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def options(**kw):
+    o = Options()
+    libs()[1].mfh_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, int(v))
+    return o
+
+
+def precompile(kind, fp32, with_hessian=True, **kw):
+    """Compile a configuration module for gfx950 into the on-disk cache (no device needed)."""
+    ms = ctypes.c_double(0)
+    _check(libs()[1].mfh_model_precompile(kind, int(fp32), int(with_hessian), ctypes.byref(options(**kw)), ctypes.byref(ms)))
+    return ms.value
+
+
+class Context:
+    """One device (HIPBackend) with its stream."""
+
+    def __init__(self, **kw):
+        self.h = ctypes.c_void_p()
+        _check(libs()[1].mfh_open(ctypes.byref(options(**kw)), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            libs()[1].mfh_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        cu = ctypes.c_int(0)
+        _check(libs()[1].mfh_device_name(self.h, buf, 256, ctypes.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    def copy_ceiling(self, nbytes=2 << 30, iters=5):
+        g = ctypes.c_double(0)
+        _check(libs()[1].mfh_copy_ceiling(self.h, float(nbytes), iters, ctypes.byref(g)))
+        return g.value
+
+    def model(self, kind, fp32=False, rng=None, with_hessian=True):
+        return Model(self, kind, fp32, rng, with_hessian)
+
+    def fit_plain(self, kind, data, start, fp32=False):
+        """One fit through the drop-in HIPBlock (reference fitter flow)."""
+        data = _f64(np.atleast_2d(data))
+        P = NTHETA[kind]
+        vals, errs = np.zeros(P), np.zeros(P)
+        out, ms = FitOut(), ctypes.c_double(0)
+        _check(libs()[1].mfh_fit_plain(self.h, kind, int(fp32), _dp(data), data.shape[1], _dp(_f64(start)), _dp(vals), _dp(errs),
+                                       ctypes.byref(out), ctypes.byref(ms)))
+        return vals, errs, out, ms.value
+
+
+class Model:
+    """A density compiled for the device: value, gradient and Hessian kernels."""
+
+    def __init__(self, ctx, kind, fp32, rng, with_hessian):
+        self.ctx, self.kind, self.fp32 = ctx, kind, fp32
+        self.h = ctypes.c_void_p()
+        r = None if rng is None else _dp(_f64(rng))
+        self._rng = None if rng is None else _f64(rng)
+        if self._rng is not None:
+            r = _dp(self._rng)
+        _check(libs()[1].mfh_model_create(ctx.h, kind, int(fp32), r, int(with_hessian), ctypes.byref(self.h)))
+        info = (ctypes.c_int * 8)()
+        ms = (ctypes.c_double * 2)()
+        _check(libs()[1].mfh_model_info(self.h, info, ms))
+        self.ntheta, self.nhoist, self.nobs = info[0], info[1], info[2]
+        self.nout = {EVAL_LH: info[3], EVAL_GRAD: info[4], EVAL_HESS: info[5]}
+        self.statements, self.graph_nodes = info[6], info[7]
+        self.codegen_ms, self.compile_ms = ms[0], ms[1]
+
+    def source(self):
+        buf = ctypes.create_string_buffer(1 << 22)
+        _check(libs()[1].mfh_model_source(self.h, buf, 1 << 22))
+        return buf.value.decode()
+
+    def batch(self, sizes):
+        return Batch(self, sizes)
+
+    def close(self):
+        if self.h:
+            libs()[1].mfh_model_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+class Batch:
+    """Resident datasets (segments) of one model."""
+
+    def __init__(self, model, sizes):
+        self.model = model
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        self.S = len(self.sizes)
+        self.h = ctypes.c_void_p()
+        arr = (ctypes.c_longlong * self.S)(*[int(s) for s in self.sizes])
+        _check(libs()[1].mfh_batch_create(model.h, arr, self.S, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            libs()[1].mfh_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def nbytes(self):
+        return libs()[1].mfh_batch_bytes(self.h)
+
+    def upload(self, seg, cols):
+        cols = _f64(np.atleast_2d(cols))
+        assert cols.shape == (self.model.nobs, self.sizes[seg])
+        _check(libs()[1].mfh_batch_upload(self.h, seg, _dp(cols), cols.shape[1]))
+
+    def download(self, seg):
+        out = np.zeros((self.model.nobs, int(self.sizes[seg])))
+        _check(libs()[1].mfh_batch_download(self.h, seg, _dp(out)))
+        return out
+
+    def generate(self, theta_gen, seeds, events_per_stream=64):
+        th = _f64(np.broadcast_to(np.asarray(theta_gen, dtype=np.float64), (self.S, self.model.ntheta)))
+        sd = (ctypes.c_ulonglong * self.S)(*[int(s) for s in seeds])
+        props = ctypes.c_ulonglong(0)
+        _check(libs()[1].mfh_batch_generate(self.h, _dp(th), sd, events_per_stream, ctypes.byref(props)))
+        return props.value
+
+    def eval(self, kind, theta, use_graph=True):
+        th = _f64(np.broadcast_to(np.asarray(theta, dtype=np.float64), (self.S, self.model.ntheta)))
+        out = np.zeros((self.S, self.model.nout[kind]))
+        _check(libs()[1].mfh_batch_eval(self.h, kind, _dp(th), int(use_graph), _dp(out)))
+        return out
+
+    def time_kernel(self, kind, iters=50):
+        ms = ctypes.c_double(0)
+        _check(libs()[1].mfh_batch_time_kernel(self.h, kind, iters, ctypes.byref(ms)))
+        return ms.value
+
+    def time_eval(self, kind, iters=50, use_graph=True):
+        us = ctypes.c_double(0)
+        _check(libs()[1].mfh_batch_time_eval(self.h, kind, iters, int(use_graph), ctypes.byref(us)))
+        return us.value
+
+    def fit(self, start, segs=None, use_graph=True):
+        P = self.model.ntheta
+        lst = list(range(self.S)) if segs is None else list(segs)
+        n = len(lst)
+        st = _f64(np.broadcast_to(np.asarray(start, dtype=np.float64), (n, P)))
+        vals, errs = np.zeros((n, P)), np.zeros((n, P))
+        outs = (FitOut * n)()
+        stats = BatchStats()
+        sp = None if segs is None else (ctypes.c_int * n)(*lst)
+        _check(libs()[1].mfh_batch_fit(self.h, sp, n, _dp(st), int(use_graph), _dp(vals), _dp(errs), outs, ctypes.byref(stats)))
+        return vals, errs, list(outs), stats
+
+
+def fit_cpu(kind, data, start, nthreads=1):
+    """One fit through the host-compiled CPU backend (reference LLVM-backend semantics)."""
+    data = _f64(np.atleast_2d(data))
+    P = NTHETA[kind]
+    vals, errs = np.zeros(P), np.zeros(P)
+    out, ms = FitOut(), ctypes.c_double(0)
+    _check(libs()[1].mfh_fit_cpu(kind, _dp(data), data.shape[1], _dp(_f64(start)), nthreads, _dp(vals), _dp(errs),
+                                 ctypes.byref(out), ctypes.byref(ms)))
+    return vals, errs, out, ms.value
+
+
+def eval_cpu(kind, data, theta, nthreads=1):
+    """-2lnL, its gradient and Hessian through the CPU backend."""
+    data = _f64(np.atleast_2d(data))
+    P = NTHETA[kind]
+    v = ctypes.c_double(0)
+    g, h = np.zeros(P), np.zeros(P * P)
+    _check(libs()[1].mfh_eval_cpu(kind, _dp(data), data.shape[1], _dp(_f64(theta)), nthreads, ctypes.byref(v), _dp(g), _dp(h)))
+    return v.value, g, h.reshape(P, P)

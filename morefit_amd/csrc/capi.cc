@@ -1,0 +1,457 @@
+// capi.cc -- libmorefit_capi.so: the host C ABI of include/morefit_capi.h over the morefit
+// C++ headers (graph, densities, fitter, HIP and CPU backends, batched fits).
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "morefit.hh"
+#include "morefit/batch.hh"
+#include "morefit_capi.h"
+
+using namespace morefit;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(const std::string& what) {
+  g_err = what;
+  return 1;
+}
+
+#define MFH_TRY try {
+#define MFH_CATCH                                  \
+  }                                                \
+  catch (const std::exception& e) {                \
+    return fail(e.what());                         \
+  }                                                \
+  catch (...) {                                    \
+    return fail("unknown C++ exception");          \
+  }                                                \
+  return 0;
+
+// the two configuration densities with the parameter limits and steps of the reference
+// benchmarks (benchmarking/morefit_massfit.cc:48-57, benchmarking/morefit_kstarmumu.cc:41-52)
+template <typename K>
+struct ModelDef {
+  int kind;
+  std::vector<std::unique_ptr<dimension<double>>> dims;
+  std::vector<std::unique_ptr<parameter<double>>> pars;
+  std::vector<std::unique_ptr<PDF<K, double>>> pdfs;
+  PDF<K, double>* pdf = nullptr;
+  std::vector<parameter<double>*> params;
+  std::vector<dimension<double>*> dimv;
+
+  ModelDef(int k, const double* range) : kind(k) {
+    if (k == MFH_MASS) {
+      double lo = range ? range[0] : 5.0, hi = range ? range[1] : 5.6;
+      dims.emplace_back(new dimension<double>("m", lo, hi));
+      pars.emplace_back(new parameter<double>("mu", 5.28, lo, hi, 0.01));
+      pars.emplace_back(new parameter<double>("sigma", 0.02, 0.005, 0.13, 0.001));
+      pars.emplace_back(new parameter<double>("f", 0.3, 0.0, 1.0, 0.01));
+      pars.emplace_back(new parameter<double>("lambda", -1.5, -10.0, 10.0, 0.01));
+      auto* g = new GaussianPDF<K, double>(dims[0].get(), pars[0].get(), pars[1].get());
+      auto* e = new ExponentialPDF<K, double>(dims[0].get(), pars[3].get());
+      pdfs.emplace_back(g);
+      pdfs.emplace_back(e);
+      pdfs.emplace_back(new SumPDF<K, double>(g, e, pars[2].get()));
+    } else if (k == MFH_KSTAR) {
+      const double dl[6] = {-1.0, 1.0, -1.0, 1.0, -M_PI, M_PI};
+      const double* r = range ? range : dl;
+      dims.emplace_back(new dimension<double>("ctl", r[0], r[1]));
+      dims.emplace_back(new dimension<double>("ctk", r[2], r[3]));
+      dims.emplace_back(new dimension<double>("phi", r[4], r[5]));
+      const char* names[8] = {"FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"};
+      const double truth[8] = {0.7, 0.0, -0.1, -0.2, 0.1, 0.0, 0.0, 0.0};
+      for (int i = 0; i < 8; i++) pars.emplace_back(new parameter<double>(names[i], truth[i], i == 0 ? 0.0 : -1.0, 1.0, 0.01));
+      pdfs.emplace_back(new KstarmumuAngularPDF<K, double>(dims[0].get(), dims[1].get(), dims[2].get(), pars[0].get(), pars[1].get(),
+                                                           pars[2].get(), pars[3].get(), pars[4].get(), pars[5].get(), pars[6].get(),
+                                                           pars[7].get()));
+    } else {
+      throw std::invalid_argument("unknown model kind " + std::to_string(k));
+    }
+    pdf = pdfs.back().get();
+    for (auto& p : pars) params.push_back(p.get());
+    for (auto& d : dims) dimv.push_back(d.get());
+  }
+};
+
+void apply_options(compute_options& co, const mfh_options* o) {
+  mfh_options d;
+  mfh_default_options(&d);
+  if (!o) o = &d;
+  co.hip_device = o->device;
+  co.hip_block_size = o->block_size > 0 ? o->block_size : 256;
+  co.hip_events_per_thread = o->events_per_thread > 0 ? o->events_per_thread : 16;
+  co.hip_grid = o->grid;
+  co.hip_fast_math = o->fast_math != 0;
+  co.hip_accumulate = o->accumulate;
+  co.hip_centre_observables = o->centre != 0;
+}
+}  // namespace
+
+struct mfh_context {
+  compute_options co;
+  std::unique_ptr<HIPBackend> be;
+};
+
+struct mfh_model {
+  mfh_context* ctx = nullptr;
+  int kind = 0;
+  bool fp32 = false;
+  std::unique_ptr<ModelDef<float>> df;
+  std::unique_ptr<ModelDef<double>> dd;
+  std::unique_ptr<HipModel<float>> mf;
+  std::unique_ptr<HipModel<double>> md;
+  template <typename F>
+  auto with(F f) {
+    return fp32 ? f(*mf) : f(*md);
+  }
+};
+
+struct mfh_batch {
+  mfh_model* model = nullptr;
+  std::unique_ptr<HipBatch<float>> bf;
+  std::unique_ptr<HipBatch<double>> bd;
+  template <typename F>
+  auto with(F f) {
+    return bf ? f(*bf) : f(*bd);
+  }
+};
+
+extern "C" {
+
+const char* mfh_last_error(void) { return g_err.c_str(); }
+
+void mfh_default_options(mfh_options* o) {
+  o->device = 0;
+  o->block_size = 256;
+  o->events_per_thread = 16;
+  o->grid = 0;
+  o->fast_math = 0;
+  o->accumulate = 1;
+  o->centre = 1;
+}
+
+int mfh_open(const mfh_options* opts, mfh_context** out) {
+  MFH_TRY
+  auto* c = new mfh_context();
+  apply_options(c->co, opts);
+  try {
+    c->be.reset(new HIPBackend(&c->co));
+  } catch (...) {
+    delete c;
+    throw;
+  }
+  *out = c;
+  MFH_CATCH
+}
+
+int mfh_close(mfh_context* ctx) {
+  delete ctx;
+  return 0;
+}
+
+int mfh_device_name(mfh_context* ctx, char* name, int len, int* cus) {
+  MFH_TRY
+  auto p = ctx->be->props();
+  std::snprintf(name, len, "%s (%s)", p.name, p.arch);
+  if (cus) *cus = p.compute_units;
+  MFH_CATCH
+}
+
+int mfh_copy_ceiling(mfh_context* ctx, double bytes, int iters, double* gbps) {
+  MFH_TRY
+  hip_check(mf_bench_copy(ctx->be->device(), (size_t)bytes, iters, gbps), "mf_bench_copy");
+  MFH_CATCH
+}
+
+int mfh_model_create(mfh_context* ctx, int kind, int fp32, const double* range, int with_hessian, mfh_model** out) {
+  MFH_TRY
+  std::unique_ptr<mfh_model> m(new mfh_model());
+  m->ctx = ctx;
+  m->kind = kind;
+  m->fp32 = fp32 != 0;
+  if (m->fp32) {
+    m->df.reset(new ModelDef<float>(kind, range));
+    m->mf.reset(new HipModel<float>(ctx->be.get(), m->df->pdf, m->df->params, with_hessian != 0));
+  } else {
+    m->dd.reset(new ModelDef<double>(kind, range));
+    m->md.reset(new HipModel<double>(ctx->be.get(), m->dd->pdf, m->dd->params, with_hessian != 0));
+  }
+  *out = m.release();
+  MFH_CATCH
+}
+
+int mfh_model_destroy(mfh_model* m) {
+  delete m;
+  return 0;
+}
+
+int mfh_model_info(mfh_model* m, int* info, double* ms) {
+  MFH_TRY
+  m->with([&](auto& hm) {
+    info[0] = hm.ntheta;
+    info[1] = hm.nhoist;
+    info[2] = hm.nobs;
+    info[3] = hm.nout[EVAL_LH];
+    info[4] = hm.nout[EVAL_GRAD];
+    info[5] = hm.nout[EVAL_HESS];
+    info[6] = (int)hm.statements;
+    info[7] = (int)hm.graph_nodes;
+    if (ms) {
+      ms[0] = hm.codegen_ms;
+      ms[1] = hm.compile_ms;
+    }
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_model_source(mfh_model* m, char* buf, int len) {
+  MFH_TRY
+  m->with([&](auto& hm) {
+    std::snprintf(buf, len, "%s", hm.source.c_str());
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_model_precompile(int kind, int fp32, int with_hessian, const mfh_options* opts, double* compile_ms) {
+  MFH_TRY
+  compute_options co;
+  apply_options(co, opts);
+  std::string src;
+  if (fp32) {
+    ModelDef<float> d(kind, nullptr);
+    src = HipModel<float>::make_source(d.pdf, d.params, co, with_hessian != 0).device_source();
+  } else {
+    ModelDef<double> d(kind, nullptr);
+    src = HipModel<double>::make_source(d.pdf, d.params, co, with_hessian != 0).device_source();
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  size_t bytes = 0;
+  hip_check(mf_module_compile_only(src.c_str(), co.hip_compile_options.c_str(), &bytes), "mf_module_compile_only");
+  if (compile_ms) *compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  MFH_CATCH
+}
+
+int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out) {
+  MFH_TRY
+  std::vector<long long> s(sizes, sizes + nseg);
+  std::unique_ptr<mfh_batch> b(new mfh_batch());
+  b->model = m;
+  if (m->fp32) b->bf.reset(new HipBatch<float>(m->mf.get(), s));
+  else b->bd.reset(new HipBatch<double>(m->md.get(), s));
+  *out = b.release();
+  MFH_CATCH
+}
+
+int mfh_batch_destroy(mfh_batch* b) {
+  delete b;
+  return 0;
+}
+
+int mfh_batch_upload(mfh_batch* b, int seg, const double* cols, long long col_stride) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    if (seg < 0 || seg >= bb.S) throw std::out_of_range("segment index");
+    bb.upload(seg, cols, col_stride);
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_batch_download(mfh_batch* b, int seg, double* cols) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    if (seg < 0 || seg >= bb.S) throw std::out_of_range("segment index");
+    bb.download(seg, cols);
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_batch_generate(mfh_batch* b, const double* theta_gen, const unsigned long long* seeds, int events_per_stream,
+                       unsigned long long* proposals) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    std::vector<double> th(theta_gen, theta_gen + (size_t)bb.S * bb.m->ntheta);
+    std::vector<uint64_t> sd(seeds, seeds + bb.S);
+    auto p = bb.generate(th, sd, events_per_stream > 0 ? events_per_stream : 64);
+    if (proposals) *proposals = p;
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_batch_eval(mfh_batch* b, int kind, const double* theta, int use_graph, double* out) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    if (kind < 1 || kind > 3 || bb.m->nout[kind] == 0) throw std::invalid_argument("bad evaluation kind");
+    for (int s = 0; s < bb.S; s++) std::memcpy(bb.theta(s), theta + (size_t)s * bb.m->ntheta, bb.m->ntheta * sizeof(double));
+    bb.run(kind, true, use_graph != 0);
+    for (int s = 0; s < bb.S; s++) std::memcpy(out + (size_t)s * bb.m->nout[kind], bb.result(s, kind), bb.m->nout[kind] * sizeof(double));
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, double* ms_per_launch) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    if (kind < 1 || kind > 3 || bb.m->nout[kind] == 0) throw std::invalid_argument("bad evaluation kind");
+    bb.set_modes(kind, true);
+    bb.prep();
+    hip_check(mf_memcpy_h2d(bb.dev, bb.segs_d[kind].ptr, bb.segs_hk[kind].ptr, bb.S * sizeof(mf_segment)), "segs");
+    bb.launch_kernel(kind);  // warm-up
+    hip_check(mf_sync(bb.dev), "sync");
+    hip_check(mf_timer_begin(bb.dev), "timer");
+    for (int i = 0; i < iters; i++) bb.launch_kernel(kind);
+    float ms = 0;
+    hip_check(mf_timer_end(bb.dev, &ms), "timer");
+    *ms_per_launch = (double)ms / iters;
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_batch_time_eval(mfh_batch* b, int kind, int iters, int use_graph, double* us_per_eval) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    bb.run(kind, true, use_graph != 0);  // warm-up (and graph capture)
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < iters; i++) bb.run(kind, true, use_graph != 0);
+    *us_per_eval = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / iters;
+    return 0;
+  });
+  MFH_CATCH
+}
+
+long long mfh_batch_bytes(mfh_batch* b) {
+  return b->with([&](auto& bb) { return (long long)bb.data_bytes(); });
+}
+
+int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, int use_graph, double* values, double* errors,
+                  mfh_fit_out* outs, mfh_batch_stats* stats) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    std::vector<int> list;
+    if (segs) list.assign(segs, segs + nfit);
+    else
+      for (int s = 0; s < bb.S; s++) list.push_back(s);
+    const int P = bb.m->ntheta;
+    std::vector<double> st(start, start + (size_t)list.size() * P);
+    typedef typename std::remove_reference<decltype(bb)>::type BT;
+    typedef typename std::remove_pointer<decltype(bb.m)>::type MT;
+    (void)sizeof(MT);
+    BatchFitter<typename std::conditional<std::is_same<BT, HipBatch<float>>::value, float, double>::type> bf(bb);
+    bf.use_graph = use_graph != 0;
+    auto res = bf.fit(list, st);
+    for (size_t i = 0; i < res.size(); i++) {
+      for (int k = 0; k < P; k++) {
+        values[i * P + k] = res[i].values.size() > (size_t)k ? res[i].values[k] : NAN;
+        errors[i * P + k] = res[i].errors.size() > (size_t)k ? res[i].errors[k] : NAN;
+      }
+      if (outs) {
+        outs[i].fval = res[i].fval;
+        outs[i].edm = res[i].edm;
+        outs[i].status = res[i].status;
+        outs[i].ncalls = res[i].ncalls;
+        outs[i].iterations = res[i].iterations;
+        outs[i].hesse_ok = res[i].hesse_ok;
+      }
+    }
+    if (stats) {
+      stats->ticks = bf.stats.ticks;
+      stats->launches_lh = bf.stats.launches[1];
+      stats->launches_grad = bf.stats.launches[2];
+      stats->launches_hess = bf.stats.launches[3];
+      stats->wall_ms = bf.stats.wall_ms;
+      stats->gpu_wait_ms = bf.stats.gpu_wait_ms;
+      stats->host_ms = bf.stats.host_ms;
+    }
+    return 0;
+  });
+  MFH_CATCH
+}
+
+}  // extern "C"
+
+namespace {
+template <typename K, typename Backend, typename Block>
+void run_fit(Backend* be, int kind, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
+             double* ms) {
+  ModelDef<K> def(kind, nullptr);
+  EventVector<double, double> ev(def.dimv, (unsigned long)n);
+  for (size_t d = 0; d < def.dimv.size(); d++)
+    for (long long i = 0; i < n; i++) ev(i, (unsigned)d) = data[d * n + i];
+  for (size_t k = 0; k < def.params.size(); k++) {
+    def.params[k]->set_value(start[k]);
+    def.params[k]->set_start_value(start[k]);
+  }
+  fitter_options fo;
+  fitter<K, double, Backend, Block> f(&fo, be);
+  f.fit(def.pdf, def.params, &ev);
+  for (size_t k = 0; k < def.params.size(); k++) {
+    values[k] = def.params[k]->get_value();
+    errors[k] = def.params[k]->get_error();
+  }
+  if (out) {
+    out->fval = f.fval_;
+    out->edm = f.edm_;
+    out->status = f.status_;
+    out->ncalls = f.ncalls_;
+    out->iterations = 0;
+    out->hesse_ok = !f.cov_.empty();
+  }
+  if (ms) *ms = f.fit_ms_;
+}
+}  // namespace
+
+extern "C" {
+
+int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, double* values,
+                  double* errors, mfh_fit_out* out, double* ms) {
+  MFH_TRY
+  if (fp32) run_fit<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms);
+  else run_fit<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms);
+  MFH_CATCH
+}
+
+int mfh_fit_cpu(int kind, const double* data, long long n, const double* start, int nthreads, double* values, double* errors,
+                mfh_fit_out* out, double* ms) {
+  MFH_TRY
+  compute_options co;
+  co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
+  CPUBackend be(&co);
+  run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms);
+  MFH_CATCH
+}
+
+int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess) {
+  MFH_TRY
+  compute_options co;
+  co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
+  CPUBackend be(&co);
+  ModelDef<double> def(kind, nullptr);
+  EventVector<double, double> ev(def.dimv, (unsigned long)n);
+  for (size_t d = 0; d < def.dimv.size(); d++)
+    for (long long i = 0; i < n; i++) ev(i, (unsigned)d) = data[d * n + i];
+  fitter_options fo;
+  fitter<double, double, CPUBackend, CPUBlock<double, double>> f(&fo, &be);
+  f.prepare(def.pdf, def.params, &ev, false);
+  for (size_t k = 0; k < def.params.size(); k++) def.params[k]->set_value(theta[k]);
+  const size_t N = def.params.size();
+  std::vector<double> g(N), h(N * N);
+  double v = f.likelihood_and_gradient((int)N, g.data());
+  f.hessian((int)N, h.data());
+  if (value) *value = v;
+  if (grad) std::memcpy(grad, g.data(), N * sizeof(double));
+  if (hess) std::memcpy(hess, h.data(), N * N * sizeof(double));
+  MFH_CATCH
+}
+
+}  // extern "C"

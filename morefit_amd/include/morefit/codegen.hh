@@ -1,0 +1,106 @@
+/**
+ * @file codegen.hh
+ * Turns graphs into the source of one likelihood module: functor structs (one per kernel
+ * kind, all outputs of a kind emitted together with CSE), the FP64 hoisted-term function
+ * mf_hoist, the generator's proposal box, and the kernel instantiation lines.  The same
+ * functor text compiles for gfx950 through hipRTC (with the device runtime prepended by
+ * libmorefit_hip) and for the host CPU backend (with host definitions of the mf_* wrappers).
+ *
+ * Reference counterpart: the string building in compute_opencl.hh:646-879 and
+ * compute_llvm.hh:535-750 (one expression per output, unsuffixed decimal constants).
+ */
+#ifndef MOREFIT_CODEGEN_HH
+#define MOREFIT_CODEGEN_HH
+
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "graph.hh"
+
+namespace morefit {
+
+struct ModuleSource {
+  bool fp32 = false;
+  int nobs = 1;
+  int ntheta = 0;
+  int nhoist = 0;
+  int block = 256;
+  int acc = 1;
+  bool fast_math = false;
+  std::vector<double> gen_lo, gen_width;  // proposal box per observable, in stored coordinates
+  std::string functors;                    // functor structs
+  std::string hoist;                       // mf_hoist definition
+  std::vector<std::string> kernels;        // instantiation lines
+  size_t statements = 0;                   // SSA statements emitted (all functors)
+
+  std::string prelude() const {
+    std::ostringstream s;
+    s << "#define MF_T " << (fp32 ? "float" : "double") << "\n"
+      << "#define MF_FP32 " << (fp32 ? 1 : 0) << "\n"
+      << "#define MF_NOBS " << nobs << "\n"
+      << "#define MF_NTHETA " << ntheta << "\n"
+      << "#define MF_NHOIST " << nhoist << "\n"
+      << "#define MF_BLOCK " << block << "\n"
+      << "#define MF_ACC " << acc << "\n"
+      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n";
+    s << "#define MF_GEN_PROPOSE(x, s) do {";
+    for (int d = 0; d < nobs; d++) {
+      double lo = d < (int)gen_lo.size() ? gen_lo[d] : 0.0, w = d < (int)gen_width.size() ? gen_width[d] : 1.0;
+      char buf[160];
+      if (fp32) std::snprintf(buf, sizeof(buf), " x[%d] = %af + %af * mf_uniform(s);", d, (double)(float)lo, (double)(float)w);
+      else std::snprintf(buf, sizeof(buf), " x[%d] = %a + %a * mf_uniform(s);", d, lo, w);
+      s << buf;
+    }
+    s << " } while (0)\n";
+    return s.str();
+  }
+  // generated module text: prelude, the runtime marker, then functors/hoist/kernels
+  std::string device_source() const {
+    std::ostringstream s;
+    s << "// generated by morefit codegen.hh\n" << prelude() << "//@MF_RUNTIME@\n" << functors << "\n" << hoist << "\n";
+    for (const auto& k : kernels) s << k << "\n";
+    return s.str();
+  }
+};
+
+// struct NAME { nout; eval(x, p, o) } computing `outputs` from observables x[d] and kernel
+// parameters p[k]
+template <typename kernelT, typename evalT>
+std::string emit_functor(const std::string& name, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& outputs,
+                         const std::vector<std::string>& observables, const std::vector<std::string>& param_names,
+                         bool fp32, size_t* statements = nullptr) {
+  KernelEmitter em(fp32);
+  for (size_t d = 0; d < observables.size(); d++) em.bindings[observables[d]] = "x[" + std::to_string(d) + "]";
+  for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
+  std::vector<std::string> res;
+  for (auto* g : outputs) res.push_back(em.emit(*g));
+  std::ostringstream s;
+  s << "struct " << name << " {\n  static constexpr int nout = " << outputs.size() << ";\n"
+    << "  __device__ __forceinline__ static void eval(const MF_T* __restrict__ x, const MF_T* __restrict__ p, MF_T* __restrict__ o) {\n"
+    << "    (void)x; (void)p;\n"
+    << em.body.str();
+  for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << res[k] << ";\n";
+  s << "  }\n};\n";
+  if (statements) *statements += (size_t)em.statements;
+  return s.str();
+}
+
+// void mf_hoist(const double* th, double* buf): FP64 values of the hoisted expressions
+template <typename kernelT, typename evalT>
+std::string emit_hoist(const std::vector<const ComputeGraphNode<kernelT, evalT>*>& exprs, const std::vector<std::string>& theta_names) {
+  KernelEmitter em(false);
+  em.prefix = "mfd_";
+  em.tmp = "h";
+  for (size_t k = 0; k < theta_names.size(); k++) em.bindings[theta_names[k]] = "th[" + std::to_string(k) + "]";
+  std::vector<std::string> res;
+  for (auto* g : exprs) res.push_back(em.emit(*g));
+  std::ostringstream s;
+  s << "__device__ void mf_hoist(const double* th, double* buf) {\n  (void)th; (void)buf;\n" << em.body.str();
+  for (size_t k = 0; k < res.size(); k++) s << "  buf[" << k << "] = " << res[k] << ";\n";
+  s << "}\n";
+  return s.str();
+}
+
+}  // namespace morefit
+#endif

@@ -1,0 +1,250 @@
+/**
+ * @file compute_cpu.hh
+ * Host CPU backend with the semantics of the reference LLVM backend (compute_llvm.hh:212-941):
+ * the same generated functor text as the GPU modules, compiled here by the host C++ compiler
+ * into a shared object and loaded with dlopen (the reference JIT-compiles C with Clang/ORC);
+ * contiguous per-thread event slices with the last thread taking the remainder, an in-loop
+ * Kahan sum per output in evalT, and a Kahan sum of the thread partials (compute_llvm.hh:
+ * 556-709, 785-850).  It runs the whole graph -> code -> fit pipeline where no GPU exists.
+ */
+#ifndef MOREFIT_COMPUTE_CPU_HH
+#define MOREFIT_COMPUTE_CPU_HH
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "codegen.hh"
+#include "compute.hh"
+#include "utils.hh"
+
+namespace morefit {
+
+class CPUBackend : public ComputeBackend {
+ public:
+  explicit CPUBackend(compute_options* opts) : ComputeBackend(opts) {}
+  int required_padding() const override { return 0; }
+};
+
+inline std::string cpu_cache_dir() {
+  const char* env = std::getenv("MOREFIT_KCACHE");
+  if (env && *env) return env;
+  const char* tmp = std::getenv("TMPDIR");
+  return std::string(tmp && *tmp ? tmp : "/tmp") + "/morefit_cpu_kcache";
+}
+
+inline uint64_t cpu_fnv1a(const std::string& s) {
+  uint64_t h = 1469598103934665603ULL;
+  for (unsigned char c : s) {
+    h ^= c;
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+// host translation unit around generated functors
+inline std::string cpu_source(const ModuleSource& ms, const std::string& kernel_fn) {
+  std::ostringstream s;
+  s << "#include <cmath>\n#include <cstddef>\n"
+    << "#define __device__\n#define __forceinline__ inline\n"
+    << "typedef " << (ms.fp32 ? "float" : "double") << " MF_T;\n";
+  const char* fns[] = {"exp", "log", "sqrt", "sin", "cos", "tan", "asin", "acos", "atan", "erf"};
+  for (auto* f : fns) {
+    s << "static inline MF_T mf_" << f << "(MF_T x) { return std::" << f << "(x); }\n";
+    s << "static inline double mfd_" << f << "(double x) { return std::" << f << "(x); }\n";
+  }
+  s << "static inline MF_T mf_rcp(MF_T x) { return (MF_T)1 / x; }\n"
+    << "static inline double mfd_rcp(double x) { return 1.0 / x; }\n"
+    << "static inline MF_T mf_pow(MF_T x, MF_T y) { return std::pow(x, y); }\n"
+    << "static inline double mfd_pow(double x, double y) { return std::pow(x, y); }\n"
+    << ms.functors << "\n";
+  // sums: in-loop Kahan per output in double (evalT), events [from, to)
+  s << "extern \"C\" void " << kernel_fn << "_sum(const MF_T* data, long long stride, long long from, long long to, const MF_T* p, double* out) {\n"
+    << "  const int NO = mf_f_block::nout;\n  double sum[NO], c[NO];\n  for (int k = 0; k < NO; k++) { sum[k] = 0.0; c[k] = 0.0; }\n"
+    << "  for (long long i = from; i < to; i++) {\n    MF_T x[" << ms.nobs << "];\n"
+    << "    for (int d = 0; d < " << ms.nobs << "; d++) x[d] = data[d * stride + i];\n"
+    << "    MF_T o[NO];\n    mf_f_block::eval(x, p, o);\n"
+    << "    for (int k = 0; k < NO; k++) { double y = (double)o[k] - c[k]; volatile double t = sum[k] + y; c[k] = (t - sum[k]) - y; sum[k] = t; }\n"
+    << "  }\n  for (int k = 0; k < NO; k++) out[k] = sum[k];\n}\n";
+  s << "extern \"C\" void " << kernel_fn << "_store(const MF_T* data, long long stride, long long from, long long to, const MF_T* p, MF_T* out, long long ostride) {\n"
+    << "  const int NO = mf_f_block::nout;\n"
+    << "  for (long long i = from; i < to; i++) {\n    MF_T x[" << ms.nobs << "];\n"
+    << "    for (int d = 0; d < " << ms.nobs << "; d++) x[d] = data[d * stride + i];\n"
+    << "    MF_T o[NO];\n    mf_f_block::eval(x, p, o);\n    for (int k = 0; k < NO; k++) out[k * ostride + i] = o[k];\n  }\n}\n";
+  return s.str();
+}
+
+// compiles (or reuses) a shared object for `src`; returns the dlopen handle
+inline void* cpu_compile(const std::string& src, const compute_options& o) {
+  std::string key = src + "\n" + o.cpu_compiler + " " + o.cpu_flags;
+  char name[64];
+  std::snprintf(name, sizeof(name), "/cpu_%016llx", (unsigned long long)cpu_fnv1a(key));
+  std::string dir = cpu_cache_dir();
+  ::mkdir(dir.c_str(), 0755);
+  std::string base = dir + name, so = base + ".so";
+  if (::access(so.c_str(), R_OK) != 0) {
+    std::string cc = base + "." + std::to_string(::getpid()) + ".cc";
+    {
+      std::ofstream f(cc);
+      f << src;
+    }
+    std::string tmpso = base + "." + std::to_string(::getpid()) + ".so";
+    std::string cmd = o.cpu_compiler + " " + o.cpu_flags + " -std=c++17 -fPIC -shared " + cc + " -o " + tmpso + " 2>&1";
+    FILE* pp = ::popen(cmd.c_str(), "r");
+    std::string log;
+    char buf[512];
+    while (pp && std::fgets(buf, sizeof(buf), pp)) log += buf;
+    int rc = pp ? ::pclose(pp) : -1;
+    ::unlink(cc.c_str());
+    if (rc != 0) throw std::runtime_error("CPU backend: compilation failed:\n" + log);
+    ::rename(tmpso.c_str(), so.c_str());
+  }
+  void* h = ::dlopen(so.c_str(), RTLD_NOW | RTLD_LOCAL);
+  if (!h) throw std::runtime_error(std::string("CPU backend: dlopen failed: ") + ::dlerror());
+  return h;
+}
+
+template <typename kernelT, typename evalT, typename seedT = uint32_t>
+class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
+  typedef void (*sum_fn)(const kernelT*, long long, long long, long long, const kernelT*, double*);
+  typedef void (*store_fn)(const kernelT*, long long, long long, long long, const kernelT*, kernelT*, long long);
+  struct State {
+    CPUBackend* be = nullptr;
+    std::shared_ptr<std::vector<kernelT>> input, output;
+    std::vector<kernelT> params;
+    std::vector<double> sums;
+    std::vector<double> centres;
+    void* handle = nullptr;
+    sum_fn fsum = nullptr;
+    store_fn fstore = nullptr;
+    bool kahan = true;
+    int nout = 0;
+    long long nevents = 0, stride = 0;
+    ~State() {
+      if (handle) ::dlclose(handle);
+    }
+  };
+  std::shared_ptr<State> s_;
+
+ public:
+  explicit CPUBlock(CPUBackend* backend) : s_(std::make_shared<State>()) { s_->be = backend; }
+  void SetCentres(const std::vector<double>& c) { s_->centres = c; }
+  bool SetupInputBuffer(unsigned long int nbytes) override {
+    s_->input = std::make_shared<std::vector<kernelT>>(nbytes / sizeof(kernelT));
+    return true;
+  }
+  bool SetupInputBuffer(ComputeBlock<kernelT, evalT>* other, bool use_data_in = false) override {
+    auto* o = dynamic_cast<CPUBlock<kernelT, evalT>*>(other);
+    if (!o) throw std::runtime_error("CPUBlock::SetupInputBuffer: source block is not a CPUBlock");
+    s_->input = use_data_in ? o->s_->input : o->s_->output;
+    return true;
+  }
+  bool SetupSeedBuffer(unsigned long int) override { return true; }
+  bool SetupParameterBuffer(unsigned long int) override { return true; }
+  bool SetupOutputBuffer(unsigned long int nbytes) override {
+    s_->output = std::make_shared<std::vector<kernelT>>(nbytes / sizeof(kernelT));
+    return true;
+  }
+  bool SetNevents(int nevents, int nevents_padded = -1) override {
+    s_->nevents = nevents;
+    s_->stride = nevents_padded > 0 ? nevents_padded : nevents;
+    return true;
+  }
+  template <typename srcT>
+  bool CopyToInputBufferAny(const EventVector<srcT, evalT>& data) {
+    const size_t nd = data.ndimensions(), np = data.nevents_padded();
+    s_->input = std::make_shared<std::vector<kernelT>>(nd * np);
+    for (size_t d = 0; d < nd; d++) {
+      double c = d < s_->centres.size() ? s_->centres[d] : 0.0;
+      for (size_t i = 0; i < np; i++) (*s_->input)[d * np + i] = (kernelT)((double)data.get_data()[d * np + i] - c);
+    }
+    return true;
+  }
+  bool CopyToInputBuffer(const EventVector<kernelT, evalT>& data) override { return CopyToInputBufferAny(data); }
+  template <typename srcT, typename std::enable_if<!std::is_same<srcT, kernelT>::value, int>::type = 0>
+  bool CopyToInputBuffer(const EventVector<srcT, evalT>& data) { return CopyToInputBufferAny(data); }
+  bool CopyToSeedBuffer(const EventVector<seedT, evalT>&) override { return true; }
+  bool CopyToParameterBuffer(const std::vector<kernelT>& params) override {
+    s_->params = params;
+    if (s_->params.empty()) s_->params.push_back(kernelT(0));
+    return true;
+  }
+  bool CopyFromOutputBuffer(EventVector<kernelT, evalT>& data) override {
+    std::copy(s_->output->begin(), s_->output->begin() + std::min(s_->output->size(), data.buffer_size() / sizeof(kernelT)), data.get_data());
+    return true;
+  }
+  bool CopyFromKahanBuffer(std::vector<evalT>& sums) override {
+    sums.assign(s_->sums.begin(), s_->sums.end());
+    return true;
+  }
+  bool ProvidesKahan() override { return true; }
+  bool MakeComputeKernel(std::string name, unsigned int nevents, std::vector<dimension<evalT>> input_signature,
+                         std::vector<dimension<evalT>> output_signature, const std::vector<std::string>& params,
+                         const std::vector<std::unique_ptr<ComputeGraphNode<kernelT, evalT>>>& graphs, bool kahan_summation = false) override {
+    ModuleSource ms;
+    ms.fp32 = std::is_same<kernelT, float>::value;
+    ms.nobs = (int)input_signature.size();
+    std::vector<std::string> obs;
+    for (auto& d : input_signature) obs.push_back(d.get_name());
+    std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
+    for (auto& g : graphs) outs.push_back(g.get());
+    ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
+    std::string src = cpu_source(ms, "mf_cpu");
+    if (s_->be->opts_->print_kernel) std::cout << src << std::endl;
+    if (s_->handle) ::dlclose(s_->handle);
+    s_->handle = cpu_compile(src, *s_->be->opts_);
+    s_->fsum = (sum_fn)::dlsym(s_->handle, "mf_cpu_sum");
+    s_->fstore = (store_fn)::dlsym(s_->handle, "mf_cpu_store");
+    if (!s_->fsum || !s_->fstore) throw std::runtime_error("CPU backend: symbols missing");
+    s_->kahan = kahan_summation;
+    s_->nout = (int)graphs.size();
+    if (s_->nevents == 0) s_->nevents = s_->stride = nevents;
+    return true;
+  }
+  bool SubmitKernel() override {
+    State& s = *s_;
+    const kernelT* data = s.input->data();
+    const kernelT* p = s.params.empty() ? nullptr : s.params.data();
+    unsigned int nt = std::max(1u, s.be->opts_->llvm_nthreads);
+    if ((long long)nt > s.nevents) nt = (unsigned)std::max<long long>(1, s.nevents);
+    if (!s.kahan) {
+      if (!s.output || (long long)s.output->size() < s.stride * s.nout) s.output = std::make_shared<std::vector<kernelT>>(s.stride * s.nout);
+      s.fstore(data, s.stride, 0, s.nevents, p, s.output->data(), s.stride);
+      return true;
+    }
+    std::vector<std::vector<double>> part(nt, std::vector<double>(s.nout, 0.0));
+    const long long size = s.nevents / nt;
+    auto slice = [&](unsigned t) {
+      long long from = t * size, to = (t == nt - 1) ? s.nevents : from + size;
+      s.fsum(data, s.stride, from, to, p, part[t].data());
+    };
+    if (nt == 1) {
+      slice(0);
+    } else {
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nt; t++) th.emplace_back(slice, t);
+      for (auto& t : th) t.join();
+    }
+    s.sums.assign(s.nout, 0.0);
+    for (int k = 0; k < s.nout; k++) {
+      std::vector<double> v;
+      for (unsigned t = 0; t < nt; t++) v.push_back(part[t][k]);
+      s.sums[k] = kahan_summation<double, double>(v);
+    }
+    return true;
+  }
+  bool Finish() override { return true; }
+};
+
+}  // namespace morefit
+#endif

@@ -1,0 +1,787 @@
+/**
+ * @file compute_hip.hh
+ * MI355X backend of the MoreFit compute boundary, over the C ABI of libmorefit_hip
+ * (include/morefit_hip.h).
+ *
+ *  HIPBackend   : ComputeBackend   (reference OpenCLBackend, compute_opencl.hh:30-206)
+ *  HIPBlock     : ComputeBlock     (reference OpenCLBlock, compute_opencl.hh:208-1012): the
+ *                 drop-in block the generic fitter/generator drive exactly as they drive the
+ *                 reference backends (host-evaluated hoisted terms, one launch per call).
+ *  HipModel     : one compiled likelihood module for a PDF and its floating parameters:
+ *                 value, gradient and Hessian kernels sharing one kernel-parameter layout, a
+ *                 device prep kernel computing the hoisted terms in FP64, and the generator.
+ *  HipBatch     : many datasets ("segments": toys, requests, or one big dataset) resident
+ *                 in one HBM arena with a static chunk table; per launch each segment is
+ *                 evaluated or skipped, the launch sequence of a kind is a captured hipGraph.
+ *
+ * FP32 kernels (kernelT = float) store observables as offsets from the range centre and
+ * receive parameter-only terms computed in FP64 and rounded once; per-lane sums are
+ * compensated and block/segment totals are FP64.
+ */
+#ifndef MOREFIT_COMPUTE_HIP_HH
+#define MOREFIT_COMPUTE_HIP_HH
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "codegen.hh"
+#include "compute.hh"
+#include "morefit_hip.h"
+#include "pdf.hh"
+#include "random.hh"
+
+namespace morefit {
+
+inline void hip_check(int rc, const char* what) {
+  if (rc != MF_OK) throw std::runtime_error(std::string(what) + ": " + mf_last_error());
+}
+
+// ---------------------------------------------------------------------------- backend
+class HIPBackend : public ComputeBackend {
+  mf_device* dev_ = nullptr;
+
+ public:
+  explicit HIPBackend(compute_options* opts) : ComputeBackend(opts) {
+    hip_check(mf_device_open(opts->hip_device, &dev_), "HIPBackend: mf_device_open");
+  }
+  ~HIPBackend() override { mf_device_close(dev_); }
+  HIPBackend(const HIPBackend&) = delete;
+  HIPBackend& operator=(const HIPBackend&) = delete;
+  // tails are masked in the kernels, so events never need padding
+  int required_padding() const override { return 0; }
+  mf_device* device() const { return dev_; }
+  mf_device_props props() const {
+    mf_device_props p;
+    hip_check(mf_device_query(dev_, &p), "mf_device_query");
+    return p;
+  }
+};
+
+// device allocation owned by one object
+struct DevBuf {
+  mf_device* dev = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() {}
+  DevBuf(mf_device* d, size_t n) { alloc(d, n); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  void alloc(mf_device* d, size_t n) {
+    if (d == dev && n <= bytes && ptr) return;
+    release();
+    dev = d;
+    hip_check(mf_malloc(d, n, &ptr), "mf_malloc");
+    bytes = n;
+  }
+  void release() {
+    if (ptr) mf_free(dev, ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return (T*)ptr; }
+};
+
+// pinned host allocation
+struct HostBuf {
+  mf_device* dev = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  HostBuf() {}
+  ~HostBuf() { release(); }
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  void alloc(mf_device* d, size_t n) {
+    if (d == dev && n <= bytes && ptr) return;
+    release();
+    dev = d;
+    hip_check(mf_host_alloc(d, n, &ptr), "mf_host_alloc");
+    bytes = n;
+  }
+  void release() {
+    if (ptr) mf_host_free(dev, ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return (T*)ptr; }
+};
+
+// one segment's chunk table [chunk_begin, chunk_end) for n events at `chunk` events per chunk
+inline void append_chunks(std::vector<mf_chunk>& chunks, mf_segment& sg, int seg, long long n, int chunk) {
+  sg.chunk_begin = (int)chunks.size();
+  if (n <= 0) {
+    chunks.push_back({seg, 0, 0, 0});  // an empty chunk still completes the segment (totals 0)
+  } else {
+    for (long long b = 0; b < n; b += chunk) chunks.push_back({seg, (int)b, (int)std::min<long long>(n, b + chunk), 0});
+  }
+  sg.chunk_end = (int)chunks.size();
+}
+
+// ---------------------------------------------------------------------------- HIPBlock
+template <typename kernelT, typename evalT, typename seedT = uint32_t>
+class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
+  struct State {
+    HIPBackend* be = nullptr;
+    std::shared_ptr<DevBuf> input, output;
+    DevBuf params, seeds_states, streams, maxprob, segs, chunks, partials, results, counters;
+    HostBuf results_h;
+    mf_module* mod = nullptr;
+    std::string kernel;
+    int nout = 0;
+    bool kahan = false, generate = false, tables_ready = false;
+    long long nevents = 0, nevents_padded = 0;
+    int nstreams = 0;
+    double maxprob_value = 1.0;
+    std::vector<double> centres;  // observables stored as x - centre (FP32 path), empty: raw
+    ~State() {
+      if (mod) mf_module_unload(mod);
+    }
+  };
+  std::shared_ptr<State> s_;
+
+  mf_device* dev() const { return s_->be->device(); }
+  const compute_options& opts() const { return *s_->be->opts_; }
+
+  void build_tables() {
+    State& s = *s_;
+    if (s.tables_ready) return;
+    mf_segment sg{};
+    sg.data_off = 0;
+    sg.stride = s.nevents_padded > 0 ? s.nevents_padded : s.nevents;
+    sg.n = (int)s.nevents;
+    sg.param_off = 0;
+    sg.out_off = 0;
+    sg.mode = 1;
+    std::vector<mf_chunk> ch;
+    append_chunks(ch, sg, 0, s.nevents, opts().hip_block_size * opts().hip_events_per_thread);
+    s.segs.alloc(dev(), sizeof(mf_segment));
+    s.chunks.alloc(dev(), ch.size() * sizeof(mf_chunk));
+    hip_check(mf_memcpy_h2d(dev(), s.segs.ptr, &sg, sizeof(sg)), "HIPBlock segs");
+    hip_check(mf_memcpy_h2d(dev(), s.chunks.ptr, ch.data(), ch.size() * sizeof(mf_chunk)), "HIPBlock chunks");
+    s.partials.alloc(dev(), ch.size() * std::max(1, s.nout) * sizeof(double));
+    s.results.alloc(dev(), std::max(1, s.nout) * sizeof(double));
+    s.results_h.alloc(dev(), std::max(1, s.nout) * sizeof(double));
+    s.counters.alloc(dev(), sizeof(unsigned int));
+    hip_check(mf_fill_zero(dev(), s.counters.ptr, sizeof(unsigned int)), "HIPBlock counters");
+    hip_check(mf_sync(dev()), "HIPBlock tables");
+    nchunks_ = (int)ch.size();
+    s.tables_ready = true;
+  }
+  int nchunks_ = 0;
+
+  void compile(const ModuleSource& ms) {
+    State& s = *s_;
+    if (s.mod) mf_module_unload(s.mod);
+    s.mod = nullptr;
+    std::string src = ms.device_source();
+    if (opts().print_kernel) std::cout << "KERNEL SOURCE\n" << src << "END KERNEL SOURCE" << std::endl;
+    hip_check(mf_module_load(dev(), src.c_str(), opts().hip_compile_options.c_str(), &s.mod), "HIPBlock: mf_module_load");
+  }
+
+ public:
+  explicit HIPBlock(HIPBackend* backend) : s_(std::make_shared<State>()) { s_->be = backend; }
+
+  // FP32 path: observables are stored as offsets from these centres (set before uploading)
+  void SetCentres(const std::vector<double>& c) { s_->centres = c; }
+  const std::vector<double>& Centres() const { return s_->centres; }
+
+  bool SetupInputBuffer(unsigned long int nbytes) override {
+    s_->input = std::make_shared<DevBuf>(dev(), nbytes);
+    return true;
+  }
+  bool SetupInputBuffer(ComputeBlock<kernelT, evalT>* other, bool use_data_in = false) override {
+    auto* o = dynamic_cast<HIPBlock<kernelT, evalT>*>(other);
+    if (!o) throw std::runtime_error("HIPBlock::SetupInputBuffer: source block is not a HIPBlock");
+    s_->input = use_data_in ? o->s_->input : o->s_->output;
+    if (use_data_in && s_->centres.empty()) s_->centres = o->s_->centres;
+    return s_->input != nullptr;
+  }
+  bool SetupSeedBuffer(unsigned long int nbytes) override { return true; }
+  bool SetupParameterBuffer(unsigned long int nbytes) override {
+    s_->params.alloc(dev(), std::max<unsigned long>(nbytes, sizeof(kernelT)));
+    return true;
+  }
+  bool SetupOutputBuffer(unsigned long int nbytes) override {
+    if (!s_->output || s_->output->bytes < nbytes) s_->output = std::make_shared<DevBuf>(dev(), std::max<unsigned long>(nbytes, 16));
+    return true;
+  }
+  bool SetupKahanBuffer(unsigned long int nbytes) override { return true; }
+  bool SetNevents(int nevents, int nevents_padded = -1) override {
+    if (nevents != s_->nevents || (nevents_padded > 0 ? nevents_padded : nevents) != s_->nevents_padded) s_->tables_ready = false;
+    s_->nevents = nevents;
+    s_->nevents_padded = nevents_padded > 0 ? nevents_padded : nevents;
+    return true;
+  }
+  // uploads SoA data, converting/centring as the kernel expects
+  template <typename srcT>
+  bool CopyToInputBufferAny(const EventVector<srcT, evalT>& data) {
+    const size_t nd = data.ndimensions(), np = data.nevents_padded();
+    if (!s_->input || s_->input->bytes < nd * np * sizeof(kernelT)) SetupInputBuffer(nd * np * sizeof(kernelT));
+    std::vector<kernelT> tmp(nd * np);
+    for (size_t d = 0; d < nd; d++) {
+      double c = d < s_->centres.size() ? s_->centres[d] : 0.0;
+      const srcT* col = data.get_data() + d * np;
+      for (size_t i = 0; i < np; i++) tmp[d * np + i] = (kernelT)((double)col[i] - c);
+    }
+    hip_check(mf_memcpy_h2d(dev(), s_->input->ptr, tmp.data(), tmp.size() * sizeof(kernelT)), "CopyToInputBuffer");
+    hip_check(mf_sync(dev()), "CopyToInputBuffer");
+    return true;
+  }
+  bool CopyToInputBuffer(const EventVector<kernelT, evalT>& data) override { return CopyToInputBufferAny(data); }
+  template <typename srcT, typename std::enable_if<!std::is_same<srcT, kernelT>::value, int>::type = 0>
+  bool CopyToInputBuffer(const EventVector<srcT, evalT>& data) { return CopyToInputBufferAny(data); }
+
+  bool CopyToSeedBuffer(const EventVector<seedT, evalT>& data) override {
+    // reference layout: 4 seed columns of one row per work item; here one row per stream
+    State& s = *s_;
+    const size_t ns = data.nevents();
+    std::vector<unsigned int> st(4 * ns);
+    for (size_t k = 0; k < ns; k++)
+      for (int j = 0; j < 4; j++) st[4 * k + j] = (unsigned int)data(k, (unsigned)j);
+    const long long n = s.nevents;
+    const long long per = ns > 0 ? (n + (long long)ns - 1) / (long long)ns : n;
+    std::vector<mf_gen_stream> streams;
+    for (size_t k = 0; k < ns; k++) {
+      long long b = (long long)k * per, e = std::min(n, b + per);
+      if (b >= e) break;
+      streams.push_back({0, (int)b, (int)e, 0});
+    }
+    s.nstreams = (int)streams.size();
+    s.seeds_states.alloc(dev(), std::max<size_t>(16, st.size() * 4));
+    s.streams.alloc(dev(), std::max<size_t>(16, streams.size() * sizeof(mf_gen_stream)));
+    hip_check(mf_memcpy_h2d(dev(), s.seeds_states.ptr, st.data(), st.size() * 4), "CopyToSeedBuffer");
+    hip_check(mf_memcpy_h2d(dev(), s.streams.ptr, streams.data(), streams.size() * sizeof(mf_gen_stream)), "CopyToSeedBuffer");
+    hip_check(mf_sync(dev()), "CopyToSeedBuffer");
+    return true;
+  }
+  bool CopyToParameterBuffer(const std::vector<kernelT>& params) override {
+    if (s_->params.bytes < params.size() * sizeof(kernelT)) SetupParameterBuffer(params.size() * sizeof(kernelT));
+    if (!params.empty()) hip_check(mf_memcpy_h2d(dev(), s_->params.ptr, params.data(), params.size() * sizeof(kernelT)), "CopyToParameterBuffer");
+    return true;
+  }
+  bool CopyFromOutputBuffer(EventVector<kernelT, evalT>& data) override {
+    const size_t nd = data.ndimensions(), np = data.nevents_padded();
+    if (!s_->output || s_->output->bytes < nd * np * sizeof(kernelT)) throw std::runtime_error("CopyFromOutputBuffer: output buffer too small");
+    hip_check(mf_memcpy_d2h(dev(), data.get_data(), s_->output->ptr, nd * np * sizeof(kernelT)), "CopyFromOutputBuffer");
+    hip_check(mf_sync(dev()), "CopyFromOutputBuffer");
+    if (s_->generate && !s_->centres.empty())
+      for (size_t d = 0; d < nd && d < s_->centres.size(); d++)
+        for (size_t i = 0; i < np; i++) data.get_data()[d * np + i] = (kernelT)((double)data.get_data()[d * np + i] + s_->centres[d]);
+    return true;
+  }
+  bool CopyFromKahanBuffer(std::vector<evalT>& sums) override {
+    sums.resize(s_->nout);
+    const double* r = s_->results_h.template as<double>();
+    for (int k = 0; k < s_->nout; k++) sums[k] = (evalT)r[k];
+    return true;
+  }
+  bool ProvidesKahan() override { return true; }
+
+  bool MakeComputeKernel(std::string name, unsigned int nevents, std::vector<dimension<evalT>> input_signature,
+                         std::vector<dimension<evalT>> output_signature, const std::vector<std::string>& params,
+                         const std::vector<std::unique_ptr<ComputeGraphNode<kernelT, evalT>>>& graphs, bool kahan_summation = false) override {
+    State& s = *s_;
+    ModuleSource ms;
+    ms.fp32 = std::is_same<kernelT, float>::value;
+    ms.nobs = (int)input_signature.size();
+    ms.ntheta = (int)params.size();
+    ms.nhoist = 0;
+    ms.block = opts().hip_block_size;
+    ms.acc = opts().hip_accumulate;
+    ms.fast_math = opts().hip_fast_math;
+    std::vector<std::string> obs;
+    for (auto& d : input_signature) obs.push_back(d.get_name());
+    std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
+    for (auto& g : graphs) outs.push_back(g.get());
+    ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
+    ms.hoist = "__device__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
+    s.kernel = name;
+    ms.kernels.push_back(std::string(kahan_summation ? "MF_EVAL_KERNEL(" : "MF_STORE_KERNEL(") + name + ", mf_f_block)");
+    compile(ms);
+    s.nout = (int)graphs.size();
+    s.kahan = kahan_summation;
+    s.generate = false;
+    if (s.nevents == 0) s.nevents = s.nevents_padded = nevents;
+    s.tables_ready = false;
+    return true;
+  }
+
+  bool MakeGenerateKernel(std::string name, unsigned int nevents, std::vector<dimension<evalT>> input_signature,
+                          std::vector<dimension<evalT>> output_signature, const std::vector<std::string>& params,
+                          const std::vector<std::unique_ptr<ComputeGraphNode<kernelT, evalT>>>& graphs, evalT maxprob = 1.0) override {
+    State& s = *s_;
+    ModuleSource ms;
+    ms.fp32 = std::is_same<kernelT, float>::value;
+    ms.nobs = (int)output_signature.size();
+    ms.ntheta = (int)params.size();
+    ms.block = opts().hip_block_size;
+    std::vector<std::string> obs;
+    for (auto& d : output_signature) {
+      obs.push_back(d.get_name());
+      ms.gen_lo.push_back(d.get_min());
+      ms.gen_width.push_back(d.get_max() - d.get_min());
+    }
+    auto lp = Log<kernelT, evalT>(graphs.at(0)->copy())->simplify();
+    ms.functors = emit_functor<kernelT, evalT>("mf_f_gen", {lp.get()}, obs, params, ms.fp32, &ms.statements);
+    ms.hoist = "__device__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
+    ms.kernels.push_back("MF_GENERATE_KERNEL(mf_f_gen)");
+    compile(ms);
+    s.generate = true;
+    s.kernel = "mf_generate";
+    s.maxprob_value = maxprob;
+    s.nevents = nevents;
+    if (s.nevents_padded < (long long)nevents) s.nevents_padded = nevents;
+    s.centres.clear();
+    return true;
+  }
+
+  bool SubmitKernel() override {
+    State& s = *s_;
+    if (!s.mod) throw std::runtime_error("HIPBlock::SubmitKernel: no kernel");
+    if (s.generate) {
+      mf_segment sg{};
+      sg.stride = s.nevents_padded;
+      sg.n = (int)s.nevents;
+      sg.mode = 1;
+      s.segs.alloc(dev(), sizeof(mf_segment));
+      s.maxprob.alloc(dev(), sizeof(double));
+      hip_check(mf_memcpy_h2d(dev(), s.segs.ptr, &sg, sizeof(sg)), "generate segs");
+      hip_check(mf_memcpy_h2d(dev(), s.maxprob.ptr, &s.maxprob_value, sizeof(double)), "generate maxprob");
+      if (!s.output) throw std::runtime_error("HIPBlock::SubmitKernel: generator needs an output buffer");
+      if (s.params.bytes == 0) SetupParameterBuffer(sizeof(kernelT));
+      mf_gen_args g{};
+      g.data = s.output->ptr;
+      g.segs = s.segs.template as<mf_segment>();
+      g.streams = s.streams.template as<mf_gen_stream>();
+      g.nstreams = s.nstreams;
+      g.states = s.seeds_states.template as<unsigned int>();
+      g.params = s.params.ptr;
+      g.maxprob = s.maxprob.template as<double>();
+      g.proposals = nullptr;
+      const int block = 64;
+      hip_check(mf_launch_generate(s.mod, &g, (s.nstreams + block - 1) / block, block), "mf_launch_generate");
+      return true;
+    }
+    build_tables();
+    if (!s.input) throw std::runtime_error("HIPBlock::SubmitKernel: no input buffer");
+    mf_eval_args a{};
+    a.data = s.input->ptr;
+    a.segs = s.segs.template as<mf_segment>();
+    a.chunks = s.chunks.template as<mf_chunk>();
+    a.nchunks = nchunks_;
+    a.nseg = 1;
+    if (s.params.bytes == 0) SetupParameterBuffer(sizeof(kernelT));
+    a.params = s.params.ptr;
+    a.partials = s.partials.template as<double>();
+    a.results = s.results.template as<double>();
+    a.counters = s.counters.template as<unsigned int>();
+    if (!s.kahan) {
+      if (!s.output) throw std::runtime_error("HIPBlock::SubmitKernel: store kernel needs an output buffer");
+      a.store = s.output->ptr;
+      a.store_stride = s.nevents_padded;
+    }
+    const int grid = opts().hip_grid > 0 ? std::min(opts().hip_grid, nchunks_) : nchunks_;
+    hip_check(mf_launch_eval(s.mod, s.kernel.c_str(), &a, grid, opts().hip_block_size), "mf_launch_eval");
+    if (s.kahan) hip_check(mf_memcpy_d2h(dev(), s.results_h.ptr, s.results.ptr, s.nout * sizeof(double)), "kahan results");
+    return true;
+  }
+  bool Finish() override {
+    hip_check(mf_sync(dev()), "HIPBlock::Finish");
+    return true;
+  }
+  mf_module* module() const { return s_->mod; }
+};
+
+// ---------------------------------------------------------------------------- engine
+enum EvalKind { EVAL_NONE = 0, EVAL_LH = 1, EVAL_GRAD = 2, EVAL_HESS = 3 };
+
+// graphs of one density for a fixed set of floating parameters (fixed ones substituted)
+template <typename kernelT, typename evalT = double>
+struct LikelihoodGraphs {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  std::vector<std::string> obs, theta;
+  std::vector<double> centres, lo, hi;
+  std::vector<Ptr> lh, grad, hess;           // per kind outputs (grad/hess include value first)
+  std::vector<std::string> hoist_names;
+  std::vector<Ptr> hoist_exprs;
+  size_t nodes = 0;
+
+  LikelihoodGraphs(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, bool centre, bool with_hessian,
+                   double hoist_threshold = 0.0) {
+    std::vector<std::string> fixed;
+    std::vector<double> fixed_v;
+    for (auto* p : params) {
+      if (p->is_constant()) {
+        fixed.push_back(p->get_name());
+        fixed_v.push_back(p->get_value());
+      } else {
+        theta.push_back(p->get_name());
+      }
+    }
+    for (auto* d : pdf->dimensions()) {
+      obs.push_back(d->get_name());
+      lo.push_back(d->get_min());
+      hi.push_back(d->get_max());
+      centres.push_back(centre ? d->get_centre() : 0.0);
+    }
+    Ptr lp = pdf->logprob_normalised()->substitute(fixed, fixed_v)->simplify();
+    if (centre) lp = centre_observables(*lp, obs, centres);
+    std::vector<Ptr> d1;
+    for (auto& t : theta) d1.emplace_back(factor_common(*lp->diff(t)->simplify()));
+    std::vector<Ptr> d2;
+    if (with_hessian)
+      for (size_t i = 0; i < theta.size(); i++)
+        for (size_t j = 0; j <= i; j++) d2.emplace_back(factor_common(*d1[i]->diff(theta[j])->simplify()));
+    auto hoist = [&](const Ptr& g) {
+      return g->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, "morefit_parambuffer_", (float)hoist_threshold)->simplify();
+    };
+    lh.emplace_back(hoist(lp));
+    grad.emplace_back(hoist(lp));
+    for (auto& g : d1) grad.emplace_back(hoist(g));
+    hess.emplace_back(hoist(lp));
+    for (auto& g : d1) hess.emplace_back(hoist(g));
+    for (auto& g : d2) hess.emplace_back(hoist(g));
+    for (auto& g : hess) nodes += g->node_count();
+  }
+  std::vector<std::string> param_names() const {
+    std::vector<std::string> r(theta);
+    r.insert(r.end(), hoist_names.begin(), hoist_names.end());
+    return r;
+  }
+  static std::vector<const ComputeGraphNode<kernelT, evalT>*> raw(const std::vector<Ptr>& v) {
+    std::vector<const ComputeGraphNode<kernelT, evalT>*> r;
+    for (auto& g : v) r.push_back(g.get());
+    return r;
+  }
+  ModuleSource module(const compute_options& o, bool with_hessian) const {
+    ModuleSource ms;
+    ms.fp32 = std::is_same<kernelT, float>::value;
+    ms.nobs = (int)obs.size();
+    ms.ntheta = (int)theta.size();
+    ms.nhoist = (int)hoist_names.size();
+    ms.block = o.hip_block_size;
+    ms.acc = o.hip_accumulate;
+    ms.fast_math = o.hip_fast_math;
+    for (size_t d = 0; d < obs.size(); d++) {
+      ms.gen_lo.push_back(lo[d] - centres[d]);
+      ms.gen_width.push_back(hi[d] - lo[d]);
+    }
+    auto pn = param_names();
+    ms.functors = emit_functor("mf_f_lh", raw(lh), obs, pn, ms.fp32, &ms.statements) +
+                  emit_functor("mf_f_grad", raw(grad), obs, pn, ms.fp32, &ms.statements);
+    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), obs, pn, ms.fp32, &ms.statements);
+    ms.hoist = emit_hoist(raw(hoist_exprs), theta);
+    ms.kernels = {"MF_EVAL_KERNEL(mf_eval_lh, mf_f_lh)", "MF_EVAL_KERNEL(mf_eval_grad, mf_f_grad)", "MF_PREP_KERNEL",
+                  "MF_GENERATE_KERNEL(mf_f_lh)"};
+    if (with_hessian) ms.kernels.push_back("MF_EVAL_KERNEL(mf_eval_hess, mf_f_hess)");
+    return ms;
+  }
+};
+
+template <typename kernelT, typename evalT = double>
+class HipModel {
+ public:
+  HIPBackend* be;
+  PDF<kernelT, evalT>* pdf;
+  std::vector<parameter<evalT>*> params;
+  std::vector<int> theta_index;  // position in params of each floating parameter
+  std::vector<std::string> obs;
+  std::vector<double> centres, lo, hi;
+  int nobs = 0, ntheta = 0, nhoist = 0, nparam = 0;
+  int nout[4] = {0, 0, 0, 0};
+  bool fp32 = std::is_same<kernelT, float>::value;
+  bool with_hessian = true;
+  std::string source;
+  size_t statements = 0, graph_nodes = 0;
+  double codegen_ms = 0.0, compile_ms = 0.0;
+  mf_module* mod = nullptr;
+
+  static ModuleSource make_source(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, const compute_options& o,
+                                  bool with_hessian, LikelihoodGraphs<kernelT, evalT>** keep = nullptr) {
+    const bool f32 = std::is_same<kernelT, float>::value;
+    auto* g = new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian);
+    ModuleSource ms = g->module(o, with_hessian);
+    if (keep) *keep = g;
+    else delete g;
+    return ms;
+  }
+
+  HipModel(HIPBackend* backend, PDF<kernelT, evalT>* p, const std::vector<parameter<evalT>*>& ps, bool hessian = true)
+      : be(backend), pdf(p), params(ps), with_hessian(hessian) {
+    auto t0 = std::chrono::steady_clock::now();
+    LikelihoodGraphs<kernelT, evalT>* g = nullptr;
+    ModuleSource ms = make_source(p, ps, *be->opts_, hessian, &g);
+    std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> keep(g);
+    for (size_t i = 0; i < ps.size(); i++)
+      if (!ps[i]->is_constant()) theta_index.push_back((int)i);
+    obs = g->obs;
+    centres = g->centres;
+    lo = g->lo;
+    hi = g->hi;
+    nobs = ms.nobs;
+    ntheta = ms.ntheta;
+    nhoist = ms.nhoist;
+    nparam = ntheta + nhoist;
+    nout[EVAL_LH] = 1;
+    nout[EVAL_GRAD] = 1 + ntheta;
+    nout[EVAL_HESS] = hessian ? 1 + ntheta + ntheta * (ntheta + 1) / 2 : 0;
+    statements = ms.statements;
+    graph_nodes = g->nodes;
+    source = ms.device_source();
+    codegen_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (be->opts_->print_kernel) std::cout << source << std::endl;
+    hip_check(mf_module_load(be->device(), source.c_str(), be->opts_->hip_compile_options.c_str(), &mod), "HipModel: mf_module_load");
+    compile_ms = mf_module_compile_ms(mod);
+  }
+  ~HipModel() { mf_module_unload(mod); }
+  HipModel(const HipModel&) = delete;
+  HipModel& operator=(const HipModel&) = delete;
+
+  // accept-reject envelope at the given floating parameter values (PDF::get_max)
+  double maxprob(const double* theta) const {
+    std::vector<evalT> saved;
+    for (size_t k = 0; k < theta_index.size(); k++) {
+      saved.push_back(params[theta_index[k]]->get_value());
+      params[theta_index[k]]->set_value(theta[k]);
+    }
+    double m = pdf->get_max();
+    for (size_t k = 0; k < theta_index.size(); k++) params[theta_index[k]]->set_value(saved[k]);
+    return m;
+  }
+};
+
+// xoshiro128++ stream states: `n` streams separated by jump() from the seeded state
+inline void xoshiro_streams(uint64_t seed, int n, unsigned int* out) {
+  Xoshiro128pp r;
+  r.setSeed(seed);
+  for (int k = 0; k < n; k++) {
+    for (int j = 0; j < 4; j++) out[4 * k + j] = r.getSeed(j);
+    r.jump();
+  }
+}
+
+template <typename kernelT, typename evalT = double>
+class HipBatch {
+ public:
+  HipModel<kernelT, evalT>* m;
+  mf_device* dev;
+  int S = 0;
+  std::vector<long long> sizes, offsets;
+  long long total_elems = 0;
+  int chunk = 0, nchunks = 0;
+  std::vector<mf_segment> segs_h;
+  DevBuf data, theta_d, params_d, segs_d[4], chunks_d, partials_d, results_d[4], counters_d;
+  HostBuf theta_h, segs_hk[4], results_h[4];
+  mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int> request_;
+
+  HipBatch(HipModel<kernelT, evalT>* model, const std::vector<long long>& seg_sizes) : m(model), dev(model->be->device()) {
+    const compute_options& o = *m->be->opts_;
+    S = (int)seg_sizes.size();
+    sizes = seg_sizes;
+    chunk = o.hip_block_size * o.hip_events_per_thread;
+    std::vector<mf_chunk> ch;
+    long long off = 0;
+    for (int s = 0; s < S; s++) {
+      mf_segment sg{};
+      sg.data_off = off;
+      sg.stride = sizes[s];
+      sg.n = (int)sizes[s];
+      sg.param_off = s * m->nparam;
+      sg.out_off = 0;
+      sg.mode = 0;
+      append_chunks(ch, sg, s, sizes[s], chunk);
+      segs_h.push_back(sg);
+      offsets.push_back(off);
+      off += sizes[s] * m->nobs;
+    }
+    total_elems = off;
+    nchunks = (int)ch.size();
+    data.alloc(dev, std::max<long long>(1, total_elems) * sizeof(kernelT));
+    theta_d.alloc(dev, std::max(1, S * m->ntheta) * sizeof(double));
+    theta_h.alloc(dev, std::max(1, S * m->ntheta) * sizeof(double));
+    params_d.alloc(dev, std::max(1, S * m->nparam) * sizeof(kernelT));
+    chunks_d.alloc(dev, ch.size() * sizeof(mf_chunk));
+    hip_check(mf_memcpy_h2d(dev, chunks_d.ptr, ch.data(), ch.size() * sizeof(mf_chunk)), "HipBatch chunks");
+    int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
+    partials_d.alloc(dev, (size_t)nchunks * nmax * sizeof(double));
+    counters_d.alloc(dev, std::max(1, S) * sizeof(unsigned int));
+    hip_check(mf_fill_zero(dev, counters_d.ptr, std::max(1, S) * sizeof(unsigned int)), "HipBatch counters");
+    for (int k = 1; k <= 3; k++) {
+      if (m->nout[k] == 0) continue;
+      segs_d[k].alloc(dev, S * sizeof(mf_segment));
+      segs_hk[k].alloc(dev, S * sizeof(mf_segment));
+      results_d[k].alloc(dev, (size_t)S * m->nout[k] * sizeof(double));
+      results_h[k].alloc(dev, (size_t)S * m->nout[k] * sizeof(double));
+      mf_segment* t = segs_hk[k].template as<mf_segment>();
+      for (int s = 0; s < S; s++) {
+        t[s] = segs_h[s];
+        t[s].out_off = s * m->nout[k];
+        t[s].mode = 0;
+      }
+    }
+    std::memset(theta_h.ptr, 0, theta_h.bytes);
+    request_.assign(S, 0);
+    hip_check(mf_sync(dev), "HipBatch init");
+  }
+  ~HipBatch() {
+    for (auto* g : graphs)
+      if (g) mf_graph_destroy(g);
+  }
+  HipBatch(const HipBatch&) = delete;
+  HipBatch& operator=(const HipBatch&) = delete;
+
+  double* theta(int s) { return theta_h.template as<double>() + (size_t)s * m->ntheta; }
+  const double* result(int s, int kind) const { return results_h[kind].template as<double>() + (size_t)s * m->nout[kind]; }
+  long long data_bytes() const { return total_elems * (long long)sizeof(kernelT); }
+
+  // host FP64 columns (SoA, col_stride elements apart) -> stored type, centred
+  void upload(int s, const double* cols, long long col_stride) {
+    const long long n = sizes[s];
+    std::vector<kernelT> tmp((size_t)n * m->nobs);
+    for (int d = 0; d < m->nobs; d++)
+      for (long long i = 0; i < n; i++) tmp[d * n + i] = (kernelT)(cols[d * col_stride + i] - m->centres[d]);
+    hip_check(mf_memcpy_h2d(dev, data.template as<kernelT>() + offsets[s], tmp.data(), tmp.size() * sizeof(kernelT)), "HipBatch upload");
+    hip_check(mf_sync(dev), "HipBatch upload");
+  }
+  // stored columns of segment s back as FP64 observables (centre added back)
+  void download(int s, double* cols) {
+    const long long n = sizes[s];
+    std::vector<kernelT> tmp((size_t)n * m->nobs);
+    hip_check(mf_memcpy_d2h(dev, tmp.data(), data.template as<kernelT>() + offsets[s], tmp.size() * sizeof(kernelT)), "HipBatch download");
+    hip_check(mf_sync(dev), "HipBatch download");
+    for (int d = 0; d < m->nobs; d++)
+      for (long long i = 0; i < n; i++) cols[d * n + i] = (double)tmp[d * n + i] + m->centres[d];
+  }
+
+  // prepare kernel parameters of every segment from the staged theta
+  void prep() {
+    hip_check(mf_memcpy_h2d(dev, theta_d.ptr, theta_h.ptr, (size_t)S * m->ntheta * sizeof(double)), "HipBatch theta");
+    hip_check(mf_launch_prep(m->mod, theta_d.template as<double>(), params_d.ptr, nullptr, S), "mf_launch_prep");
+  }
+
+  // On-device accept-reject into every segment with mode != 0 semantics ignored: all segments.
+  // theta_gen: S x ntheta generation values; seeds: one 64-bit seed per segment (streams
+  // separated by jump()).  Returns total proposals.
+  unsigned long long generate(const std::vector<double>& theta_gen, const std::vector<uint64_t>& seeds, int events_per_stream = 64) {
+    for (int s = 0; s < S; s++) std::memcpy(theta(s), &theta_gen[(size_t)s * m->ntheta], m->ntheta * sizeof(double));
+    prep();
+    std::vector<mf_gen_stream> st;
+    std::vector<unsigned int> states;
+    std::vector<double> mp(S);
+    for (int s = 0; s < S; s++) {
+      mp[s] = m->maxprob(&theta_gen[(size_t)s * m->ntheta]);
+      int ns = (int)((sizes[s] + events_per_stream - 1) / events_per_stream);
+      size_t base = states.size();
+      states.resize(base + 4 * (size_t)ns);
+      xoshiro_streams(seeds[s], ns, states.data() + base);
+      for (int k = 0; k < ns; k++)
+        st.push_back({s, (int)((long long)k * events_per_stream), (int)std::min<long long>(sizes[s], (long long)(k + 1) * events_per_stream), 0});
+    }
+    DevBuf st_d(dev, std::max<size_t>(16, st.size() * sizeof(mf_gen_stream)));
+    DevBuf states_d(dev, std::max<size_t>(16, states.size() * 4));
+    DevBuf mp_d(dev, S * sizeof(double));
+    DevBuf seg_d(dev, S * sizeof(mf_segment));
+    DevBuf props_d(dev, S * sizeof(unsigned long long));
+    hip_check(mf_memcpy_h2d(dev, st_d.ptr, st.data(), st.size() * sizeof(mf_gen_stream)), "generate streams");
+    hip_check(mf_memcpy_h2d(dev, states_d.ptr, states.data(), states.size() * 4), "generate states");
+    hip_check(mf_memcpy_h2d(dev, mp_d.ptr, mp.data(), S * sizeof(double)), "generate maxprob");
+    hip_check(mf_memcpy_h2d(dev, seg_d.ptr, segs_h.data(), S * sizeof(mf_segment)), "generate segs");
+    hip_check(mf_fill_zero(dev, props_d.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    mf_gen_args g{};
+    g.data = data.ptr;
+    g.segs = seg_d.template as<mf_segment>();
+    g.streams = st_d.template as<mf_gen_stream>();
+    g.nstreams = (int)st.size();
+    g.states = states_d.template as<unsigned int>();
+    g.params = params_d.ptr;
+    g.maxprob = mp_d.template as<double>();
+    g.proposals = props_d.template as<unsigned long long>();
+    const int block = 64;
+    hip_check(mf_launch_generate(m->mod, &g, (int)((st.size() + block - 1) / block), block), "mf_launch_generate");
+    std::vector<unsigned long long> props(S);
+    hip_check(mf_memcpy_d2h(dev, props.data(), props_d.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    hip_check(mf_sync(dev), "generate");
+    unsigned long long tot = 0;
+    for (auto p : props) tot += p;
+    return tot;
+  }
+
+  void request(int s, int kind) { request_[s] = kind; }
+  int requested(int s) const { return request_[s]; }
+
+  static const char* kernel_name(int kind) {
+    return kind == EVAL_LH ? "mf_eval_lh" : (kind == EVAL_GRAD ? "mf_eval_grad" : "mf_eval_hess");
+  }
+  int grid() const {
+    const int g = m->be->opts_->hip_grid;
+    return g > 0 ? std::min(g, nchunks) : nchunks;
+  }
+  mf_eval_args eval_args(int kind) const {
+    mf_eval_args a{};
+    a.data = data.ptr;
+    a.segs = segs_d[kind].template as<mf_segment>();
+    a.chunks = chunks_d.template as<mf_chunk>();
+    a.nchunks = nchunks;
+    a.nseg = S;
+    a.params = params_d.ptr;
+    a.partials = partials_d.template as<double>();
+    a.results = results_d[kind].template as<double>();
+    a.counters = counters_d.template as<unsigned int>();
+    return a;
+  }
+  // just the evaluation kernel of `kind` (segment table and parameters already on device)
+  void launch_kernel(int kind) {
+    mf_eval_args a = eval_args(kind);
+    hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->be->opts_->hip_block_size), "mf_launch_eval");
+  }
+  // the full per-call sequence of one kind: theta -> prep -> segment modes -> kernel -> totals
+  void enqueue(int kind) {
+    prep();
+    hip_check(mf_memcpy_h2d(dev, segs_d[kind].ptr, segs_hk[kind].ptr, S * sizeof(mf_segment)), "HipBatch segs");
+    launch_kernel(kind);
+    hip_check(mf_memcpy_d2h(dev, results_h[kind].ptr, results_d[kind].ptr, (size_t)S * m->nout[kind] * sizeof(double)), "HipBatch results");
+  }
+  void set_modes(int kind, bool all) {
+    mf_segment* t = segs_hk[kind].template as<mf_segment>();
+    for (int s = 0; s < S; s++) t[s].mode = (all || request_[s] == kind) ? 1 : 0;
+  }
+  // evaluate `kind` on the requested (or all) segments; as a replayed hipGraph when use_graph
+  void run(int kind, bool all, bool use_graph) {
+    if (m->nout[kind] == 0) throw std::runtime_error("HipBatch::run: kind not compiled");
+    set_modes(kind, all);
+    if (use_graph) {
+      if (!graphs[kind]) {
+        hip_check(mf_sync(dev), "graph capture");
+        hip_check(mf_graph_begin(dev), "mf_graph_begin");
+        enqueue(kind);
+        hip_check(mf_graph_end(dev, &graphs[kind]), "mf_graph_end");
+      }
+      hip_check(mf_graph_launch(graphs[kind]), "mf_graph_launch");
+    } else {
+      enqueue(kind);
+    }
+    hip_check(mf_sync(dev), "HipBatch run");
+  }
+  // one launch per kind that any segment requested; requests are cleared
+  int run_requested(bool use_graph) {
+    int launches = 0;
+    for (int k = 1; k <= 3; k++) {
+      bool any = false;
+      for (int s = 0; s < S && !any; s++) any = request_[s] == k;
+      if (!any) continue;
+      run(k, false, use_graph);
+      launches++;
+    }
+    std::fill(request_.begin(), request_.end(), 0);
+    return launches;
+  }
+};
+
+}  // namespace morefit
+#endif

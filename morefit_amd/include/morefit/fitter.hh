@@ -1,0 +1,384 @@
+/**
+ * @file fitter.hh
+ * Unbinned maximum-likelihood fitter over any ComputeBackend/ComputeBlock pair, with the
+ * reference's interface and semantics (fitter.hh:38-1476):
+ *
+ *  make_kernels  log p normalised, fixed parameters substituted, simplified, optionally
+ *                weighted (w * log p); first and second derivatives; parameter-only terms
+ *                hoisted into the parameter buffer [floating theta..., hoisted...]; one
+ *                value, one value+gradient and one Hessian kernel (fitter.hh:321-658)
+ *  likelihood    -2 sum log p (+ extended term)                         (fitter.hh:1152-1220)
+ *  likelihood_and_gradient  -2 sums, zeros for fixed parameters          (fitter.hh:1054-1151)
+ *  hessian       -2 sum d2 log p into h[i + j*n] (+2N/nu^2 for yields)  (fitter.hh:1221-1326)
+ *  fit           MIGRAD (strategy 2, analytic gradient and Hessian, max 4000 calls,
+ *                tolerance 0.01, errordef 1) then HESSE; values and errors written back
+ *                into the parameters                                     (fitter.hh:660-1002)
+ *
+ * The minimiser is this package's Minuit-algorithm MIGRAD (minimizer.hh); the vendored
+ * Minuit2 of the reference is not part of this build.  The Hessian kernel here writes the
+ * Hessian of the floating parameters into their positions in the full parameter matrix
+ * (the reference indexes it by floating index, which only agrees when nothing is fixed).
+ */
+#ifndef MOREFIT_FITTER_HH
+#define MOREFIT_FITTER_HH
+
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "compute.hh"
+#include "eventvector.hh"
+#include "minimizer.hh"
+#include "parametervector.hh"
+#include "pdf.hh"
+#include "utils.hh"
+
+namespace morefit {
+
+struct fitter_options {
+  enum minimizer_type { TMinuit, Minuit2 };
+  minimizer_type minimizer = Minuit2;
+  bool analytic_gradient = true;
+  bool analytic_hessian = true;
+  bool analytic_fisher = false;
+  bool optimize_parameters = true;
+  bool optimize_dimensions = false;
+  bool kahan_on_accelerator = true;
+  int print_level = 0;
+  bool correct_weighted_fit = true;
+  double minuit_strategy = 2;
+  int minuit_printlevel = 0;
+  double minuit_maxiterations = 4000;
+  double minuit_maxcalls = 4000;
+  double minuit_tolerance = 0.1;     // TMinuit only in the reference; MIGRAD here uses 0.01 like Minuit2's default
+  double minuit2_tolerance = 0.01;
+  double minuit_errordef = 1.0;
+  float buffering_cost_threshold = 0.0f;  // hoist every parameter-only operation
+  bool postrun_hesse = true;
+  void print() const {
+    const int w = 40;
+    std::cout << "FITTER OPTIONS\n"
+              << std::setw(w) << std::left << "  analytic gradient " << (analytic_gradient ? "YES" : "NO") << "\n"
+              << std::setw(w) << std::left << "  analytic hessian " << (analytic_hessian ? "YES" : "NO") << "\n"
+              << std::setw(w) << std::left << "  strategy " << minuit_strategy << "\n"
+              << std::setw(w) << std::left << "  postrun hesse " << (postrun_hesse ? "YES" : "NO") << std::endl;
+  }
+};
+
+// blocks that store observables as offsets (FP32 backends) expose SetCentres
+template <typename B, typename = void> struct has_set_centres : std::false_type {};
+template <typename B>
+struct has_set_centres<B, std::void_t<decltype(std::declval<B&>().SetCentres(std::vector<double>()))>> : std::true_type {};
+
+template <typename kernelT, typename evalT, typename backendT, typename computeT>
+class fitter {
+ public:
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  fitter_options* opts_;
+  backendT* backend_;
+  computeT block_, grad_block_, hessian_block_;
+  std::vector<parameter<evalT>*> params_;
+  PDF<kernelT, evalT>* pdf_ = nullptr;
+  std::vector<Ptr> buffer_expressions_, grad_buffer_expressions_, hessian_buffer_expressions_;
+  std::vector<std::string> buffer_names_, grad_buffer_names_, hessian_buffer_names_;
+  std::vector<std::string> floating_names_;
+  std::vector<double> centres_;
+  size_t nevents_ = 0;
+  bool weighted_ = false;
+  // results of the last fit
+  int status_ = -1;
+  double fval_ = NAN, edm_ = NAN;
+  int ncalls_ = 0, ngrad_ = 0, nhess_ = 0;
+  std::vector<double> cov_;
+  double fit_ms_ = 0.0, kernel_ms_ = 0.0;
+
+  fitter(fitter_options* options, backendT* backend)
+      : opts_(options), backend_(backend), block_(backend), grad_block_(backend), hessian_block_(backend) {}
+
+  bool centring() const {
+    return std::is_same<kernelT, float>::value && has_set_centres<computeT>::value && backend_->opts_->hip_centre_observables;
+  }
+
+  // kernel graphs and compilation (fitter.hh:321-658)
+  bool make_kernels(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, const std::vector<dimension<evalT>*>& dims,
+                    const std::string& weight_name) {
+    std::vector<std::string> fixed;
+    std::vector<double> fixed_v;
+    floating_names_.clear();
+    for (auto* p : params) {
+      if (p->is_constant()) {
+        fixed.push_back(p->get_name());
+        fixed_v.push_back(p->get_value());
+      } else {
+        floating_names_.push_back(p->get_name());
+      }
+    }
+    std::vector<std::string> obs;
+    centres_.clear();
+    for (auto* d : dims) {
+      obs.push_back(d->get_name());
+      centres_.push_back(centring() && d->get_name() != weight_name ? d->get_centre() : 0.0);
+    }
+    Ptr lp = pdf->logprob_normalised()->substitute(fixed, fixed_v)->simplify();
+    if (!weight_name.empty()) lp = (Variable<kernelT, evalT>(weight_name) * std::move(lp))->simplify();
+    if (centring()) lp = centre_observables(*lp, obs, centres_);
+    std::vector<Ptr> d1;
+    for (auto& t : floating_names_) d1.emplace_back(factor_common(*lp->diff(t)->simplify()));
+    auto hoisted = [&](std::vector<Ptr>& graphs, std::vector<std::string>& names, std::vector<Ptr>& exprs, const std::string& prefix) {
+      names.clear();
+      exprs.clear();
+      std::vector<Ptr> out;
+      for (auto& g : graphs)
+        out.emplace_back(opts_->optimize_parameters
+                             ? g->optimize_buffering_constant_terms(names, exprs, floating_names_, prefix, opts_->buffering_cost_threshold)->simplify()
+                             : g->copy());
+      return out;
+    };
+    std::vector<dimension<evalT>> in_sig;
+    for (auto* d : dims) in_sig.push_back(*d);
+    auto param_sig = [&](const std::vector<std::string>& names) {
+      std::vector<std::string> r(floating_names_);
+      r.insert(r.end(), names.begin(), names.end());
+      return r;
+    };
+    dimension<evalT> res("res", 0.0, 1.0);
+    {
+      std::vector<Ptr> g;
+      g.emplace_back(lp->copy());
+      auto k = hoisted(g, buffer_names_, buffer_expressions_, "morefit_parambuffer_");
+      block_.MakeComputeKernel("lh_kernel", nevents_, in_sig, {res}, param_sig(buffer_names_), k, true);
+    }
+    if (opts_->analytic_gradient) {
+      std::vector<Ptr> g;
+      g.emplace_back(lp->copy());
+      for (auto& d : d1) g.emplace_back(d->copy());
+      auto k = hoisted(g, grad_buffer_names_, grad_buffer_expressions_, "morefit_parambuffer_");
+      std::vector<dimension<evalT>> out(k.size(), res);
+      grad_block_.MakeComputeKernel("lh_grad_kernel", nevents_, in_sig, out, param_sig(grad_buffer_names_), k, true);
+    }
+    if (opts_->analytic_hessian) {
+      std::vector<Ptr> g;
+      for (size_t i = 0; i < d1.size(); i++)
+        for (size_t j = 0; j <= i; j++) g.emplace_back(factor_common(*d1[i]->diff(floating_names_[j])->simplify()));
+      auto k = hoisted(g, hessian_buffer_names_, hessian_buffer_expressions_, "morefit_parambuffer_");
+      std::vector<dimension<evalT>> out(k.size(), res);
+      hessian_block_.MakeComputeKernel("lh_hessian_kernel", nevents_, in_sig, out, param_sig(hessian_buffer_names_), k, true);
+    }
+    return true;
+  }
+
+  template <typename B>
+  static void apply_centres(B& b, const std::vector<double>& c) {
+    if constexpr (has_set_centres<B>::value) b.SetCentres(c);
+  }
+
+  // upload a dataset (FP64 host data may feed an FP32 kernel) and bind the blocks to it
+  template <typename dataT>
+  void bind_data(EventVector<dataT, evalT>* data) {
+    nevents_ = data->nevents();
+    apply_centres(block_, centres_);
+    block_.SetupInputBuffer(data->nevents_padded() * data->ndimensions() * sizeof(kernelT));
+    block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
+    block_.SetupParameterBuffer((floating_names_.size() + buffer_names_.size() + 1) * sizeof(kernelT));
+    if constexpr (std::is_same<dataT, kernelT>::value) block_.CopyToInputBuffer(*data);
+    else block_.CopyToInputBuffer(*data);
+    if (opts_->analytic_gradient) {
+      apply_centres(grad_block_, centres_);
+      grad_block_.SetupInputBuffer(&block_, true);
+      grad_block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
+      grad_block_.SetupParameterBuffer((floating_names_.size() + grad_buffer_names_.size() + 1) * sizeof(kernelT));
+    }
+    if (opts_->analytic_hessian) {
+      apply_centres(hessian_block_, centres_);
+      hessian_block_.SetupInputBuffer(&block_, true);
+      hessian_block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
+      hessian_block_.SetupParameterBuffer((floating_names_.size() + hessian_buffer_names_.size() + 1) * sizeof(kernelT));
+    }
+  }
+
+  template <typename dataT>
+  bool prepare(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, bool refit) {
+    pdf_ = pdf;
+    params_ = params;
+    nevents_ = data->nevents();
+    weighted_ = data->has_event_weight();
+    if (!refit) make_kernels(pdf, params, data->get_dimensions(), data->event_weight_name());
+    bind_data(data);
+    return true;
+  }
+
+  // parameter buffer: floating values then the hoisted expressions evaluated in FP64 on the host
+  std::vector<kernelT> parameter_buffer(const std::vector<Ptr>& exprs) const {
+    std::vector<std::string> names;
+    std::vector<double> values;
+    std::vector<kernelT> buf;
+    for (auto* p : params_)
+      if (!p->is_constant()) {
+        names.push_back(p->get_name());
+        values.push_back(p->get_value());
+        buf.push_back((kernelT)p->get_value());
+      }
+    for (auto& e : exprs) buf.push_back((kernelT)e->eval(names, values));
+    return buf;
+  }
+  double total_yield() const {
+    double t = 0.0;
+    for (auto* p : pdf_->parameters_) t += p->get_value();
+    return t;
+  }
+  bool is_yield(const parameter<evalT>* p) const {
+    for (auto* q : pdf_->parameters_)
+      if (q->get_name() == p->get_name()) return true;
+    return false;
+  }
+  template <typename B>
+  std::vector<evalT> run(B& b, const std::vector<Ptr>& exprs) {
+    auto t0 = std::chrono::steady_clock::now();
+    b.CopyToParameterBuffer(parameter_buffer(exprs));
+    b.SubmitKernel();
+    b.Finish();
+    std::vector<evalT> sums;
+    b.CopyFromKahanBuffer(sums);
+    kernel_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return sums;
+  }
+
+  double likelihood() { return likelihood(pdf_, params_, (EventVector<kernelT, evalT>*)nullptr); }
+  template <typename dataT = kernelT>
+  evalT likelihood(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data) {
+    auto s = run(block_, buffer_expressions_);
+    double r = s.at(0);
+    if (pdf_->is_extended()) {
+      double nu = total_yield();
+      r += nevents_ * std::log(nu) - nu;
+    }
+    return -2.0 * r;
+  }
+  double likelihood_and_gradient(int npar, double* grad) { return likelihood_and_gradient(pdf_, params_, (EventVector<kernelT, evalT>*)nullptr, npar, grad); }
+  template <typename dataT = kernelT>
+  evalT likelihood_and_gradient(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, int npar,
+                                double* grad) {
+    auto s = run(grad_block_, grad_buffer_expressions_);
+    double r = -2.0 * s.at(0);
+    int k = 0;
+    for (size_t i = 0; i < params_.size(); i++) grad[i] = params_[i]->is_constant() ? 0.0 : -2.0 * s.at(1 + k++);
+    if (pdf_->is_extended()) {
+      double nu = total_yield();
+      r -= 2.0 * (nevents_ * std::log(nu) - nu);
+      for (size_t i = 0; i < params_.size(); i++)
+        if (!params_[i]->is_constant() && is_yield(params_[i])) grad[i] -= 2.0 * (nevents_ / nu - 1.0);
+    }
+    return r;
+  }
+  bool hessian(int npar, double* h) { return hessian(pdf_, params_, (EventVector<kernelT, evalT>*)nullptr, npar, h); }
+  template <typename dataT = kernelT>
+  bool hessian(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, int npar, double* h) {
+    auto s = run(hessian_block_, hessian_buffer_expressions_);
+    const size_t N = params_.size();
+    for (size_t i = 0; i < N * N; i++) h[i] = 0.0;
+    std::vector<int> fl;
+    for (size_t i = 0; i < N; i++)
+      if (!params_[i]->is_constant()) fl.push_back((int)i);
+    size_t idx = 0;
+    for (size_t i = 0; i < fl.size(); i++)
+      for (size_t j = 0; j <= i; j++) {
+        double v = -2.0 * s.at(idx++);
+        h[fl[i] + N * fl[j]] = v;
+        h[fl[j] + N * fl[i]] = v;
+      }
+    if (pdf_->is_extended()) {
+      double nu = total_yield();
+      for (size_t i = 0; i < fl.size(); i++)
+        for (size_t j = 0; j < fl.size(); j++)
+          if (is_yield(params_[fl[i]]) && is_yield(params_[fl[j]])) h[fl[i] + N * fl[j]] += 2.0 * nevents_ / (nu * nu);
+    }
+    return true;
+  }
+
+  void update_parameters(const double* x) {
+    for (size_t i = 0; i < params_.size(); i++) params_[i]->set_value(x[i]);
+  }
+
+  struct Fcn : MinimizerFcn {
+    fitter* f;
+    explicit Fcn(fitter* ff) : f(ff) {}
+    double value(const double* x) override {
+      f->update_parameters(x);
+      return f->likelihood();
+    }
+    double value_grad(const double* x, double* g) override {
+      f->update_parameters(x);
+      return f->likelihood_and_gradient((int)f->params_.size(), g);
+    }
+    bool hessian(const double* x, double* h) override {
+      f->update_parameters(x);
+      return f->hessian((int)f->params_.size(), h);
+    }
+  };
+
+  std::vector<MinimizerParam> minimizer_params(bool reset) {
+    std::vector<MinimizerParam> mp;
+    int idx = 0;
+    for (auto* p : params_) {
+      if (reset) p->set_value(p->get_start_value());
+      MinimizerParam q;
+      q.name = p->get_name();
+      q.value = p->get_value();
+      q.step = p->get_step_size();
+      q.fixed = p->is_constant();
+      if (!p->is_unlimited()) {
+        q.lo = p->get_min();
+        q.hi = p->get_max();
+        q.has_lo = std::isfinite(q.lo);
+        q.has_hi = std::isfinite(q.hi);
+      }
+      p->set_minuit_idx(q.fixed ? -1 : idx++);
+      mp.push_back(q);
+    }
+    return mp;
+  }
+
+  // full fit (fitter.hh:660-1002): returns true when MIGRAD converged
+  template <typename dataT>
+  bool fit(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, bool refit = false) {
+    auto t0 = std::chrono::steady_clock::now();
+    kernel_ms_ = 0.0;
+    prepare(pdf, params, data, refit);
+    Migrad m(minimizer_params(true));
+    m.strategy = (int)opts_->minuit_strategy;
+    m.max_calls = (int)opts_->minuit_maxcalls;
+    m.max_iterations = (int)opts_->minuit_maxiterations;
+    m.tolerance = opts_->minuit2_tolerance;
+    m.up = opts_->minuit_errordef;
+    Fcn fcn(this);
+    m.minimize(fcn);
+    update_parameters(m.x.data());
+    if (opts_->postrun_hesse && opts_->analytic_hessian) m.hesse(fcn);
+    for (size_t i = 0; i < params_.size(); i++)
+      if (!params_[i]->is_constant()) {
+        params_[i]->set_value(m.x[i]);
+        params_[i]->set_error(m.errors[i]);
+      }
+    status_ = m.status;
+    fval_ = m.fval;
+    edm_ = m.edm;
+    ncalls_ = m.ncalls;
+    ngrad_ = m.ngrad;
+    nhess_ = m.nhess;
+    cov_ = m.cov;
+    fit_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (opts_->print_level > 0)
+      std::cout << "fit status " << status_ << " fval " << std::setprecision(12) << fval_ << " edm " << edm_ << " calls " << ncalls_
+                << " time " << fit_ms_ << " ms" << std::endl;
+    return status_ == 0;
+  }
+};
+
+}  // namespace morefit
+#endif

@@ -1,0 +1,995 @@
+/**
+ * @file graph.hh
+ * Symbolic compute graph of MoreFit densities: construction, exact derivatives,
+ * simplification, parameter substitution, hoisting of parameter-only subexpressions,
+ * host evaluation and code emission.
+ *
+ * API parity with the reference graph.hh: the node class names (ConstantNode, VariableNode,
+ * SumNode, ProdNode, NegNode, InvNode, ExpNode, LogNode, PowNode, SqrtNode, SinNode, CosNode,
+ * TanNode, AsinNode, AcosNode, AtanNode, ErfNode, ConditionalNode), the virtual interface of
+ * ComputeGraphNode (graph.hh:93-230: copy, get_kernel, print, diff, substitute, simplify,
+ * cost, eval, variable_in_tree, contains_other_variables, optimize_buffering_constant_terms,
+ * operator==, find_in_list) and the builder functions/operators (graph.hh:2491-3010) are
+ * kept, so PDF code written against the reference compiles unchanged.
+ *
+ * Design differences (MI355X-first):
+ *  - one node representation, dispatched on a kind tag, instead of one class per operation;
+ *  - correct derivative rules for asin/acos/tan (the reference omits the square root in
+ *    asin/acos and uses 1+x^2 for tan, graph.hh:1554-1565, 1758-1769, 1862-1873);
+ *  - emit_kernel(): all outputs of a kernel are emitted together as SSA statements with
+ *    structural common-subexpression elimination (the reference emits one independent
+ *    expression string per output, graph.hh:245-2290), and constants are emitted as exact
+ *    hexadecimal literals (the reference prints 16 digits, graph.hh:252);
+ *  - centre_observables(): rewrites x -> x' + c so an FP32 kernel works on offsets from the
+ *    range centre, with parameter-only parts moved into hoisted FP64 terms.
+ */
+#ifndef MOREFIT_GRAPH_HH
+#define MOREFIT_GRAPH_HH
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <string>
+#include <type_traits>
+#include <unordered_map>
+#include <vector>
+
+namespace morefit {
+
+enum class NodeKind {
+  Constant, Variable, Sum, Prod, Neg, Inv, Exp, Log, Pow, Sqrt,
+  Sin, Cos, Tan, Asin, Acos, Atan, Erf, Conditional
+};
+
+template <typename kernelT, typename evalT> class ComputeGraphNode;
+template <typename kernelT, typename evalT> using GraphPtr = std::unique_ptr<ComputeGraphNode<kernelT, evalT>>;
+
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> make_node(NodeKind kind, std::vector<GraphPtr<kernelT, evalT>> children,
+                                   double value = 0.0, const std::string& name = "", int cmp = 0);
+
+template <typename T> inline std::string type_name_str() {
+  if constexpr (std::is_same_v<T, float>) return "float";
+  else if constexpr (std::is_same_v<T, double>) return "double";
+  else if constexpr (std::is_same_v<T, long double>) return "long double";
+  else return "unknown";
+}
+
+inline std::string hex_double(double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%a", v);
+  return buf;
+}
+
+template <typename kernelT, typename evalT>
+class ComputeGraphNode {
+ public:
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  enum comparator_type { Equal, Unequal, Larger, Smaller, LargerEqual, SmallerEqual };  // all against zero
+  std::vector<Ptr> children_;
+
+ protected:
+  NodeKind kind_;
+  double value_ = 0.0;   // Constant
+  std::string name_;     // Variable
+  int cmp_ = 0;          // Conditional
+
+ public:
+  ComputeGraphNode(NodeKind kind, std::vector<Ptr> children, double value = 0.0, const std::string& name = "", int cmp = 0)
+      : children_(std::move(children)), kind_(kind), value_(value), name_(name), cmp_(cmp) {}
+  virtual ~ComputeGraphNode() = default;
+
+  NodeKind kind() const { return kind_; }
+  double value() const { return value_; }
+  const std::string& name() const { return name_; }
+  int comparator() const { return cmp_; }
+  bool is_constant() const { return kind_ == NodeKind::Constant; }
+  bool is_variable() const { return kind_ == NodeKind::Variable; }
+  const ComputeGraphNode& child(size_t i) const { return *children_.at(i); }
+
+  std::string kernelT_str() const { return type_name_str<kernelT>(); }
+  std::string evalT_str() const { return type_name_str<evalT>(); }
+
+  // ---------------------------------------------------------------- copy
+  std::vector<Ptr> copy_children() const {
+    std::vector<Ptr> r;
+    r.reserve(children_.size());
+    for (const auto& c : children_) r.emplace_back(c->copy());
+    return r;
+  }
+  virtual Ptr copy() const { return make_node<kernelT, evalT>(kind_, copy_children(), value_, name_, cmp_); }
+
+  virtual void rename_variable(const std::string& variable, const std::string& new_name) {
+    if (kind_ == NodeKind::Variable && name_ == variable) name_ = new_name;
+    for (auto& c : children_) c->rename_variable(variable, new_name);
+  }
+
+  // ---------------------------------------------------------------- queries
+  virtual bool variable_in_tree(const std::string& variable) const {
+    if (kind_ == NodeKind::Variable) return name_ == variable;
+    for (const auto& c : children_)
+      if (c->variable_in_tree(variable)) return true;
+    return false;
+  }
+  // true if the tree holds a variable that is not in `variables`
+  virtual bool contains_other_variables(const std::vector<std::string>& variables) const {
+    if (kind_ == NodeKind::Variable) return std::find(variables.begin(), variables.end(), name_) == variables.end();
+    for (const auto& c : children_)
+      if (c->contains_other_variables(variables)) return true;
+    return false;
+  }
+  // true if the tree holds any of `variables`
+  bool contains_any(const std::vector<std::string>& variables) const {
+    if (kind_ == NodeKind::Variable) return std::find(variables.begin(), variables.end(), name_) != variables.end();
+    for (const auto& c : children_)
+      if (c->contains_any(variables)) return true;
+    return false;
+  }
+  void collect_variables(std::set<std::string>& out) const {
+    if (kind_ == NodeKind::Variable) out.insert(name_);
+    for (const auto& c : children_) c->collect_variables(out);
+  }
+  size_t node_count() const {
+    size_t n = 1;
+    for (const auto& c : children_) n += c->node_count();
+    return n;
+  }
+
+  // structural equality
+  bool equals(const ComputeGraphNode& o) const {
+    if (kind_ != o.kind_ || children_.size() != o.children_.size()) return false;
+    if (kind_ == NodeKind::Constant && !(value_ == o.value_ || (std::isnan(value_) && std::isnan(o.value_)))) return false;
+    if (kind_ == NodeKind::Variable && name_ != o.name_) return false;
+    if (kind_ == NodeKind::Conditional && cmp_ != o.cmp_) return false;
+    for (size_t i = 0; i < children_.size(); i++)
+      if (!children_[i]->equals(*o.children_[i])) return false;
+    return true;
+  }
+  virtual bool operator==(const Ptr& rhs) const { return rhs && equals(*rhs); }
+  virtual int find_in_list(std::vector<Ptr>& list) const {
+    for (size_t i = 0; i < list.size(); i++)
+      if (equals(*list[i])) return (int)i;
+    return -1;
+  }
+
+  // ---------------------------------------------------------------- cost heuristic
+  virtual double cost() const {
+    double c = 0.0;
+    for (const auto& ch : children_) c += ch->cost();
+    switch (kind_) {
+      case NodeKind::Constant:
+      case NodeKind::Variable: return 0.0;
+      case NodeKind::Sum:
+      case NodeKind::Prod: return c + (double)(children_.size() - 1);
+      case NodeKind::Neg: return c + 0.5;
+      case NodeKind::Inv:
+      case NodeKind::Sqrt: return c + 4.0;
+      case NodeKind::Pow: return c + 30.0;
+      case NodeKind::Conditional: return c + 1.0;
+      default: return c + 20.0;  // transcendental
+    }
+  }
+
+  // ---------------------------------------------------------------- host evaluation
+  static int cmp_holds(int cmp, double v) {
+    switch (cmp) {
+      case Equal: return v == 0.0;
+      case Unequal: return v != 0.0;
+      case Larger: return v > 0.0;
+      case Smaller: return v < 0.0;
+      case LargerEqual: return v >= 0.0;
+      default: return v <= 0.0;
+    }
+  }
+  static double apply_unary(NodeKind k, double a) {
+    switch (k) {
+      case NodeKind::Neg: return -a;
+      case NodeKind::Inv: return 1.0 / a;
+      case NodeKind::Exp: return std::exp(a);
+      case NodeKind::Log: return std::log(a);
+      case NodeKind::Sqrt: return std::sqrt(a);
+      case NodeKind::Sin: return std::sin(a);
+      case NodeKind::Cos: return std::cos(a);
+      case NodeKind::Tan: return std::tan(a);
+      case NodeKind::Asin: return std::asin(a);
+      case NodeKind::Acos: return std::acos(a);
+      case NodeKind::Atan: return std::atan(a);
+      case NodeKind::Erf: return std::erf(a);
+      default: return NAN;
+    }
+  }
+  // evaluates with the named variables bound; unbound variables give NaN (graph.hh:2224-2234)
+  virtual evalT eval(const std::vector<std::string>& variables, const std::vector<double>& values) const {
+    switch (kind_) {
+      case NodeKind::Constant: return value_;
+      case NodeKind::Variable: {
+        for (size_t i = 0; i < variables.size(); i++)
+          if (variables[i] == name_) return values.at(i);
+        return NAN;
+      }
+      case NodeKind::Sum: {
+        double s = 0.0;
+        for (const auto& c : children_) s += c->eval(variables, values);
+        return s;
+      }
+      case NodeKind::Prod: {
+        double p = 1.0;
+        for (const auto& c : children_) p *= c->eval(variables, values);
+        return p;
+      }
+      case NodeKind::Pow: return std::pow((double)children_[0]->eval(variables, values), (double)children_[1]->eval(variables, values));
+      case NodeKind::Conditional:
+        return cmp_holds(cmp_, children_[0]->eval(variables, values)) ? children_[1]->eval(variables, values)
+                                                                       : children_[2]->eval(variables, values);
+      default: return apply_unary(kind_, children_[0]->eval(variables, values));
+    }
+  }
+
+  // ---------------------------------------------------------------- substitution
+  virtual Ptr substitute(const std::vector<std::string>& variables, const std::vector<double>& values) const {
+    if (kind_ == NodeKind::Variable) {
+      for (size_t i = 0; i < variables.size(); i++)
+        if (variables[i] == name_) return make_node<kernelT, evalT>(NodeKind::Constant, {}, values.at(i));
+      return copy();
+    }
+    std::vector<Ptr> ch;
+    for (const auto& c : children_) ch.emplace_back(c->substitute(variables, values));
+    return make_node<kernelT, evalT>(kind_, std::move(ch), value_, name_, cmp_);
+  }
+  // replaces every occurrence of the variable `variable` by a copy of `replacement`
+  Ptr replace_variable(const std::string& variable, const ComputeGraphNode& replacement) const {
+    if (kind_ == NodeKind::Variable) return name_ == variable ? replacement.copy() : copy();
+    std::vector<Ptr> ch;
+    for (const auto& c : children_) ch.emplace_back(c->replace_variable(variable, replacement));
+    return make_node<kernelT, evalT>(kind_, std::move(ch), value_, name_, cmp_);
+  }
+
+  // ---------------------------------------------------------------- derivatives
+  static Ptr K(double c) { return make_node<kernelT, evalT>(NodeKind::Constant, {}, c); }
+  static Ptr U(NodeKind k, Ptr a) {
+    std::vector<Ptr> v;
+    v.emplace_back(std::move(a));
+    return make_node<kernelT, evalT>(k, std::move(v));
+  }
+  static Ptr B(NodeKind k, Ptr a, Ptr b) {
+    std::vector<Ptr> v;
+    v.emplace_back(std::move(a));
+    v.emplace_back(std::move(b));
+    return make_node<kernelT, evalT>(k, std::move(v));
+  }
+  static Ptr P3(Ptr a, Ptr b, Ptr c) {
+    std::vector<Ptr> v;
+    v.emplace_back(std::move(a));
+    v.emplace_back(std::move(b));
+    v.emplace_back(std::move(c));
+    return make_node<kernelT, evalT>(NodeKind::Prod, std::move(v));
+  }
+
+  virtual Ptr diff(std::string variable) const {
+    if (!variable_in_tree(variable)) return K(0.0);
+    const ComputeGraphNode* a = children_.empty() ? nullptr : children_[0].get();
+    switch (kind_) {
+      case NodeKind::Variable: return K(name_ == variable ? 1.0 : 0.0);
+      case NodeKind::Constant: return K(0.0);
+      case NodeKind::Sum: {
+        std::vector<Ptr> terms;
+        for (const auto& c : children_)
+          if (c->variable_in_tree(variable)) terms.emplace_back(c->diff(variable));
+        return make_node<kernelT, evalT>(NodeKind::Sum, std::move(terms));
+      }
+      case NodeKind::Prod: {  // product rule over all factors
+        std::vector<Ptr> terms;
+        for (size_t i = 0; i < children_.size(); i++) {
+          if (!children_[i]->variable_in_tree(variable)) continue;
+          std::vector<Ptr> f;
+          for (size_t j = 0; j < children_.size(); j++) f.emplace_back(i == j ? children_[j]->diff(variable) : children_[j]->copy());
+          terms.emplace_back(make_node<kernelT, evalT>(NodeKind::Prod, std::move(f)));
+        }
+        return make_node<kernelT, evalT>(NodeKind::Sum, std::move(terms));
+      }
+      case NodeKind::Neg: return U(NodeKind::Neg, a->diff(variable));
+      case NodeKind::Inv:  // -(a') / a^2
+        return U(NodeKind::Neg, P3(a->diff(variable), U(NodeKind::Inv, a->copy()), U(NodeKind::Inv, a->copy())));
+      case NodeKind::Exp: return B(NodeKind::Prod, copy(), a->diff(variable));
+      case NodeKind::Log: return B(NodeKind::Prod, a->diff(variable), U(NodeKind::Inv, a->copy()));
+      case NodeKind::Sqrt: return P3(K(0.5), a->diff(variable), U(NodeKind::Inv, copy()));
+      case NodeKind::Sin: return B(NodeKind::Prod, U(NodeKind::Cos, a->copy()), a->diff(variable));
+      case NodeKind::Cos: return U(NodeKind::Neg, B(NodeKind::Prod, U(NodeKind::Sin, a->copy()), a->diff(variable)));
+      case NodeKind::Tan: {  // (1 + tan^2 a) a'
+        std::vector<Ptr> s;
+        s.emplace_back(K(1.0));
+        s.emplace_back(B(NodeKind::Prod, copy(), copy()));
+        return B(NodeKind::Prod, make_node<kernelT, evalT>(NodeKind::Sum, std::move(s)), a->diff(variable));
+      }
+      case NodeKind::Asin:
+      case NodeKind::Acos: {  // +-a' / sqrt(1 - a^2)
+        std::vector<Ptr> s;
+        s.emplace_back(K(1.0));
+        s.emplace_back(U(NodeKind::Neg, B(NodeKind::Prod, a->copy(), a->copy())));
+        Ptr r = B(NodeKind::Prod, a->diff(variable), U(NodeKind::Inv, U(NodeKind::Sqrt, make_node<kernelT, evalT>(NodeKind::Sum, std::move(s)))));
+        return kind_ == NodeKind::Asin ? std::move(r) : U(NodeKind::Neg, std::move(r));
+      }
+      case NodeKind::Atan: {  // a' / (1 + a^2)
+        std::vector<Ptr> s;
+        s.emplace_back(K(1.0));
+        s.emplace_back(B(NodeKind::Prod, a->copy(), a->copy()));
+        return B(NodeKind::Prod, a->diff(variable), U(NodeKind::Inv, make_node<kernelT, evalT>(NodeKind::Sum, std::move(s))));
+      }
+      case NodeKind::Erf:  // 2/sqrt(pi) exp(-a^2) a'
+        return P3(K(1.1283791670955126), U(NodeKind::Exp, U(NodeKind::Neg, B(NodeKind::Prod, a->copy(), a->copy()))), a->diff(variable));
+      case NodeKind::Pow: {
+        const auto& e = children_[1];
+        if (!e->variable_in_tree(variable)) {  // e * a^(e-1) * a'
+          std::vector<Ptr> s;
+          s.emplace_back(e->copy());
+          s.emplace_back(K(-1.0));
+          return P3(e->copy(), B(NodeKind::Pow, a->copy(), make_node<kernelT, evalT>(NodeKind::Sum, std::move(s))), a->diff(variable));
+        }
+        // a^e (e' log a + e a'/a)
+        std::vector<Ptr> s;
+        s.emplace_back(B(NodeKind::Prod, e->diff(variable), U(NodeKind::Log, a->copy())));
+        s.emplace_back(P3(e->copy(), a->diff(variable), U(NodeKind::Inv, a->copy())));
+        return B(NodeKind::Prod, copy(), make_node<kernelT, evalT>(NodeKind::Sum, std::move(s)));
+      }
+      case NodeKind::Conditional: {
+        std::vector<Ptr> ch;
+        ch.emplace_back(children_[0]->copy());
+        ch.emplace_back(children_[1]->diff(variable));
+        ch.emplace_back(children_[2]->diff(variable));
+        return make_node<kernelT, evalT>(NodeKind::Conditional, std::move(ch), 0.0, "", cmp_);
+      }
+    }
+    return K(0.0);
+  }
+
+  // ---------------------------------------------------------------- simplification
+  virtual Ptr simplify() const {
+    std::vector<Ptr> ch;
+    for (const auto& c : children_) ch.emplace_back(c->simplify());
+    auto all_const = [&ch]() {
+      for (const auto& c : ch)
+        if (!c->is_constant()) return false;
+      return true;
+    };
+    switch (kind_) {
+      case NodeKind::Constant:
+      case NodeKind::Variable: return copy();
+      case NodeKind::Sum: {
+        std::vector<Ptr> flat;
+        double csum = 0.0;
+        bool has_c = false;
+        std::vector<Ptr> stack;
+        for (auto& c : ch) stack.emplace_back(std::move(c));
+        for (size_t i = 0; i < stack.size(); i++) {
+          Ptr& c = stack[i];
+          if (c->kind() == NodeKind::Sum) {
+            for (auto& g : c->children_) flat.emplace_back(std::move(g));
+          } else {
+            flat.emplace_back(std::move(c));
+          }
+        }
+        std::vector<Ptr> out;
+        for (auto& c : flat) {
+          if (c->is_constant()) { csum += c->value(); has_c = true; }
+          else out.emplace_back(std::move(c));
+        }
+        if (has_c && csum != 0.0) out.emplace_back(K(csum));
+        if (out.empty()) return K(has_c ? csum : 0.0);
+        if (out.size() == 1) return std::move(out[0]);
+        return make_node<kernelT, evalT>(NodeKind::Sum, std::move(out));
+      }
+      case NodeKind::Prod: {
+        std::vector<Ptr> flat;
+        for (auto& c : ch) {
+          if (c->kind() == NodeKind::Prod)
+            for (auto& g : c->children_) flat.emplace_back(std::move(g));
+          else
+            flat.emplace_back(std::move(c));
+        }
+        double cprod = 1.0;
+        std::vector<Ptr> out;
+        bool negate = false;
+        for (auto& c : flat) {
+          if (c->is_constant()) cprod *= c->value();
+          else if (c->kind() == NodeKind::Neg) { negate = !negate; out.emplace_back(std::move(c->children_[0])); }
+          else out.emplace_back(std::move(c));
+        }
+        if (negate) cprod = -cprod;
+        if (cprod == 0.0) return K(0.0);
+        if (out.empty()) return K(cprod);
+        if (cprod == 1.0 && out.size() == 1) return std::move(out[0]);
+        if (cprod == -1.0 && out.size() == 1) return U(NodeKind::Neg, std::move(out[0]));
+        if (cprod != 1.0) out.insert(out.begin(), K(cprod));
+        return make_node<kernelT, evalT>(NodeKind::Prod, std::move(out));
+      }
+      case NodeKind::Neg: {
+        Ptr& a = ch[0];
+        if (a->is_constant()) return K(-a->value());
+        if (a->kind() == NodeKind::Neg) return std::move(a->children_[0]);
+        if (a->kind() == NodeKind::Prod && a->children_[0]->is_constant()) {
+          Ptr r = a->copy();
+          r->children_[0] = K(-a->children_[0]->value());
+          if (r->children_[0]->value() == 1.0) {
+            r->children_.erase(r->children_.begin());
+            if (r->children_.size() == 1) return std::move(r->children_[0]);
+          }
+          return r;
+        }
+        return U(NodeKind::Neg, std::move(a));
+      }
+      case NodeKind::Inv: {
+        Ptr& a = ch[0];
+        if (a->is_constant()) return K(1.0 / a->value());
+        if (a->kind() == NodeKind::Inv) return std::move(a->children_[0]);
+        if (a->kind() == NodeKind::Neg) return U(NodeKind::Neg, U(NodeKind::Inv, std::move(a->children_[0])));
+        return U(NodeKind::Inv, std::move(a));
+      }
+      case NodeKind::Exp:
+        if (ch[0]->is_constant()) return K(std::exp(ch[0]->value()));
+        if (ch[0]->kind() == NodeKind::Log) return std::move(ch[0]->children_[0]);
+        return U(kind_, std::move(ch[0]));
+      case NodeKind::Log:
+        if (ch[0]->is_constant()) return K(std::log(ch[0]->value()));
+        if (ch[0]->kind() == NodeKind::Exp) return std::move(ch[0]->children_[0]);
+        return U(kind_, std::move(ch[0]));
+      case NodeKind::Pow: {
+        if (all_const()) return K(std::pow(ch[0]->value(), ch[1]->value()));
+        if (ch[1]->is_constant()) {
+          if (ch[1]->value() == 0.0) return K(1.0);
+          if (ch[1]->value() == 1.0) return std::move(ch[0]);
+        }
+        return B(NodeKind::Pow, std::move(ch[0]), std::move(ch[1]));
+      }
+      case NodeKind::Conditional: {
+        if (ch[0]->is_constant()) return cmp_holds(cmp_, ch[0]->value()) ? std::move(ch[1]) : std::move(ch[2]);
+        if (ch[1]->equals(*ch[2])) return std::move(ch[1]);
+        return make_node<kernelT, evalT>(NodeKind::Conditional, std::move(ch), 0.0, "", cmp_);
+      }
+      default:
+        if (ch[0]->is_constant()) return K(apply_unary(kind_, ch[0]->value()));
+        return U(kind_, std::move(ch[0]));
+    }
+  }
+
+  // ---------------------------------------------------------------- hoisting
+  // Replaces subtrees that depend only on `variables` (the floating parameters) and
+  // constants by new variables prefix+k whose expressions are appended to
+  // bufferexpressions (deduplicated structurally).  Parameter-only operands of a Sum or
+  // Prod that also has event-dependent operands are grouped and hoisted together.
+  static std::string hoist(Ptr expr, std::vector<std::string>& names, std::vector<Ptr>& exprs, const std::string& prefix) {
+    for (size_t i = 0; i < exprs.size(); i++)
+      if (exprs[i]->equals(*expr)) return names[i];
+    std::string n = prefix + std::to_string(exprs.size());
+    names.push_back(n);
+    exprs.emplace_back(std::move(expr));
+    return n;
+  }
+  virtual Ptr optimize_buffering_constant_terms(std::vector<std::string>& buffernames, std::vector<Ptr>& bufferexpressions,
+                                                const std::vector<std::string>& variables,
+                                                const std::string& prefix = "morefit_buffer_",
+                                                float buffering_cost_threshold = 2.0) {
+    if (kind_ == NodeKind::Constant || kind_ == NodeKind::Variable) return copy();
+    if (!contains_other_variables(variables)) {
+      if (cost() > buffering_cost_threshold)
+        return make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, hoist(copy(), buffernames, bufferexpressions, prefix));
+      return copy();
+    }
+    std::vector<Ptr> ch;
+    if (kind_ == NodeKind::Sum || kind_ == NodeKind::Prod) {
+      // parameter-only operands (constants join them when at least one parameter term exists)
+      bool has_param = false;
+      for (auto& c : children_) has_param |= !c->contains_other_variables(variables) && !c->is_constant();
+      std::vector<Ptr> group;
+      for (auto& c : children_) {
+        if (has_param && !c->contains_other_variables(variables)) group.emplace_back(c->copy());
+        else ch.emplace_back(c->optimize_buffering_constant_terms(buffernames, bufferexpressions, variables, prefix, buffering_cost_threshold));
+      }
+      if (!group.empty()) {
+        Ptr g = group.size() == 1 ? std::move(group[0]) : make_node<kernelT, evalT>(kind_, std::move(group));
+        if (g->cost() > buffering_cost_threshold || g->kind() == kind_)
+          ch.emplace_back(make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, hoist(std::move(g), buffernames, bufferexpressions, prefix)));
+        else
+          ch.emplace_back(std::move(g));
+      }
+      return make_node<kernelT, evalT>(kind_, std::move(ch));
+    }
+    for (auto& c : children_)
+      ch.emplace_back(c->optimize_buffering_constant_terms(buffernames, bufferexpressions, variables, prefix, buffering_cost_threshold));
+    return make_node<kernelT, evalT>(kind_, std::move(ch), value_, name_, cmp_);
+  }
+
+  // ---------------------------------------------------------------- printing
+  static const char* kind_str(NodeKind k) {
+    switch (k) {
+      case NodeKind::Constant: return "Constant";
+      case NodeKind::Variable: return "Variable";
+      case NodeKind::Sum: return "Sum";
+      case NodeKind::Prod: return "Prod";
+      case NodeKind::Neg: return "Neg";
+      case NodeKind::Inv: return "Inv";
+      case NodeKind::Exp: return "exp";
+      case NodeKind::Log: return "log";
+      case NodeKind::Pow: return "pow";
+      case NodeKind::Sqrt: return "sqrt";
+      case NodeKind::Sin: return "sin";
+      case NodeKind::Cos: return "cos";
+      case NodeKind::Tan: return "tan";
+      case NodeKind::Asin: return "asin";
+      case NodeKind::Acos: return "acos";
+      case NodeKind::Atan: return "atan";
+      case NodeKind::Erf: return "erf";
+      case NodeKind::Conditional: return "Conditional";
+    }
+    return "?";
+  }
+  static const char* cmp_str(int c) {
+    static const char* s[] = {"==", "!=", ">", "<", ">=", "<="};
+    return s[c];
+  }
+  // single C expression (no CSE); kernels use emit_kernel() instead
+  virtual std::string get_kernel() const {
+    switch (kind_) {
+      case NodeKind::Constant: return "(" + hex_double(value_) + ")";
+      case NodeKind::Variable: return name_;
+      case NodeKind::Sum:
+      case NodeKind::Prod: {
+        std::string r = "(";
+        for (size_t i = 0; i < children_.size(); i++) r += (i ? (kind_ == NodeKind::Sum ? " + " : " * ") : "") + children_[i]->get_kernel();
+        return r + ")";
+      }
+      case NodeKind::Neg: return "(-" + children_[0]->get_kernel() + ")";
+      case NodeKind::Inv: return "(1.0/" + children_[0]->get_kernel() + ")";
+      case NodeKind::Pow: return "pow(" + children_[0]->get_kernel() + ", " + children_[1]->get_kernel() + ")";
+      case NodeKind::Conditional:
+        return "((" + children_[0]->get_kernel() + cmp_str(cmp_) + "0.0) ? " + children_[1]->get_kernel() + " : " + children_[2]->get_kernel() + ")";
+      default: return std::string(kind_str(kind_)) + "(" + children_[0]->get_kernel() + ")";
+    }
+  }
+  virtual void print(unsigned int level = 0) const {
+    std::cout << std::string(2 * level, ' ') << kind_str(kind_);
+    if (kind_ == NodeKind::Constant) std::cout << " " << value_;
+    if (kind_ == NodeKind::Variable) std::cout << " " << name_;
+    if (kind_ == NodeKind::Conditional) std::cout << " " << cmp_str(cmp_) << "0";
+    std::cout << std::endl;
+    for (const auto& c : children_) c->print(level + 1);
+  }
+  virtual std::string draw_node() const {
+    std::string label = kind_ == NodeKind::Constant ? hex_double(value_) : (kind_ == NodeKind::Variable ? name_ : kind_str(kind_));
+    std::string r = "\"" + label + "\"";
+    if (!children_.empty()) {
+      r += " -> {";
+      for (size_t i = 0; i < children_.size(); i++) r += (i ? ", " : "") + children_[i]->draw_node();
+      r += "}";
+    }
+    return r;
+  }
+  void draw(std::string filename) const {
+    std::ofstream f(filename);
+    f << "\\documentclass[tikz,border=10pt]{standalone}\n\\usetikzlibrary{graphdrawing,graphs}\n\\usegdlibrary{trees}\n"
+         "\\begin{document}\n\\begin{tikzpicture}\n\\graph [tree layout, fresh nodes]\n{\n"
+      << draw_node() << "};\n\\end{tikzpicture}\n\\end{document}\n";
+  }
+};
+
+// ==================================================================== node classes
+#define MOREFIT_UNARY_NODE(NAME, KIND)                                                           \
+  template <typename kernelT, typename evalT>                                                  \
+  class NAME : public ComputeGraphNode<kernelT, evalT> {                                       \
+   public:                                                                                     \
+    explicit NAME(GraphPtr<kernelT, evalT> a)                                                  \
+        : ComputeGraphNode<kernelT, evalT>(NodeKind::KIND, [&] {                               \
+            std::vector<GraphPtr<kernelT, evalT>> v;                                           \
+            v.emplace_back(std::move(a));                                                      \
+            return v;                                                                          \
+          }()) {}                                                                              \
+    NAME(NodeKind k, std::vector<GraphPtr<kernelT, evalT>> ch) : ComputeGraphNode<kernelT, evalT>(k, std::move(ch)) {} \
+  };
+
+MOREFIT_UNARY_NODE(NegNode, Neg)
+MOREFIT_UNARY_NODE(InvNode, Inv)
+MOREFIT_UNARY_NODE(ExpNode, Exp)
+MOREFIT_UNARY_NODE(LogNode, Log)
+MOREFIT_UNARY_NODE(SqrtNode, Sqrt)
+MOREFIT_UNARY_NODE(SinNode, Sin)
+MOREFIT_UNARY_NODE(CosNode, Cos)
+MOREFIT_UNARY_NODE(TanNode, Tan)
+MOREFIT_UNARY_NODE(AsinNode, Asin)
+MOREFIT_UNARY_NODE(AcosNode, Acos)
+MOREFIT_UNARY_NODE(AtanNode, Atan)
+MOREFIT_UNARY_NODE(ErfNode, Erf)
+#undef MOREFIT_UNARY_NODE
+
+template <typename kernelT, typename evalT>
+class ConstantNode : public ComputeGraphNode<kernelT, evalT> {
+ public:
+  explicit ConstantNode(double c) : ComputeGraphNode<kernelT, evalT>(NodeKind::Constant, {}, c) {}
+};
+
+template <typename kernelT, typename evalT>
+class VariableNode : public ComputeGraphNode<kernelT, evalT> {
+ public:
+  explicit VariableNode(std::string name) : ComputeGraphNode<kernelT, evalT>(NodeKind::Variable, {}, 0.0, name) {}
+};
+
+#define MOREFIT_NARY_NODE(NAME, KIND)                                                                     \
+  template <typename kernelT, typename evalT>                                                           \
+  class NAME : public ComputeGraphNode<kernelT, evalT> {                                                \
+   public:                                                                                              \
+    explicit NAME(std::vector<GraphPtr<kernelT, evalT>> ch) : ComputeGraphNode<kernelT, evalT>(NodeKind::KIND, std::move(ch)) {} \
+    NAME(GraphPtr<kernelT, evalT> a, GraphPtr<kernelT, evalT> b)                                        \
+        : ComputeGraphNode<kernelT, evalT>(NodeKind::KIND, [&] {                                        \
+            std::vector<GraphPtr<kernelT, evalT>> v;                                                    \
+            v.emplace_back(std::move(a));                                                               \
+            v.emplace_back(std::move(b));                                                               \
+            return v;                                                                                   \
+          }()) {}                                                                                       \
+  };
+MOREFIT_NARY_NODE(SumNode, Sum)
+MOREFIT_NARY_NODE(ProdNode, Prod)
+MOREFIT_NARY_NODE(PowNode, Pow)
+#undef MOREFIT_NARY_NODE
+
+template <typename kernelT, typename evalT>
+class ConditionalNode : public ComputeGraphNode<kernelT, evalT> {
+ public:
+  using typename ComputeGraphNode<kernelT, evalT>::comparator_type;
+  ConditionalNode(int comparator, GraphPtr<kernelT, evalT> expression, GraphPtr<kernelT, evalT> a, GraphPtr<kernelT, evalT> b)
+      : ComputeGraphNode<kernelT, evalT>(NodeKind::Conditional, [&] {
+          std::vector<GraphPtr<kernelT, evalT>> v;
+          v.emplace_back(std::move(expression));
+          v.emplace_back(std::move(a));
+          v.emplace_back(std::move(b));
+          return v;
+        }(), 0.0, "", comparator) {}
+};
+
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> make_node(NodeKind kind, std::vector<GraphPtr<kernelT, evalT>> ch, double value, const std::string& name, int cmp) {
+  switch (kind) {
+    case NodeKind::Constant: return std::make_unique<ConstantNode<kernelT, evalT>>(value);
+    case NodeKind::Variable: return std::make_unique<VariableNode<kernelT, evalT>>(name);
+    case NodeKind::Sum: return std::make_unique<SumNode<kernelT, evalT>>(std::move(ch));
+    case NodeKind::Prod: return std::make_unique<ProdNode<kernelT, evalT>>(std::move(ch));
+    case NodeKind::Pow: return std::make_unique<PowNode<kernelT, evalT>>(std::move(ch));
+    case NodeKind::Conditional:
+      return std::make_unique<ConditionalNode<kernelT, evalT>>(cmp, std::move(ch[0]), std::move(ch[1]), std::move(ch[2]));
+    case NodeKind::Neg: return std::make_unique<NegNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Inv: return std::make_unique<InvNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Exp: return std::make_unique<ExpNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Log: return std::make_unique<LogNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Sqrt: return std::make_unique<SqrtNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Sin: return std::make_unique<SinNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Cos: return std::make_unique<CosNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Tan: return std::make_unique<TanNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Asin: return std::make_unique<AsinNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Acos: return std::make_unique<AcosNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Atan: return std::make_unique<AtanNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Erf: return std::make_unique<ErfNode<kernelT, evalT>>(kind, std::move(ch));
+  }
+  return nullptr;
+}
+
+// ==================================================================== builders
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Variable(std::string name) { return std::make_unique<VariableNode<kernelT, evalT>>(name); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Constant(evalT c) { return std::make_unique<ConstantNode<kernelT, evalT>>(c); }
+
+template <typename kernelT, typename evalT>
+bool operator==(const GraphPtr<kernelT, evalT>& lhs, const GraphPtr<kernelT, evalT>& rhs) { return lhs->equals(*rhs); }
+
+#define MOREFIT_CONDITIONAL_BUILDER(NAME, CMP)                                                                      \
+  template <typename kernelT, typename evalT>                                                                   \
+  GraphPtr<kernelT, evalT> NAME(GraphPtr<kernelT, evalT> e, GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { \
+    return std::make_unique<ConditionalNode<kernelT, evalT>>(ComputeGraphNode<kernelT, evalT>::CMP, std::move(e), std::move(A), std::move(B)); \
+  }
+MOREFIT_CONDITIONAL_BUILDER(ConditionalEqual, Equal)
+MOREFIT_CONDITIONAL_BUILDER(ConditionalUnequal, Unequal)
+MOREFIT_CONDITIONAL_BUILDER(ConditionalLarger, Larger)
+MOREFIT_CONDITIONAL_BUILDER(ConditionalLargerEqual, LargerEqual)
+MOREFIT_CONDITIONAL_BUILDER(ConditionalSmaller, Smaller)
+MOREFIT_CONDITIONAL_BUILDER(ConditionalSmallerEqual, SmallerEqual)
+#undef MOREFIT_CONDITIONAL_BUILDER
+
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Sum(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return std::make_unique<SumNode<kernelT, evalT>>(std::move(A), std::move(B)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Prod(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return std::make_unique<ProdNode<kernelT, evalT>>(std::move(A), std::move(B)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Neg(GraphPtr<kernelT, evalT> A) { return std::make_unique<NegNode<kernelT, evalT>>(std::move(A)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Neg(std::string A) { return Neg<kernelT, evalT>(Variable<kernelT, evalT>(A)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Neg(evalT A) { return Constant<kernelT, evalT>(-A); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Inv(GraphPtr<kernelT, evalT> A) { return std::make_unique<InvNode<kernelT, evalT>>(std::move(A)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Inv(std::string A) { return Inv<kernelT, evalT>(Variable<kernelT, evalT>(A)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Inv(evalT A) { return Constant<kernelT, evalT>(1.0 / A); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> Pow(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return std::make_unique<PowNode<kernelT, evalT>>(std::move(A), std::move(B)); }
+
+#define MOREFIT_UNARY_BUILDER(NAME, NODE)                                                                                    \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> NAME(GraphPtr<kernelT, evalT> A) { return std::make_unique<NODE<kernelT, evalT>>(std::move(A)); } \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> NAME(std::string A) { return NAME<kernelT, evalT>(Variable<kernelT, evalT>(A)); } \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> NAME(evalT A) { return NAME<kernelT, evalT>(Constant<kernelT, evalT>(A)); }
+MOREFIT_UNARY_BUILDER(Sqrt, SqrtNode)
+MOREFIT_UNARY_BUILDER(Exp, ExpNode)
+MOREFIT_UNARY_BUILDER(Log, LogNode)
+MOREFIT_UNARY_BUILDER(Sin, SinNode)
+MOREFIT_UNARY_BUILDER(Cos, CosNode)
+MOREFIT_UNARY_BUILDER(Tan, TanNode)
+MOREFIT_UNARY_BUILDER(Asin, AsinNode)
+MOREFIT_UNARY_BUILDER(Acos, AcosNode)
+MOREFIT_UNARY_BUILDER(Atan, AtanNode)
+MOREFIT_UNARY_BUILDER(Erf, ErfNode)
+#undef MOREFIT_UNARY_BUILDER
+
+// a-b -> Sum(a, Neg(b)); a/b -> Prod(a, Inv(b)) (the reference's conventions, graph.hh:2661-2664, 2710-2713)
+#define MOREFIT_BINOP(OP, MAKE)                                                                                               \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> operator OP(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return MAKE(std::move(A), std::move(B)); } \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> operator OP(evalT A, GraphPtr<kernelT, evalT> B) { return MAKE(Constant<kernelT, evalT>(A), std::move(B)); } \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> operator OP(GraphPtr<kernelT, evalT> A, evalT B) { return MAKE(std::move(A), Constant<kernelT, evalT>(B)); } \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> operator OP(std::string A, GraphPtr<kernelT, evalT> B) { return MAKE(Variable<kernelT, evalT>(A), std::move(B)); } \
+  template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> operator OP(GraphPtr<kernelT, evalT> A, std::string B) { return MAKE(std::move(A), Variable<kernelT, evalT>(B)); }
+
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> mf_add(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return Sum<kernelT, evalT>(std::move(A), std::move(B)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> mf_sub(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return Sum<kernelT, evalT>(std::move(A), Neg<kernelT, evalT>(std::move(B))); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> mf_mul(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return Prod<kernelT, evalT>(std::move(A), std::move(B)); }
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> mf_div(GraphPtr<kernelT, evalT> A, GraphPtr<kernelT, evalT> B) { return Prod<kernelT, evalT>(std::move(A), Inv<kernelT, evalT>(std::move(B))); }
+MOREFIT_BINOP(+, mf_add)
+MOREFIT_BINOP(-, mf_sub)
+MOREFIT_BINOP(*, mf_mul)
+MOREFIT_BINOP(/, mf_div)
+#undef MOREFIT_BINOP
+
+template <typename kernelT, typename evalT> GraphPtr<kernelT, evalT> operator-(GraphPtr<kernelT, evalT> A) { return Neg<kernelT, evalT>(std::move(A)); }
+
+// ==================================================================== graph transforms used by the backends
+
+// Is the node free of every name in `observables` (i.e. depends only on parameters/constants)?
+template <typename kernelT, typename evalT>
+bool parameter_only(const ComputeGraphNode<kernelT, evalT>& n, const std::vector<std::string>& observables) {
+  return !n.contains_any(observables);
+}
+
+// One pass of the rewrites that let hoisting pull constant offsets out of centred observables:
+//  (1) p1*...*pk * (e + q)  ->  p1*...*pk*e + p1*...*pk*q   (single event-dependent Sum factor
+//      with a parameter-only part, all other factors parameter-only)
+//  (2) exp(e + q)           ->  exp(e) * exp(q)              (q parameter-only)
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> centring_rewrite(const ComputeGraphNode<kernelT, evalT>& n, const std::vector<std::string>& obs) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  std::vector<Ptr> ch;
+  for (const auto& c : n.children_) ch.emplace_back(centring_rewrite(*c, obs));
+  if (n.kind() == NodeKind::Prod) {
+    int nonparam = -1, count = 0;
+    for (size_t i = 0; i < ch.size(); i++)
+      if (!parameter_only(*ch[i], obs)) { nonparam = (int)i; count++; }
+    if (count == 1 && ch.size() > 1 && ch[nonparam]->kind() == NodeKind::Sum) {
+      const auto& s = *ch[nonparam];
+      bool has_param_part = false, has_event_part = false;
+      for (const auto& t : s.children_) (parameter_only(*t, obs) ? has_param_part : has_event_part) = true;
+      if (has_param_part && has_event_part) {
+        std::vector<Ptr> terms;
+        for (const auto& t : s.children_) {
+          std::vector<Ptr> f;
+          for (size_t i = 0; i < ch.size(); i++) f.emplace_back(i == (size_t)nonparam ? t->copy() : ch[i]->copy());
+          terms.emplace_back(make_node<kernelT, evalT>(NodeKind::Prod, std::move(f)));
+        }
+        return make_node<kernelT, evalT>(NodeKind::Sum, std::move(terms));
+      }
+    }
+  }
+  if (n.kind() == NodeKind::Exp && ch[0]->kind() == NodeKind::Sum) {
+    std::vector<Ptr> ev, pa;
+    for (const auto& t : ch[0]->children_) (parameter_only(*t, obs) ? pa : ev).emplace_back(t->copy());
+    if (!ev.empty() && !pa.empty()) {
+      std::vector<Ptr> f;
+      f.emplace_back(Exp<kernelT, evalT>(ev.size() == 1 ? std::move(ev[0]) : make_node<kernelT, evalT>(NodeKind::Sum, std::move(ev))));
+      f.emplace_back(Exp<kernelT, evalT>(pa.size() == 1 ? std::move(pa[0]) : make_node<kernelT, evalT>(NodeKind::Sum, std::move(pa))));
+      return make_node<kernelT, evalT>(NodeKind::Prod, std::move(f));
+    }
+  }
+  return make_node<kernelT, evalT>(n.kind(), std::move(ch), n.value(), n.name(), n.comparator());
+}
+
+// x -> x + c for each observable (the kernel then reads centred offsets), simplified and
+// rewritten until stable so that parameter-only offsets can be hoisted.
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> centre_observables(const ComputeGraphNode<kernelT, evalT>& g, const std::vector<std::string>& obs,
+                                            const std::vector<double>& centres) {
+  GraphPtr<kernelT, evalT> r = g.copy();
+  for (size_t i = 0; i < obs.size(); i++) {
+    if (centres.at(i) == 0.0) continue;
+    auto rep = Sum<kernelT, evalT>(Variable<kernelT, evalT>(obs[i]), Constant<kernelT, evalT>(centres[i]));
+    r = r->replace_variable(obs[i], *rep);
+  }
+  r = r->simplify();
+  for (int pass = 0; pass < 4; pass++) {
+    auto nxt = centring_rewrite(*r, obs)->simplify();
+    bool same = nxt->equals(*r);
+    r = std::move(nxt);
+    if (same) break;
+  }
+  return r;
+}
+
+// structural hash (equal graphs hash equally)
+template <typename kernelT, typename evalT>
+uint64_t structural_hash(const ComputeGraphNode<kernelT, evalT>& n) {
+  uint64_t h = 1469598103934665603ULL ^ (uint64_t)n.kind() * 0x9e3779b97f4a7c15ULL;
+  auto mix = [&h](uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+  };
+  if (n.kind() == NodeKind::Constant) {
+    uint64_t b;
+    double v = n.value();
+    std::memcpy(&b, &v, sizeof(b));
+    mix(b);
+  }
+  if (n.kind() == NodeKind::Variable) mix(std::hash<std::string>()(n.name()));
+  mix((uint64_t)n.comparator());
+  for (const auto& c : n.children_) mix(structural_hash(*c));
+  return h;
+}
+
+// Pulls factors shared by every term of a Sum out in front:  a*b*c + a*d  ->  a*(b*c + d).
+// Applied bottom-up to derivative graphs before hoisting, so event-dependent factors
+// (exp(...), 1/p) are multiplied once and the parameter-only remainders of a sum can be
+// hoisted into one FP64 term instead of being combined per event in FP32.
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> factor_common(const ComputeGraphNode<kernelT, evalT>& n) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  std::vector<Ptr> ch;
+  for (const auto& c : n.children_) ch.emplace_back(factor_common(*c));
+  if (n.kind() != NodeKind::Sum || ch.size() < 2)
+    return make_node<kernelT, evalT>(n.kind(), std::move(ch), n.value(), n.name(), n.comparator())->simplify();
+  // factor lists of each term
+  std::vector<std::vector<Ptr>> fac(ch.size());
+  std::vector<std::vector<uint64_t>> hs(ch.size());
+  for (size_t t = 0; t < ch.size(); t++) {
+    if (ch[t]->kind() == NodeKind::Prod)
+      for (auto& f : ch[t]->children_) fac[t].emplace_back(f->copy());
+    else
+      fac[t].emplace_back(ch[t]->copy());
+    for (auto& f : fac[t]) hs[t].push_back(structural_hash(*f));
+  }
+  std::vector<Ptr> common;
+  for (size_t i = 0; i < fac[0].size(); i++) {
+    if (fac[0][i]->is_constant()) continue;
+    std::vector<int> pos(ch.size(), -1);
+    pos[0] = (int)i;
+    bool everywhere = true;
+    for (size_t t = 1; t < ch.size() && everywhere; t++) {
+      for (size_t j = 0; j < fac[t].size(); j++)
+        if (fac[t][j] && hs[t][j] == hs[0][i] && fac[t][j]->equals(*fac[0][i])) {
+          pos[t] = (int)j;
+          break;
+        }
+      everywhere = pos[t] >= 0;
+    }
+    if (!everywhere) continue;
+    common.emplace_back(fac[0][i]->copy());
+    for (size_t t = 0; t < ch.size(); t++) fac[t][pos[t]].reset();
+  }
+  if (common.empty()) return make_node<kernelT, evalT>(NodeKind::Sum, std::move(ch))->simplify();
+  std::vector<Ptr> terms;
+  for (size_t t = 0; t < ch.size(); t++) {
+    std::vector<Ptr> rest;
+    for (auto& f : fac[t])
+      if (f) rest.emplace_back(std::move(f));
+    if (rest.empty()) terms.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 1.0));
+    else if (rest.size() == 1) terms.emplace_back(std::move(rest[0]));
+    else terms.emplace_back(make_node<kernelT, evalT>(NodeKind::Prod, std::move(rest)));
+  }
+  auto inner = factor_common(*make_node<kernelT, evalT>(NodeKind::Sum, std::move(terms))->simplify());
+  common.emplace_back(std::move(inner));
+  return make_node<kernelT, evalT>(NodeKind::Prod, std::move(common))->simplify();
+}
+
+// ==================================================================== code emission with CSE
+// Emits the outputs of several graphs as one straight-line block of SSA statements
+//   const T t17 = ...;
+// identical subtrees (structural key over already-numbered operands) are computed once.
+// `bindings` maps variable names to C expressions (observables / parameter slots).
+// Math goes through mf_* wrappers that the kernel prologue defines per precision.
+struct KernelEmitter {
+  std::string T;          // "float" or "double"
+  bool fp32 = false;
+  std::ostringstream body;
+  std::unordered_map<std::string, std::string> memo;
+  std::map<std::string, std::string> bindings;
+  int next = 0;
+  int statements = 0;
+  std::string prefix = "mf_";  // math wrapper prefix ("mfd_" for the FP64 hoisted terms)
+  std::string tmp = "t";       // SSA name prefix
+
+  explicit KernelEmitter(bool is_fp32) : T(is_fp32 ? "float" : "double"), fp32(is_fp32) {}
+
+  std::string literal(double v) const {
+    if (std::isinf(v)) return v > 0 ? "(" + T + ")(1.0/0.0)" : "(" + T + ")(-1.0/0.0)";
+    if (std::isnan(v)) return "(" + T + ")(0.0/0.0)";
+    char buf[64];
+    if (fp32) std::snprintf(buf, sizeof(buf), "%af", (double)(float)v);
+    else std::snprintf(buf, sizeof(buf), "%a", v);
+    return buf;
+  }
+  std::string fresh(const std::string& key, const std::string& expr) {
+    std::string n = tmp + std::to_string(next++);
+    body << "  const " << T << " " << n << " = " << expr << ";\n";
+    statements++;
+    memo[key] = n;
+    return n;
+  }
+  std::string fn(NodeKind k) const {
+    switch (k) {
+      case NodeKind::Exp: return prefix + "exp";
+      case NodeKind::Log: return prefix + "log";
+      case NodeKind::Sqrt: return prefix + "sqrt";
+      case NodeKind::Sin: return prefix + "sin";
+      case NodeKind::Cos: return prefix + "cos";
+      case NodeKind::Tan: return prefix + "tan";
+      case NodeKind::Asin: return prefix + "asin";
+      case NodeKind::Acos: return prefix + "acos";
+      case NodeKind::Atan: return prefix + "atan";
+      case NodeKind::Erf: return prefix + "erf";
+      case NodeKind::Inv: return prefix + "rcp";
+      default: return "";
+    }
+  }
+
+  template <typename kernelT, typename evalT>
+  std::string emit(const ComputeGraphNode<kernelT, evalT>& n) {
+    switch (n.kind()) {
+      case NodeKind::Constant: return literal(n.value());
+      case NodeKind::Variable: {
+        auto it = bindings.find(n.name());
+        if (it == bindings.end()) throw std::runtime_error("unbound variable in kernel graph: " + n.name());
+        return it->second;
+      }
+      default: break;
+    }
+    std::vector<std::string> ops;
+    for (const auto& c : n.children_) ops.push_back(emit(*c));
+    std::string key = std::to_string((int)n.kind()) + ":" + std::to_string(n.comparator()) + "(";
+    {
+      std::vector<std::string> k2 = ops;
+      if (n.kind() == NodeKind::Sum || n.kind() == NodeKind::Prod) std::sort(k2.begin(), k2.end());
+      for (auto& s : k2) key += s + ",";
+      key += ")";
+    }
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+    std::string e;
+    switch (n.kind()) {
+      case NodeKind::Sum:
+        for (size_t i = 0; i < ops.size(); i++) e += (i ? " + " : "") + ops[i];
+        break;
+      case NodeKind::Prod:
+        for (size_t i = 0; i < ops.size(); i++) e += (i ? " * " : "") + ops[i];
+        break;
+      case NodeKind::Neg: e = "-" + ops[0]; break;
+      case NodeKind::Pow: {
+        const auto& ex = n.child(1);
+        if (ex.is_constant() && ex.value() == std::floor(ex.value()) && std::fabs(ex.value()) <= 4 && ex.value() != 0) {
+          int k = (int)std::fabs(ex.value());
+          for (int i = 0; i < k; i++) e += (i ? " * " : "") + ops[0];
+          if (ex.value() < 0) e = prefix + "rcp(" + e + ")";
+        } else {
+          e = prefix + "pow(" + ops[0] + ", " + ops[1] + ")";
+        }
+        break;
+      }
+      case NodeKind::Conditional: {
+        static const char* cs[] = {"==", "!=", ">", "<", ">=", "<="};
+        e = "(" + ops[0] + " " + cs[n.comparator()] + " (" + T + ")0) ? " + ops[1] + " : " + ops[2];
+        break;
+      }
+      default: e = fn(n.kind()) + "(" + ops[0] + ")"; break;
+    }
+    return fresh(key, e);
+  }
+};
+
+}  // namespace morefit
+
+#endif

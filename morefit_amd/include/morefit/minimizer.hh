@@ -1,0 +1,391 @@
+/**
+ * @file minimizer.hh
+ * Variable-metric minimiser with MIGRAD's control flow and HESSE-style errors, written from
+ * the published MINUIT algorithm description (F. James, "MINUIT Function Minimization and
+ * Error Analysis", CERN D506; F. James & M. Winkler, Minuit2 user's guide):
+ *
+ *  - parameters with two-sided limits live in an internal space through
+ *    ext = lo + (hi - lo)/2 (sin(int) + 1); one-sided limits use the square-root maps;
+ *  - seed: analytic Hessian (strategy 2 with an analytic Hessian, the setting the reference
+ *    fitter uses, fitter.hh:762-767, 806-813) transformed to the internal space and
+ *    inverted, forced positive-definite by a diagonal shift when needed;
+ *  - iteration: Newton step -V g, a parabolic line search, the BFGS update of V;
+ *  - convergence: EDM = g^T V g / 2 < 0.002 * tolerance * up (tolerance 0.01, up 1 as in the
+ *    reference's Minuit2 defaults), confirmed with a fresh analytic Hessian (strategy 2);
+ *  - HESSE: covariance = 2 up H^-1 in the internal space, mapped back through the Jacobian.
+ *
+ * The function interface is synchronous (value, value+gradient, Hessian).  The batched toy
+ * engine runs one instance per toy in a fibre and answers the calls from shared launches.
+ */
+#ifndef MOREFIT_MINIMIZER_HH
+#define MOREFIT_MINIMIZER_HH
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace morefit {
+
+// objective: all calls take the full external parameter vector (fixed ones included)
+struct MinimizerFcn {
+  virtual ~MinimizerFcn() = default;
+  virtual double value(const double* x) = 0;
+  virtual double value_grad(const double* x, double* grad) = 0;  // returns the value as well
+  virtual bool hessian(const double* x, double* h) = 0;             // n x n, row-major, all params
+};
+
+struct MinimizerParam {
+  std::string name;
+  double value = 0.0, step = 0.1, lo = 0.0, hi = 0.0;
+  bool has_lo = false, has_hi = false, fixed = false;
+};
+
+// small dense linear algebra (n <= ~50)
+namespace la {
+// in-place inverse of a symmetric positive-definite matrix by Cholesky; false if not PD
+inline bool spd_inverse(std::vector<double>& a, int n) {
+  std::vector<double> L(n * n, 0.0);
+  for (int j = 0; j < n; j++) {
+    double d = a[j * n + j];
+    for (int k = 0; k < j; k++) d -= L[j * n + k] * L[j * n + k];
+    if (!(d > 0.0) || !std::isfinite(d)) return false;
+    L[j * n + j] = std::sqrt(d);
+    for (int i = j + 1; i < n; i++) {
+      double s = a[i * n + j];
+      for (int k = 0; k < j; k++) s -= L[i * n + k] * L[j * n + k];
+      L[i * n + j] = s / L[j * n + j];
+    }
+  }
+  // inverse of L
+  std::vector<double> Li(n * n, 0.0);
+  for (int i = 0; i < n; i++) {
+    Li[i * n + i] = 1.0 / L[i * n + i];
+    for (int j = 0; j < i; j++) {
+      double s = 0.0;
+      for (int k = j; k < i; k++) s -= L[i * n + k] * Li[k * n + j];
+      Li[i * n + j] = s / L[i * n + i];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j <= i; j++) {
+      double s = 0.0;
+      for (int k = i; k < n; k++) s += Li[k * n + i] * Li[k * n + j];
+      a[i * n + j] = a[j * n + i] = s;
+    }
+  return true;
+}
+// symmetric Jacobi eigenvalues (for the positive-definite check/shift)
+inline double min_eigenvalue(std::vector<double> a, int n) {
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = a[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+      }
+  }
+  double m = a[0];
+  for (int i = 1; i < n; i++) m = std::min(m, a[i * n + i]);
+  return m;
+}
+}  // namespace la
+
+class Migrad {
+ public:
+  // settings (reference fitter_options / Minuit2 defaults)
+  int strategy = 2;
+  int max_calls = 4000;
+  int max_iterations = 4000;
+  double tolerance = 0.01;
+  double up = 1.0;
+  int print_level = 0;
+
+  // results
+  std::vector<double> x;      // external values (all parameters)
+  std::vector<double> errors; // external errors (0 for fixed)
+  std::vector<double> cov;    // covariance of the floating parameters (nf x nf, row-major)
+  double fval = NAN, edm = NAN;
+  int ncalls = 0, ngrad = 0, nhess = 0, iterations = 0;
+  int status = -1;            // 0 ok, 1 call limit, 2 line search failure, 3 not positive-definite, 4 non-finite
+  bool valid = false, cov_accurate = false;
+
+  explicit Migrad(std::vector<MinimizerParam> params) : p_(std::move(params)) {
+    for (size_t i = 0; i < p_.size(); i++)
+      if (!p_[i].fixed) free_.push_back((int)i);
+    x.resize(p_.size());
+    for (size_t i = 0; i < p_.size(); i++) x[i] = p_[i].value;
+    errors.assign(p_.size(), 0.0);
+  }
+  int nfree() const { return (int)free_.size(); }
+
+  // ---------------------------------------------------------------- transforms
+  double int2ext(int k, double u) const {
+    const auto& q = p_[free_[k]];
+    if (q.has_lo && q.has_hi) return q.lo + 0.5 * (q.hi - q.lo) * (std::sin(u) + 1.0);
+    if (q.has_lo) return q.lo - 1.0 + std::sqrt(u * u + 1.0);
+    if (q.has_hi) return q.hi + 1.0 - std::sqrt(u * u + 1.0);
+    return u;
+  }
+  double ext2int(int k, double v) const {
+    const auto& q = p_[free_[k]];
+    if (q.has_lo && q.has_hi) {
+      double y = 2.0 * (v - q.lo) / (q.hi - q.lo) - 1.0;
+      y = std::max(-1.0, std::min(1.0, y));
+      return std::asin(y);
+    }
+    if (q.has_lo) { double y = v - q.lo + 1.0; return std::sqrt(std::max(0.0, y * y - 1.0)); }
+    if (q.has_hi) { double y = q.hi - v + 1.0; return std::sqrt(std::max(0.0, y * y - 1.0)); }
+    return v;
+  }
+  double dext(int k, double u) const {  // d ext / d int
+    const auto& q = p_[free_[k]];
+    if (q.has_lo && q.has_hi) return 0.5 * (q.hi - q.lo) * std::cos(u);
+    if (q.has_lo) return u / std::sqrt(u * u + 1.0);
+    if (q.has_hi) return -u / std::sqrt(u * u + 1.0);
+    return 1.0;
+  }
+  double d2ext(int k, double u) const {  // d2 ext / d int2
+    const auto& q = p_[free_[k]];
+    if (q.has_lo && q.has_hi) return -0.5 * (q.hi - q.lo) * std::sin(u);
+    double r = std::pow(u * u + 1.0, -1.5);
+    if (q.has_lo) return r;
+    if (q.has_hi) return -r;
+    return 0.0;
+  }
+
+  // ---------------------------------------------------------------- minimisation
+  int minimize(MinimizerFcn& fcn) {
+    const int n = nfree();
+    status = -1;
+    valid = false;
+    std::vector<double> u(n), g(n), V(n * n), xe(x);
+    for (int k = 0; k < n; k++) u[k] = ext2int(k, p_[free_[k]].value);
+    auto to_ext = [&](const std::vector<double>& uu) {
+      for (int k = 0; k < n; k++) xe[free_[k]] = int2ext(k, uu[k]);
+    };
+    std::vector<double> ge(p_.size()), he(p_.size() * p_.size());
+    auto eval_fg = [&](const std::vector<double>& uu, std::vector<double>& gg) {
+      to_ext(uu);
+      double f = fcn.value_grad(xe.data(), ge.data());
+      ncalls++;
+      ngrad++;
+      for (int k = 0; k < n; k++) gg[k] = ge[free_[k]] * dext(k, uu[k]);
+      return f;
+    };
+    auto eval_f = [&](const std::vector<double>& uu) {
+      to_ext(uu);
+      ncalls++;
+      return fcn.value(xe.data());
+    };
+    // internal Hessian -> V = H^-1 (shifted to positive-definite); false on failure
+    auto seed_metric = [&](const std::vector<double>& uu, const std::vector<double>& gg) {
+      to_ext(uu);
+      if (!fcn.hessian(xe.data(), he.data())) return false;
+      nhess++;
+      ncalls++;
+      std::vector<double> H(n * n);
+      const int N = (int)p_.size();
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+          double v = he[free_[i] * N + free_[j]] * dext(i, uu[i]) * dext(j, uu[j]);
+          if (i == j && dext(i, uu[i]) != 0.0) v += (gg[i] / dext(i, uu[i])) * d2ext(i, uu[i]);
+          H[i * n + j] = v;
+        }
+      last_hint_ = H;
+      std::vector<double> Hi = H;
+      if (!la::spd_inverse(Hi, n)) {
+        double lmin = la::min_eigenvalue(H, n);
+        double dmax = 0.0;
+        for (int i = 0; i < n; i++) dmax = std::max(dmax, std::fabs(H[i * n + i]));
+        double shift = -lmin + 1e-3 * std::max(dmax, 1e-10);
+        Hi = H;
+        for (int i = 0; i < n; i++) Hi[i * n + i] += shift;
+        if (!la::spd_inverse(Hi, n)) return false;
+        pd_forced_ = true;
+      }
+      V = Hi;
+      return true;
+    };
+    auto compute_edm = [&](const std::vector<double>& gg) {
+      double e = 0.0;
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) e += gg[i] * V[i * n + j] * gg[j];
+      return 0.5 * e;
+    };
+    const double edm_goal = 0.002 * tolerance * up;
+
+    double f = eval_fg(u, g);
+    if (!std::isfinite(f)) return finish(4, f, NAN, u, n);
+    if (!seed_metric(u, g)) {  // diagonal fallback from the step sizes
+      std::fill(V.begin(), V.end(), 0.0);
+      for (int k = 0; k < n; k++) {
+        double s = p_[free_[k]].step / std::max(1e-300, std::fabs(dext(k, u[k])));
+        V[k * n + k] = s * s;
+      }
+    }
+    double e = compute_edm(g);
+    std::vector<double> d(n), u1(n), g1(n), s(n), y(n), Vy(n);
+    bool confirmed = false;
+    for (iterations = 0; iterations < max_iterations; iterations++) {
+      if (e < edm_goal) {
+        if (strategy >= 2 && !confirmed) {  // confirm with a fresh Hessian
+          confirmed = true;
+          if (seed_metric(u, g)) {
+            e = compute_edm(g);
+            if (e < edm_goal) break;
+          } else {
+            break;
+          }
+        } else {
+          break;
+        }
+      }
+      if (ncalls >= max_calls) return finish(1, f, e, u, n);
+      // Newton direction
+      double slope = 0.0;
+      for (int i = 0; i < n; i++) {
+        double t = 0.0;
+        for (int j = 0; j < n; j++) t -= V[i * n + j] * g[j];
+        d[i] = t;
+        slope += g[i] * t;
+      }
+      if (!(slope < 0.0)) {  // not a descent direction: restart from the Hessian
+        if (!seed_metric(u, g)) return finish(3, f, e, u, n);
+        continue;
+      }
+      // parabolic line search on alpha in f(u + alpha d)
+      double alpha = 1.0, fa = 0.0, best_a = 0.0, best_f = f;
+      for (int ls = 0; ls < 12; ls++) {
+        for (int i = 0; i < n; i++) u1[i] = u[i] + alpha * d[i];
+        fa = eval_f(u1);
+        if (std::isfinite(fa) && fa < best_f) { best_f = fa; best_a = alpha; }
+        double c2 = std::isfinite(fa) ? (fa - f - slope * alpha) / (alpha * alpha) : INFINITY;
+        double next;
+        if (std::isfinite(c2) && c2 > 0.0) next = -slope / (2.0 * c2);
+        else next = best_a > 0.0 ? 2.0 * alpha : 0.25 * alpha;
+        next = std::max(0.1 * alpha, std::min(next, 4.0 * std::max(alpha, best_a)));
+        if (best_a > 0.0 && std::fabs(next - best_a) < 0.15 * best_a) break;  // parabola agrees: done
+        if (best_a > 0.0 && ls >= 2) break;
+        if (std::fabs(next - alpha) < 1e-12 * alpha) break;
+        alpha = next;
+      }
+      if (best_a == 0.0) {
+        // no decrease along the Newton direction: refresh the metric once, else give up
+        if (!refreshed_) {
+          refreshed_ = true;
+          if (seed_metric(u, g)) { e = compute_edm(g); continue; }
+        }
+        return finish(e < 100.0 * edm_goal ? 0 : 2, f, e, u, n);
+      }
+      refreshed_ = false;
+      for (int i = 0; i < n; i++) {
+        u1[i] = u[i] + best_a * d[i];
+        s[i] = best_a * d[i];
+      }
+      double f1 = eval_fg(u1, g1);
+      if (!std::isfinite(f1)) return finish(4, f, e, u, n);
+      // BFGS update of the inverse metric
+      double sy = 0.0;
+      for (int i = 0; i < n; i++) { y[i] = g1[i] - g[i]; sy += s[i] * y[i]; }
+      if (sy > 0.0) {
+        double yVy = 0.0;
+        for (int i = 0; i < n; i++) {
+          double t = 0.0;
+          for (int j = 0; j < n; j++) t += V[i * n + j] * y[j];
+          Vy[i] = t;
+          yVy += y[i] * t;
+        }
+        double rho = 1.0 / sy;
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < n; j++)
+            V[i * n + j] += (1.0 + rho * yVy) * rho * s[i] * s[j] - rho * (Vy[i] * s[j] + s[i] * Vy[j]);
+      }
+      u = u1;
+      g = g1;
+      f = f1;
+      e = compute_edm(g);
+      confirmed = false;
+    }
+    if (iterations >= max_iterations) return finish(1, f, e, u, n);
+    return finish(0, f, e, u, n);
+  }
+
+  // errors and covariance from the analytic Hessian at the current point
+  bool hesse(MinimizerFcn& fcn) {
+    const int n = nfree(), N = (int)p_.size();
+    std::vector<double> u(n), ge(N), he(N * N);
+    for (int k = 0; k < n; k++) u[k] = ext2int(k, x[free_[k]]);
+    double f = fcn.value_grad(x.data(), ge.data());
+    ncalls++;
+    if (!fcn.hessian(x.data(), he.data())) return false;
+    nhess++;
+    ncalls++;
+    std::vector<double> H(n * n);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        double v = he[free_[i] * N + free_[j]] * dext(i, u[i]) * dext(j, u[j]);
+        if (i == j) v += ge[free_[i]] * d2ext(i, u[i]);
+        H[i * n + j] = v;
+      }
+    std::vector<double> C = H;
+    cov_accurate = la::spd_inverse(C, n);
+    if (!cov_accurate) return false;
+    cov.assign(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) cov[i * n + j] = 2.0 * up * C[i * n + j] * dext(i, u[i]) * dext(j, u[j]);
+    for (int k = 0; k < n; k++) errors[free_[k]] = std::sqrt(cov[k * n + k]);
+    double e = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) e += ge[free_[i]] * dext(i, u[i]) * C[i * n + j] * ge[free_[j]] * dext(j, u[j]);
+    edm = 0.5 * e;
+    fval = f;
+    return true;
+  }
+
+ private:
+  std::vector<MinimizerParam> p_;
+  std::vector<int> free_;
+  std::vector<double> last_hint_;
+  bool pd_forced_ = false;
+  bool refreshed_ = false;
+
+  int finish(int st, double f, double e, const std::vector<double>& u, int n) {
+    for (int k = 0; k < n; k++) x[free_[k]] = int2ext(k, u[k]);
+    fval = f;
+    edm = e;
+    status = st;
+    valid = st == 0;
+    // approximate errors from the last internal Hessian (hesse() replaces them)
+    if (!last_hint_.empty()) {
+      std::vector<double> C = last_hint_;
+      if (la::spd_inverse(C, n)) {
+        cov.assign(n * n, 0.0);
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < n; j++) cov[i * n + j] = 2.0 * up * C[i * n + j] * dext(i, u[i]) * dext(j, u[j]);
+        for (int k = 0; k < n; k++) errors[free_[k]] = std::sqrt(cov[k * n + k]);
+      }
+    }
+    return st;
+  }
+};
+
+}  // namespace morefit
+#endif

@@ -210,7 +210,7 @@ double orc_get_max(int model, const double* th, const double* r) {
 #define ORC_MAXOUT (1 + 8 + 36)
 
 typedef struct {
-  int model, mode, nout, P;
+  int model, mode, nout, P, absval;
   const double* data;
   long stride, from, to;
   const double* theta;
@@ -249,6 +249,8 @@ static void* slice_run(void* arg) {
   for (long i = J->from; i < J->to; i++) {
     for (int d = 0; d < nobs; d++) x[d] = J->data[d * J->stride + i];
     event_outputs(J->model, J->mode, x, J->theta, J->range, &mh, o);
+    if (J->absval)
+      for (int k = 0; k < J->nout; k++) o[k] = fabs(o[k]);
     for (int k = 0; k < J->nout; k++) { /* in-loop Kahan, compute_llvm.hh:694-700 */
       double y = o[k] - c[k];
       volatile double t = sum[k] + y;
@@ -284,6 +286,11 @@ static int nll_precise(int model, const double* data, long n, long stride, const
 
 int orc_nll(int model, const double* data, long n, long stride, const double* theta,
             const double* range, int mode, int nthreads, double* out) {
+  int absval = mode >= 10;
+  if (absval) {
+    mode -= 10;
+    if (nthreads == 0) nthreads = 1;
+  }
   if (model != ORC_MASS && model != ORC_KSTAR) return -1;
   if (mode < 0 || mode > 2 || n < 0) return -1;
   if (nthreads == 0) return nll_precise(model, data, n, stride, theta, range, mode, out);
@@ -295,7 +302,7 @@ int orc_nll(int model, const double* data, long n, long stride, const double* th
   long size = n / nthreads; /* compute_llvm.hh:800-808: the last thread takes the remainder */
   for (int t = 0; t < nthreads; t++) {
     slice_job* J = &jobs[t];
-    J->model = model; J->mode = mode; J->nout = nout; J->P = orc_npar(model);
+    J->model = model; J->mode = mode; J->nout = nout; J->P = orc_npar(model); J->absval = absval;
     J->data = data; J->stride = stride; J->theta = theta; J->range = range;
     J->from = t * size;
     J->to = (t == nthreads - 1) ? n : t * size + size;

@@ -56,6 +56,7 @@ double orc_get_max(int model, const double* theta, const double* range);
  * mode 0: out[0] = sum log p
  * mode 1: + out[1..P] = sum dlog p/dtheta
  * mode 2: + out[1+P ..] = sum d2 log p/dtheta_i dtheta_j, i>=j packed row by row
+ * mode + 10: the same outputs summed as absolute values (error scales for tolerances)
  * nthreads >= 1: compute_llvm.hh semantics (contiguous slices, Kahan per output, Kahan of
  *                partials); nthreads == 0: long-double Kahan sum in one pass (the precise sum).
  * Returns 0 on success. */

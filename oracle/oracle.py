@@ -97,14 +97,16 @@ def get_max(model, theta, rng=None):
 
 def nll_sums(model, data, theta, mode=1, nthreads=1, rng=None):
     """Per-output sums (sum log p, sum grad, sum Hessian packed i>=j) over SoA data
-    of shape (nobs, n).  nthreads=0 selects the long-double precise sum."""
+    of shape (nobs, n).  nthreads=0 selects the long-double precise sum; mode+10 sums
+    absolute values (tolerance scales)."""
     data = _f64(np.atleast_2d(data))
     nobs, n = data.shape
     assert nobs == NOBS[model]
     theta = _f64(theta)
     rng = _f64(RANGES[model] if rng is None else rng)
     P = NPAR[model]
-    nout = 1 if mode == 0 else (1 + P if mode == 1 else 1 + P + P * (P + 1) // 2)
+    m = mode % 10
+    nout = 1 if m == 0 else (1 + P if m == 1 else 1 + P + P * (P + 1) // 2)
     out = np.zeros(nout)
     rc = lib().orc_nll(model, _dp(data), n, n, _dp(theta), _dp(rng), mode, nthreads, _dp(out))
     if rc != 0:
@@ -118,12 +120,15 @@ def xoshiro_seed(seed):
     return s
 
 
-def generate(model, theta, n, seed, nstreams=1, rng=None, maxprob=None):
-    """Accept-reject sample of n events in `nstreams` streams separated by jump()."""
+def generate(model, theta, n, seed, nstreams=1, rng=None, maxprob=None, events_per_stream=None):
+    """Accept-reject sample of n events in `nstreams` streams separated by jump(); with
+    events_per_stream, stream k owns events [k*eps, min(n, (k+1)*eps))."""
     theta = _f64(theta)
     rng = _f64(RANGES[model] if rng is None else rng)
     if maxprob is None:
         maxprob = get_max(model, theta, rng)
+    if events_per_stream:
+        nstreams = (n + events_per_stream - 1) // events_per_stream
     states = (ctypes.c_uint32 * (4 * nstreams))()
     s = xoshiro_seed(seed)
     for k in range(nstreams):
@@ -132,7 +137,7 @@ def generate(model, theta, n, seed, nstreams=1, rng=None, maxprob=None):
         lib().orc_xoshiro128pp_jump(s)
     offs = (ctypes.c_long * (nstreams + 1))()
     for k in range(nstreams + 1):
-        offs[k] = (n * k) // nstreams
+        offs[k] = min(n, k * events_per_stream) if events_per_stream else (n * k) // nstreams
     out = np.zeros((NOBS[model], n))
     lib().orc_generate(model, _dp(theta), _dp(rng), maxprob, states, nstreams, offs, _dp(out), n)
     return out

@@ -1,0 +1,79 @@
+"""The C ABI libraries load, export every symbol their headers declare, and report errors
+through status codes (no device needed: only calls that fail before touching a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import morefit_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header, prefix):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", txt)))
+
+
+def _exported(lib):
+    out = subprocess.check_output(["nm", "-D", "--defined-only", os.path.join(M.LIBDIR, lib)]).decode()
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+@pytest.mark.parametrize("header,lib,prefix,listed", [
+    ("morefit_hip.h", "libmorefit_hip.so", "mf_", M.HIP_SYMBOLS),
+    ("morefit_capi.h", "libmorefit_capi.so", "mfh_", M.CAPI_SYMBOLS),
+])
+def test_every_declared_symbol_is_exported(header, lib, prefix, listed):
+    M.libs()
+    declared = _declared(header, prefix)
+    assert declared, header
+    exp = _exported(lib)
+    missing = [s for s in declared if s not in exp]
+    assert not missing, missing
+    assert sorted(listed) == declared
+
+
+def test_compile_error_is_reported_not_fatal():
+    hip, _ = M.libs()
+    hip.mf_module_compile_only.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
+    hip.mf_last_error.restype = ctypes.c_char_p
+    n = ctypes.c_size_t(0)
+    rc = hip.mf_module_compile_only(b"#define MF_T double\n//@MF_RUNTIME@\nthis is not C++;\n", b"", ctypes.byref(n))
+    assert rc == 3  # MF_ERR_COMPILE
+    assert b"error" in hip.mf_last_error()
+
+
+def test_injected_failure():
+    hip, _ = M.libs()
+    hip.mf_debug_fail_next.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    hip.mf_module_compile_only.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]
+    assert hip.mf_debug_fail_next(b"mf_module_compile_only", 1) == 0
+    n = ctypes.c_size_t(0)
+    assert hip.mf_module_compile_only(b"x", b"", ctypes.byref(n)) == 6  # MF_ERR_INJECTED
+    hip.mf_last_error.restype = ctypes.c_char_p
+    assert b"injected" in hip.mf_last_error()
+
+
+def test_null_arguments_rejected():
+    hip, _ = M.libs()
+    hip.mf_malloc.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    assert hip.mf_malloc(None, 16, None) == 1  # MF_ERR_INVALID
+    hip.mf_launch_eval.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    assert hip.mf_launch_eval(None, b"k", None, 1, 256) == 1
+
+
+def test_generated_modules_compile_for_gfx950():
+    """the four configuration modules (mass/angular x FP64/FP32) compile with hipRTC"""
+    for kind in (M.MASS, M.KSTAR):
+        for fp32 in (False, True):
+            M.precompile(kind, fp32)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_open_without_gpu_raises():
+    with pytest.raises(M.MorefitError):
+        M.Context()

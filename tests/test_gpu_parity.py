@@ -1,0 +1,245 @@
+"""GPU parity of the MI355X likelihood path against the FP64 oracle, through the C ABI.
+
+Bars (written here as the tolerances):
+  FP64 kernels: -2lnL within 1e-12 relative of the oracle's long-double sum; gradient and
+    Hessian sums within 1e-12 of the sum of absolute per-event values (the rounding scale).
+  FP32 kernels (observables stored as centred FP32, parameter-only terms FP64, compensated
+    per-lane sums, FP64 totals): every sum within 2e-6 of the sum of absolute per-event
+    values, checked against the oracle evaluated on the same stored FP32 observables.
+  Fits: FP64 GPU fit == CPU-backend fit of the same data to 1e-4 sigma; FP32 fits within
+    1e-3 sigma of the FP64 fit.
+  Segments: a segment's result is bitwise the same alone and in a mixed launch.
+  Generator (FP64): the event stream equals the oracle's accept-reject stream exactly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import morefit_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _hess_full(P, packed):
+    H = np.zeros((P, P))
+    k = 0
+    for i in range(P):
+        for j in range(i + 1):
+            H[i, j] = H[j, i] = packed[k]
+            k += 1
+    return H
+
+
+def _stored(model, data, fp32):
+    """the observables exactly as the FP32 kernel stores them (centred, rounded), back in FP64"""
+    if not fp32:
+        return data
+    rng = O.RANGES[model].reshape(-1, 2)
+    c = rng.mean(axis=1)[:, None]
+    return (data - c).astype(np.float32).astype(np.float64) + c
+
+
+@pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_eval_matches_oracle(ctx, model, fp32):
+    n = 100000
+    data = O.generate(model, M.TRUTH[model], n, seed=1234, nstreams=16)
+    theta = np.array(M.START[model])
+    say("eval parity", "model", model, "fp32", fp32, "n", n)
+    mdl = ctx.model(model, fp32=fp32)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    stored = _stored(model, data, fp32)
+    ref = O.nll_sums(model, stored, theta, mode=2, nthreads=0)
+    absref = O.nll_sums(model, stored, theta, mode=12, nthreads=4)
+    P = O.NPAR[model]
+    tol = 2e-6 if fp32 else 1e-12
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        say("  kind", kind)
+        out = b.eval(kind, theta)[0]
+        k = len(out)
+        if fp32:
+            assert np.all(np.abs(out - ref[:k]) <= tol * absref[:k]), (kind, out - ref[:k], absref[:k])
+        else:
+            assert abs(out[0] - ref[0]) <= 1e-12 * abs(ref[0])
+            assert np.all(np.abs(out - ref[:k]) <= tol * absref[:k]), (kind, out - ref[:k])
+    b.close()
+    mdl.close()
+
+
+def test_graph_replays_match_eager(ctx):
+    """hipGraph replays (several, with eager work between them) == eager launches, bitwise"""
+    n = 300000
+    data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=5, nstreams=16)
+    mdl = ctx.model(M.MASS, fp32=True)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    thetas = [M.MASS_START, M.MASS_TRUTH, np.array([5.3, 0.025, 0.25, -1.2])]
+    say("graph replay")
+    for t in thetas:
+        eager = b.eval(M.EVAL_GRAD, t, use_graph=False)
+        for _ in range(3):
+            g = b.eval(M.EVAL_GRAD, t, use_graph=True)
+            assert np.array_equal(g, eager)
+            b.eval(M.EVAL_LH, t, use_graph=False)
+    b.close()
+    mdl.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_segments_alone_equal_mixed(ctx, fp32):
+    """ragged segments (0, 1, 63, 4096, 4097, 100000 events): each segment's sums are bitwise
+    identical evaluated alone and together with the others, and match the oracle."""
+    sizes = [0, 1, 63, 4096, 4097, 100000]
+    rs = np.random.default_rng(9)
+    datas = [O.generate(M.MASS, M.MASS_TRUTH, s, seed=100 + i, nstreams=2) if s else np.zeros((1, 0)) for i, s in enumerate(sizes)]
+    thetas = np.array([M.MASS_START + rs.normal(0, 1e-3, 4) * [1, 0.1, 1, 1] for _ in sizes])
+    say("segments", sizes, "fp32", fp32)
+    mdl = ctx.model(M.MASS, fp32=fp32)
+    mixed = mdl.batch(sizes)
+    for i, d in enumerate(datas):
+        if sizes[i]:
+            mixed.upload(i, d)
+    both = mixed.eval(M.EVAL_HESS, thetas)
+    for i, d in enumerate(datas):
+        alone = mdl.batch([sizes[i]])
+        if sizes[i]:
+            alone.upload(0, d)
+        r = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
+        assert np.array_equal(r[0], both[i]), (i, r[0] - both[i])
+        if sizes[i] == 0:
+            assert np.all(r[0] == 0)
+        else:
+            ref = O.nll_sums(M.MASS, _stored(M.MASS, d, fp32), thetas[i], mode=2, nthreads=0)
+            absref = O.nll_sums(M.MASS, _stored(M.MASS, d, fp32), thetas[i], mode=12, nthreads=1)
+            tol = 2e-6 if fp32 else 1e-12
+            assert np.all(np.abs(r[0] - ref) <= tol * absref + 1e-300)
+        alone.close()
+    mixed.close()
+    mdl.close()
+
+
+@pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
+def test_generator_fp64_matches_oracle_stream(ctx, model):
+    """on-device accept-reject in FP64 reproduces the oracle's xoshiro128++ streams"""
+    n, eps = 20000, 64
+    seed = 229387429
+    say("generator", model)
+    mdl = ctx.model(model, fp32=False)
+    b = mdl.batch([n])
+    props = b.generate(M.TRUTH[model], [seed], events_per_stream=eps)
+    got = b.download(0)
+    ref = O.generate(model, M.TRUTH[model], n, seed=seed, events_per_stream=eps)
+    same = np.mean(np.all(got == ref, axis=0))
+    assert same > 0.999, same
+    assert props >= n
+    b.close()
+    mdl.close()
+
+
+def test_generator_fp32_statistics(ctx):
+    """FP32 on-device generation: right count, inside the range, mean/width as the oracle sample"""
+    n = 200000
+    say("generator fp32")
+    mdl = ctx.model(M.MASS, fp32=True)
+    b = mdl.batch([n, 5000, 1])
+    b.generate(M.MASS_TRUTH, [1, 2, 3])
+    g = b.download(0)[0]
+    assert g.min() >= 5.0 and g.max() <= 5.6
+    ref = O.generate(M.MASS, M.MASS_TRUTH, n, seed=77, nstreams=8)[0]
+    assert abs(g.mean() - ref.mean()) < 5 * ref.std() / np.sqrt(n) * 1.5
+    assert abs(g.std() - ref.std()) < 0.01 * ref.std()
+    b.close()
+    mdl.close()
+
+
+def test_fit_fp64_gpu_equals_cpu_backend(ctx):
+    n = 100000
+    data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=229387429, nstreams=16)
+    say("plain fit fp64 gpu")
+    v, e, out, ms = ctx.fit_plain(M.MASS, data, M.MASS_START, fp32=False)
+    assert out.status == 0
+    vc, ec, outc, msc = M.fit_cpu(M.MASS, data, M.MASS_START, nthreads=4)
+    assert np.max(np.abs(v - vc) / ec) < 1e-4
+    np.testing.assert_allclose(e, ec, rtol=1e-6)
+    say("plain fit fp32 gpu")
+    v32, e32, out32, _ = ctx.fit_plain(M.MASS, data, M.MASS_START, fp32=True)
+    assert out32.status == 0
+    assert np.max(np.abs(v32 - vc) / ec) < 1e-3
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_batched_fits_match_single_fits(ctx, fp32):
+    """each toy fitted inside one batched study == the same toy fitted alone (CPU backend,
+    FP64) within 1e-3 sigma (FP32) / 1e-4 sigma (FP64)"""
+    rs = np.random.default_rng(4)
+    sizes = list(rs.poisson(5000, 24))
+    datas = [O.generate(M.MASS, M.MASS_TRUTH, int(s), seed=1000 + i, nstreams=4) for i, s in enumerate(sizes)]
+    say("batched fits", len(sizes), "fp32", fp32)
+    mdl = ctx.model(M.MASS, fp32=fp32)
+    b = mdl.batch(sizes)
+    for i, d in enumerate(datas):
+        b.upload(i, d)
+    vals, errs, outs, stats = b.fit(M.MASS_START)
+    lim = 1e-3 if fp32 else 1e-4
+    for i, d in enumerate(datas):
+        assert outs[i].status == 0, i
+        vc, ec, oc, _ = M.fit_cpu(M.MASS, d, M.MASS_START, nthreads=1)
+        assert np.max(np.abs(vals[i] - vc) / ec) < lim, (i, (vals[i] - vc) / ec)
+    assert stats.ticks > 0
+    b.close()
+    mdl.close()
+
+
+def test_kstar_fit_gpu(ctx):
+    n = 200000
+    data = O.generate(M.KSTAR, M.KSTAR_TRUTH, n, seed=7, nstreams=16)
+    say("kstar plain fit")
+    v, e, out, ms = ctx.fit_plain(M.KSTAR, data, M.KSTAR_START, fp32=False)
+    assert out.status == 0
+    vc, ec, oc, _ = M.fit_cpu(M.KSTAR, data, M.KSTAR_START, nthreads=4)
+    assert np.max(np.abs(v - vc) / ec) < 1e-4
+    mdl = ctx.model(M.KSTAR, fp32=True)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    vals, errs, outs, stats = b.fit(M.KSTAR_START)
+    assert outs[0].status == 0
+    assert np.max(np.abs(vals[0] - vc) / ec) < 1e-3
+    b.close()
+    mdl.close()
+
+
+def test_full_size_properties(ctx):
+    """N = 1e7 (the headline configuration): FP32 and FP64 agree within the FP32 bar,
+    repeated evaluations are bitwise identical, the value sum is extensive (two halves)."""
+    n = 10_000_000
+    say("full size generate+eval", n)
+    m64 = ctx.model(M.MASS, fp32=False)
+    b64 = m64.batch([n])
+    b64.generate(M.MASS_TRUTH, [42])
+    data = b64.download(0)
+    r1 = b64.eval(M.EVAL_GRAD, M.MASS_START)
+    r2 = b64.eval(M.EVAL_GRAD, M.MASS_START)
+    assert np.array_equal(r1, r2)
+    halves = m64.batch([n // 2, n - n // 2])
+    halves.upload(0, data[:, : n // 2])
+    halves.upload(1, data[:, n // 2:])
+    rh = halves.eval(M.EVAL_GRAD, M.MASS_START)
+    assert abs(rh[:, 0].sum() - r1[0, 0]) <= 1e-12 * abs(r1[0, 0])
+    m32 = ctx.model(M.MASS, fp32=True)
+    b32 = m32.batch([n])
+    b32.upload(0, data)
+    r32 = b32.eval(M.EVAL_GRAD, M.MASS_START)
+    absref = O.nll_sums(M.MASS, data[:, :200000], M.MASS_START, mode=11, nthreads=8) * (n / 200000)
+    assert np.all(np.abs(r32[0] - r1[0]) <= 4e-6 * absref), (r32[0] - r1[0]) / absref
+    for x in (b64, halves, b32):
+        x.close()
+    m64.close()
+    m32.close()
