@@ -14,7 +14,7 @@
  *   mfh_fit_plain        fitter::fit with the drop-in HIPBlock (FP64 or FP32), one launch per call
  *   mfh_fit_cpu          fitter::fit with the host-compiled CPU backend (reference LLVM semantics)
  *   mfh_batch_fit        toystudy::toy's fit loop (toystudy.hh:55-111) as one batched study
- *   mfh_serve_run        no counterpart: continuous-batching fit service
+ *   mfh_serve_run        no counterpart: continuous-batching fit service (serve.hh)
  * Status: 0 on success, else nonzero with mfh_last_error().
  */
 #ifndef MOREFIT_CAPI_H
@@ -91,6 +91,18 @@ long long mfh_batch_bytes(mfh_batch* b);
 /* fits of the listed segments (NULL = all) from start values (nfit x ntheta) */
 int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, int use_graph, double* values, double* errors,
                   mfh_fit_out* outs, mfh_batch_stats* stats);
+
+/* Fit service with continuous batching over the given models (request class = index into
+ * `models`, all of one precision): requests arrive at arrival_s (seconds after the start),
+ * are admitted within `budget_bytes` of HBM and `slots` per class, get their dataset
+ * generated on the device from `seeds` at the class's generation values theta_gen (per class,
+ * ntheta values) and are fitted from `start` (per class).  Per request: timings[3] =
+ * arrival, admission, completion (s); values/errors (max ntheta per request, NaN-padded);
+ * status.  stats[0..6] = ticks, launches, peak live fits, rejected, wall s, peak bytes,
+ * generation ms. */
+int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen, const double* start, const int* cls,
+                  const long long* n, const unsigned long long* seeds, const double* arrival_s, int nreq, double budget_bytes,
+                  int slots, double* timings, double* values, double* errors, int* status, double* stats);
 
 /* one fit through the drop-in HIPBlock (reference fitter flow); data SoA nobs x n */
 int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, double* values,

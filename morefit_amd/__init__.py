@@ -62,7 +62,7 @@ CAPI_SYMBOLS = [
     "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
-    "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu",
+    "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run",
 ]
 
 
@@ -102,6 +102,9 @@ def libs():
         L.mfh_fit_plain.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_fit_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_eval_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
+        L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_ulonglong), dp, ip,
+                                    ctypes.c_double, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_int), dp]
         _capi = L
     return _hip, _capi
 
@@ -267,6 +270,35 @@ class Batch:
         sp = None if segs is None else (ctypes.c_int * n)(*lst)
         _check(libs()[1].mfh_batch_fit(self.h, sp, n, _dp(st), int(use_graph), _dp(vals), _dp(errs), outs, ctypes.byref(stats)))
         return vals, errs, list(outs), stats
+
+
+def serve(models, theta_gen, start, cls, n, seeds, arrival_s, budget_bytes, slots=256):
+    """Run the continuous-batching fit service over `models` (request class = index).
+    Returns dict with per-request arrival/admit/done times, values, errors, status, stats."""
+    nm = len(models)
+    maxp = max(m.ntheta for m in models)
+    tg = np.full((nm, maxp), np.nan)
+    st = np.full((nm, maxp), np.nan)
+    for i, m in enumerate(models):
+        tg[i, :m.ntheta] = theta_gen[i]
+        st[i, :m.ntheta] = start[i]
+    nreq = len(cls)
+    mh = (ctypes.c_void_p * nm)(*[m.h.value for m in models])
+    ci = (ctypes.c_int * nreq)(*[int(c) for c in cls])
+    ni = (ctypes.c_longlong * nreq)(*[int(x) for x in n])
+    si = (ctypes.c_ulonglong * nreq)(*[int(x) for x in seeds])
+    arr = _f64(arrival_s)
+    timings = np.zeros((nreq, 3))
+    vals = np.zeros((nreq, maxp))
+    errs = np.zeros((nreq, maxp))
+    status = (ctypes.c_int * nreq)()
+    stats = np.zeros(7)
+    tg, st = _f64(tg), _f64(st)
+    _check(libs()[1].mfh_serve_run(mh, nm, _dp(tg), _dp(st), ci, ni, si, _dp(arr), nreq, float(budget_bytes), int(slots),
+                                   _dp(timings), _dp(vals), _dp(errs), status, _dp(stats)))
+    return {"arrival": timings[:, 0], "admit": timings[:, 1], "done": timings[:, 2], "values": vals, "errors": errs,
+            "status": np.array(list(status)), "ticks": int(stats[0]), "launches": int(stats[1]), "peak_live": int(stats[2]),
+            "rejected": int(stats[3]), "wall_s": stats[4], "peak_bytes": stats[5], "generate_ms": stats[6]}
 
 
 def fit_cpu(kind, data, start, nthreads=1):

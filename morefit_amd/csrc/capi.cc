@@ -9,6 +9,7 @@
 
 #include "morefit.hh"
 #include "morefit/batch.hh"
+#include "morefit/serve.hh"
 #include "morefit_capi.h"
 
 using namespace morefit;
@@ -375,6 +376,64 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
     }
     return 0;
   });
+  MFH_CATCH
+}
+
+int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen, const double* start, const int* cls,
+                  const long long* n, const unsigned long long* seeds, const double* arrival_s, int nreq, double budget_bytes,
+                  int slots, double* timings, double* values, double* errors, int* status, double* stats) {
+  MFH_TRY
+  if (nmodels <= 0) throw std::invalid_argument("no models");
+  const bool fp32 = models[0]->fp32;
+  int maxp = 0;
+  for (int i = 0; i < nmodels; i++) {
+    if (models[i]->fp32 != fp32) throw std::invalid_argument("all service models must share one precision");
+    maxp = std::max(maxp, models[i]->with([](auto& hm) { return hm.ntheta; }));
+  }
+  std::vector<ServeRequest> reqs(nreq);
+  for (int i = 0; i < nreq; i++) {
+    reqs[i].id = i;
+    reqs[i].cls = cls[i];
+    reqs[i].n = n[i];
+    reqs[i].seed = seeds[i];
+    reqs[i].arrival_s = arrival_s[i];
+    if (cls[i] < 0 || cls[i] >= nmodels) throw std::out_of_range("request class");
+  }
+  std::stable_sort(reqs.begin(), reqs.end(), [](const ServeRequest& a, const ServeRequest& b) { return a.arrival_s < b.arrival_s; });
+  auto serve = [&](auto tag) {
+    typedef decltype(tag) K;
+    FitServer<K> srv((size_t)budget_bytes, slots);
+    for (int c = 0; c < nmodels; c++) {
+      HipModel<K>* hm;
+      if constexpr (std::is_same<K, float>::value) hm = models[c]->mf.get();
+      else hm = models[c]->md.get();
+      std::vector<double> tg(theta_gen + (size_t)c * maxp, theta_gen + (size_t)c * maxp + hm->ntheta);
+      std::vector<double> st(start + (size_t)c * maxp, start + (size_t)c * maxp + hm->ntheta);
+      srv.add_class(hm, tg, st);
+    }
+    auto res = srv.run(reqs);
+    for (auto& r : res) {
+      const long long i = r.id;
+      timings[3 * i + 0] = r.arrival_s;
+      timings[3 * i + 1] = r.admit_s;
+      timings[3 * i + 2] = r.done_s;
+      for (int k = 0; k < maxp; k++) {
+        values[i * maxp + k] = k < (int)r.fit.values.size() ? r.fit.values[k] : NAN;
+        errors[i * maxp + k] = k < (int)r.fit.errors.size() ? r.fit.errors[k] : NAN;
+      }
+      status[i] = r.fit.status;
+    }
+    stats[0] = srv.stats.ticks;
+    stats[1] = srv.stats.launches;
+    stats[2] = srv.stats.peak_live;
+    stats[3] = srv.stats.rejected;
+    stats[4] = srv.stats.wall_s;
+    stats[5] = srv.stats.peak_bytes;
+    stats[6] = srv.stats.generate_ms;
+    return 0;
+  };
+  if (fp32) serve(float());
+  else serve(double());
   MFH_CATCH
 }
 

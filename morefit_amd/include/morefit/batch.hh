@@ -1,11 +1,11 @@
 /**
  * @file batch.hh
- * Many fits at once over one HipBatch: each fit runs the ordinary blocking MIGRAD/HESSE code
- * in its own fibre (ucontext); whenever a fit asks for a value, a gradient or a Hessian it
- * posts its parameters and yields.  When every live fibre has yielded, the scheduler issues
- * one shared launch per requested kind over all requesting segments (each kind's launch
- * sequence a replayed hipGraph), hands the totals back and resumes the fibres.  Converged
- * fits leave; the others keep going.  This inverts Minuit's synchronous FCN callback
+ * Many fits at once.  Each fit runs the ordinary blocking MIGRAD/HESSE code in its own fibre
+ * (ucontext); when a fit asks for a value, a gradient or a Hessian it posts its parameters
+ * into its segment's slot and yields.  Once every live fibre has yielded, the scheduler
+ * issues one shared launch per (batch, kind) over all requesting segments, and resumes the
+ * fibres with their totals.  Converged fits leave, new fits may join between ticks (the fit
+ * service), the others keep going.  This inverts Minuit's synchronous FCN callback
  * (reference toystudy.hh:55-111 runs one fit after another, each with its own launches).
  */
 #ifndef MOREFIT_BATCH_HH
@@ -16,8 +16,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <functional>
+#include <list>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <vector>
 
@@ -40,10 +41,22 @@ struct BatchFitStats {
 };
 
 template <typename kernelT, typename evalT = double>
-class BatchFitter {
+class FitScheduler {
  public:
   typedef HipBatch<kernelT, evalT> Batch;
-  Batch& b;
+  struct Task {
+    FitScheduler* owner = nullptr;
+    Batch* b = nullptr;
+    int seg = 0;
+    long long id = 0;
+    std::vector<double> start;
+    std::vector<char> stack;
+    ucontext_t ctx;
+    int pending = 0;
+    bool done = false;
+    BatchFitResult result;
+  };
+
   bool use_graph = true;
   bool run_hesse = true;
   int strategy = 2;
@@ -51,95 +64,101 @@ class BatchFitter {
   size_t stack_bytes = 256 * 1024;
   BatchFitStats stats;
 
-  explicit BatchFitter(Batch& batch) : b(batch) {}
+  FitScheduler() {}
+  FitScheduler(const FitScheduler&) = delete;
+  FitScheduler& operator=(const FitScheduler&) = delete;
 
-  // fits every segment in `segs` from its start values; parameter limits from the model's
-  // parameter objects.  Results in segment order of `segs`.
-  std::vector<BatchFitResult> fit(const std::vector<int>& segs, const std::vector<double>& start /* per listed seg: ntheta */) {
-    auto t0 = std::chrono::steady_clock::now();
-    const int F = (int)segs.size(), P = b.m->ntheta;
-    fibres_.clear();
-    fibres_.resize(F);
-    for (int i = 0; i < F; i++) {
-      Fibre& f = fibres_[i];
-      f.owner = this;
-      f.seg = segs[i];
-      f.start.assign(start.begin() + (size_t)i * P, start.begin() + (size_t)(i + 1) * P);
-      f.stack.resize(stack_bytes);
-      getcontext(&f.ctx);
-      f.ctx.uc_stack.ss_sp = f.stack.data();
-      f.ctx.uc_stack.ss_size = f.stack.size();
-      f.ctx.uc_link = &main_;
-      uintptr_t p = (uintptr_t)&f;
-      makecontext(&f.ctx, (void (*)())&BatchFitter::trampoline, 2, (unsigned int)(p & 0xffffffffu), (unsigned int)(p >> 32));
-    }
-    int live = F;
-    double wait_ms = 0.0;
-    while (live > 0) {
-      for (auto& f : fibres_) {
-        if (f.done || f.pending) continue;
-        current_ = &f;
-        swapcontext(&main_, &f.ctx);
-        current_ = nullptr;
-        if (f.done) live--;
+  // a new fit of segment `seg` of batch `b` from `start` (ntheta values)
+  Task* add(Batch* b, int seg, const double* start, long long id = 0) {
+    tasks_.emplace_back();
+    Task& t = tasks_.back();
+    t.owner = this;
+    t.b = b;
+    t.seg = seg;
+    t.id = id;
+    t.start.assign(start, start + b->m->ntheta);
+    t.stack.resize(stack_bytes);
+    getcontext(&t.ctx);
+    t.ctx.uc_stack.ss_sp = t.stack.data();
+    t.ctx.uc_stack.ss_size = t.stack.size();
+    t.ctx.uc_link = &main_;
+    uintptr_t p = (uintptr_t)&t;
+    makecontext(&t.ctx, (void (*)())&FitScheduler::trampoline, 2, (unsigned int)(p & 0xffffffffu), (unsigned int)(p >> 32));
+    batches_.insert(b);
+    return &t;
+  }
+  size_t live() const { return tasks_.size(); }
+
+  // Runs every task that is not waiting until it posts a request or finishes; finished
+  // tasks are handed to `on_done` and removed.  Returns the number still live.
+  template <typename F>
+  size_t step(F on_done) {
+    for (auto it = tasks_.begin(); it != tasks_.end();) {
+      Task& t = *it;
+      if (!t.pending && !t.done) swapcontext(&main_, &t.ctx);
+      if (t.done) {
+        on_done(t);
+        it = tasks_.erase(it);
+      } else {
+        ++it;
       }
+    }
+    return tasks_.size();
+  }
+  // one shared launch per (batch, kind) over every pending request, then wakes the tasks
+  void launch() {
+    auto tw = std::chrono::steady_clock::now();
+    for (Batch* b : batches_) {
       bool any = false;
-      for (auto& f : fibres_) any |= f.pending != 0;
-      if (!any) break;
-      for (auto& f : fibres_)
-        if (f.pending) b.request(f.seg, f.pending);
-      auto tw = std::chrono::steady_clock::now();
+      for (auto& t : tasks_)
+        if (t.b == b && t.pending) {
+          b->request(t.seg, t.pending);
+          any = true;
+        }
+      if (!any) continue;
       for (int k = 1; k <= 3; k++) {
         bool want = false;
-        for (auto& f : fibres_) want |= f.pending == k;
+        for (int s = 0; s < b->S && !want; s++) want = b->requested(s) == k;
         if (!want) continue;
-        b.set_modes(k, false);
-        b.run(k, false, use_graph);
+        b->run(k, false, use_graph);
         stats.launches[k]++;
       }
-      std::fill(b.request_.begin(), b.request_.end(), 0);
-      wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
-      for (auto& f : fibres_) f.pending = 0;  // results are in the batch's pinned buffers
-      stats.ticks++;
+      std::fill(b->request_.begin(), b->request_.end(), 0);
     }
-    std::vector<BatchFitResult> out(F);
-    for (int i = 0; i < F; i++) out[i] = fibres_[i].result;
-    stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    stats.gpu_wait_ms = wait_ms;
-    stats.host_ms = stats.wall_ms - wait_ms;
-    fibres_.clear();
-    return out;
+    for (auto& t : tasks_) t.pending = 0;
+    stats.ticks++;
+    stats.gpu_wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
   }
 
  private:
-  struct Fibre;
   struct Fcn : MinimizerFcn {
-    Fibre* f;
-    explicit Fcn(Fibre* ff) : f(ff) {}
+    Task* t;
+    explicit Fcn(Task* tt) : t(tt) {}
     void post(const double* x, int kind) {
-      double* th = f->owner->b.theta(f->seg);
-      for (size_t k = 0; k < f->owner->b.m->theta_index.size(); k++) th[k] = x[f->owner->b.m->theta_index[k]];
-      f->pending = kind;
-      swapcontext(&f->ctx, &f->owner->main_);
+      double* th = t->b->theta(t->seg);
+      const auto& ti = t->b->m->theta_index;
+      for (size_t k = 0; k < ti.size(); k++) th[k] = x[ti[k]];
+      t->pending = kind;
+      swapcontext(&t->ctx, &t->owner->main_);
     }
     double value(const double* x) override {
       post(x, EVAL_LH);
-      return -2.0 * f->owner->b.result(f->seg, EVAL_LH)[0];
+      return -2.0 * t->b->result(t->seg, EVAL_LH)[0];
     }
     double value_grad(const double* x, double* g) override {
       post(x, EVAL_GRAD);
-      const double* r = f->owner->b.result(f->seg, EVAL_GRAD);
-      const auto& ti = f->owner->b.m->theta_index;
-      const size_t N = f->owner->b.m->params.size();
+      const double* r = t->b->result(t->seg, EVAL_GRAD);
+      const auto& ti = t->b->m->theta_index;
+      const size_t N = t->b->m->params.size();
       for (size_t i = 0; i < N; i++) g[i] = 0.0;
       for (size_t k = 0; k < ti.size(); k++) g[ti[k]] = -2.0 * r[1 + k];
       return -2.0 * r[0];
     }
     bool hessian(const double* x, double* h) override {
       post(x, EVAL_HESS);
-      const double* r = f->owner->b.result(f->seg, EVAL_HESS);
-      const auto& ti = f->owner->b.m->theta_index;
-      const size_t N = f->owner->b.m->params.size(), P = ti.size();
+      const double* r = t->b->result(t->seg, EVAL_HESS);
+      const auto& ti = t->b->m->theta_index;
+      const size_t N = t->b->m->params.size(), P = ti.size();
       for (size_t i = 0; i < N * N; i++) h[i] = 0.0;
       size_t idx = 1 + P;
       for (size_t i = 0; i < P; i++)
@@ -151,36 +170,27 @@ class BatchFitter {
       return true;
     }
   };
-  struct Fibre {
-    BatchFitter* owner = nullptr;
-    int seg = 0;
-    std::vector<double> start;
-    std::vector<char> stack;
-    ucontext_t ctx;
-    int pending = 0;
-    bool done = false;
-    BatchFitResult result;
-  };
-  std::vector<Fibre> fibres_;
+
+  std::list<Task> tasks_;  // stable addresses: contexts point into them
+  std::set<Batch*> batches_;
   ucontext_t main_;
-  Fibre* current_ = nullptr;
 
   static void trampoline(unsigned int lo, unsigned int hi) {
-    Fibre* f = (Fibre*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
-    f->owner->run_fit(*f);
-    f->done = true;
-    f->pending = 0;
+    Task* t = (Task*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
+    t->owner->run_fit(*t);
+    t->done = true;
+    t->pending = 0;
   }
 
-  void run_fit(Fibre& f) {
-    const auto& ps = b.m->params;
+  void run_fit(Task& t) {
+    const auto& ps = t.b->m->params;
     std::vector<MinimizerParam> mp;
     size_t k = 0;
     for (size_t i = 0; i < ps.size(); i++) {
       MinimizerParam q;
       q.name = ps[i]->get_name();
       q.fixed = ps[i]->is_constant();
-      q.value = q.fixed ? ps[i]->get_value() : f.start[k++];
+      q.value = q.fixed ? ps[i]->get_value() : t.start[k++];
       q.step = ps[i]->get_step_size();
       if (!ps[i]->is_unlimited()) {
         q.lo = ps[i]->get_min();
@@ -193,22 +203,48 @@ class BatchFitter {
     Migrad m(mp);
     m.strategy = strategy;
     m.tolerance = tolerance;
-    Fcn fcn(&f);
+    Fcn fcn(&t);
     try {
       m.minimize(fcn);
-      if (run_hesse) f.result.hesse_ok = m.hesse(fcn);
+      if (run_hesse) t.result.hesse_ok = m.hesse(fcn);
     } catch (...) {
       m.status = 4;
     }
-    for (size_t i = 0; i < b.m->theta_index.size(); i++) {
-      f.result.values.push_back(m.x[b.m->theta_index[i]]);
-      f.result.errors.push_back(m.errors[b.m->theta_index[i]]);
+    for (size_t i = 0; i < t.b->m->theta_index.size(); i++) {
+      t.result.values.push_back(m.x[t.b->m->theta_index[i]]);
+      t.result.errors.push_back(m.errors[t.b->m->theta_index[i]]);
     }
-    f.result.fval = m.fval;
-    f.result.edm = m.edm;
-    f.result.status = m.status;
-    f.result.ncalls = m.ncalls;
-    f.result.iterations = m.iterations;
+    t.result.fval = m.fval;
+    t.result.edm = m.edm;
+    t.result.status = m.status;
+    t.result.ncalls = m.ncalls;
+    t.result.iterations = m.iterations;
+  }
+};
+
+// fits of a fixed set of segments of one batch (a toy study)
+template <typename kernelT, typename evalT = double>
+class BatchFitter {
+ public:
+  HipBatch<kernelT, evalT>& b;
+  bool use_graph = true;
+  bool run_hesse = true;
+  BatchFitStats stats;
+  explicit BatchFitter(HipBatch<kernelT, evalT>& batch) : b(batch) {}
+
+  std::vector<BatchFitResult> fit(const std::vector<int>& segs, const std::vector<double>& start /* ntheta per listed seg */) {
+    auto t0 = std::chrono::steady_clock::now();
+    FitScheduler<kernelT, evalT> sch;
+    sch.use_graph = use_graph;
+    sch.run_hesse = run_hesse;
+    const int P = b.m->ntheta;
+    for (size_t i = 0; i < segs.size(); i++) sch.add(&b, segs[i], &start[i * P], (long long)i);
+    std::vector<BatchFitResult> out(segs.size());
+    while (sch.step([&](auto& t) { out[t.id] = t.result; }) > 0) sch.launch();
+    stats = sch.stats;
+    stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats.host_ms = stats.wall_ms - stats.gpu_wait_ms;
+    return out;
   }
 };
 

@@ -565,59 +565,185 @@ inline void xoshiro_streams(uint64_t seed, int n, unsigned int* out) {
   }
 }
 
+// first-fit allocator over an arena of `capacity` elements (offsets aligned to 64 elements)
+class ArenaAllocator {
+  std::vector<std::pair<long long, long long>> free_;  // (offset, length), sorted by offset
+
+ public:
+  long long capacity = 0;
+  explicit ArenaAllocator(long long cap = 0) { reset(cap); }
+  void reset(long long cap) {
+    capacity = cap;
+    free_.clear();
+    if (cap > 0) free_.push_back({0, cap});
+  }
+  static long long round(long long n) { return (std::max<long long>(n, 1) + 63) / 64 * 64; }
+  long long alloc(long long n) {
+    n = round(n);
+    for (size_t i = 0; i < free_.size(); i++)
+      if (free_[i].second >= n) {
+        long long off = free_[i].first;
+        free_[i].first += n;
+        free_[i].second -= n;
+        if (free_[i].second == 0) free_.erase(free_.begin() + i);
+        return off;
+      }
+    return -1;
+  }
+  void release(long long off, long long n) {
+    n = round(n);
+    auto it = free_.begin();
+    while (it != free_.end() && it->first < off) ++it;
+    it = free_.insert(it, {off, n});
+    size_t i = it - free_.begin();
+    if (i + 1 < free_.size() && free_[i].first + free_[i].second == free_[i + 1].first) {
+      free_[i].second += free_[i + 1].second;
+      free_.erase(free_.begin() + i + 1);
+    }
+    if (i > 0 && free_[i - 1].first + free_[i - 1].second == free_[i].first) {
+      free_[i - 1].second += free_[i].second;
+      free_.erase(free_.begin() + i);
+    }
+  }
+  long long largest_free() const {
+    long long m = 0;
+    for (auto& f : free_) m = std::max(m, f.second);
+    return m;
+  }
+};
+
+// Segments of one model resident in one HBM arena.  Built either with fixed sizes (a toy
+// study, one big dataset) or with a slot capacity and an arena budget, in which case
+// segments are added and removed while other segments keep being evaluated (the fit
+// service).  Chunk tables are rebuilt when membership changes; a changed table drops the
+// captured graphs, which are recaptured on the next graph launch.
 template <typename kernelT, typename evalT = double>
 class HipBatch {
  public:
   HipModel<kernelT, evalT>* m;
   mf_device* dev;
-  int S = 0;
-  std::vector<long long> sizes, offsets;
-  long long total_elems = 0;
-  int chunk = 0, nchunks = 0;
+  int S = 0;                             // slot capacity
+  std::vector<long long> sizes, offsets; // per slot (events, element offset)
+  std::vector<char> active;
+  ArenaAllocator arena;
+  long long total_elems = 0;             // arena capacity in elements
+  int chunk = 0, nchunks = 0, chunk_cap = 0;
+  bool dirty = true;
   std::vector<mf_segment> segs_h;
   DevBuf data, theta_d, params_d, segs_d[4], chunks_d, partials_d, results_d[4], counters_d;
   HostBuf theta_h, segs_hk[4], results_h[4];
   mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> request_;
 
+  // fixed segments, packed back to back
   HipBatch(HipModel<kernelT, evalT>* model, const std::vector<long long>& seg_sizes) : m(model), dev(model->be->device()) {
+    long long tot = 0;
+    for (auto n : seg_sizes) tot += ArenaAllocator::round(n * m->nobs);
+    init((int)seg_sizes.size(), tot);
+    for (size_t s = 0; s < seg_sizes.size(); s++)
+      if (add_segment(seg_sizes[s]) != (int)s) throw std::runtime_error("HipBatch: packing failed");
+    rebuild();
+  }
+  // `capacity` slots in an arena of `arena_bytes`; segments come and go via add/remove
+  HipBatch(HipModel<kernelT, evalT>* model, int capacity, size_t arena_bytes) : m(model), dev(model->be->device()) {
+    init(capacity, (long long)(arena_bytes / sizeof(kernelT)));
+  }
+  ~HipBatch() { drop_graphs(); }
+  HipBatch(const HipBatch&) = delete;
+  HipBatch& operator=(const HipBatch&) = delete;
+
+  void init(int capacity, long long elems) {
     const compute_options& o = *m->be->opts_;
-    S = (int)seg_sizes.size();
-    sizes = seg_sizes;
+    S = capacity;
     chunk = o.hip_block_size * o.hip_events_per_thread;
-    std::vector<mf_chunk> ch;
-    long long off = 0;
-    for (int s = 0; s < S; s++) {
-      mf_segment sg{};
-      sg.data_off = off;
-      sg.stride = sizes[s];
-      sg.n = (int)sizes[s];
-      sg.param_off = s * m->nparam;
-      sg.out_off = 0;
-      sg.mode = 0;
-      append_chunks(ch, sg, s, sizes[s], chunk);
-      segs_h.push_back(sg);
-      offsets.push_back(off);
-      off += sizes[s] * m->nobs;
-    }
-    total_elems = off;
-    nchunks = (int)ch.size();
-    data.alloc(dev, std::max<long long>(1, total_elems) * sizeof(kernelT));
+    arena.reset(elems);
+    total_elems = elems;
+    sizes.assign(S, 0);
+    offsets.assign(S, 0);
+    active.assign(S, 0);
+    segs_h.assign(S, mf_segment{});
+    data.alloc(dev, std::max<long long>(1, elems) * sizeof(kernelT));
     theta_d.alloc(dev, std::max(1, S * m->ntheta) * sizeof(double));
     theta_h.alloc(dev, std::max(1, S * m->ntheta) * sizeof(double));
     params_d.alloc(dev, std::max(1, S * m->nparam) * sizeof(kernelT));
-    chunks_d.alloc(dev, ch.size() * sizeof(mf_chunk));
-    hip_check(mf_memcpy_h2d(dev, chunks_d.ptr, ch.data(), ch.size() * sizeof(mf_chunk)), "HipBatch chunks");
+    chunk_cap = (int)(elems / m->nobs / chunk) + 2 * S + 1;
+    chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
     int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
-    partials_d.alloc(dev, (size_t)nchunks * nmax * sizeof(double));
+    partials_d.alloc(dev, (size_t)chunk_cap * nmax * sizeof(double));
     counters_d.alloc(dev, std::max(1, S) * sizeof(unsigned int));
     hip_check(mf_fill_zero(dev, counters_d.ptr, std::max(1, S) * sizeof(unsigned int)), "HipBatch counters");
     for (int k = 1; k <= 3; k++) {
       if (m->nout[k] == 0) continue;
-      segs_d[k].alloc(dev, S * sizeof(mf_segment));
-      segs_hk[k].alloc(dev, S * sizeof(mf_segment));
-      results_d[k].alloc(dev, (size_t)S * m->nout[k] * sizeof(double));
-      results_h[k].alloc(dev, (size_t)S * m->nout[k] * sizeof(double));
+      segs_d[k].alloc(dev, std::max(1, S) * sizeof(mf_segment));
+      segs_hk[k].alloc(dev, std::max(1, S) * sizeof(mf_segment));
+      results_d[k].alloc(dev, (size_t)std::max(1, S) * m->nout[k] * sizeof(double));
+      results_h[k].alloc(dev, (size_t)std::max(1, S) * m->nout[k] * sizeof(double));
+      std::memset(results_h[k].ptr, 0, results_h[k].bytes);
+    }
+    std::memset(theta_h.ptr, 0, theta_h.bytes);
+    request_.assign(S, 0);
+    hip_check(mf_sync(dev), "HipBatch init");
+  }
+
+  // ---------------------------------------------------------------- membership
+  // returns the slot, or -1 when no slot or no contiguous arena space is left
+  int add_segment(long long n) {
+    int slot = -1;
+    for (int s = 0; s < S && slot < 0; s++)
+      if (!active[s]) slot = s;
+    if (slot < 0) return -1;
+    long long off = arena.alloc(n * m->nobs);
+    if (off < 0) return -1;
+    active[slot] = 1;
+    sizes[slot] = n;
+    offsets[slot] = off;
+    dirty = true;
+    return slot;
+  }
+  void remove_segment(int s) {
+    if (s < 0 || s >= S || !active[s]) return;
+    arena.release(offsets[s], sizes[s] * m->nobs);
+    active[s] = 0;
+    sizes[s] = 0;
+    request_[s] = 0;
+    dirty = true;
+  }
+  bool fits(long long n) const {
+    bool slot = false;
+    for (int s = 0; s < S && !slot; s++) slot = !active[s];
+    return slot && arena.largest_free() >= ArenaAllocator::round(n * m->nobs);
+  }
+  int active_count() const {
+    int c = 0;
+    for (auto a : active) c += a;
+    return c;
+  }
+  void drop_graphs() {
+    for (auto*& g : graphs)
+      if (g) {
+        mf_graph_destroy(g);
+        g = nullptr;
+      }
+  }
+  // chunk table over the active slots, in slot order
+  void rebuild() {
+    if (!dirty) return;
+    std::vector<mf_chunk> ch;
+    for (int s = 0; s < S; s++) {
+      mf_segment sg{};
+      sg.data_off = offsets[s];
+      sg.stride = sizes[s];
+      sg.n = (int)sizes[s];
+      sg.param_off = s * m->nparam;
+      sg.mode = 0;
+      if (active[s]) append_chunks(ch, sg, s, sizes[s], chunk);
+      segs_h[s] = sg;
+    }
+    if ((int)ch.size() > chunk_cap) throw std::runtime_error("HipBatch: chunk table over capacity");
+    nchunks = (int)ch.size();
+    if (nchunks) hip_check(mf_memcpy_h2d(dev, chunks_d.ptr, ch.data(), ch.size() * sizeof(mf_chunk)), "HipBatch chunks");
+    for (int k = 1; k <= 3; k++) {
+      if (m->nout[k] == 0) continue;
       mf_segment* t = segs_hk[k].template as<mf_segment>();
       for (int s = 0; s < S; s++) {
         t[s] = segs_h[s];
@@ -625,20 +751,18 @@ class HipBatch {
         t[s].mode = 0;
       }
     }
-    std::memset(theta_h.ptr, 0, theta_h.bytes);
-    request_.assign(S, 0);
-    hip_check(mf_sync(dev), "HipBatch init");
+    hip_check(mf_sync(dev), "HipBatch rebuild");
+    drop_graphs();
+    dirty = false;
   }
-  ~HipBatch() {
-    for (auto* g : graphs)
-      if (g) mf_graph_destroy(g);
-  }
-  HipBatch(const HipBatch&) = delete;
-  HipBatch& operator=(const HipBatch&) = delete;
 
   double* theta(int s) { return theta_h.template as<double>() + (size_t)s * m->ntheta; }
   const double* result(int s, int kind) const { return results_h[kind].template as<double>() + (size_t)s * m->nout[kind]; }
-  long long data_bytes() const { return total_elems * (long long)sizeof(kernelT); }
+  long long data_bytes() const {
+    long long b = 0;
+    for (int s = 0; s < S; s++) b += active[s] ? sizes[s] * m->nobs : 0;
+    return b * (long long)sizeof(kernelT);
+  }
 
   // host FP64 columns (SoA, col_stride elements apart) -> stored type, centred
   void upload(int s, const double* cols, long long col_stride) {
@@ -659,30 +783,38 @@ class HipBatch {
       for (long long i = 0; i < n; i++) cols[d * n + i] = (double)tmp[d * n + i] + m->centres[d];
   }
 
-  // prepare kernel parameters of every segment from the staged theta
+  // prepare kernel parameters of every slot from the staged theta
   void prep() {
     hip_check(mf_memcpy_h2d(dev, theta_d.ptr, theta_h.ptr, (size_t)S * m->ntheta * sizeof(double)), "HipBatch theta");
     hip_check(mf_launch_prep(m->mod, theta_d.template as<double>(), params_d.ptr, nullptr, S), "mf_launch_prep");
   }
 
-  // On-device accept-reject into every segment with mode != 0 semantics ignored: all segments.
-  // theta_gen: S x ntheta generation values; seeds: one 64-bit seed per segment (streams
-  // separated by jump()).  Returns total proposals.
-  unsigned long long generate(const std::vector<double>& theta_gen, const std::vector<uint64_t>& seeds, int events_per_stream = 64) {
-    for (int s = 0; s < S; s++) std::memcpy(theta(s), &theta_gen[(size_t)s * m->ntheta], m->ntheta * sizeof(double));
+  // On-device accept-reject into the listed slots (all active slots when `slots` is empty).
+  // theta_gen: ntheta generation values per listed slot; seeds: one per listed slot (its
+  // streams are separated by jump()).  Returns the total number of proposals.
+  unsigned long long generate(const std::vector<double>& theta_gen, const std::vector<uint64_t>& seeds, int events_per_stream = 64,
+                              std::vector<int> slots = {}) {
+    rebuild();
+    if (slots.empty())
+      for (int s = 0; s < S; s++)
+        if (active[s]) slots.push_back(s);
+    const int L = (int)slots.size();
+    for (int i = 0; i < L; i++) std::memcpy(theta(slots[i]), &theta_gen[(size_t)i * m->ntheta], m->ntheta * sizeof(double));
     prep();
     std::vector<mf_gen_stream> st;
     std::vector<unsigned int> states;
-    std::vector<double> mp(S);
-    for (int s = 0; s < S; s++) {
-      mp[s] = m->maxprob(&theta_gen[(size_t)s * m->ntheta]);
+    std::vector<double> mp(S, 1.0);
+    for (int i = 0; i < L; i++) {
+      const int s = slots[i];
+      mp[s] = m->maxprob(&theta_gen[(size_t)i * m->ntheta]);
       int ns = (int)((sizes[s] + events_per_stream - 1) / events_per_stream);
       size_t base = states.size();
       states.resize(base + 4 * (size_t)ns);
-      xoshiro_streams(seeds[s], ns, states.data() + base);
+      xoshiro_streams(seeds[i], ns, states.data() + base);
       for (int k = 0; k < ns; k++)
         st.push_back({s, (int)((long long)k * events_per_stream), (int)std::min<long long>(sizes[s], (long long)(k + 1) * events_per_stream), 0});
     }
+    if (st.empty()) return 0;
     DevBuf st_d(dev, std::max<size_t>(16, st.size() * sizeof(mf_gen_stream)));
     DevBuf states_d(dev, std::max<size_t>(16, states.size() * 4));
     DevBuf mp_d(dev, S * sizeof(double));
@@ -737,6 +869,7 @@ class HipBatch {
   }
   // just the evaluation kernel of `kind` (segment table and parameters already on device)
   void launch_kernel(int kind) {
+    if (nchunks == 0) return;
     mf_eval_args a = eval_args(kind);
     hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->be->opts_->hip_block_size), "mf_launch_eval");
   }
@@ -749,11 +882,12 @@ class HipBatch {
   }
   void set_modes(int kind, bool all) {
     mf_segment* t = segs_hk[kind].template as<mf_segment>();
-    for (int s = 0; s < S; s++) t[s].mode = (all || request_[s] == kind) ? 1 : 0;
+    for (int s = 0; s < S; s++) t[s].mode = active[s] && (all || request_[s] == kind) ? 1 : 0;
   }
-  // evaluate `kind` on the requested (or all) segments; as a replayed hipGraph when use_graph
+  // evaluate `kind` on the requested (or all active) segments; as a replayed hipGraph when use_graph
   void run(int kind, bool all, bool use_graph) {
     if (m->nout[kind] == 0) throw std::runtime_error("HipBatch::run: kind not compiled");
+    rebuild();
     set_modes(kind, all);
     if (use_graph) {
       if (!graphs[kind]) {

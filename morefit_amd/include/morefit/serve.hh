@@ -1,0 +1,150 @@
+/**
+ * @file serve.hh
+ * Fit service with continuous batching.  Requests (a density class, an event count, a
+ * seed, an arrival time) arrive while other fits run.  A request is admitted when its
+ * class's pool has a free slot and the service's HBM budget covers its events; its dataset
+ * is generated on the device into the pool's arena, and its fit joins the shared fibre
+ * scheduler.  Every tick issues one launch per (class, kind) over all fits that are waiting
+ * for a value, a gradient or a Hessian, so the likelihood evaluations of concurrent fits
+ * share launches; a finished fit releases its slot and memory at once.  Results are
+ * bitwise the same alone and in the mix: a segment's sums depend only on its own chunks.
+ *
+ * No reference counterpart (the reference fits one dataset at a time, fitter.hh:660-1002).
+ */
+#ifndef MOREFIT_SERVE_HH
+#define MOREFIT_SERVE_HH
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "batch.hh"
+
+namespace morefit {
+
+struct ServeRequest {
+  long long id = 0;
+  int cls = 0;
+  long long n = 0;
+  uint64_t seed = 0;
+  double arrival_s = 0.0;
+};
+
+struct ServeResult {
+  long long id = 0;
+  int cls = 0;
+  long long n = 0;
+  double arrival_s = 0.0, admit_s = -1.0, done_s = -1.0;
+  BatchFitResult fit;
+};
+
+struct ServeStats {
+  int ticks = 0, launches = 0, peak_live = 0, rejected = 0;
+  double wall_s = 0.0, peak_bytes = 0.0, generate_ms = 0.0;
+};
+
+template <typename kernelT, typename evalT = double>
+class FitServer {
+ public:
+  struct Class {
+    HipModel<kernelT, evalT>* model;
+    std::vector<double> theta_gen, start;  // generation values and fit start values
+    std::unique_ptr<HipBatch<kernelT, evalT>> pool;
+  };
+  std::vector<Class> classes;
+  size_t budget_bytes;
+  ServeStats stats;
+  bool use_graph = false;  // membership changes every few ticks: direct launches
+
+  FitServer(size_t hbm_budget_bytes, int slots_per_class) : budget_bytes(hbm_budget_bytes), slots_(slots_per_class) {}
+
+  void add_class(HipModel<kernelT, evalT>* m, const std::vector<double>& theta_gen, const std::vector<double>& start) {
+    Class c;
+    c.model = m;
+    c.theta_gen = theta_gen;
+    c.start = start;
+    c.pool.reset(new HipBatch<kernelT, evalT>(m, slots_, budget_bytes));
+    classes.push_back(std::move(c));
+  }
+
+  // Serves `reqs` (sorted by arrival) in real time; returns one result per request.
+  std::vector<ServeResult> run(const std::vector<ServeRequest>& reqs) {
+    std::vector<ServeResult> res(reqs.size());
+    for (size_t i = 0; i < reqs.size(); i++) {
+      res[i].id = reqs[i].id;
+      res[i].cls = reqs[i].cls;
+      res[i].n = reqs[i].n;
+      res[i].arrival_s = reqs[i].arrival_s;
+    }
+    FitScheduler<kernelT, evalT> sch;
+    sch.use_graph = use_graph;
+    std::deque<size_t> queue;
+    size_t next = 0, finished = 0;
+    double used = 0.0;
+    std::vector<int> slot_of(reqs.size(), -1);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    while (finished < reqs.size()) {
+      const double t = now();
+      while (next < reqs.size() && reqs[next].arrival_s <= t) queue.push_back(next++);
+      // admission in arrival order, bounded by slots and the HBM budget
+      while (!queue.empty()) {
+        size_t i = queue.front();
+        const ServeRequest& r = reqs[i];
+        Class& c = classes.at(r.cls);
+        const double need = (double)ArenaAllocator::round(r.n * c.model->nobs) * sizeof(kernelT);
+        if (need > (double)budget_bytes) {  // can never fit: reject
+          queue.pop_front();
+          stats.rejected++;
+          res[i].fit.status = -2;
+          res[i].done_s = now();
+          finished++;
+          continue;
+        }
+        if (used + need > (double)budget_bytes || !c.pool->fits(r.n)) break;
+        int slot = c.pool->add_segment(r.n);
+        if (slot < 0) break;
+        queue.pop_front();
+        used += need;
+        stats.peak_bytes = std::max(stats.peak_bytes, used);
+        slot_of[i] = slot;
+        auto tg = std::chrono::steady_clock::now();
+        c.pool->generate(c.theta_gen, {r.seed}, 64, {slot});
+        stats.generate_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
+        res[i].admit_s = now();
+        sch.add(c.pool.get(), slot, c.start.data(), (long long)i);
+      }
+      if (sch.live() == 0) {
+        if (next < reqs.size() && queue.empty()) {
+          double wait = reqs[next].arrival_s - now();
+          if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+        }
+        continue;
+      }
+      stats.peak_live = std::max(stats.peak_live, (int)sch.live());
+      sch.step([&](auto& task) {
+        size_t i = (size_t)task.id;
+        res[i].fit = task.result;
+        res[i].done_s = now();
+        Class& c = classes.at(reqs[i].cls);
+        c.pool->remove_segment(slot_of[i]);
+        used -= (double)ArenaAllocator::round(reqs[i].n * c.model->nobs) * sizeof(kernelT);
+        finished++;
+      });
+      if (sch.live() > 0) sch.launch();
+    }
+    stats.ticks = sch.stats.ticks;
+    stats.launches = sch.stats.launches[1] + sch.stats.launches[2] + sch.stats.launches[3];
+    stats.wall_s = now();
+    return res;
+  }
+
+ private:
+  int slots_;
+};
+
+}  // namespace morefit
+#endif
