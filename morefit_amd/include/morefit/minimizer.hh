@@ -9,7 +9,8 @@
  *  - seed: analytic Hessian (strategy 2 with an analytic Hessian, the setting the reference
  *    fitter uses, fitter.hh:762-767, 806-813) transformed to the internal space and
  *    inverted, forced positive-definite by a diagonal shift when needed;
- *  - iteration: Newton step -V g, a parabolic line search, the BFGS update of V;
+ *  - iteration: Newton step -V g taken whole when value and directional derivative agree
+ *    (one value+gradient call), else a parabolic line search on the value; BFGS update of V;
  *  - convergence: EDM = g^T V g / 2 < 0.002 * tolerance * up (tolerance 0.01, up 1 as in the
  *    reference's Minuit2 defaults), confirmed with a fresh analytic Hessian (strategy 2);
  *  - HESSE: covariance = 2 up H^-1 in the internal space, mapped back through the Jacobian.
@@ -271,37 +272,52 @@ class Migrad {
         if (!seed_metric(u, g)) return finish(3, f, e, u, n);
         continue;
       }
-      // parabolic line search on alpha in f(u + alpha d)
-      double alpha = 1.0, fa = 0.0, best_a = 0.0, best_f = f;
-      for (int ls = 0; ls < 12; ls++) {
-        for (int i = 0; i < n; i++) u1[i] = u[i] + alpha * d[i];
-        fa = eval_f(u1);
-        if (std::isfinite(fa) && fa < best_f) { best_f = fa; best_a = alpha; }
-        double c2 = std::isfinite(fa) ? (fa - f - slope * alpha) / (alpha * alpha) : INFINITY;
-        double next;
-        if (std::isfinite(c2) && c2 > 0.0) next = -slope / (2.0 * c2);
-        else next = best_a > 0.0 ? 2.0 * alpha : 0.25 * alpha;
-        next = std::max(0.1 * alpha, std::min(next, 4.0 * std::max(alpha, best_a)));
-        if (best_a > 0.0 && std::fabs(next - best_a) < 0.15 * best_a) break;  // parabola agrees: done
-        if (best_a > 0.0 && ls >= 2) break;
-        if (std::fabs(next - alpha) < 1e-12 * alpha) break;
-        alpha = next;
-      }
-      if (best_a == 0.0) {
-        // no decrease along the Newton direction: refresh the metric once, else give up
-        if (!refreshed_) {
-          refreshed_ = true;
-          if (seed_metric(u, g)) { e = compute_edm(g); continue; }
+      // Full Newton step first, with value and gradient in one call.  It is accepted on the
+      // usual sufficient decrease, or -- near the minimum, where the change of the value is
+      // comparable to its rounding noise (FP32 sums) -- when the directional derivative shows
+      // the step went most of the way to the line minimum and the value did not rise by more
+      // than a quarter of the predicted decrease.
+      for (int i = 0; i < n; i++) u1[i] = u[i] + d[i];
+      double f1 = eval_fg(u1, g1);
+      double slope1 = 0.0;
+      for (int i = 0; i < n; i++) slope1 += g1[i] * d[i];
+      bool accept = std::isfinite(f1) && (f1 <= f + 1e-4 * slope ||
+                                          (std::fabs(slope1) <= 0.5 * std::fabs(slope) && f1 - f <= 0.25 * std::fabs(slope)));
+      double best_a = accept ? 1.0 : 0.0;
+      if (!accept) {
+        // parabolic search on the value along d
+        double alpha = 1.0, best_f = f;
+        double fa = f1;
+        for (int ls = 0; ls < 12; ls++) {
+          if (ls > 0) {
+            for (int i = 0; i < n; i++) u1[i] = u[i] + alpha * d[i];
+            fa = eval_f(u1);
+          }
+          if (std::isfinite(fa) && fa < best_f) { best_f = fa; best_a = alpha; }
+          double c2 = std::isfinite(fa) ? (fa - f - slope * alpha) / (alpha * alpha) : INFINITY;
+          double next;
+          if (std::isfinite(c2) && c2 > 0.0) next = -slope / (2.0 * c2);
+          else next = best_a > 0.0 ? 2.0 * alpha : 0.25 * alpha;
+          next = std::max(0.1 * alpha, std::min(next, 4.0 * std::max(alpha, best_a)));
+          if (best_a > 0.0 && std::fabs(next - best_a) < 0.15 * best_a) break;  // parabola agrees: done
+          if (best_a > 0.0 && ls >= 2) break;
+          if (std::fabs(next - alpha) < 1e-12 * alpha) break;
+          alpha = next;
         }
-        return finish(e < 100.0 * edm_goal ? 0 : 2, f, e, u, n);
+        if (best_a == 0.0) {
+          // no decrease along the Newton direction: refresh the metric once, else give up
+          if (!refreshed_) {
+            refreshed_ = true;
+            if (seed_metric(u, g)) { e = compute_edm(g); continue; }
+          }
+          return finish(e < edm_goal ? 0 : 2, f, e, u, n);
+        }
+        for (int i = 0; i < n; i++) u1[i] = u[i] + best_a * d[i];
+        f1 = eval_fg(u1, g1);
+        if (!std::isfinite(f1)) return finish(4, f, e, u, n);
       }
       refreshed_ = false;
-      for (int i = 0; i < n; i++) {
-        u1[i] = u[i] + best_a * d[i];
-        s[i] = best_a * d[i];
-      }
-      double f1 = eval_fg(u1, g1);
-      if (!std::isfinite(f1)) return finish(4, f, e, u, n);
+      for (int i = 0; i < n; i++) s[i] = best_a * d[i];
       // BFGS update of the inverse metric
       double sy = 0.0;
       for (int i = 0; i < n; i++) { y[i] = g1[i] - g[i]; sy += s[i] * y[i]; }
