@@ -149,7 +149,7 @@ def main():
         t0 = time.perf_counter()
         bt.generate(M.MASS_TRUTH, list(range(1, args.toys + 1)))
         tgen = time.perf_counter() - t0
-        starts = np.tile(M.MASS_START, (args.toys, 1))
+        starts = M.toy_starts(M.MASS, sizes, seed=229387429)
         t0 = time.perf_counter()
         tv, te, to, tst = bt.fit(starts)
         tfit = time.perf_counter() - t0
@@ -161,7 +161,7 @@ def main():
         for i in range(args.plain_toys):
             d = bt.download(i)
             t1 = time.perf_counter()
-            ctx.fit_plain(M.MASS, d, M.MASS_START, fp32=False)
+            ctx.fit_plain(M.MASS, d, starts[i], fp32=False)
             tplain += time.perf_counter() - t1
         plain_rate = args.plain_toys / tplain
         toys = {"toys": args.toys, "mean_events": float(sizes.mean()), "generate_s": tgen, "fit_s": tfit,

@@ -24,6 +24,17 @@ TRUTH = {MASS: MASS_TRUTH, KSTAR: KSTAR_TRUTH}
 MASS_START = np.array([5.28 + 0.06, 0.022, 0.33, -1.65])
 KSTAR_START = np.array([0.77, 0.0, -0.11, -0.22, 0.11, 0.0, 0.0, 0.0])
 START = {MASS: MASS_START, KSTAR: KSTAR_START}
+# statistical errors of one fit at the truth, per parameter, scaled as 1/sqrt(N) from N_REF events
+SIGMA_REF = {MASS: (np.array([1.57e-4, 1.41e-4, 1.93e-3, 2.24e-2]), 100000),
+             KSTAR: (np.array([2.9e-3, 3.8e-3, 4.7e-3, 4.7e-3, 2.6e-3, 4.8e-3, 4.8e-3, 3.8e-3]), 50000)}
+
+
+def toy_starts(kind, sizes, seed):
+    """independent start values per toy: truth + N(0, 1) * sigma_stat(N), fixed by the seed"""
+    sizes = np.asarray(sizes, dtype=float)
+    sig, nref = SIGMA_REF[kind]
+    rs = np.random.default_rng(seed)
+    return TRUTH[kind] + rs.standard_normal((len(sizes), len(sig))) * sig * np.sqrt(nref / np.maximum(sizes, 1.0))[:, None]
 
 
 class MorefitError(RuntimeError):

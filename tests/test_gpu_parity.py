@@ -187,12 +187,15 @@ def test_batched_fits_match_single_fits(ctx, fp32):
     b = mdl.batch(sizes)
     for i, d in enumerate(datas):
         b.upload(i, d)
-    vals, errs, outs, stats = b.fit(M.MASS_START)
+    starts = M.toy_starts(M.MASS, sizes, seed=4)
+    vals, errs, outs, stats = b.fit(starts)
     lim = 1e-3 if fp32 else 1e-4
     for i, d in enumerate(datas):
         assert outs[i].status == 0, i
-        vc, ec, oc, _ = M.fit_cpu(M.MASS, d, M.MASS_START, nthreads=1)
+        vc, ec, oc, _ = M.fit_cpu(M.MASS, d, starts[i], nthreads=1)
+        assert oc.status == 0 and np.all(ec > 0), i
         assert np.max(np.abs(vals[i] - vc) / ec) < lim, (i, (vals[i] - vc) / ec)
+        np.testing.assert_allclose(errs[i], ec, rtol=1e-3 if fp32 else 1e-6)
     assert stats.ticks > 0
     b.close()
     mdl.close()

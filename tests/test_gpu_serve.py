@@ -28,7 +28,7 @@ def test_service_alone_equals_mixed(ctx):
     kst = ctx.model(M.KSTAR, fp32=True)
     models = [mass, kst]
     tg = [M.MASS_TRUTH, M.KSTAR_TRUTH]
-    st = [M.MASS_START, M.KSTAR_START]
+    st = [M.MASS_TRUTH, M.KSTAR_TRUTH]  # start at the truth: requests as small as 1e3 events
     say("service mixed", nreq, "requests")
     mixed = M.serve(models, tg, st, cls, n, seeds, arrival, budget_bytes=256 << 20, slots=64)
     assert np.all(mixed["status"] == 0), mixed["status"]
@@ -51,7 +51,7 @@ def test_service_budget_limits_concurrency(ctx):
     n = [100000] * 6 + [10_000_000]
     budget = 2.2 * 100000 * 4
     say("service budget")
-    r = M.serve([mass], [M.MASS_TRUTH], [M.MASS_START], [0] * len(n), n, list(range(1, len(n) + 1)), [0.0] * len(n),
+    r = M.serve([mass], [M.MASS_TRUTH], [M.MASS_TRUTH], [0] * len(n), n, list(range(1, len(n) + 1)), [0.0] * len(n),
                 budget_bytes=budget, slots=16)
     assert r["peak_bytes"] <= budget
     assert r["rejected"] == 1 and r["status"][-1] == -2
