@@ -13,6 +13,9 @@
  *    (one value+gradient call), else a parabolic line search on the value; BFGS update of V;
  *  - convergence: EDM = g^T V g / 2 < 0.002 * tolerance * up (tolerance 0.01, up 1 as in the
  *    reference's Minuit2 defaults), confirmed with a fresh analytic Hessian (strategy 2);
+ *  - polish (on by default, not in MINUIT): from the converged point, Newton steps with that
+ *    exact Hessian down to EDM 1e-10, so the result is the minimum itself and not any point
+ *    inside MIGRAD's EDM tolerance (which allows ~6e-3 sigma);
  *  - HESSE: covariance = 2 up H^-1 in the internal space, mapped back through the Jacobian.
  *
  * The function interface is synchronous (value, value+gradient, Hessian).  The batched toy
@@ -118,6 +121,10 @@ class Migrad {
   double tolerance = 0.01;
   double up = 1.0;
   int print_level = 0;
+  // after convergence: Newton steps with a fresh exact Hessian down to this EDM
+  bool polish = true;
+  double polish_edm = 1e-10;
+  int polish_steps = 2;
 
   // results
   std::vector<double> x;      // external values (all parameters)
@@ -178,6 +185,7 @@ class Migrad {
     status = -1;
     valid = false;
     std::vector<double> u(n), g(n), V(n * n), xe(x);
+    std::vector<double> d(n), u1(n), g1(n), s(n), y(n), Vy(n);
     for (int k = 0; k < n; k++) u[k] = ext2int(k, p_[free_[k]].value);
     auto to_ext = [&](const std::vector<double>& uu) {
       for (int k = 0; k < n; k++) xe[free_[k]] = int2ext(k, uu[k]);
@@ -243,19 +251,45 @@ class Migrad {
       }
     }
     double e = compute_edm(g);
-    std::vector<double> d(n), u1(n), g1(n), s(n), y(n), Vy(n);
     bool confirmed = false;
+    // Newton steps with the exact metric until EDM < polish_edm (at most polish_steps); a
+    // step is kept only when it lowers EDM.  Takes a converged point to the minimum itself,
+    // so fits in different precisions land within rounding of each other.
+    auto polish_at_minimum = [&]() {
+      for (int k = 0; k < polish_steps && e > polish_edm; k++) {
+        for (int i = 0; i < n; i++) {
+          double t = 0.0;
+          for (int j = 0; j < n; j++) t -= V[i * n + j] * g[j];
+          u1[i] = u[i] + t;
+        }
+        double f1 = eval_fg(u1, g1);
+        if (!std::isfinite(f1)) break;
+        double e1 = compute_edm(g1);
+        if (!(e1 < e)) break;
+        u = u1;
+        g = g1;
+        f = f1;
+        e = e1;
+      }
+    };
     for (iterations = 0; iterations < max_iterations; iterations++) {
       if (e < edm_goal) {
         if (strategy >= 2 && !confirmed) {  // confirm with a fresh Hessian
           confirmed = true;
           if (seed_metric(u, g)) {
             e = compute_edm(g);
-            if (e < edm_goal) break;
+            if (e < edm_goal) {
+              if (polish) polish_at_minimum();
+              break;
+            }
           } else {
             break;
           }
         } else {
+          if (polish && seed_metric(u, g)) {
+            e = compute_edm(g);
+            polish_at_minimum();
+          }
           break;
         }
       }
