@@ -145,6 +145,10 @@ class FitScheduler {
       post(x, EVAL_LH);
       return -2.0 * t->b->result(t->seg, EVAL_LH)[0];
     }
+    // FP32: ~1e-7 absolute per event in log p, adding up as a random walk
+    double value_noise() const override {
+      return std::is_same<kernelT, float>::value ? 1e-6 * std::sqrt((double)std::max<long long>(1, t->b->sizes[t->seg])) : 0.0;
+    }
     double value_grad(const double* x, double* g) override {
       post(x, EVAL_GRAD);
       const double* r = t->b->result(t->seg, EVAL_GRAD);

@@ -320,6 +320,9 @@ class fitter {
       f->update_parameters(x);
       return f->hessian((int)f->params_.size(), h);
     }
+    double value_noise() const override {
+      return std::is_same<kernelT, float>::value ? 1e-6 * std::sqrt((double)std::max<size_t>(1, f->nevents_)) : 0.0;
+    }
   };
 
   std::vector<MinimizerParam> minimizer_params(bool reset) {

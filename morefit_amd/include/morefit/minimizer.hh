@@ -38,6 +38,9 @@ struct MinimizerFcn {
   virtual double value(const double* x) = 0;
   virtual double value_grad(const double* x, double* grad) = 0;  // returns the value as well
   virtual bool hessian(const double* x, double* h) = 0;             // n x n, row-major, all params
+  // rounding noise of value() (e.g. FP32 per-event sums); value differences below it carry
+  // no information and the search then relies on the gradient
+  virtual double value_noise() const { return 0.0; }
 };
 
 struct MinimizerParam {
@@ -315,8 +318,9 @@ class Migrad {
       double f1 = eval_fg(u1, g1);
       double slope1 = 0.0;
       for (int i = 0; i < n; i++) slope1 += g1[i] * d[i];
-      bool accept = std::isfinite(f1) && (f1 <= f + 1e-4 * slope ||
-                                          (std::fabs(slope1) <= 0.5 * std::fabs(slope) && f1 - f <= 0.25 * std::fabs(slope)));
+      const double noise = fcn.value_noise();
+      bool accept = std::isfinite(f1) && (f1 <= f + 1e-4 * slope + noise ||
+                                          (std::fabs(slope1) <= 0.5 * std::fabs(slope) && f1 - f <= 0.25 * std::fabs(slope) + noise));
       double best_a = accept ? 1.0 : 0.0;
       if (!accept) {
         // parabolic search on the value along d
@@ -339,7 +343,13 @@ class Migrad {
           alpha = next;
         }
         if (best_a == 0.0) {
-          // no decrease along the Newton direction: refresh the metric once, else give up
+          // no decrease along the Newton direction.  Inside the value's noise floor the
+          // gradient decides: Newton steps that lower EDM; otherwise refresh the metric once.
+          if (-slope <= 4.0 * noise && seed_metric(u, g)) {
+            e = compute_edm(g);
+            polish_at_minimum();
+            return finish(e < edm_goal ? 0 : 2, f, e, u, n);
+          }
           if (!refreshed_) {
             refreshed_ = true;
             if (seed_metric(u, g)) { e = compute_edm(g); continue; }
