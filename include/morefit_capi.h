@@ -39,6 +39,7 @@ typedef struct mfh_options {
   int fast_math;           /* FP32 hardware exp/log/rcp */
   int accumulate;          /* FP32: 1 compensated FP32 per lane, 0 FP64 per event */
   int centre;              /* FP32: observables stored as offsets from the range centre */
+  int unroll;              /* events per thread whose loads are issued together */
 } mfh_options;
 
 typedef struct mfh_fit_out {

@@ -55,7 +55,9 @@ extern "C" {
  * parameter-buffer order, fitter.hh:1063-1072) start at params[param_off].  Its events
  * are split into chunks [begin, end) listed contiguously at chunks[chunk_begin ..
  * chunk_end); each chunk's sums go to partials[chunk * nout ...] and the segment's totals
- * (FP64) to results[out_off ...].  mode 0 skips the segment in a launch. */
+ * (FP64) to results[out_off ...].  mode 0 skips the segment in a launch.  mf_launch_eval runs
+ * the per-chunk kernel <kernel> and then, when the module has it, <kernel>_final (one
+ * wavefront per segment, fixed-order reduction of the segment's chunk partials). */
 typedef struct mf_segment {
   long long data_off;
   long long stride;
@@ -84,7 +86,7 @@ typedef struct mf_eval_args {
   const void* params;         /* T* kernel parameters */
   double* partials;           /* [nchunks][nout] */
   double* results;            /* [..][nout] per segment at out_off */
-  unsigned int* counters;     /* [nseg] arrival counters, left at zero after every launch */
+  unsigned int* reserved;     /* unused, NULL */
   void* store;                /* T* per-event outputs (store kernels only) or NULL */
   long long store_stride;     /* column stride of `store` */
   int flags;

@@ -43,7 +43,8 @@ class MorefitError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("block_size", ctypes.c_int), ("events_per_thread", ctypes.c_int),
-                ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int)]
+                ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int),
+                ("unroll", ctypes.c_int)]
 
 
 class FitOut(ctypes.Structure):

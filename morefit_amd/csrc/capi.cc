@@ -90,6 +90,7 @@ void apply_options(compute_options& co, const mfh_options* o) {
   co.hip_fast_math = o->fast_math != 0;
   co.hip_accumulate = o->accumulate;
   co.hip_centre_observables = o->centre != 0;
+  co.hip_unroll = o->unroll > 0 ? o->unroll : 4;
 }
 }  // namespace
 
@@ -134,6 +135,7 @@ void mfh_default_options(mfh_options* o) {
   o->fast_math = 0;
   o->accumulate = 1;
   o->centre = 1;
+  o->unroll = 4;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {

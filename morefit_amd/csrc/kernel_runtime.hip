@@ -10,6 +10,7 @@
 //   MF_BLOCK        work-group size of the evaluation kernels (multiple of 64)
 //   MF_ACC          0: FP64 accumulation per event, 1: compensated (Kahan) FP32 per lane
 //   MF_FAST_MATH    1: hardware exp2/log2/rcp approximations in FP32
+//   MF_UNROLL       events per thread whose loads are issued together (default 4)
 //   MF_GEN_LO<d>, MF_GEN_W<d>  generator proposal box per observable (stored coordinates)
 // and, after it, functor structs { static constexpr int nout; static void eval(x, p, o); },
 // mf_hoist(theta, buf) and the kernel instantiations (MF_EVAL_KERNEL etc. below).
@@ -17,13 +18,16 @@
 // Semantics it implements (reference compute_opencl.hh:646-995): per event the log density
 // and its derivatives; per output the compensated sum over the events of each segment.
 // Differences by design: the sum is fused into the per-event kernel (no N x outputs
-// round trip through memory, no ceil(log32 N) extra passes), partial sums are indexed by
-// (segment, chunk) and reduced in a fixed order by the last-arriving block of each segment,
-// so a result is bitwise the same whatever else shares the launch.
+// round trip through memory, no ceil(log32 N) extra passes); partial sums are indexed by
+// (segment, chunk) and a second small kernel reduces each segment in a fixed order, so a
+// result is bitwise the same whatever else shares the launch, with no fences or atomics.
 
 #define MF_NPARAM (MF_NTHETA + MF_NHOIST)
 #define MF_GEN_MAX_TRIES (1u << 20)
 #define MF_NWAVE (MF_BLOCK / 64)
+#ifndef MF_UNROLL
+#define MF_UNROLL 4
+#endif
 
 // ------------------------------------------------------------------ math
 #if MF_FP32
@@ -34,7 +38,7 @@ __device__ __forceinline__ float mf_rcp(float x) { return __builtin_amdgcn_rcpf(
 #else
 __device__ __forceinline__ float mf_exp(float x) { return expf(x); }
 __device__ __forceinline__ float mf_log(float x) { return logf(x); }
-__device__ __forceinline__ float mf_rcp(float x) { return 1.0f / x; }
+__device__ __forceinline__ float mf_rcp(float x) { return __builtin_amdgcn_rcpf(x); }  // 1 ulp
 #endif
 __device__ __forceinline__ float mf_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ float mf_sin(float x) { return sinf(x); }
@@ -130,50 +134,16 @@ __device__ __forceinline__ void mf_two_sum(double a, double b, double& s, double
   e = (a - (s - bb)) + (b - bb);
 }
 
-// Reduce the chunk partials of one segment in a fixed order (compensated per lane, then a
-// fixed tree over lanes and waves) and write the segment totals.  Called by all threads
-// of the block that completed the segment's last chunk.
-template <int NO>
-__device__ void mf_segment_final(const mf_eval_args& a, const mf_segment& sg, double (*s_red)[NO]) {
-  const int nch = sg.chunk_end - sg.chunk_begin;
-  double s[NO], e[NO];
-#pragma unroll
-  for (int k = 0; k < NO; k++) { s[k] = 0.0; e[k] = 0.0; }
-  for (int j = threadIdx.x; j < nch; j += MF_BLOCK) {
-    const double* p = a.partials + (long long)(sg.chunk_begin + j) * NO;
-#pragma unroll
-    for (int k = 0; k < NO; k++) {
-      double t, err;
-      mf_two_sum(s[k], __builtin_nontemporal_load(p + k), t, err);
-      s[k] = t;
-      e[k] += err;
-    }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NO; k++) {
-    double v = mf_wave_sum(s[k] + e[k]);
-    if (lane == 0) s_red[wave][k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < NO) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
-    a.results[sg.out_off + threadIdx.x] = t;
-  }
-}
-
 // ------------------------------------------------------------------ fused evaluation
 // Grid-stride over the chunk table (a persistent grid when gridDim < nchunks).  Per chunk:
 // coalesced SoA loads, per-event outputs from the generated functor F, per-lane
-// accumulation, wave64 butterfly + LDS reduction to one FP64 partial per output, then the
-// segment's arrival counter; the last chunk of a segment reduces that segment.
+// accumulation, wave64 butterfly + LDS reduction to one FP64 partial per output, written to
+// partials[chunk][k].  No fences or atomics: the segment totals come from the separate
+// final kernel below, launched right after on the same stream.
 template <class F>
 __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
   constexpr int NO = F::nout;
   __shared__ double s_red[MF_NWAVE][NO];
-  __shared__ int s_last;
   const MF_T* __restrict__ data = (const MF_T*)a.data;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
@@ -184,7 +154,22 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
     const MF_T* __restrict__ col = data + sg.data_off;
     mf_acc<NO> acc;
     acc.zero();
-    for (int i = ch.begin + (int)threadIdx.x; i < ch.end; i += MF_BLOCK) {
+    int i = ch.begin + (int)threadIdx.x;
+    // MF_UNROLL events per thread per step: their loads are issued together, then computed
+    for (; i + (MF_UNROLL - 1) * MF_BLOCK < ch.end; i += MF_UNROLL * MF_BLOCK) {
+      MF_T x[MF_UNROLL][MF_NOBS];
+#pragma unroll
+      for (int u = 0; u < MF_UNROLL; u++)
+#pragma unroll
+        for (int d = 0; d < MF_NOBS; d++) x[u][d] = __builtin_nontemporal_load(col + d * sg.stride + i + u * MF_BLOCK);
+#pragma unroll
+      for (int u = 0; u < MF_UNROLL; u++) {
+        MF_T o[NO];
+        F::eval(x[u], P, o);
+        acc.add(o);
+      }
+    }
+    for (; i < ch.end; i += MF_BLOCK) {
       MF_T x[MF_NOBS];
 #pragma unroll
       for (int d = 0; d < MF_NOBS; d++) x[d] = __builtin_nontemporal_load(col + d * sg.stride + i);
@@ -206,19 +191,31 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
       a.partials[(long long)c * NO + threadIdx.x] = t;
     }
-    __threadfence();  // release: this chunk's partials before the arrival count
     __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned int prev = __hip_atomic_fetch_add(a.counters + ch.seg, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      s_last = (prev + 1u == (unsigned int)(sg.chunk_end - sg.chunk_begin));
+  }
+}
+
+// Segment totals: one wavefront per segment reduces its chunk partials in chunk order
+// (compensated two-sum per lane over a fixed lane-strided assignment, then the fixed
+// butterfly), so a segment's totals are bitwise the same whatever else shares the launch.
+template <int NO>
+__device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
+  const int s = blockIdx.x;
+  if (s >= a.nseg) return;
+  const mf_segment sg = a.segs[s];
+  if (sg.mode == 0) return;
+  const int lane = threadIdx.x;
+  const int nch = sg.chunk_end - sg.chunk_begin;
+  for (int k = 0; k < NO; k++) {
+    double acc = 0.0, err = 0.0;
+    for (int j = lane; j < nch; j += 64) {
+      double t, e;
+      mf_two_sum(acc, a.partials[(long long)(sg.chunk_begin + j) * NO + k], t, e);
+      acc = t;
+      err += e;
     }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();  // acquire: every chunk's partials of this segment
-      mf_segment_final<NO>(a, sg, s_red);
-      if (threadIdx.x == 0) __hip_atomic_store(a.counters + ch.seg, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
+    const double tot = mf_wave_sum(acc + err);
+    if (lane == 0) a.results[sg.out_off + k] = tot;
   }
 }
 
@@ -247,8 +244,9 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
   }
 }
 
-#define MF_EVAL_KERNEL(NAME, FUNCTOR) \
-  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR>(a); }
+#define MF_EVAL_KERNEL(NAME, FUNCTOR)                                                                      \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR>(a); } \
+  extern "C" __global__ void __launch_bounds__(64) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR::nout>(a); }
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }
 

@@ -417,9 +417,25 @@ int mf_module_unload(mf_module* mod) {
 
 double mf_module_compile_ms(mf_module* mod) { return mod ? mod->compile_ms : 0.0; }
 
-static int get_fn(mf_module* mod, const char* kernel, hipFunction_t* fn) {
+static bool mod_has(mf_module* mod, const char* kernel, hipFunction_t* fn) {
   auto it = mod->fns.find(kernel);
   if (it != mod->fns.end()) {
+    *fn = it->second;
+    return it->second != nullptr;
+  }
+  hipError_t e = hipModuleGetFunction(fn, mod->mod, kernel);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    mod->fns[kernel] = nullptr;  // remember the absence
+    return false;
+  }
+  mod->fns[kernel] = *fn;
+  return true;
+}
+
+static int get_fn(mf_module* mod, const char* kernel, hipFunction_t* fn) {
+  auto it = mod->fns.find(kernel);
+  if (it != mod->fns.end() && it->second) {
     *fn = it->second;
     return MF_OK;
   }
@@ -449,6 +465,10 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
   mf_eval_args a = *args;
   void* kp[] = {(void*)&a};
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval");
+  std::string fin = std::string(kernel) + "_final";
+  hipFunction_t ff;
+  if (args->nseg > 0 && mod_has(mod, fin.c_str(), &ff))
+    MF_HIP(hipModuleLaunchKernel(ff, args->nseg, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval final");
   return MF_OK;
 }
 

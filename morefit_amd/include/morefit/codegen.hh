@@ -28,6 +28,7 @@ struct ModuleSource {
   int block = 256;
   int acc = 1;
   bool fast_math = false;
+  int unroll = 4;
   std::vector<double> gen_lo, gen_width;  // proposal box per observable, in stored coordinates
   std::string functors;                    // functor structs
   std::string hoist;                       // mf_hoist definition
@@ -43,7 +44,8 @@ struct ModuleSource {
       << "#define MF_NHOIST " << nhoist << "\n"
       << "#define MF_BLOCK " << block << "\n"
       << "#define MF_ACC " << acc << "\n"
-      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n";
+      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n"
+      << "#define MF_UNROLL " << unroll << "\n";
     s << "#define MF_GEN_PROPOSE(x, s) do {";
     for (int d = 0; d < nobs; d++) {
       double lo = d < (int)gen_lo.size() ? gen_lo[d] : 0.0, w = d < (int)gen_width.size() ? gen_width[d] : 1.0;

@@ -47,6 +47,7 @@ struct compute_options {
   int hip_block_size = 256;          // work-group size of the evaluation kernels
   int hip_events_per_thread = 16;    // chunk = block * events_per_thread events
   int hip_grid = 0;                  // 0: one work-group per chunk, >0: persistent grid of that many
+  int hip_unroll = 4;                // events per thread whose loads are issued together
   bool hip_fast_math = false;        // FP32: hardware exp/log/rcp approximations
   int hip_accumulate = 1;            // FP32: 1 = compensated FP32 per lane, 0 = FP64 per event
   bool hip_centre_observables = true;// FP32: store observables as offsets from the range centre

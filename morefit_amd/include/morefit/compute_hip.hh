@@ -128,7 +128,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
   struct State {
     HIPBackend* be = nullptr;
     std::shared_ptr<DevBuf> input, output;
-    DevBuf params, seeds_states, streams, maxprob, segs, chunks, partials, results, counters;
+    DevBuf params, seeds_states, streams, maxprob, segs, chunks, partials, results;
     HostBuf results_h;
     mf_module* mod = nullptr;
     std::string kernel;
@@ -166,8 +166,6 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     s.partials.alloc(dev(), ch.size() * std::max(1, s.nout) * sizeof(double));
     s.results.alloc(dev(), std::max(1, s.nout) * sizeof(double));
     s.results_h.alloc(dev(), std::max(1, s.nout) * sizeof(double));
-    s.counters.alloc(dev(), sizeof(unsigned int));
-    hip_check(mf_fill_zero(dev(), s.counters.ptr, sizeof(unsigned int)), "HIPBlock counters");
     hip_check(mf_sync(dev()), "HIPBlock tables");
     nchunks_ = (int)ch.size();
     s.tables_ready = true;
@@ -294,6 +292,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     ms.block = opts().hip_block_size;
     ms.acc = opts().hip_accumulate;
     ms.fast_math = opts().hip_fast_math;
+    ms.unroll = opts().hip_unroll;
     std::vector<std::string> obs;
     for (auto& d : input_signature) obs.push_back(d.get_name());
     std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
@@ -379,7 +378,6 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     a.params = s.params.ptr;
     a.partials = s.partials.template as<double>();
     a.results = s.results.template as<double>();
-    a.counters = s.counters.template as<unsigned int>();
     if (!s.kahan) {
       if (!s.output) throw std::runtime_error("HIPBlock::SubmitKernel: store kernel needs an output buffer");
       a.store = s.output->ptr;
@@ -467,6 +465,7 @@ struct LikelihoodGraphs {
     ms.block = o.hip_block_size;
     ms.acc = o.hip_accumulate;
     ms.fast_math = o.hip_fast_math;
+    ms.unroll = o.hip_unroll;
     for (size_t d = 0; d < obs.size(); d++) {
       ms.gen_lo.push_back(lo[d] - centres[d]);
       ms.gen_width.push_back(hi[d] - lo[d]);
@@ -630,7 +629,7 @@ class HipBatch {
   int chunk = 0, nchunks = 0, chunk_cap = 0;
   bool dirty = true;
   std::vector<mf_segment> segs_h;
-  DevBuf data, theta_d, params_d, segs_d[4], chunks_d, partials_d, results_d[4], counters_d;
+  DevBuf data, theta_d, params_d, segs_d[4], chunks_d, partials_d, results_d[4];
   HostBuf theta_h, segs_hk[4], results_h[4];
   mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> request_;
@@ -670,8 +669,6 @@ class HipBatch {
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
     int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
     partials_d.alloc(dev, (size_t)chunk_cap * nmax * sizeof(double));
-    counters_d.alloc(dev, std::max(1, S) * sizeof(unsigned int));
-    hip_check(mf_fill_zero(dev, counters_d.ptr, std::max(1, S) * sizeof(unsigned int)), "HipBatch counters");
     for (int k = 1; k <= 3; k++) {
       if (m->nout[k] == 0) continue;
       segs_d[k].alloc(dev, std::max(1, S) * sizeof(mf_segment));
@@ -864,7 +861,6 @@ class HipBatch {
     a.params = params_d.ptr;
     a.partials = partials_d.template as<double>();
     a.results = results_d[kind].template as<double>();
-    a.counters = counters_d.template as<unsigned int>();
     return a;
   }
   // just the evaluation kernel of `kind` (segment table and parameters already on device)
