@@ -170,7 +170,7 @@ def main():
                 "pull_mean": pulls.mean(axis=0).round(4).tolist(), "pull_width": pulls.std(axis=0).round(4).tolist(),
                 "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
                 "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms,
-                "launch_shape": {"block": 256, "events_per_thread": 16}}
+                "launch_shape": {"block": 256, "events_per_thread": 64, "unroll": 8}}
         log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain ({batched_rate / plain_rate:.1f}x)")
         bt.close()
         mt.close()
@@ -207,7 +207,7 @@ def main():
         "data": "synthetic: seeded accept-reject sample of the density, generated on the device",
         "config": {"workload": "mass fit f*Gauss+(1-f)*Exp on [5.0,5.6], N=1e7 events, one -2lnL+gradient evaluation per step",
                    "events": n, "precision": "FP32 per-event arithmetic, compensated per-lane sums, FP64 totals",
-                   "block": 256, "events_per_thread": 16},
+                   "block": 256, "events_per_thread": 64, "unroll": 8, "fast_math": True},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "mf_eval_grad (fp32)", "kernel_ms": k32_ms, "algorithmic_bytes": bytes32},
         "cpu_baseline": cpu,

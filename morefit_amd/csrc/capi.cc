@@ -85,12 +85,12 @@ void apply_options(compute_options& co, const mfh_options* o) {
   if (!o) o = &d;
   co.hip_device = o->device;
   co.hip_block_size = o->block_size > 0 ? o->block_size : 256;
-  co.hip_events_per_thread = o->events_per_thread > 0 ? o->events_per_thread : 16;
+  co.hip_events_per_thread = o->events_per_thread > 0 ? o->events_per_thread : 64;
   co.hip_grid = o->grid;
   co.hip_fast_math = o->fast_math != 0;
   co.hip_accumulate = o->accumulate;
   co.hip_centre_observables = o->centre != 0;
-  co.hip_unroll = o->unroll > 0 ? o->unroll : 4;
+  co.hip_unroll = o->unroll > 0 ? o->unroll : 8;
 }
 }  // namespace
 
@@ -130,12 +130,12 @@ const char* mfh_last_error(void) { return g_err.c_str(); }
 void mfh_default_options(mfh_options* o) {
   o->device = 0;
   o->block_size = 256;
-  o->events_per_thread = 16;
+  o->events_per_thread = 64;
   o->grid = 0;
-  o->fast_math = 0;
+  o->fast_math = 1;
   o->accumulate = 1;
   o->centre = 1;
-  o->unroll = 4;
+  o->unroll = 8;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {

@@ -45,10 +45,10 @@ struct compute_options {
   // MI355X backend
   int hip_device = 0;
   int hip_block_size = 256;          // work-group size of the evaluation kernels
-  int hip_events_per_thread = 16;    // chunk = block * events_per_thread events
+  int hip_events_per_thread = 64;    // chunk = block * events_per_thread events
   int hip_grid = 0;                  // 0: one work-group per chunk, >0: persistent grid of that many
-  int hip_unroll = 4;                // events per thread whose loads are issued together
-  bool hip_fast_math = false;        // FP32: hardware exp/log/rcp approximations
+  int hip_unroll = 8;                // events per thread whose loads are issued together
+  bool hip_fast_math = true;         // FP32: hardware exp2/log2/rcp (error measured against the oracle)
   int hip_accumulate = 1;            // FP32: 1 = compensated FP32 per lane, 0 = FP64 per event
   bool hip_centre_observables = true;// FP32: store observables as offsets from the range centre
   int hip_generator_events_per_stream = 64;
