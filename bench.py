@@ -11,6 +11,9 @@ batched toy study (4096 toys, Poisson(5000) events) against the plain per-fit FP
 the CPU baseline (the oracle, single thread) are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-toys] [--no-cpu]
+       python bench.py --profile   (counter workload: the headline FP32 gradient and value
+                                    kernels at N = 10^7 and the copy kernel over 40 MB, 20
+                                    launches each, no JSON; for rocprofv3 --pmc passes)
 Progress goes to stderr; stdout carries exactly one JSON line.
 """
 import argparse
@@ -32,6 +35,19 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def profile_workload():
+    """Headline kernels only, for hardware-counter passes: FP32 mf_eval_grad and mf_eval_lh at
+    N = 10^7 (generated on the device), then the copy kernel over 40 MB (the counters' scale)."""
+    import morefit_amd as M
+    ctx = M.Context(device=0)
+    m32 = ctx.model(M.MASS, fp32=True)
+    b32 = m32.batch([N_EVENTS])
+    b32.generate(M.MASS_TRUTH, [229387429])
+    log("profile: grad", b32.time_kernel(M.EVAL_GRAD, iters=20) * 1e3, "us")
+    log("profile: lh", b32.time_kernel(M.EVAL_LH, iters=20) * 1e3, "us")
+    log("profile: copy 40 MB", ctx.copy_ceiling(40_000_000, iters=20), "GB/s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,11 +59,16 @@ def main():
     ap.add_argument("--toys", type=int, default=4096)
     ap.add_argument("--plain-toys", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--profile", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
-              "(the likelihood path is measured on one MI355X; see DESIGN.md for how it shards).", file=sys.stderr)
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
+              "(the likelihood path is measured on one MI355X's share; multi-GPU sharding is a design note only).",
+              file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        profile_workload()
+        return
 
     import morefit_amd as M
     from oracle import oracle as O
@@ -112,15 +133,26 @@ def main():
         "fp32_rel_err_m2lnL_same_stored_data": float(abs(r32[0, 0] - ref_st[0]) / abs(ref_st[0])),
         "fp32_max_grad_err_scaled": float(np.max(np.abs(r32[0, 1:] - ref_st[1:]) / absref[1:])),
     }
-    # fitted parameters: FP32-compensated vs FP64 on the same data, in units of sigma
+    # fitted parameters: FP32-compensated vs FP64 on the same data, in units of sigma, both
+    # started 1 sigma (N(0,1)-smeared) from the generating values
     t0 = time.perf_counter()
-    v32f, e32f, o32, st32 = b32.fit(start)
-    v64f, e64f, o64, st64 = b64.fit(start)
+    near = M.toy_starts(M.MASS, [n], seed=5)[0]
+    v32f, e32f, o32, st32 = b32.fit(near)
+    v64f, e64f, o64, st64 = b64.fit(near)
     acc["fit_fp32_vs_fp64_max_dtheta_over_sigma"] = float(np.max(np.abs(v32f[0] - v64f[0]) / e64f[0]))
     acc["fit_status"] = [o32[0].status, o64[0].status]
+    acc["fit_fp32"] = v32f[0].tolist()
+    acc["fit_fp64"] = v64f[0].tolist()
+    acc["fit_sigma_fp64"] = e64f[0].tolist()
     acc["fit_fp32_ms"] = st32.wall_ms
     acc["fit_fp64_ms"] = st64.wall_ms
     log(f"fits fp32/fp64 in {time.perf_counter() - t0:.1f} s: dtheta/sigma {acc['fit_fp32_vs_fp64_max_dtheta_over_sigma']:.2e}")
+    # the same from the configuration's start values, thousands of sigma away at this N
+    f32, _, fo32, _ = b32.fit(start)
+    f64, _, fo64, _ = b64.fit(start)
+    acc["far_start"] = {"start": list(map(float, start)), "status": [fo32[0].status, fo64[0].status],
+                        "fp32_vs_fp64_max_dtheta_over_sigma": float(np.max(np.abs(f32[0] - f64[0]) / e64f[0])),
+                        "fp64_vs_near_start_max_dtheta_over_sigma": float(np.max(np.abs(f64[0] - v64f[0]) / e64f[0]))}
 
     # ---------------------------------------------------------------- HBM ceilings
     copy_gbs = ctx.copy_ceiling(1 << 31, iters=5)

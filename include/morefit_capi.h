@@ -69,11 +69,18 @@ int mfh_copy_ceiling(mfh_context* ctx, double bytes, int iters, double* gbps);
 int mfh_model_create(mfh_context* ctx, int kind, int fp32, const double* range, int with_hessian, mfh_model** out);
 int mfh_model_destroy(mfh_model* m);
 /* info[0..7] = ntheta, nhoist, nobs, nout_lh, nout_grad, nout_hess, statements, graph_nodes;
- * ms[0..1] = codegen ms, hipRTC compile ms (0 when the code object came from the cache) */
+ * info[8..10] = unroll of the value, gradient and Hessian kernels (fitted to the register
+ * budget, 0 when not compiled), info[11..13] = their VGPR counts;
+ * ms[0..1] = codegen ms, hipRTC compile + load ms (small when served from the cache) */
 int mfh_model_info(mfh_model* m, int* info, double* ms);
 int mfh_model_source(mfh_model* m, char* buf, int len);
 /* generated source without a device (warms the on-disk code-object cache) */
 int mfh_model_precompile(int kind, int fp32, int with_hessian, const mfh_options* opts, double* compile_ms);
+/* without a device: compile a configuration module and report, for the value, gradient and
+ * Hessian kernels, info[0..2] = unroll, info[3..5] = VGPRs, info[6..8] = AGPRs, info[9..11] =
+ * scratch bytes per lane.  fit = 1 fits each kernel's unroll to the register budget (as a
+ * model does), fit = 0 compiles every kernel at opts->unroll as given. */
+int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* opts, int fit, int* info);
 
 int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out);
 int mfh_batch_destroy(mfh_batch* b);
@@ -108,9 +115,10 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
 /* one fit through the drop-in HIPBlock (reference fitter flow); data SoA nobs x n */
 int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, double* values,
                   double* errors, mfh_fit_out* out, double* ms);
-/* one fit through the host-compiled CPU backend (no device needed) */
-int mfh_fit_cpu(int kind, const double* data, long long n, const double* start, int nthreads, double* values, double* errors,
-                mfh_fit_out* out, double* ms);
+/* one fit through the host-compiled CPU backend (no device needed); fp32 = 1 runs the FP32
+ * kernels (FP32-rounded parameters, centred FP32 observables) as the device's FP32 path does */
+int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, double* values,
+                double* errors, mfh_fit_out* out, double* ms);
 /* -2lnL, gradient (-2 scale) and Hessian through the CPU backend at theta (no device needed) */
 int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess);
 

@@ -20,6 +20,7 @@
  *   mf_memcpy_h2d                   CopyToInputBuffer / CopyToParameterBuffer / CopyToSeedBuffer (compute_opencl.hh:351-400)
  *   mf_memcpy_d2h                   CopyFromOutputBuffer / CopyFromKahanBuffer (compute_opencl.hh:401-424, 977-986)
  *   mf_module_load                  MakeComputeKernel / MakeGenerateKernel build step (compute_opencl.hh:425-804)
+ *   mf_module_resources             no counterpart (register budget of each compiled kernel)
  *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
  *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
@@ -151,9 +152,34 @@ int mf_fill_zero(mf_device* dev, void* dptr, size_t bytes);                    /
 int mf_sync(mf_device* dev);
 
 /* modules: generated source (+ the kernel runtime) compiled by hipRTC for gfx950 and cached
- * in memory and on disk by a hash of source and options */
+ * in memory and on disk by a hash of source and options.
+ *
+ * mf_module_load refuses (MF_ERR_COMPILE) a module in which any kernel allocates AGPRs.
+ * When a kernel's register demand exceeds the 256 architectural VGPRs, the ROCm 7.2 hipRTC
+ * compiler keeps loop-carried values in AGPRs and was found to place the VGPR->AGPR copies
+ * inside a divergent branch (under a narrowed exec mask) while reloading them for every lane
+ * at the loop latch, so lanes outside the branch read unset registers -- in the event loop
+ * these were load addresses.  Hosts choose the unroll of each evaluation kernel with
+ * mf_module_resources so that no kernel needs AGPRs. */
 int mf_module_load(mf_device* dev, const char* generated_src, const char* options, mf_module** out);
 int mf_module_compile_only(const char* generated_src, const char* options, size_t* code_bytes);
+
+/* resources of one compiled kernel, read from the code object's AMDGPU metadata note */
+typedef struct mf_kernel_resources {
+  char name[64];
+  int vgpr_count;             /* architectural VGPRs per lane */
+  int agpr_count;             /* accumulation VGPRs per lane */
+  int sgpr_count;
+  int vgpr_spill_count;
+  int sgpr_spill_count;
+  int private_segment_bytes;  /* scratch per work-item */
+  int group_segment_bytes;    /* LDS per work-group */
+  int pad;
+} mf_kernel_resources;
+
+/* compile (or fetch from the cache) without a device and list the resources of every kernel
+ * in the module: up to `cap` entries in out[], the number of kernels in *count */
+int mf_module_resources(const char* generated_src, const char* options, mf_kernel_resources* out, int cap, int* count);
 int mf_module_unload(mf_module* mod);
 int mf_module_has_kernel(mf_module* mod, const char* kernel);
 double mf_module_compile_ms(mf_module* mod);  /* 0 when served from a cache */

@@ -67,11 +67,11 @@ HIP_SYMBOLS = [
     "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_module_load",
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
-    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next",
+    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources",
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
-    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile",
+    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run",
@@ -100,6 +100,7 @@ def libs():
         L.mfh_model_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
         L.mfh_model_source.argtypes = [vp, ctypes.c_char_p, ip]
         L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
+        L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
         L.mfh_batch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, pp]
         L.mfh_batch_destroy.argtypes = [vp]
         L.mfh_batch_upload.argtypes = [vp, ip, dp, ctypes.c_longlong]
@@ -112,7 +113,7 @@ def libs():
         L.mfh_batch_bytes.restype = ctypes.c_longlong
         L.mfh_batch_fit.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ip, dp, ip, dp, dp, ctypes.POINTER(FitOut), ctypes.POINTER(BatchStats)]
         L.mfh_fit_plain.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, dp, dp, ctypes.POINTER(FitOut), dp]
-        L.mfh_fit_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
+        L.mfh_fit_cpu.argtypes = [ip, ip, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_eval_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
         L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_ulonglong), dp, ip,
@@ -140,6 +141,16 @@ def options(**kw):
     for k, v in kw.items():
         setattr(o, k, int(v))
     return o
+
+
+def kernel_resources(kind, fp32, with_hessian=True, fit=True, **kw):
+    """Without a device: per evaluation kind {EVAL_LH, EVAL_GRAD, EVAL_HESS} the kernel's
+    unroll, VGPRs, AGPRs and scratch bytes, fitted to the register budget (fit=True, as a
+    model is built) or at the requested unroll as given (fit=False)."""
+    info = (ctypes.c_int * 12)()
+    _check(libs()[1].mfh_model_kernels(kind, int(fp32), int(with_hessian), ctypes.byref(options(**kw)), int(fit), info))
+    return {k: {"unroll": info[j], "vgpr": info[3 + j], "agpr": info[6 + j], "scratch": info[9 + j]}
+            for j, k in enumerate((EVAL_LH, EVAL_GRAD, EVAL_HESS))}
 
 
 def precompile(kind, fp32, with_hessian=True, **kw):
@@ -197,12 +208,14 @@ class Model:
         if self._rng is not None:
             r = _dp(self._rng)
         _check(libs()[1].mfh_model_create(ctx.h, kind, int(fp32), r, int(with_hessian), ctypes.byref(self.h)))
-        info = (ctypes.c_int * 8)()
+        info = (ctypes.c_int * 14)()
         ms = (ctypes.c_double * 2)()
         _check(libs()[1].mfh_model_info(self.h, info, ms))
         self.ntheta, self.nhoist, self.nobs = info[0], info[1], info[2]
         self.nout = {EVAL_LH: info[3], EVAL_GRAD: info[4], EVAL_HESS: info[5]}
         self.statements, self.graph_nodes = info[6], info[7]
+        self.unroll = {EVAL_LH: info[8], EVAL_GRAD: info[9], EVAL_HESS: info[10]}
+        self.vgprs = {EVAL_LH: info[11], EVAL_GRAD: info[12], EVAL_HESS: info[13]}
         self.codegen_ms, self.compile_ms = ms[0], ms[1]
 
     def source(self):
@@ -313,13 +326,14 @@ def serve(models, theta_gen, start, cls, n, seeds, arrival_s, budget_bytes, slot
             "rejected": int(stats[3]), "wall_s": stats[4], "peak_bytes": stats[5], "generate_ms": stats[6]}
 
 
-def fit_cpu(kind, data, start, nthreads=1):
-    """One fit through the host-compiled CPU backend (reference LLVM-backend semantics)."""
+def fit_cpu(kind, data, start, nthreads=1, fp32=False):
+    """One fit through the host-compiled CPU backend (reference LLVM-backend semantics);
+    fp32=True runs the FP32 kernels the way the device's FP32 path does."""
     data = _f64(np.atleast_2d(data))
     P = NTHETA[kind]
     vals, errs = np.zeros(P), np.zeros(P)
     out, ms = FitOut(), ctypes.c_double(0)
-    _check(libs()[1].mfh_fit_cpu(kind, _dp(data), data.shape[1], _dp(_f64(start)), nthreads, _dp(vals), _dp(errs),
+    _check(libs()[1].mfh_fit_cpu(kind, int(fp32), _dp(data), data.shape[1], _dp(_f64(start)), nthreads, _dp(vals), _dp(errs),
                                  ctypes.byref(out), ctypes.byref(ms)))
     return vals, errs, out, ms.value
 

@@ -37,6 +37,7 @@ int fail(const std::string& what) {
 // benchmarks (benchmarking/morefit_massfit.cc:48-57, benchmarking/morefit_kstarmumu.cc:41-52)
 template <typename K>
 struct ModelDef {
+  typedef K kernel_type;
   int kind;
   std::vector<std::unique_ptr<dimension<double>>> dims;
   std::vector<std::unique_ptr<parameter<double>>> pars;
@@ -204,6 +205,11 @@ int mfh_model_info(mfh_model* m, int* info, double* ms) {
     info[5] = hm.nout[EVAL_HESS];
     info[6] = (int)hm.statements;
     info[7] = (int)hm.graph_nodes;
+    const char* names[3] = {"mf_eval_lh", "mf_eval_grad", "mf_eval_hess"};
+    for (int k = 0; k < 3; k++) {
+      info[8 + k] = hm.kernel_unroll(names[k]);
+      info[11 + k] = hm.kernel_vgprs(names[k]);
+    }
     if (ms) {
       ms[0] = hm.codegen_ms;
       ms[1] = hm.compile_ms;
@@ -238,6 +244,47 @@ int mfh_model_precompile(int kind, int fp32, int with_hessian, const mfh_options
   size_t bytes = 0;
   hip_check(mf_module_compile_only(src.c_str(), co.hip_compile_options.c_str(), &bytes), "mf_module_compile_only");
   if (compile_ms) *compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  MFH_CATCH
+}
+
+int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* opts, int fit, int* info) {
+  MFH_TRY
+  compute_options co;
+  apply_options(co, opts);
+  auto report = [&](auto& d) {
+    typedef typename std::remove_reference<decltype(d)>::type::kernel_type K;
+    ModuleSource ms;
+    std::vector<mf_kernel_resources> res;
+    if (fit) {
+      ms = HipModel<K>::make_source(d.pdf, d.params, co, with_hessian != 0, nullptr, &res);
+    } else {
+      LikelihoodGraphs<K> g(d.pdf, d.params, std::is_same<K, float>::value && co.hip_centre_observables, with_hessian != 0);
+      ms = g.module(co, with_hessian != 0);
+      mf_kernel_resources r[64];
+      int n = 0;
+      hip_check(mf_module_resources(ms.device_source().c_str(), co.hip_compile_options.c_str(), r, 64, &n), "mf_module_resources");
+      res.assign(r, r + std::min(n, 64));
+    }
+    const char* names[3] = {"mf_eval_lh", "mf_eval_grad", "mf_eval_hess"};
+    for (int k = 0; k < 12; k++) info[k] = 0;
+    for (int k = 0; k < 3; k++) {
+      for (auto& e : ms.evals)
+        if (e.name == names[k]) info[k] = e.unroll;
+      for (auto& r : res)
+        if (std::string(r.name) == names[k]) {
+          info[3 + k] = r.vgpr_count;
+          info[6 + k] = r.agpr_count;
+          info[9 + k] = r.private_segment_bytes;
+        }
+    }
+  };
+  if (fp32) {
+    ModelDef<float> d(kind, nullptr);
+    report(d);
+  } else {
+    ModelDef<double> d(kind, nullptr);
+    report(d);
+  }
   MFH_CATCH
 }
 
@@ -482,13 +529,14 @@ int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long
   MFH_CATCH
 }
 
-int mfh_fit_cpu(int kind, const double* data, long long n, const double* start, int nthreads, double* values, double* errors,
-                mfh_fit_out* out, double* ms) {
+int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, double* values,
+                double* errors, mfh_fit_out* out, double* ms) {
   MFH_TRY
   compute_options co;
   co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
   CPUBackend be(&co);
-  run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms);
+  if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms);
+  else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms);
   MFH_CATCH
 }
 

@@ -10,10 +10,12 @@
 //   MF_BLOCK        work-group size of the evaluation kernels (multiple of 64)
 //   MF_ACC          0: FP64 accumulation per event, 1: compensated (Kahan) FP32 per lane
 //   MF_FAST_MATH    1: hardware exp2/log2/rcp approximations in FP32
-//   MF_UNROLL       events per thread whose loads are issued together (default 4)
 //   MF_GEN_LO<d>, MF_GEN_W<d>  generator proposal box per observable (stored coordinates)
 // and, after it, functor structs { static constexpr int nout; static void eval(x, p, o); },
-// mf_hoist(theta, buf) and the kernel instantiations (MF_EVAL_KERNEL etc. below).
+// mf_hoist(theta, buf) and the kernel instantiations (MF_EVAL_KERNEL etc. below).  The
+// unroll of each evaluation kernel (events per thread whose loads are issued together) is an
+// argument of MF_EVAL_KERNEL, chosen per kernel by the host so that the kernel's registers
+// stay in the 256 architectural VGPRs (see mf_module_load).
 //
 // Semantics it implements (reference compute_opencl.hh:646-995): per event the log density
 // and its derivatives; per output the compensated sum over the events of each segment.
@@ -25,9 +27,6 @@
 #define MF_NPARAM (MF_NTHETA + MF_NHOIST)
 #define MF_GEN_MAX_TRIES (1u << 20)
 #define MF_NWAVE (MF_BLOCK / 64)
-#ifndef MF_UNROLL
-#define MF_UNROLL 4
-#endif
 
 // ------------------------------------------------------------------ math
 #if MF_FP32
@@ -140,7 +139,7 @@ __device__ __forceinline__ void mf_two_sum(double a, double b, double& s, double
 // accumulation, wave64 butterfly + LDS reduction to one FP64 partial per output, written to
 // partials[chunk][k].  No fences or atomics: the segment totals come from the separate
 // final kernel below, launched right after on the same stream.
-template <class F>
+template <class F, int U>
 __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
   constexpr int NO = F::nout;
   __shared__ double s_red[MF_NWAVE][NO];
@@ -155,15 +154,15 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
     mf_acc<NO> acc;
     acc.zero();
     int i = ch.begin + (int)threadIdx.x;
-    // MF_UNROLL events per thread per step: their loads are issued together, then computed
-    for (; i + (MF_UNROLL - 1) * MF_BLOCK < ch.end; i += MF_UNROLL * MF_BLOCK) {
-      MF_T x[MF_UNROLL][MF_NOBS];
+    // U events per thread per step: their loads are issued together, then computed
+    for (; i + (U - 1) * MF_BLOCK < ch.end; i += U * MF_BLOCK) {
+      MF_T x[U][MF_NOBS];
 #pragma unroll
-      for (int u = 0; u < MF_UNROLL; u++)
+      for (int u = 0; u < U; u++)
 #pragma unroll
         for (int d = 0; d < MF_NOBS; d++) x[u][d] = __builtin_nontemporal_load(col + d * sg.stride + i + u * MF_BLOCK);
 #pragma unroll
-      for (int u = 0; u < MF_UNROLL; u++) {
+      for (int u = 0; u < U; u++) {
         MF_T o[NO];
         F::eval(x[u], P, o);
         acc.add(o);
@@ -244,8 +243,8 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
   }
 }
 
-#define MF_EVAL_KERNEL(NAME, FUNCTOR)                                                                      \
-  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR>(a); } \
+#define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL)                                                                    \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL>(a); } \
   extern "C" __global__ void __launch_bounds__(64) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR::nout>(a); }
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }

@@ -169,6 +169,168 @@ int get_code(const char* generated, const char* options, std::vector<char>& code
   return MF_OK;
 }
 
+// ---------------------------------------------------------------- code-object metadata
+// A gfx950 code object is an ELF file whose SHT_NOTE section holds an NT_AMDGPU_METADATA
+// (type 32, owner "AMDGPU") note: a MessagePack map with "amdhsa.kernels", one map per
+// kernel carrying .name, .vgpr_count, .agpr_count, ... .  Only what that needs is read here.
+struct MpVal {
+  enum Kind { NIL, BOOL, INT, FLOAT, STR, BIN, ARR, MAP, EXT } kind = NIL;
+  long long i = 0;
+  std::string s;
+  std::vector<MpVal> items;  // array items; for a map: key, value, key, value, ...
+  const MpVal* get(const char* key) const {
+    if (kind != MAP) return nullptr;
+    for (size_t k = 0; k + 1 < items.size(); k += 2)
+      if (items[k].kind == STR && items[k].s == key) return &items[k + 1];
+    return nullptr;
+  }
+};
+
+struct MpReader {
+  const unsigned char *p, *e;
+  bool ok = true;
+  uint64_t be(int n) {
+    if (e - p < n) {
+      ok = false;
+      return 0;
+    }
+    uint64_t v = 0;
+    for (int k = 0; k < n; k++) v = (v << 8) | p[k];
+    p += n;
+    return v;
+  }
+  void bytes(size_t n, std::string* out) {
+    if ((size_t)(e - p) < n) {
+      ok = false;
+      return;
+    }
+    if (out) out->assign((const char*)p, n);
+    p += n;
+  }
+  void seq(MpVal& v, uint64_t n, bool map, int depth) {
+    v.kind = map ? MpVal::MAP : MpVal::ARR;
+    uint64_t m = map ? 2 * n : n;
+    if (m > (uint64_t)(e - p)) {  // every element takes at least one byte
+      ok = false;
+      return;
+    }
+    v.items.resize(m);
+    for (uint64_t k = 0; k < m && ok; k++) value(v.items[k], depth + 1);
+  }
+  void value(MpVal& v, int depth = 0) {
+    if (depth > 64 || p >= e) {
+      ok = false;
+      return;
+    }
+    unsigned c = *p++;
+    if (c <= 0x7f) { v.kind = MpVal::INT; v.i = c; }
+    else if (c >= 0xe0) { v.kind = MpVal::INT; v.i = (int)c - 256; }
+    else if ((c & 0xf0) == 0x80) seq(v, c & 0x0f, true, depth);
+    else if ((c & 0xf0) == 0x90) seq(v, c & 0x0f, false, depth);
+    else if ((c & 0xe0) == 0xa0) { v.kind = MpVal::STR; bytes(c & 0x1f, &v.s); }
+    else switch (c) {
+      case 0xc0: v.kind = MpVal::NIL; break;
+      case 0xc2: case 0xc3: v.kind = MpVal::BOOL; v.i = c == 0xc3; break;
+      case 0xc4: case 0xc5: case 0xc6: v.kind = MpVal::BIN; bytes(be(1 << (c - 0xc4)), nullptr); break;
+      case 0xc7: case 0xc8: case 0xc9: { v.kind = MpVal::EXT; uint64_t n = be(1 << (c - 0xc7)); be(1); bytes(n, nullptr); break; }
+      case 0xca: v.kind = MpVal::FLOAT; be(4); break;
+      case 0xcb: v.kind = MpVal::FLOAT; be(8); break;
+      case 0xcc: case 0xcd: case 0xce: case 0xcf: v.kind = MpVal::INT; v.i = (long long)be(1 << (c - 0xcc)); break;
+      case 0xd0: v.kind = MpVal::INT; v.i = (int8_t)be(1); break;
+      case 0xd1: v.kind = MpVal::INT; v.i = (int16_t)be(2); break;
+      case 0xd2: v.kind = MpVal::INT; v.i = (int32_t)be(4); break;
+      case 0xd3: v.kind = MpVal::INT; v.i = (long long)be(8); break;
+      case 0xd4: case 0xd5: case 0xd6: case 0xd7: case 0xd8: v.kind = MpVal::EXT; be(1); bytes((size_t)1 << (c - 0xd4), nullptr); break;
+      case 0xd9: case 0xda: case 0xdb: v.kind = MpVal::STR; bytes(be(1 << (c - 0xd9)), &v.s); break;
+      case 0xdc: case 0xdd: seq(v, be(c == 0xdc ? 2 : 4), false, depth); break;
+      case 0xde: case 0xdf: seq(v, be(c == 0xde ? 2 : 4), true, depth); break;
+      default: ok = false;
+    }
+  }
+};
+
+template <typename T>
+bool rd_le(const std::vector<char>& b, size_t off, T& v) {
+  if (off + sizeof(T) > b.size()) return false;
+  std::memcpy(&v, b.data() + off, sizeof(T));
+  return true;
+}
+
+// metadata note of a code object (an ELF64 image, or a clang offload bundle holding one)
+bool code_metadata(const std::vector<char>& code, MpVal& meta) {
+  static const char kBundle[] = "__CLANG_OFFLOAD_BUNDLE__";
+  if (code.size() > 32 && std::memcmp(code.data(), kBundle, 24) == 0) {
+    uint64_t n = 0;
+    rd_le(code, 24, n);
+    size_t q = 32;
+    for (uint64_t k = 0; k < n && k < 64; k++) {
+      uint64_t off = 0, size = 0, idlen = 0;
+      if (!rd_le(code, q, off) || !rd_le(code, q + 8, size) || !rd_le(code, q + 16, idlen)) return false;
+      if (q + 24 + idlen > code.size()) return false;
+      std::string id(code.data() + q + 24, idlen);
+      q += 24 + idlen;
+      if (id.find(kArch) != std::string::npos && off + size <= code.size()) {
+        std::vector<char> inner(code.begin() + off, code.begin() + off + size);
+        return code_metadata(inner, meta);
+      }
+    }
+    return false;
+  }
+  if (code.size() < 64 || std::memcmp(code.data(), "\x7f" "ELF", 4) != 0 || code[4] != 2 || code[5] != 1) return false;
+  uint64_t shoff = 0;
+  uint16_t shentsize = 0, shnum = 0;
+  if (!rd_le(code, 0x28, shoff) || !rd_le(code, 0x3a, shentsize) || !rd_le(code, 0x3c, shnum)) return false;
+  for (uint16_t k = 0; k < shnum; k++) {
+    size_t sh = shoff + (size_t)k * shentsize;
+    uint32_t type = 0;
+    uint64_t off = 0, size = 0;
+    if (!rd_le(code, sh + 4, type) || !rd_le(code, sh + 0x18, off) || !rd_le(code, sh + 0x20, size)) return false;
+    if (type != 7 || off + size > code.size()) continue;  // SHT_NOTE
+    size_t q = off, end = off + size;
+    while (q + 12 <= end) {
+      uint32_t namesz = 0, descsz = 0, ntype = 0;
+      rd_le(code, q, namesz);
+      rd_le(code, q + 4, descsz);
+      rd_le(code, q + 8, ntype);
+      size_t name = q + 12, desc = name + ((namesz + 3u) & ~3u);
+      if (desc + descsz > end) break;
+      if (ntype == 32 && namesz >= 6 && std::memcmp(code.data() + name, "AMDGPU", 6) == 0) {
+        MpReader r{(const unsigned char*)code.data() + desc, (const unsigned char*)code.data() + desc + descsz};
+        r.value(meta);
+        return r.ok && meta.kind == MpVal::MAP;
+      }
+      q = desc + ((descsz + 3u) & ~3u);
+    }
+  }
+  return false;
+}
+
+int code_resources(const std::vector<char>& code, std::vector<mf_kernel_resources>& out) {
+  MpVal meta;
+  if (!code_metadata(code, meta)) return set_error(MF_ERR_COMPILE, "code object: AMDGPU metadata note not found or malformed");
+  const MpVal* ks = meta.get("amdhsa.kernels");
+  if (!ks || ks->kind != MpVal::ARR) return set_error(MF_ERR_COMPILE, "code object: no amdhsa.kernels in metadata");
+  out.clear();
+  for (const MpVal& k : ks->items) {
+    mf_kernel_resources r{};
+    auto num = [&](const char* key) {
+      const MpVal* v = k.get(key);
+      return v && v->kind == MpVal::INT ? (int)v->i : 0;
+    };
+    const MpVal* nm = k.get(".name");
+    if (nm && nm->kind == MpVal::STR) snprintf(r.name, sizeof(r.name), "%s", nm->s.c_str());
+    r.vgpr_count = num(".vgpr_count");
+    r.agpr_count = num(".agpr_count");
+    r.sgpr_count = num(".sgpr_count");
+    r.vgpr_spill_count = num(".vgpr_spill_count");
+    r.sgpr_spill_count = num(".sgpr_spill_count");
+    r.private_segment_bytes = num(".private_segment_fixed_size");
+    r.group_segment_bytes = num(".group_segment_fixed_size");
+    out.push_back(r);
+  }
+  return MF_OK;
+}
+
 // built-in kernels (compiled into this library for gfx950)
 __global__ void k_fill_zero(unsigned int* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,6 +528,20 @@ int mf_module_compile_only(const char* generated_src, const char* options, size_
   return rc;
 }
 
+int mf_module_resources(const char* generated_src, const char* options, mf_kernel_resources* out, int cap, int* count) {
+  MF_INJECT("mf_module_resources");
+  if (!generated_src || !count || (cap > 0 && !out)) return set_error(MF_ERR_INVALID, "mf_module_resources: null argument");
+  std::vector<char> code;
+  int rc = get_code(generated_src, options, code, nullptr, nullptr);
+  if (rc != MF_OK) return rc;
+  std::vector<mf_kernel_resources> res;
+  rc = code_resources(code, res);
+  if (rc != MF_OK) return rc;
+  *count = (int)res.size();
+  for (int k = 0; k < cap && k < (int)res.size(); k++) out[k] = res[k];
+  return MF_OK;
+}
+
 int mf_module_load(mf_device* dev, const char* generated_src, const char* options, mf_module** out) {
   MF_INJECT("mf_module_load");
   if (!dev || !generated_src || !out) return set_error(MF_ERR_INVALID, "mf_module_load: null argument");
@@ -389,6 +565,15 @@ int mf_module_load(mf_device* dev, const char* generated_src, const char* option
   }
   int rc = get_code(generated_src, options, code, &h, &ms);
   if (rc != MF_OK) return rc;
+  std::vector<mf_kernel_resources> res;
+  rc = code_resources(code, res);
+  if (rc != MF_OK) return rc;
+  for (const auto& r : res)
+    if (r.agpr_count > 0)
+      return set_error(MF_ERR_COMPILE,
+                       "mf_module_load: kernel %s allocates %d AGPRs (%d VGPRs); modules whose kernels need more than the "
+                       "256 architectural VGPRs are refused -- lower that kernel's unroll",
+                       r.name, r.agpr_count, r.vgpr_count);
   MF_HIP(hipSetDevice(dev->ordinal), "mf_module_load");
   hipModule_t hm;
   MF_HIP(hipModuleLoadData(&hm, code.data()), "mf_module_load");

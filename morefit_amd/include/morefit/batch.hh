@@ -15,6 +15,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -61,6 +62,9 @@ class FitScheduler {
   bool run_hesse = true;
   int strategy = 2;
   double tolerance = 0.01;
+  // MIGRAD trace level of every fit (>= 3: per-iteration lines on stderr); the environment
+  // variable MOREFIT_PRINT_LEVEL sets it for fits driven through the C API
+  int print_level = std::getenv("MOREFIT_PRINT_LEVEL") ? std::atoi(std::getenv("MOREFIT_PRINT_LEVEL")) : 0;
   size_t stack_bytes = 256 * 1024;
   BatchFitStats stats;
 
@@ -145,9 +149,8 @@ class FitScheduler {
       post(x, EVAL_LH);
       return -2.0 * t->b->result(t->seg, EVAL_LH)[0];
     }
-    // FP32: ~1e-7 absolute per event in log p, adding up as a random walk
     double value_noise() const override {
-      return std::is_same<kernelT, float>::value ? 1e-6 * std::sqrt((double)std::max<long long>(1, t->b->sizes[t->seg])) : 0.0;
+      return std::is_same<kernelT, float>::value ? fp32_sum_noise((double)t->b->sizes[t->seg]) : 0.0;
     }
     double value_grad(const double* x, double* g) override {
       post(x, EVAL_GRAD);
@@ -207,6 +210,8 @@ class FitScheduler {
     Migrad m(mp);
     m.strategy = strategy;
     m.tolerance = tolerance;
+    m.print_level = print_level;
+    m.trace_id = t.id;
     Fcn fcn(&t);
     try {
       m.minimize(fcn);

@@ -20,6 +20,12 @@
 
 namespace morefit {
 
+// one MF_EVAL_KERNEL instantiation: kernel name, functor, events per thread per load batch
+struct EvalKernelSpec {
+  std::string name, functor;
+  int unroll = 1;
+};
+
 struct ModuleSource {
   bool fp32 = false;
   int nobs = 1;
@@ -28,11 +34,11 @@ struct ModuleSource {
   int block = 256;
   int acc = 1;
   bool fast_math = false;
-  int unroll = 4;
   std::vector<double> gen_lo, gen_width;  // proposal box per observable, in stored coordinates
   std::string functors;                    // functor structs
   std::string hoist;                       // mf_hoist definition
-  std::vector<std::string> kernels;        // instantiation lines
+  std::vector<EvalKernelSpec> evals;       // evaluation kernels (unroll chosen per kernel)
+  std::vector<std::string> kernels;        // other instantiation lines
   size_t statements = 0;                   // SSA statements emitted (all functors)
 
   std::string prelude() const {
@@ -44,8 +50,7 @@ struct ModuleSource {
       << "#define MF_NHOIST " << nhoist << "\n"
       << "#define MF_BLOCK " << block << "\n"
       << "#define MF_ACC " << acc << "\n"
-      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n"
-      << "#define MF_UNROLL " << unroll << "\n";
+      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n";
     s << "#define MF_GEN_PROPOSE(x, s) do {";
     for (int d = 0; d < nobs; d++) {
       double lo = d < (int)gen_lo.size() ? gen_lo[d] : 0.0, w = d < (int)gen_width.size() ? gen_width[d] : 1.0;
@@ -61,6 +66,7 @@ struct ModuleSource {
   std::string device_source() const {
     std::ostringstream s;
     s << "// generated by morefit codegen.hh\n" << prelude() << "//@MF_RUNTIME@\n" << functors << "\n" << hoist << "\n";
+    for (const auto& e : evals) s << "MF_EVAL_KERNEL(" << e.name << ", " << e.functor << ", " << e.unroll << ")\n";
     for (const auto& k : kernels) s << k << "\n";
     return s.str();
   }

@@ -41,6 +41,40 @@ inline void hip_check(int rc, const char* what) {
   if (rc != MF_OK) throw std::runtime_error(std::string(what) + ": " + mf_last_error());
 }
 
+// Halve the unroll of every evaluation kernel whose registers do not fit the 256
+// architectural VGPRs (it allocates AGPRs or scratch) until every kernel fits.  A kernel that
+// needs AGPRs even at unroll 1 is an error: mf_module_load refuses such modules, because the
+// compiler was found to leave loop-carried AGPR copies unset in lanes outside a divergent
+// branch.  Each pass compiles through the code-object cache, so the load that follows finds
+// the final code object there.  `res` receives the resources of the final module.
+inline void fit_register_budget(ModuleSource& ms, const std::string& options, std::vector<mf_kernel_resources>* res = nullptr) {
+  constexpr int kMaxKernels = 64;
+  mf_kernel_resources r[kMaxKernels];
+  for (;;) {
+    const std::string src = ms.device_source();
+    int n = 0;
+    hip_check(mf_module_resources(src.c_str(), options.c_str(), r, kMaxKernels, &n), "mf_module_resources");
+    n = std::min(n, kMaxKernels);
+    bool changed = false;
+    for (auto& e : ms.evals)
+      for (int k = 0; k < n; k++) {
+        if (e.name != r[k].name || (r[k].agpr_count == 0 && r[k].private_segment_bytes == 0)) continue;
+        if (e.unroll > 1) {
+          e.unroll /= 2;
+          changed = true;
+        } else if (r[k].agpr_count > 0) {
+          throw std::runtime_error("kernel " + e.name + " needs " + std::to_string(r[k].vgpr_count) + " VGPRs + " +
+                                   std::to_string(r[k].agpr_count) + " AGPRs at unroll 1; more than the 256 architectural "
+                                   "VGPRs is not supported");
+        }
+      }
+    if (!changed) {
+      if (res) res->assign(r, r + n);
+      return;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------- backend
 class HIPBackend : public ComputeBackend {
   mf_device* dev_ = nullptr;
@@ -172,10 +206,11 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
   }
   int nchunks_ = 0;
 
-  void compile(const ModuleSource& ms) {
+  void compile(ModuleSource& ms) {
     State& s = *s_;
     if (s.mod) mf_module_unload(s.mod);
     s.mod = nullptr;
+    fit_register_budget(ms, opts().hip_compile_options);
     std::string src = ms.device_source();
     if (opts().print_kernel) std::cout << "KERNEL SOURCE\n" << src << "END KERNEL SOURCE" << std::endl;
     hip_check(mf_module_load(dev(), src.c_str(), opts().hip_compile_options.c_str(), &s.mod), "HIPBlock: mf_module_load");
@@ -292,7 +327,6 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     ms.block = opts().hip_block_size;
     ms.acc = opts().hip_accumulate;
     ms.fast_math = opts().hip_fast_math;
-    ms.unroll = opts().hip_unroll;
     std::vector<std::string> obs;
     for (auto& d : input_signature) obs.push_back(d.get_name());
     std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
@@ -300,7 +334,8 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
     ms.hoist = "__device__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
     s.kernel = name;
-    ms.kernels.push_back(std::string(kahan_summation ? "MF_EVAL_KERNEL(" : "MF_STORE_KERNEL(") + name + ", mf_f_block)");
+    if (kahan_summation) ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll)});
+    else ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
     compile(ms);
     s.nout = (int)graphs.size();
     s.kahan = kahan_summation;
@@ -465,7 +500,6 @@ struct LikelihoodGraphs {
     ms.block = o.hip_block_size;
     ms.acc = o.hip_accumulate;
     ms.fast_math = o.hip_fast_math;
-    ms.unroll = o.hip_unroll;
     for (size_t d = 0; d < obs.size(); d++) {
       ms.gen_lo.push_back(lo[d] - centres[d]);
       ms.gen_width.push_back(hi[d] - lo[d]);
@@ -475,9 +509,10 @@ struct LikelihoodGraphs {
                   emit_functor("mf_f_grad", raw(grad), obs, pn, ms.fp32, &ms.statements);
     if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), obs, pn, ms.fp32, &ms.statements);
     ms.hoist = emit_hoist(raw(hoist_exprs), theta);
-    ms.kernels = {"MF_EVAL_KERNEL(mf_eval_lh, mf_f_lh)", "MF_EVAL_KERNEL(mf_eval_grad, mf_f_grad)", "MF_PREP_KERNEL",
-                  "MF_GENERATE_KERNEL(mf_f_lh)"};
-    if (with_hessian) ms.kernels.push_back("MF_EVAL_KERNEL(mf_eval_hess, mf_f_hess)");
+    const int u = std::max(1, o.hip_unroll);
+    ms.evals = {{"mf_eval_lh", "mf_f_lh", u}, {"mf_eval_grad", "mf_f_grad", u}};
+    if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u});
+    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_lh)"};
     return ms;
   }
 };
@@ -499,23 +534,43 @@ class HipModel {
   size_t statements = 0, graph_nodes = 0;
   double codegen_ms = 0.0, compile_ms = 0.0;
   mf_module* mod = nullptr;
+  std::vector<std::pair<std::string, int>> unrolls;  // evaluation kernel -> unroll
+  std::vector<mf_kernel_resources> resources;        // every kernel of the module
 
+  // generated module with the unroll of each evaluation kernel fitted to the register budget
   static ModuleSource make_source(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, const compute_options& o,
-                                  bool with_hessian, LikelihoodGraphs<kernelT, evalT>** keep = nullptr) {
+                                  bool with_hessian, LikelihoodGraphs<kernelT, evalT>** keep = nullptr,
+                                  std::vector<mf_kernel_resources>* res = nullptr, double* compile_ms = nullptr) {
     const bool f32 = std::is_same<kernelT, float>::value;
-    auto* g = new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian);
+    std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> g(
+        new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian));
     ModuleSource ms = g->module(o, with_hessian);
-    if (keep) *keep = g;
-    else delete g;
+    auto t0 = std::chrono::steady_clock::now();
+    fit_register_budget(ms, o.hip_compile_options, res);
+    if (compile_ms) *compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (keep) *keep = g.release();
     return ms;
+  }
+  // unroll and register count of an evaluation kernel ("mf_eval_lh", ...) in the loaded module
+  int kernel_unroll(const std::string& name) const {
+    for (auto& e : unrolls)
+      if (e.first == name) return e.second;
+    return 0;
+  }
+  int kernel_vgprs(const std::string& name) const {
+    for (auto& r : resources)
+      if (name == r.name) return r.vgpr_count;
+    return 0;
   }
 
   HipModel(HIPBackend* backend, PDF<kernelT, evalT>* p, const std::vector<parameter<evalT>*>& ps, bool hessian = true)
       : be(backend), pdf(p), params(ps), with_hessian(hessian) {
     auto t0 = std::chrono::steady_clock::now();
     LikelihoodGraphs<kernelT, evalT>* g = nullptr;
-    ModuleSource ms = make_source(p, ps, *be->opts_, hessian, &g);
+    double fit_ms = 0.0;
+    ModuleSource ms = make_source(p, ps, *be->opts_, hessian, &g, &resources, &fit_ms);
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> keep(g);
+    for (auto& e : ms.evals) unrolls.push_back({e.name, e.unroll});
     for (size_t i = 0; i < ps.size(); i++)
       if (!ps[i]->is_constant()) theta_index.push_back((int)i);
     obs = g->obs;
@@ -532,10 +587,13 @@ class HipModel {
     statements = ms.statements;
     graph_nodes = g->nodes;
     source = ms.device_source();
-    codegen_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // codegen_ms: graphs and source; compile_ms: hipRTC passes of the register fit (0 when the
+    // code objects come from the cache) plus the module load
+    codegen_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() - fit_ms;
     if (be->opts_->print_kernel) std::cout << source << std::endl;
+    auto t1 = std::chrono::steady_clock::now();
     hip_check(mf_module_load(be->device(), source.c_str(), be->opts_->hip_compile_options.c_str(), &mod), "HipModel: mf_module_load");
-    compile_ms = mf_module_compile_ms(mod);
+    compile_ms = fit_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
   }
   ~HipModel() { mf_module_unload(mod); }
   HipModel(const HipModel&) = delete;

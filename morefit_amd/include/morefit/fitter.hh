@@ -321,7 +321,7 @@ class fitter {
       return f->hessian((int)f->params_.size(), h);
     }
     double value_noise() const override {
-      return std::is_same<kernelT, float>::value ? 1e-6 * std::sqrt((double)std::max<size_t>(1, f->nevents_)) : 0.0;
+      return std::is_same<kernelT, float>::value ? fp32_sum_noise((double)f->nevents_) : 0.0;
     }
   };
 

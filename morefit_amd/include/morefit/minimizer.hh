@@ -9,8 +9,10 @@
  *  - seed: analytic Hessian (strategy 2 with an analytic Hessian, the setting the reference
  *    fitter uses, fitter.hh:762-767, 806-813) transformed to the internal space and
  *    inverted, forced positive-definite by a diagonal shift when needed;
- *  - iteration: Newton step -V g taken whole when value and directional derivative agree
- *    (one value+gradient call), else a parabolic line search on the value; BFGS update of V;
+ *  - iteration: Newton step -V g taken whole on sufficient decrease (one value+gradient
+ *    call), else a parabolic line search on the value; where the value's rounding noise
+ *    (FP32 sums) exceeds the predicted decrease, the directional derivative decides instead
+ *    (secant search for its zero); BFGS update of V;
  *  - convergence: EDM = g^T V g / 2 < 0.002 * tolerance * up (tolerance 0.01, up 1 as in the
  *    reference's Minuit2 defaults), confirmed with a fresh analytic Hessian (strategy 2);
  *  - polish (on by default, not in MINUIT): from the converged point, Newton steps with that
@@ -26,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 #include <string>
 #include <vector>
@@ -42,6 +45,13 @@ struct MinimizerFcn {
   // no information and the search then relies on the gradient
   virtual double value_noise() const { return 0.0; }
 };
+
+// Rounding floor, in -2 lnL units, of a likelihood summed in FP32 over n events.  The kernel
+// parameters (floating values and hoisted terms) are rounded to FP32 once per call, so all
+// events see the same relative error of up to 2^-24 in each: as theta moves, the sum jumps
+// coherently by up to ~n 2^-24 per parameter-dependent term, which dominates the
+// random-walk rounding of the per-event terms.  Four such terms, times 2 for -2 lnL.
+inline double fp32_sum_noise(double nevents) { return 8.0 * std::ldexp(1.0, -24) * std::max(1.0, nevents); }
 
 struct MinimizerParam {
   std::string name;
@@ -123,11 +133,12 @@ class Migrad {
   int max_iterations = 4000;
   double tolerance = 0.01;
   double up = 1.0;
-  int print_level = 0;
+  int print_level = 0;         // >= 3: one line per iteration on stderr
+  int trace_id = -1;           // printed with the trace lines (fit id in a batch)
   // after convergence: Newton steps with a fresh exact Hessian down to this EDM
   bool polish = true;
   double polish_edm = 1e-10;
-  int polish_steps = 2;
+  int polish_steps = 4;
 
   // results
   std::vector<double> x;      // external values (all parameters)
@@ -268,6 +279,7 @@ class Migrad {
         double f1 = eval_fg(u1, g1);
         if (!std::isfinite(f1)) break;
         double e1 = compute_edm(g1);
+        if (print_level >= 3) std::fprintf(stderr, "[migrad %d]   polish edm %.3g -> %.3g f1-f %.3g\n", trace_id, e, e1, f1 - f);
         if (!(e1 < e)) break;
         u = u1;
         g = g1;
@@ -309,20 +321,27 @@ class Migrad {
         if (!seed_metric(u, g)) return finish(3, f, e, u, n);
         continue;
       }
-      // Full Newton step first, with value and gradient in one call.  It is accepted on the
-      // usual sufficient decrease, or -- near the minimum, where the change of the value is
-      // comparable to its rounding noise (FP32 sums) -- when the directional derivative shows
-      // the step went most of the way to the line minimum and the value did not rise by more
-      // than a quarter of the predicted decrease.
+      // Full Newton step first, with value and gradient in one call, accepted on sufficient
+      // decrease (Armijo, widened by the value's rounding noise).  Where the predicted
+      // decrease is itself inside the noise (FP32 sums near the minimum) the value says
+      // nothing, and the step is judged by the directional derivative instead: accepted when
+      // it moved toward the line minimum, else a secant search for the derivative's zero.
       for (int i = 0; i < n; i++) u1[i] = u[i] + d[i];
       double f1 = eval_fg(u1, g1);
       double slope1 = 0.0;
       for (int i = 0; i < n; i++) slope1 += g1[i] * d[i];
       const double noise = fcn.value_noise();
-      bool accept = std::isfinite(f1) && (f1 <= f + 1e-4 * slope + noise ||
-                                          (std::fabs(slope1) <= 0.5 * std::fabs(slope) && f1 - f <= 0.25 * std::fabs(slope) + noise));
+      const bool noisy = noise > 0.0 && -slope <= 16.0 * noise;
+      bool accept = std::isfinite(f1) && f1 <= f + 1e-4 * slope + noise &&
+                    (!noisy || std::fabs(slope1) < std::fabs(slope));
       double best_a = accept ? 1.0 : 0.0;
-      if (!accept) {
+      if (print_level >= 3)
+        std::fprintf(stderr, "[migrad %d] it %d f %.10g edm %.3g slope %.3g f1-f %.3g slope1 %.3g noise %.3g newton %s\n", trace_id,
+                     iterations, f, e, slope, f1 - f, slope1, noise, accept ? "accepted" : "rejected");
+      if (!accept && noisy) {
+        best_a = derivative_search(d, slope, f, noise, 1.0, f1, slope1, u, u1, g1, eval_fg, f1);
+        if (print_level >= 3) std::fprintf(stderr, "[migrad %d]   derivative search alpha %.3g\n", trace_id, best_a);
+      } else if (!accept) {
         // parabolic search on the value along d
         double alpha = 1.0, best_f = f;
         double fa = f1;
@@ -342,10 +361,18 @@ class Migrad {
           if (std::fabs(next - alpha) < 1e-12 * alpha) break;
           alpha = next;
         }
+        if (print_level >= 3) std::fprintf(stderr, "[migrad %d]   line search best alpha %.3g\n", trace_id, best_a);
+        if (best_a > 0.0) {
+          for (int i = 0; i < n; i++) u1[i] = u[i] + best_a * d[i];
+          f1 = eval_fg(u1, g1);
+          if (!std::isfinite(f1)) return finish(4, f, e, u, n);
+        }
+      }
+      if (!accept) {
         if (best_a == 0.0) {
-          // no decrease along the Newton direction.  Inside the value's noise floor the
-          // gradient decides: Newton steps that lower EDM; otherwise refresh the metric once.
-          if (-slope <= 4.0 * noise && seed_metric(u, g)) {
+          // no acceptable point along the Newton direction.  Inside the value's noise floor
+          // the gradient decides: Newton steps that lower EDM; otherwise refresh the metric once.
+          if (noisy && seed_metric(u, g)) {
             e = compute_edm(g);
             polish_at_minimum();
             return finish(e < edm_goal ? 0 : 2, f, e, u, n);
@@ -356,9 +383,6 @@ class Migrad {
           }
           return finish(e < edm_goal ? 0 : 2, f, e, u, n);
         }
-        for (int i = 0; i < n; i++) u1[i] = u[i] + best_a * d[i];
-        f1 = eval_fg(u1, g1);
-        if (!std::isfinite(f1)) return finish(4, f, e, u, n);
       }
       refreshed_ = false;
       for (int i = 0; i < n; i++) s[i] = best_a * d[i];
@@ -427,7 +451,56 @@ class Migrad {
   bool pd_forced_ = false;
   bool refreshed_ = false;
 
+  // Zero of the directional derivative phi'(a) = g(u + a d) . d by safeguarded secant steps,
+  // starting from phi'(0) = slope0 < 0 and the probe (a1, f_a1, s_a1).  Accepts a point whose
+  // |phi'| is at most half of |slope0| and whose value is within the noise of f0; on success
+  // u1/g1/f_out hold that point and the step length is returned, else 0.
+  template <class EvalFG>
+  double derivative_search(const std::vector<double>& d, double slope0, double f0, double noise, double a1, double f_a1,
+                           double s_a1, const std::vector<double>& u, std::vector<double>& u1, std::vector<double>& g1,
+                           EvalFG& eval_fg, double& f_out) {
+    const int n = (int)d.size();
+    double a_lo = 0.0, s_lo = slope0;        // phi' < 0 here
+    double a_hi = NAN, s_hi = NAN;           // phi' > 0 (or not finite) here, once bracketed
+    double a_prev = 0.0, s_prev = slope0;    // previous descending point, for extrapolation
+    double a = a1, fa = f_a1, sa = s_a1;
+    for (int it = 0; it < 10; it++) {
+      if (std::isfinite(fa) && std::fabs(sa) <= 0.5 * std::fabs(slope0) && fa <= f0 + noise) {
+        f_out = fa;
+        return a;
+      }
+      if (!std::isfinite(fa) || sa > 0.0) {
+        a_hi = a;
+        s_hi = std::isfinite(fa) ? sa : NAN;
+      } else {
+        a_prev = a_lo;
+        s_prev = s_lo;
+        a_lo = a;
+        s_lo = sa;
+      }
+      double next;
+      if (std::isfinite(a_hi)) {
+        const double w = a_hi - a_lo;
+        next = std::isfinite(s_hi) ? a_lo - s_lo * w / (s_hi - s_lo) : a_lo + 0.5 * w;
+        next = std::min(std::max(next, a_lo + 0.1 * w), a_hi - 0.1 * w);
+      } else {
+        const double c = (s_lo - s_prev) / (a_lo - a_prev);  // curvature along d
+        next = c > 0.0 ? a_lo - s_lo / c : 2.0 * a_lo;
+        next = std::min(std::max(next, 1.5 * a_lo), 4.0 * a_lo);
+      }
+      a = next;
+      for (int i = 0; i < n; i++) u1[i] = u[i] + a * d[i];
+      fa = eval_fg(u1, g1);
+      sa = 0.0;
+      for (int i = 0; i < n; i++) sa += g1[i] * d[i];
+    }
+    return 0.0;
+  }
+
   int finish(int st, double f, double e, const std::vector<double>& u, int n) {
+    if (print_level >= 3)
+      std::fprintf(stderr, "[migrad %d] finish status %d f %.10g edm %.3g calls %d iterations %d\n", trace_id, st, f, e, ncalls,
+                   iterations);
     for (int k = 0; k < n; k++) x[free_[k]] = int2ext(k, u[k]);
     fval = f;
     edm = e;

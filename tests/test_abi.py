@@ -77,3 +77,61 @@ def test_generated_modules_compile_for_gfx950():
 def test_open_without_gpu_raises():
     with pytest.raises(M.MorefitError):
         M.Context()
+
+
+class _Res(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64)] + [(f, ctypes.c_int) for f in (
+        "vgpr_count", "agpr_count", "sgpr_count", "vgpr_spill_count", "sgpr_spill_count",
+        "private_segment_bytes", "group_segment_bytes", "pad")]
+
+
+_LDS_MODULE = b"""#define MF_T float
+#define MF_FP32 1
+#define MF_NOBS 1
+#define MF_NTHETA 1
+#define MF_NHOIST 0
+#define MF_BLOCK 256
+#define MF_ACC 1
+#define MF_FAST_MATH 1
+#define MF_GEN_PROPOSE(x, s) do { } while (0)
+//@MF_RUNTIME@
+extern "C" __global__ void k_lds_probe(float* p) {
+  __shared__ float s[1000];
+  s[threadIdx.x] = p[threadIdx.x];
+  __syncthreads();
+  p[threadIdx.x] = s[999 - threadIdx.x];
+}
+"""
+
+
+def test_module_resources_reads_code_object_metadata():
+    """mf_module_resources parses the AMDGPU metadata note: a known kernel with a 4000-byte
+    LDS array reports that size, its name, and registers within the architectural VGPRs"""
+    hip, _ = M.libs()
+    hip.mf_module_resources.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_Res), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int)]
+    out = (_Res * 8)()
+    n = ctypes.c_int(0)
+    assert hip.mf_module_resources(_LDS_MODULE, b"", out, 8, ctypes.byref(n)) == 0
+    res = {out[k].name.decode(): out[k] for k in range(min(n.value, 8))}
+    assert "k_lds_probe" in res, list(res)
+    r = res["k_lds_probe"]
+    assert r.group_segment_bytes == 4000
+    assert r.agpr_count == 0 and r.private_segment_bytes == 0
+    assert 0 < r.vgpr_count <= 256
+
+
+def test_kernels_fit_register_budget():
+    """Every evaluation kernel of the configuration modules is fitted to the 256 architectural
+    VGPRs: no AGPRs and no scratch.  At unroll 8 the FP64 angular Hessian kernel needs AGPRs
+    (the condition under which the compiler left loop-carried address registers unset in
+    lanes outside a divergent branch); the fit lowers its unroll until it needs none."""
+    raw = M.kernel_resources(M.KSTAR, False, fit=False, unroll=8)
+    assert raw[M.EVAL_HESS]["agpr"] > 0
+    for kind in (M.MASS, M.KSTAR):
+        for fp32 in (False, True):
+            fit = M.kernel_resources(kind, fp32, fit=True, unroll=8)
+            for k, r in fit.items():
+                assert 1 <= r["unroll"] <= 8, (kind, fp32, k, r)
+                assert r["agpr"] == 0 and r["scratch"] == 0, (kind, fp32, k, r)
+    assert M.kernel_resources(M.KSTAR, False, fit=True, unroll=8)[M.EVAL_HESS]["unroll"] < 8
