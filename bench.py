@@ -35,6 +35,35 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def shape(model):
+    """launch shape of a model's evaluation kernels (options of its context, fitted unrolls)"""
+    import morefit_amd as M
+    o = M.options()
+    return {"block": o.block_size, "events_per_thread": o.events_per_thread, "accumulate": o.accumulate,
+            "fast_math": bool(o.fast_math), "unroll": {"lh": model.unroll[M.EVAL_LH], "grad": model.unroll[M.EVAL_GRAD],
+                                                       "hess": model.unroll[M.EVAL_HESS]},
+            "vgprs": {"lh": model.vgprs[M.EVAL_LH], "grad": model.vgprs[M.EVAL_GRAD], "hess": model.vgprs[M.EVAL_HESS]}}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp32_mass_1e7_traffic.txt")
+
+
+def measured_traffic(current_shape):
+    """HBM bytes per launch of the FP32 mf_eval_grad at N = 10^7, from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy kernel's reading), when that
+    summary was taken with the current launch shape; else None"""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    kv = dict(l.split(":", 1) for l in lines if l.startswith(("traffic_bytes_per_launch:", "launch_shape:")))
+    want = "block=%d ept=%d acc=%d unroll=%d" % (current_shape["block"], current_shape["events_per_thread"],
+                                                current_shape["accumulate"], current_shape["unroll"]["grad"])
+    if kv.get("launch_shape", "").strip() != want:
+        return None
+    return float(kv["traffic_bytes_per_launch"])
+
+
 def profile_workload():
     """Headline kernels only, for hardware-counter passes: FP32 mf_eval_grad and mf_eval_lh at
     N = 10^7 (generated on the device), then the copy kernel over 40 MB (the counters' scale)."""
@@ -101,12 +130,16 @@ def main():
     value = args.steps / dt
     log(f"fp32 -2lnL+grad: {value:.1f} evals/s ({ms_per_step:.4f} ms/eval)")
 
-    # kernel-only device time (HIP events on the library's stream)
-    k32_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
-    k32_lh_ms = b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps))
+    # device time of the per-chunk kernel alone and with its final pass (HIP events on the
+    # library's stream, around back-to-back launches)
+    k32_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True)
+    k32_pair_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
+    k32_lh_ms = b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps), main_only=True)
     bytes32 = n * 4
     achieved = bytes32 / (k32_ms * 1e-3) / 1e9
-    log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s; lh kernel {k32_lh_ms * 1e3:.2f} us")
+    log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s (with final pass {k32_pair_ms * 1e3:.2f} us); "
+        f"lh kernel {k32_lh_ms * 1e3:.2f} us")
+    traffic = measured_traffic(shape(m32))
 
     # ---------------------------------------------------------------- FP64 path
     for _ in range(max(5, args.warmup // 2)):
@@ -202,7 +235,7 @@ def main():
                 "pull_mean": pulls.mean(axis=0).round(4).tolist(), "pull_width": pulls.std(axis=0).round(4).tolist(),
                 "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
                 "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms,
-                "launch_shape": {"block": 256, "events_per_thread": 64, "unroll": 8}}
+                "launch_shape": shape(mt)}
         log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain ({batched_rate / plain_rate:.1f}x)")
         bt.close()
         mt.close()
@@ -238,10 +271,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded accept-reject sample of the density, generated on the device",
         "config": {"workload": "mass fit f*Gauss+(1-f)*Exp on [5.0,5.6], N=1e7 events, one -2lnL+gradient evaluation per step",
-                   "events": n, "precision": "FP32 per-event arithmetic, compensated per-lane sums, FP64 totals",
-                   "block": 256, "events_per_thread": 64, "unroll": 8, "fast_math": True},
+                   "events": n, "precision": "FP32 per-event arithmetic, FP32 sums of 8-event groups added in FP64 per lane, FP64 totals",
+                   "launch_shape": shape(m32)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "mf_eval_grad (fp32)", "kernel_ms": k32_ms, "algorithmic_bytes": bytes32},
+                     "traffic": traffic, "kernel": "mf_eval_grad (fp32, per-chunk kernel)", "kernel_ms": k32_ms,
+                     "algorithmic_bytes": bytes32, "with_final_pass_ms": k32_pair_ms},
         "cpu_baseline": cpu,
         "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9},
         "lh_kernel_ms_fp32": k32_lh_ms,

@@ -37,7 +37,7 @@ typedef struct mfh_options {
   int events_per_thread;   /* chunk = block_size * events_per_thread */
   int grid;                /* 0 = one work-group per chunk, else persistent grid */
   int fast_math;           /* FP32 hardware exp/log/rcp */
-  int accumulate;          /* FP32: 1 compensated FP32 per lane, 0 FP64 per event */
+  int accumulate;          /* FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group then FP64 */
   int centre;              /* FP32: observables stored as offsets from the range centre */
   int unroll;              /* events per thread whose loads are issued together */
 } mfh_options;
@@ -91,8 +91,10 @@ int mfh_batch_generate(mfh_batch* b, const double* theta_gen, const unsigned lon
 /* kind 1: value, 2: value+gradient, 3: value+gradient+Hessian (packed i>=j).  theta: nseg x ntheta.
  * out: nseg x nout(kind) raw sums of log p and its derivatives (not scaled by -2). */
 int mfh_batch_eval(mfh_batch* b, int kind, const double* theta, int use_graph, double* out);
-/* mean device time of the evaluation kernel alone (HIP events around `iters` launches) */
-int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, double* ms_per_launch);
+/* mean device time per launch of the evaluation kernels (HIP events on the library's stream
+ * around `iters` back-to-back launches): the per-chunk kernel and its final pass, or with
+ * main_only = 1 the per-chunk kernel alone (totals left unreduced) */
+int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, int main_only, double* ms_per_launch);
 /* mean wall time of the full per-call sequence (upload, prep, kernel, totals, sync) */
 int mfh_batch_time_eval(mfh_batch* b, int kind, int iters, int use_graph, double* us_per_eval);
 long long mfh_batch_bytes(mfh_batch* b);

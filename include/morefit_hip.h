@@ -17,6 +17,7 @@
  * Entry points and the reference interface each one replaces:
  *   mf_device_open/close            OpenCLBackend ctor: platform/device/context/queue (compute_opencl.hh:30-206)
  *   mf_malloc/mf_free               OpenCLBlock::Setup*Buffer / dtor (compute_opencl.hh:258-350)
+ *   mf_host_alloc_mapped            no counterpart (zero-copy staging of parameters and totals)
  *   mf_memcpy_h2d                   CopyToInputBuffer / CopyToParameterBuffer / CopyToSeedBuffer (compute_opencl.hh:351-400)
  *   mf_memcpy_d2h                   CopyFromOutputBuffer / CopyFromKahanBuffer (compute_opencl.hh:401-424, 977-986)
  *   mf_module_load                  MakeComputeKernel / MakeGenerateKernel build step (compute_opencl.hh:425-804)
@@ -58,7 +59,8 @@ extern "C" {
  * chunk_end); each chunk's sums go to partials[chunk * nout ...] and the segment's totals
  * (FP64) to results[out_off ...].  mode 0 skips the segment in a launch.  mf_launch_eval runs
  * the per-chunk kernel <kernel> and then, when the module has it, <kernel>_final (one
- * wavefront per segment, fixed-order reduction of the segment's chunk partials). */
+ * 256-thread work-group per segment, fixed-order reduction of the partials of segments with
+ * more than one chunk; a one-chunk segment's totals come from the per-chunk kernel). */
 typedef struct mf_segment {
   long long data_off;
   long long stride;
@@ -90,9 +92,14 @@ typedef struct mf_eval_args {
   unsigned int* reserved;     /* unused, NULL */
   void* store;                /* T* per-event outputs (store kernels only) or NULL */
   long long store_stride;     /* column stride of `store` */
-  int flags;
+  int flags;                  /* MF_EVAL_* bits */
   int pad;
 } mf_eval_args;
+
+/* mf_eval_args.flags: launch only the per-chunk kernel.  Correct when no active segment has
+ * more than one chunk (the per-chunk kernel then writes the totals itself); otherwise the
+ * totals are left unreduced (used to time the per-chunk kernel alone). */
+#define MF_EVAL_SKIP_FINAL 1
 
 /* argument block of the generated generator kernel (mf_generate):
  * stream k owns events [begin, end) of segment seg and the xoshiro128++ state
@@ -144,6 +151,10 @@ int mf_device_query(mf_device* dev, mf_device_props* props);
 int mf_malloc(mf_device* dev, size_t bytes, void** dptr);
 int mf_free(mf_device* dev, void* dptr);
 int mf_host_alloc(mf_device* dev, size_t bytes, void** hptr);
+/* coherent pinned host memory that kernels read and write in place: *hptr for the host, *dptr
+ * for kernel arguments (staged parameters in, totals out, with no copy in between); freed
+ * with mf_host_free */
+int mf_host_alloc_mapped(mf_device* dev, size_t bytes, void** hptr, void** dptr);
 int mf_host_free(mf_device* dev, void* hptr);
 int mf_memcpy_h2d(mf_device* dev, void* dst, const void* src, size_t bytes);   /* async, stream-ordered */
 int mf_memcpy_d2h(mf_device* dev, void* dst, const void* src, size_t bytes);   /* async, stream-ordered */
@@ -186,7 +197,11 @@ double mf_module_compile_ms(mf_module* mod);  /* 0 when served from a cache */
 
 /* launches (on the device stream) */
 int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
-int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg);
+/* kernel parameters of segments seg_list[0..nseg) (all 0..nseg-1 when seg_list is NULL) from
+ * their FP64 theta; with seg_src set, each of those segments' table entries is also copied
+ * seg_src[s] -> seg_dst[s] (a mapped host table staged into device memory in the same launch) */
+int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg,
+                   const mf_segment* seg_src, mf_segment* seg_dst);
 int mf_launch_generate(mf_module* mod, const mf_gen_args* args, int grid, int block);
 
 /* hipGraph capture of everything enqueued on the device stream between begin and end */

@@ -67,7 +67,7 @@ HIP_SYMBOLS = [
     "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_module_load",
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
-    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources",
+    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped",
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
@@ -107,7 +107,7 @@ def libs():
         L.mfh_batch_download.argtypes = [vp, ip, dp]
         L.mfh_batch_generate.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_ulonglong), ip, ctypes.POINTER(ctypes.c_ulonglong)]
         L.mfh_batch_eval.argtypes = [vp, ip, dp, ip, dp]
-        L.mfh_batch_time_kernel.argtypes = [vp, ip, ip, dp]
+        L.mfh_batch_time_kernel.argtypes = [vp, ip, ip, ip, dp]
         L.mfh_batch_time_eval.argtypes = [vp, ip, ip, ip, dp]
         L.mfh_batch_bytes.argtypes = [vp]
         L.mfh_batch_bytes.restype = ctypes.c_longlong
@@ -274,9 +274,11 @@ class Batch:
         _check(libs()[1].mfh_batch_eval(self.h, kind, _dp(th), int(use_graph), _dp(out)))
         return out
 
-    def time_kernel(self, kind, iters=50):
+    def time_kernel(self, kind, iters=50, main_only=False):
+        """ms per launch of the evaluation kernels of `kind` (HIP events, back-to-back launches);
+        main_only: the per-chunk kernel without its final pass"""
         ms = ctypes.c_double(0)
-        _check(libs()[1].mfh_batch_time_kernel(self.h, kind, iters, ctypes.byref(ms)))
+        _check(libs()[1].mfh_batch_time_kernel(self.h, kind, iters, int(main_only), ctypes.byref(ms)))
         return ms.value
 
     def time_eval(self, kind, iters=50, use_graph=True):

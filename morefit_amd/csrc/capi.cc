@@ -131,10 +131,10 @@ const char* mfh_last_error(void) { return g_err.c_str(); }
 void mfh_default_options(mfh_options* o) {
   o->device = 0;
   o->block_size = 256;
-  o->events_per_thread = 64;
+  o->events_per_thread = 32;
   o->grid = 0;
   o->fast_math = 1;
-  o->accumulate = 1;
+  o->accumulate = 2;
   o->centre = 1;
   o->unroll = 8;
 }
@@ -349,17 +349,16 @@ int mfh_batch_eval(mfh_batch* b, int kind, const double* theta, int use_graph, d
   MFH_CATCH
 }
 
-int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, double* ms_per_launch) {
+int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, int main_only, double* ms_per_launch) {
   MFH_TRY
   b->with([&](auto& bb) {
     if (kind < 1 || kind > 3 || bb.m->nout[kind] == 0) throw std::invalid_argument("bad evaluation kind");
     bb.set_modes(kind, true);
-    bb.prep();
-    hip_check(mf_memcpy_h2d(bb.dev, bb.segs_d[kind].ptr, bb.segs_hk[kind].ptr, bb.S * sizeof(mf_segment)), "segs");
-    bb.launch_kernel(kind);  // warm-up
+    bb.prep(kind);
+    bb.launch_kernel(kind, main_only != 0);  // warm-up
     hip_check(mf_sync(bb.dev), "sync");
     hip_check(mf_timer_begin(bb.dev), "timer");
-    for (int i = 0; i < iters; i++) bb.launch_kernel(kind);
+    for (int i = 0; i < iters; i++) bb.launch_kernel(kind, main_only != 0);
     float ms = 0;
     hip_check(mf_timer_end(bb.dev, &ms), "timer");
     *ms_per_launch = (double)ms / iters;

@@ -78,8 +78,10 @@ __device__ __forceinline__ double mfd_pow(double x, double y) { return pow(x, y)
 
 // ------------------------------------------------------------------ accumulation
 // Per-lane accumulator of NO outputs.  MF_ACC == 1: Kahan in FP32 (4 VALU ops per output
-// and event), converted to FP64 once per chunk; otherwise every event's value is added in
-// FP64 (1 convert + 1 FP64 add).
+// and event), converted to FP64 once per chunk; otherwise FP64 sums.  With MF_ACC == 0 every
+// event's value is added in FP64 (1 convert + 1 FP64 add); with MF_ACC == 2 the event loop
+// first sums the U events of one unrolled group in FP32 (U - 1 adds) and adds that group sum
+// in FP64, about 1 + 2/U VALU ops per output and event.
 template <int NO>
 struct mf_acc {
 #if MF_FP32 && MF_ACC == 1
@@ -161,12 +163,25 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       for (int u = 0; u < U; u++)
 #pragma unroll
         for (int d = 0; d < MF_NOBS; d++) x[u][d] = __builtin_nontemporal_load(col + d * sg.stride + i + u * MF_BLOCK);
+#if MF_FP32 && MF_ACC == 2
+      MF_T g[NO];
+      F::eval(x[0], P, g);
+#pragma unroll
+      for (int u = 1; u < U; u++) {
+        MF_T o[NO];
+        F::eval(x[u], P, o);
+#pragma unroll
+        for (int k = 0; k < NO; k++) g[k] += o[k];
+      }
+      acc.add(g);
+#else
 #pragma unroll
       for (int u = 0; u < U; u++) {
         MF_T o[NO];
         F::eval(x[u], P, o);
         acc.add(o);
       }
+#endif
     }
     for (; i < ch.end; i += MF_BLOCK) {
       MF_T x[MF_NOBS];
@@ -189,32 +204,60 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
 #pragma unroll
       for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
       a.partials[(long long)c * NO + threadIdx.x] = t;
+      // a segment of one chunk is complete here: its totals need no second pass
+      if (sg.chunk_end - sg.chunk_begin == 1) a.results[sg.out_off + threadIdx.x] = t;
     }
     __syncthreads();
   }
 }
 
-// Segment totals: one wavefront per segment reduces its chunk partials in chunk order
-// (compensated two-sum per lane over a fixed lane-strided assignment, then the fixed
-// butterfly), so a segment's totals are bitwise the same whatever else shares the launch.
+// Segment totals of segments with more than one chunk: one work-group of MF_FINAL_BLOCK
+// threads per segment.  Thread t sums chunks t, t + MF_FINAL_BLOCK, ... with compensated
+// (two-sum) FP64 additions, MF_FINAL_KB outputs at a time; the lanes are combined by the fixed
+// wave64 butterfly and the waves in wave order through LDS.  The order of every addition
+// depends only on the segment's own chunk count, so a segment's totals are bitwise the same
+// whatever else shares the launch.
+#define MF_FINAL_BLOCK 256
+#define MF_FINAL_KB 8
 template <int NO>
 __device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
+  __shared__ double s_red[MF_FINAL_BLOCK / 64][MF_FINAL_KB];
   const int s = blockIdx.x;
   if (s >= a.nseg) return;
   const mf_segment sg = a.segs[s];
-  if (sg.mode == 0) return;
-  const int lane = threadIdx.x;
   const int nch = sg.chunk_end - sg.chunk_begin;
-  for (int k = 0; k < NO; k++) {
-    double acc = 0.0, err = 0.0;
-    for (int j = lane; j < nch; j += 64) {
-      double t, e;
-      mf_two_sum(acc, a.partials[(long long)(sg.chunk_begin + j) * NO + k], t, e);
-      acc = t;
-      err += e;
+  if (sg.mode == 0 || nch <= 1) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const double* __restrict__ part = a.partials + (long long)sg.chunk_begin * NO;
+  for (int k0 = 0; k0 < NO; k0 += MF_FINAL_KB) {
+    double acc[MF_FINAL_KB], err[MF_FINAL_KB];
+#pragma unroll
+    for (int kk = 0; kk < MF_FINAL_KB; kk++) acc[kk] = err[kk] = 0.0;
+#pragma unroll 4
+    for (int j = t; j < nch; j += MF_FINAL_BLOCK) {
+#pragma unroll
+      for (int kk = 0; kk < MF_FINAL_KB; kk++) {
+        if (k0 + kk < NO) {
+          double u, e;
+          mf_two_sum(acc[kk], part[(long long)j * NO + k0 + kk], u, e);
+          acc[kk] = u;
+          err[kk] += e;
+        }
+      }
     }
-    const double tot = mf_wave_sum(acc + err);
-    if (lane == 0) a.results[sg.out_off + k] = tot;
+#pragma unroll
+    for (int kk = 0; kk < MF_FINAL_KB; kk++) {
+      const double w = mf_wave_sum(acc[kk] + err[kk]);
+      if (lane == 0) s_red[wave][kk] = w;
+    }
+    __syncthreads();
+    if (t < MF_FINAL_KB && k0 + t < NO) {
+      double tot = 0.0;
+#pragma unroll
+      for (int w = 0; w < MF_FINAL_BLOCK / 64; w++) tot += s_red[w][t];
+      a.results[sg.out_off + k0 + t] = tot;
+    }
+    __syncthreads();
   }
 }
 
@@ -245,20 +288,24 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
 
 #define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL)                                                                    \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL>(a); } \
-  extern "C" __global__ void __launch_bounds__(64) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR::nout>(a); }
+  extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR::nout>(a); }
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }
 
 // ------------------------------------------------------------------ parameter preparation
 // One thread per segment: hoisted parameter-only terms in FP64 from the segment's floating
 // parameters, then [theta, hoisted] rounded once to MF_T into the kernel parameter block.
+// theta may live in mapped host memory; with seg_src the segment's table entry is staged
+// from there into the device table the evaluation kernels read.
 __device__ void mf_hoist(const double* th, double* buf);
 
 #define MF_PREP_KERNEL                                                                                  \
-  extern "C" __global__ void mf_prep(const double* theta, MF_T* params, const int* seg_list, int nseg) { \
+  extern "C" __global__ void mf_prep(const double* theta, MF_T* params, const int* seg_list, int nseg,  \
+                                     const mf_segment* seg_src, mf_segment* seg_dst) {                  \
     const int i = blockIdx.x * blockDim.x + threadIdx.x;                                                \
     if (i >= nseg) return;                                                                              \
     const int s = seg_list ? seg_list[i] : i;                                                           \
+    if (seg_src) seg_dst[s] = seg_src[s];                                                               \
     const double* th = theta + (long long)s * MF_NTHETA;                                                \
     MF_T* out = params + (long long)s * MF_NPARAM;                                                      \
     double hb[MF_NHOIST > 0 ? MF_NHOIST : 1];                                                           \

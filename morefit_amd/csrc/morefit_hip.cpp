@@ -468,6 +468,23 @@ int mf_host_alloc(mf_device* dev, size_t bytes, void** hptr) {
   return MF_OK;
 }
 
+int mf_host_alloc_mapped(mf_device* dev, size_t bytes, void** hptr, void** dptr) {
+  MF_INJECT("mf_host_alloc_mapped");
+  if (!dev || !hptr || !dptr) return set_error(MF_ERR_INVALID, "mf_host_alloc_mapped: null argument");
+  *hptr = *dptr = nullptr;
+  if (bytes == 0) return MF_OK;
+  MF_HIP(hipSetDevice(dev->ordinal), "mf_host_alloc_mapped");
+  hipError_t e = hipHostMalloc(hptr, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+  if (e != hipSuccess) return set_error(MF_ERR_NOMEM, "mf_host_alloc_mapped(%zu): %s", bytes, hipGetErrorString(e));
+  e = hipHostGetDevicePointer(dptr, *hptr, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(*hptr);
+    *hptr = nullptr;
+    return set_error(MF_ERR_HIP, "mf_host_alloc_mapped: hipHostGetDevicePointer: %s", hipGetErrorString(e));
+  }
+  return MF_OK;
+}
+
 int mf_host_free(mf_device* dev, void* hptr) {
   if (!dev) return set_error(MF_ERR_INVALID, "mf_host_free: null device");
   if (!hptr) return MF_OK;
@@ -652,19 +669,21 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval");
   std::string fin = std::string(kernel) + "_final";
   hipFunction_t ff;
-  if (args->nseg > 0 && mod_has(mod, fin.c_str(), &ff))
-    MF_HIP(hipModuleLaunchKernel(ff, args->nseg, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval final");
+  if (args->nseg > 0 && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff))
+    MF_HIP(hipModuleLaunchKernel(ff, args->nseg, 1, 1, 256, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval final");
   return MF_OK;
 }
 
-int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg) {
+int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg,
+                   const mf_segment* seg_src, mf_segment* seg_dst) {
   MF_INJECT("mf_launch_prep");
-  if (!mod || !theta || !params || nseg < 0) return set_error(MF_ERR_INVALID, "mf_launch_prep: bad argument");
+  if (!mod || !theta || !params || nseg < 0 || (seg_src && !seg_dst))
+    return set_error(MF_ERR_INVALID, "mf_launch_prep: bad argument");
   if (nseg == 0) return MF_OK;
   hipFunction_t fn;
   int rc = get_fn(mod, "mf_prep", &fn);
   if (rc != MF_OK) return rc;
-  void* kp[] = {(void*)&theta, (void*)&params, (void*)&seg_list, (void*)&nseg};
+  void* kp[] = {(void*)&theta, (void*)&params, (void*)&seg_list, (void*)&nseg, (void*)&seg_src, (void*)&seg_dst};
   int grid = (nseg + 63) / 64;
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep");
   return MF_OK;

@@ -45,11 +45,11 @@ struct compute_options {
   // MI355X backend
   int hip_device = 0;
   int hip_block_size = 256;          // work-group size of the evaluation kernels
-  int hip_events_per_thread = 64;    // chunk = block * events_per_thread events
+  int hip_events_per_thread = 32;    // chunk = block * events_per_thread events
   int hip_grid = 0;                  // 0: one work-group per chunk, >0: persistent grid of that many
   int hip_unroll = 8;                // events per thread whose loads are issued together
   bool hip_fast_math = true;         // FP32: hardware exp2/log2/rcp (error measured against the oracle)
-  int hip_accumulate = 1;            // FP32: 1 = compensated FP32 per lane, 0 = FP64 per event
+  int hip_accumulate = 2;            // FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group + FP64
   bool hip_centre_observables = true;// FP32: store observables as offsets from the range centre
   int hip_generator_events_per_stream = 64;
   std::string hip_compile_options;   // extra hipRTC options
@@ -66,7 +66,8 @@ struct compute_options {
               << std::setw(w) << std::left << "  hip events per thread " << hip_events_per_thread << "\n"
               << std::setw(w) << std::left << "  hip grid " << hip_grid << "\n"
               << std::setw(w) << std::left << "  hip fast math " << (hip_fast_math ? "YES" : "NO") << "\n"
-              << std::setw(w) << std::left << "  hip accumulate " << (hip_accumulate ? "FP32 compensated" : "FP64") << "\n"
+              << std::setw(w) << std::left << "  hip accumulate "
+              << (hip_accumulate == 1 ? "FP32 compensated" : hip_accumulate == 2 ? "FP32 groups + FP64" : "FP64") << "\n"
               << std::setw(w) << std::left << "  cpu threads " << llvm_nthreads << std::endl;
   }
 };

@@ -125,6 +125,7 @@ struct DevBuf {
 struct HostBuf {
   mf_device* dev = nullptr;
   void* ptr = nullptr;
+  void* dptr = nullptr;  // device address of mapped buffers (alloc_mapped), else null
   size_t bytes = 0;
   HostBuf() {}
   ~HostBuf() { release(); }
@@ -137,9 +138,17 @@ struct HostBuf {
     hip_check(mf_host_alloc(d, n, &ptr), "mf_host_alloc");
     bytes = n;
   }
+  // coherent pinned memory that kernels read and write in place through dptr
+  void alloc_mapped(mf_device* d, size_t n) {
+    if (d == dev && n <= bytes && ptr && dptr) return;
+    release();
+    dev = d;
+    hip_check(mf_host_alloc_mapped(d, n, &ptr, &dptr), "mf_host_alloc_mapped");
+    bytes = n;
+  }
   void release() {
     if (ptr) mf_host_free(dev, ptr);
-    ptr = nullptr;
+    ptr = dptr = nullptr;
     bytes = 0;
   }
   template <typename T> T* as() const { return (T*)ptr; }
@@ -413,6 +422,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     a.params = s.params.ptr;
     a.partials = s.partials.template as<double>();
     a.results = s.results.template as<double>();
+    if (nchunks_ <= 1) a.flags |= MF_EVAL_SKIP_FINAL;  // one chunk: the per-chunk kernel writes the totals
     if (!s.kahan) {
       if (!s.output) throw std::runtime_error("HIPBlock::SubmitKernel: store kernel needs an output buffer");
       a.store = s.output->ptr;
@@ -685,9 +695,12 @@ class HipBatch {
   ArenaAllocator arena;
   long long total_elems = 0;             // arena capacity in elements
   int chunk = 0, nchunks = 0, chunk_cap = 0;
+  int max_seg_chunks = 0;                // most chunks of any active segment (<= 1: no final pass)
   bool dirty = true;
   std::vector<mf_segment> segs_h;
-  DevBuf data, theta_d, params_d, segs_d[4], chunks_d, partials_d, results_d[4];
+  DevBuf data, params_d, segs_d[4], chunks_d, partials_d;
+  // staging in mapped host memory, read and written by the kernels in place: theta and the
+  // per-kind segment tables (modes) in, totals out -- a call's graph holds kernels only
   HostBuf theta_h, segs_hk[4], results_h[4];
   mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> request_;
@@ -720,8 +733,7 @@ class HipBatch {
     active.assign(S, 0);
     segs_h.assign(S, mf_segment{});
     data.alloc(dev, std::max<long long>(1, elems) * sizeof(kernelT));
-    theta_d.alloc(dev, std::max(1, S * m->ntheta) * sizeof(double));
-    theta_h.alloc(dev, std::max(1, S * m->ntheta) * sizeof(double));
+    theta_h.alloc_mapped(dev, std::max(1, S * m->ntheta) * sizeof(double));
     params_d.alloc(dev, std::max(1, S * m->nparam) * sizeof(kernelT));
     chunk_cap = (int)(elems / m->nobs / chunk) + 2 * S + 1;
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
@@ -730,9 +742,8 @@ class HipBatch {
     for (int k = 1; k <= 3; k++) {
       if (m->nout[k] == 0) continue;
       segs_d[k].alloc(dev, std::max(1, S) * sizeof(mf_segment));
-      segs_hk[k].alloc(dev, std::max(1, S) * sizeof(mf_segment));
-      results_d[k].alloc(dev, (size_t)std::max(1, S) * m->nout[k] * sizeof(double));
-      results_h[k].alloc(dev, (size_t)std::max(1, S) * m->nout[k] * sizeof(double));
+      segs_hk[k].alloc_mapped(dev, std::max(1, S) * sizeof(mf_segment));
+      results_h[k].alloc_mapped(dev, (size_t)std::max(1, S) * m->nout[k] * sizeof(double));
       std::memset(results_h[k].ptr, 0, results_h[k].bytes);
     }
     std::memset(theta_h.ptr, 0, theta_h.bytes);
@@ -784,6 +795,7 @@ class HipBatch {
   void rebuild() {
     if (!dirty) return;
     std::vector<mf_chunk> ch;
+    max_seg_chunks = 0;
     for (int s = 0; s < S; s++) {
       mf_segment sg{};
       sg.data_off = offsets[s];
@@ -792,6 +804,7 @@ class HipBatch {
       sg.param_off = s * m->nparam;
       sg.mode = 0;
       if (active[s]) append_chunks(ch, sg, s, sizes[s], chunk);
+      max_seg_chunks = std::max(max_seg_chunks, sg.chunk_end - sg.chunk_begin);
       segs_h[s] = sg;
     }
     if ((int)ch.size() > chunk_cap) throw std::runtime_error("HipBatch: chunk table over capacity");
@@ -838,10 +851,12 @@ class HipBatch {
       for (long long i = 0; i < n; i++) cols[d * n + i] = (double)tmp[d * n + i] + m->centres[d];
   }
 
-  // prepare kernel parameters of every slot from the staged theta
-  void prep() {
-    hip_check(mf_memcpy_h2d(dev, theta_d.ptr, theta_h.ptr, (size_t)S * m->ntheta * sizeof(double)), "HipBatch theta");
-    hip_check(mf_launch_prep(m->mod, theta_d.template as<double>(), params_d.ptr, nullptr, S), "mf_launch_prep");
+  // kernel parameters of every slot from the staged theta; with kind > 0 the same launch
+  // stages that kind's segment table (modes) into device memory
+  void prep(int kind = 0) {
+    const mf_segment* src = kind > 0 ? (const mf_segment*)segs_hk[kind].dptr : nullptr;
+    mf_segment* dst = kind > 0 ? segs_d[kind].template as<mf_segment>() : nullptr;
+    hip_check(mf_launch_prep(m->mod, (const double*)theta_h.dptr, params_d.ptr, nullptr, S, src, dst), "mf_launch_prep");
   }
 
   // On-device accept-reject into the listed slots (all active slots when `slots` is empty).
@@ -918,21 +933,23 @@ class HipBatch {
     a.nseg = S;
     a.params = params_d.ptr;
     a.partials = partials_d.template as<double>();
-    a.results = results_d[kind].template as<double>();
+    a.results = (double*)results_h[kind].dptr;
+    if (max_seg_chunks <= 1) a.flags |= MF_EVAL_SKIP_FINAL;
     return a;
   }
-  // just the evaluation kernel of `kind` (segment table and parameters already on device)
-  void launch_kernel(int kind) {
+  // just the evaluation kernels of `kind` (segment table and parameters already on device);
+  // main_only launches the per-chunk kernel without the final pass (for timing it alone)
+  void launch_kernel(int kind, bool main_only = false) {
     if (nchunks == 0) return;
     mf_eval_args a = eval_args(kind);
+    if (main_only) a.flags |= MF_EVAL_SKIP_FINAL;
     hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->be->opts_->hip_block_size), "mf_launch_eval");
   }
-  // the full per-call sequence of one kind: theta -> prep -> segment modes -> kernel -> totals
+  // the full per-call sequence of one kind: prep (theta and segment modes from mapped host
+  // memory) -> per-chunk kernel -> final pass (totals into mapped host memory)
   void enqueue(int kind) {
-    prep();
-    hip_check(mf_memcpy_h2d(dev, segs_d[kind].ptr, segs_hk[kind].ptr, S * sizeof(mf_segment)), "HipBatch segs");
+    prep(kind);
     launch_kernel(kind);
-    hip_check(mf_memcpy_d2h(dev, results_h[kind].ptr, results_d[kind].ptr, (size_t)S * m->nout[kind] * sizeof(double)), "HipBatch results");
   }
   void set_modes(int kind, bool all) {
     mf_segment* t = segs_hk[kind].template as<mf_segment>();

@@ -74,6 +74,32 @@ def test_eval_matches_oracle(ctx, model, fp32):
     mdl.close()
 
 
+@pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
+@pytest.mark.parametrize("accumulate", [0, 1, 2])
+def test_fp32_accumulation_modes(model, accumulate):
+    """FP32 kernels with each per-lane accumulation (0: FP64 per event, 1: Kahan FP32, 2: FP32
+    sums of each unrolled group added in FP64), at a size with many chunks per segment and
+    a ragged tail: every sum within 2e-6 of the sum of absolute per-event values"""
+    n = 1_000_003
+    data = O.generate(model, M.TRUTH[model], n, seed=77, nstreams=64)
+    theta = np.array(M.START[model])
+    say("fp32 accumulation", accumulate, "model", model, "n", n)
+    cx = M.Context(accumulate=accumulate)
+    mdl = cx.model(model, fp32=True)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    stored = _stored(model, data, True)
+    ref = O.nll_sums(model, stored, theta, mode=2, nthreads=8)
+    absref = O.nll_sums(model, stored, theta, mode=12, nthreads=8)
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        out = b.eval(kind, theta)[0]
+        k = len(out)
+        assert np.all(np.abs(out - ref[:k]) <= 2e-6 * absref[:k]), (kind, (out - ref[:k]) / absref[:k])
+    b.close()
+    mdl.close()
+    cx.close()
+
+
 def test_graph_replays_match_eager(ctx):
     """hipGraph replays (several, with eager work between them) == eager launches, bitwise"""
     n = 300000
