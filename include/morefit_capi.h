@@ -81,6 +81,12 @@ int mfh_model_precompile(int kind, int fp32, int with_hessian, const mfh_options
  * scratch bytes per lane.  fit = 1 fits each kernel's unroll to the register budget (as a
  * model does), fit = 0 compiles every kernel at opts->unroll as given. */
 int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* opts, int fit, int* info);
+/* without a device: the per-event outputs of the value (which = 1), gradient (2) or Hessian (3)
+ * kernel of a configuration module exactly as generated (observables centred for fp32, base-2
+ * transcendentals, hoisted terms, output scales applied), evaluated on the host in FP64 at
+ * theta for events data (SoA nobs x n): out[i * nout + k].  Checks the code generator's
+ * rewrites against the density's definition. */
+int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double* out);
 
 int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out);
 int mfh_batch_destroy(mfh_batch* b);

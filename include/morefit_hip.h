@@ -89,7 +89,9 @@ typedef struct mf_eval_args {
   const void* params;         /* T* kernel parameters */
   double* partials;           /* [nchunks][nout] */
   double* results;            /* [..][nout] per segment at out_off */
-  unsigned int* reserved;     /* unused, NULL */
+  const double* hoist64;      /* FP64 hoisted terms [segment][nhoist] from mf_prep: totals of outputs
+                                 with a scale (functor scale(k) >= 0) are multiplied by
+                                 hoist64[seg * nhoist + scale(k)]; NULL: no scaling */
   void* store;                /* T* per-event outputs (store kernels only) or NULL */
   long long store_stride;     /* column stride of `store` */
   int flags;                  /* MF_EVAL_* bits */
@@ -198,9 +200,11 @@ double mf_module_compile_ms(mf_module* mod);  /* 0 when served from a cache */
 /* launches (on the device stream) */
 int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
 /* kernel parameters of segments seg_list[0..nseg) (all 0..nseg-1 when seg_list is NULL) from
- * their FP64 theta; with seg_src set, each of those segments' table entries is also copied
- * seg_src[s] -> seg_dst[s] (a mapped host table staged into device memory in the same launch) */
-int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg,
+ * their FP64 theta: params[s * (ntheta + nhoist) ...] in the module's type and, when hoist64 is
+ * set, the hoisted terms in FP64 at hoist64[s * nhoist ...].  With seg_src set, each of those
+ * segments' table entries is also copied seg_src[s] -> seg_dst[s] (a mapped host table staged
+ * into device memory in the same launch). */
+int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* hoist64, const int* seg_list, int nseg,
                    const mf_segment* seg_src, mf_segment* seg_dst);
 int mf_launch_generate(mf_module* mod, const mf_gen_args* args, int grid, int block);
 

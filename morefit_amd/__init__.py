@@ -71,7 +71,7 @@ HIP_SYMBOLS = [
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
-    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels",
+    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run",
@@ -101,6 +101,7 @@ def libs():
         L.mfh_model_source.argtypes = [vp, ctypes.c_char_p, ip]
         L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
         L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
+        L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, dp]
         L.mfh_batch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, pp]
         L.mfh_batch_destroy.argtypes = [vp]
         L.mfh_batch_upload.argtypes = [vp, ip, dp, ctypes.c_longlong]
@@ -151,6 +152,17 @@ def kernel_resources(kind, fp32, with_hessian=True, fit=True, **kw):
     _check(libs()[1].mfh_model_kernels(kind, int(fp32), int(with_hessian), ctypes.byref(options(**kw)), int(fit), info))
     return {k: {"unroll": info[j], "vgpr": info[3 + j], "agpr": info[6 + j], "scratch": info[9 + j]}
             for j, k in enumerate((EVAL_LH, EVAL_GRAD, EVAL_HESS))}
+
+
+def host_outputs(kind, fp32, which, theta, data):
+    """Per-event outputs of a configuration module's value/gradient/Hessian kernel (which = 1/2/3)
+    as generated, evaluated on the host in FP64 (no device): array (n, nout)."""
+    data = _f64(np.atleast_2d(data))
+    n = data.shape[1]
+    nout = {1: 1, 2: 1 + NTHETA[kind], 3: 1 + NTHETA[kind] + NTHETA[kind] * (NTHETA[kind] + 1) // 2}[which]
+    out = np.zeros(n * nout)
+    _check(libs()[1].mfh_model_host_outputs(kind, int(fp32), which, _dp(_f64(theta)), _dp(data), n, _dp(out)))
+    return out.reshape(n, nout)
 
 
 def precompile(kind, fp32, with_hessian=True, **kw):

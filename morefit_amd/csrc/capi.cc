@@ -288,6 +288,42 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
   MFH_CATCH
 }
 
+int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double* out) {
+  MFH_TRY
+  if (which < 1 || which > 3) throw std::invalid_argument("which must be 1, 2 or 3");
+  auto run = [&](auto& d) {
+    typedef typename std::remove_reference<decltype(d)>::type::kernel_type K;
+    LikelihoodGraphs<K> g(d.pdf, d.params, fp32 != 0, true, fp32 != 0);
+    std::vector<std::string> names(g.theta);
+    std::vector<double> vals(theta, theta + g.theta.size());
+    const size_t nt = names.size();
+    std::vector<double> hv;
+    for (auto& e : g.hoist_exprs) hv.push_back(e->eval(std::vector<std::string>(names.begin(), names.begin() + nt),
+                                                       std::vector<double>(vals.begin(), vals.begin() + nt)));
+    names.insert(names.end(), g.hoist_names.begin(), g.hoist_names.end());
+    vals.insert(vals.end(), hv.begin(), hv.end());
+    const size_t off = names.size();
+    names.insert(names.end(), g.obs.begin(), g.obs.end());
+    vals.resize(names.size());
+    const auto& outs = which == 1 ? g.lh : (which == 2 ? g.grad : g.hess);
+    const auto& sc = which == 1 ? g.lh_scale : (which == 2 ? g.grad_scale : g.hess_scale);
+    const size_t no = outs.size();
+    for (long long i = 0; i < n; i++) {
+      for (size_t dd = 0; dd < g.obs.size(); dd++) vals[off + dd] = data[dd * n + i] - g.centres[dd];
+      for (size_t k = 0; k < no; k++)
+        out[i * no + k] = (double)outs[k]->eval(names, vals) * (sc[k] >= 0 ? hv[sc[k]] : 1.0);
+    }
+  };
+  if (fp32) {
+    ModelDef<float> d(kind, nullptr);
+    run(d);
+  } else {
+    ModelDef<double> d(kind, nullptr);
+    run(d);
+  }
+  MFH_CATCH
+}
+
 int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out) {
   MFH_TRY
   std::vector<long long> s(sizes, sizes + nseg);

@@ -34,10 +34,14 @@
 __device__ __forceinline__ float mf_exp(float x) { return __expf(x); }
 __device__ __forceinline__ float mf_log(float x) { return __logf(x); }
 __device__ __forceinline__ float mf_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float mf_exp2(float x) { return __builtin_amdgcn_exp2f(x); }  // v_exp_f32
+__device__ __forceinline__ float mf_log2(float x) { return __builtin_amdgcn_logf(x); }   // v_log_f32
 #else
 __device__ __forceinline__ float mf_exp(float x) { return expf(x); }
 __device__ __forceinline__ float mf_log(float x) { return logf(x); }
 __device__ __forceinline__ float mf_rcp(float x) { return __builtin_amdgcn_rcpf(x); }  // 1 ulp
+__device__ __forceinline__ float mf_exp2(float x) { return exp2f(x); }
+__device__ __forceinline__ float mf_log2(float x) { return log2f(x); }
 #endif
 __device__ __forceinline__ float mf_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ float mf_sin(float x) { return sinf(x); }
@@ -51,6 +55,8 @@ __device__ __forceinline__ float mf_pow(float x, float y) { return powf(x, y); }
 #else
 __device__ __forceinline__ double mf_exp(double x) { return exp(x); }
 __device__ __forceinline__ double mf_log(double x) { return log(x); }
+__device__ __forceinline__ double mf_exp2(double x) { return exp2(x); }
+__device__ __forceinline__ double mf_log2(double x) { return log2(x); }
 __device__ __forceinline__ double mf_rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ double mf_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ double mf_sin(double x) { return sin(x); }
@@ -64,6 +70,8 @@ __device__ __forceinline__ double mf_pow(double x, double y) { return pow(x, y);
 #endif
 // FP64 versions for the hoisted terms (always double)
 __device__ __forceinline__ double mfd_exp(double x) { return exp(x); }
+__device__ __forceinline__ double mfd_exp2(double x) { return exp2(x); }
+__device__ __forceinline__ double mfd_log2(double x) { return log2(x); }
 __device__ __forceinline__ double mfd_log(double x) { return log(x); }
 __device__ __forceinline__ double mfd_rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ double mfd_sqrt(double x) { return sqrt(x); }
@@ -133,6 +141,14 @@ __device__ __forceinline__ void mf_two_sum(double a, double b, double& s, double
   s = a + b;
   double bb = s - a;
   e = (a - (s - bb)) + (b - bb);
+}
+
+// FP64 factor of output k's total for segment `seg`: the parameter-only factor the code
+// generator split off the per-event term (functor scale(k) >= 0), else 1
+template <class F>
+__device__ __forceinline__ double mf_output_scale(const mf_eval_args& a, int seg, int k) {
+  const int si = F::scale(k);
+  return (si >= 0 && a.hoist64) ? a.hoist64[(long long)seg * MF_NHOIST + si] : 1.0;
 }
 
 // ------------------------------------------------------------------ fused evaluation
@@ -205,7 +221,7 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
       a.partials[(long long)c * NO + threadIdx.x] = t;
       // a segment of one chunk is complete here: its totals need no second pass
-      if (sg.chunk_end - sg.chunk_begin == 1) a.results[sg.out_off + threadIdx.x] = t;
+      if (sg.chunk_end - sg.chunk_begin == 1) a.results[sg.out_off + threadIdx.x] = t * mf_output_scale<F>(a, ch.seg, threadIdx.x);
     }
     __syncthreads();
   }
@@ -219,8 +235,9 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
 // whatever else shares the launch.
 #define MF_FINAL_BLOCK 256
 #define MF_FINAL_KB 8
-template <int NO>
+template <class F>
 __device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
+  constexpr int NO = F::nout;
   __shared__ double s_red[MF_FINAL_BLOCK / 64][MF_FINAL_KB];
   const int s = blockIdx.x;
   if (s >= a.nseg) return;
@@ -255,7 +272,7 @@ __device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
       double tot = 0.0;
 #pragma unroll
       for (int w = 0; w < MF_FINAL_BLOCK / 64; w++) tot += s_red[w][t];
-      a.results[sg.out_off + k0 + t] = tot;
+      a.results[sg.out_off + k0 + t] = tot * mf_output_scale<F>(a, s, k0 + t);
     }
     __syncthreads();
   }
@@ -288,30 +305,34 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
 
 #define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL)                                                                    \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL>(a); } \
-  extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR::nout>(a); }
+  extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR>(a); }
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }
 
 // ------------------------------------------------------------------ parameter preparation
 // One thread per segment: hoisted parameter-only terms in FP64 from the segment's floating
 // parameters, then [theta, hoisted] rounded once to MF_T into the kernel parameter block.
-// theta may live in mapped host memory; with seg_src the segment's table entry is staged
-// from there into the device table the evaluation kernels read.
+// theta may live in mapped host memory (read once, before anything is stored); with seg_src
+// the segment's table entry is staged from there into the device table the evaluation kernels
+// read; with hoist64 the hoisted terms are also kept in FP64 (output scales, mf_eval_args).
 __device__ void mf_hoist(const double* th, double* buf);
 
 #define MF_PREP_KERNEL                                                                                  \
-  extern "C" __global__ void mf_prep(const double* theta, MF_T* params, const int* seg_list, int nseg,  \
-                                     const mf_segment* seg_src, mf_segment* seg_dst) {                  \
+  extern "C" __global__ void mf_prep(const double* theta, MF_T* params, double* hoist64, const int* seg_list, \
+                                     int nseg, const mf_segment* seg_src, mf_segment* seg_dst) {        \
     const int i = blockIdx.x * blockDim.x + threadIdx.x;                                                \
     if (i >= nseg) return;                                                                              \
     const int s = seg_list ? seg_list[i] : i;                                                           \
+    double th[MF_NTHETA > 0 ? MF_NTHETA : 1];                                                           \
+    for (int k = 0; k < MF_NTHETA; k++) th[k] = theta[(long long)s * MF_NTHETA + k];                    \
     if (seg_src) seg_dst[s] = seg_src[s];                                                               \
-    const double* th = theta + (long long)s * MF_NTHETA;                                                \
     MF_T* out = params + (long long)s * MF_NPARAM;                                                      \
     double hb[MF_NHOIST > 0 ? MF_NHOIST : 1];                                                           \
     mf_hoist(th, hb);                                                                                   \
     for (int k = 0; k < MF_NTHETA; k++) out[k] = (MF_T)th[k];                                           \
     for (int j = 0; j < MF_NHOIST; j++) out[MF_NTHETA + j] = (MF_T)hb[j];                               \
+    if (hoist64)                                                                                        \
+      for (int j = 0; j < MF_NHOIST; j++) hoist64[(long long)s * MF_NHOIST + j] = hb[j];                \
   }
 
 // ------------------------------------------------------------------ generator
@@ -333,7 +354,8 @@ __device__ __forceinline__ MF_T mf_uniform(unsigned int* s) { return (MF_T)(mf_x
 
 // Accept-reject (compute_opencl.hh:540-575): propose each observable uniformly in the box,
 // accept when maxprob * u < p(x).  Each stream fills its own fixed event range, so the
-// output does not depend on scheduling.  L is the log-density functor (nout == 1).
+// output does not depend on scheduling.  L is the log-density functor (nout == 1, its value
+// the log density itself: no output scale).
 template <class L>
 __device__ __forceinline__ void mf_generate_impl(const mf_gen_args& g) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

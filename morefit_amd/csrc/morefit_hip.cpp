@@ -674,7 +674,7 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
   return MF_OK;
 }
 
-int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int* seg_list, int nseg,
+int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* hoist64, const int* seg_list, int nseg,
                    const mf_segment* seg_src, mf_segment* seg_dst) {
   MF_INJECT("mf_launch_prep");
   if (!mod || !theta || !params || nseg < 0 || (seg_src && !seg_dst))
@@ -683,7 +683,7 @@ int mf_launch_prep(mf_module* mod, const double* theta, void* params, const int*
   hipFunction_t fn;
   int rc = get_fn(mod, "mf_prep", &fn);
   if (rc != MF_OK) return rc;
-  void* kp[] = {(void*)&theta, (void*)&params, (void*)&seg_list, (void*)&nseg, (void*)&seg_src, (void*)&seg_dst};
+  void* kp[] = {(void*)&theta, (void*)&params, (void*)&hoist64, (void*)&seg_list, (void*)&nseg, (void*)&seg_src, (void*)&seg_dst};
   int grid = (nseg + 63) / 64;
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep");
   return MF_OK;

@@ -72,15 +72,17 @@ struct ModuleSource {
   }
 };
 
-// struct NAME { nout; eval(x, p, o) } computing `outputs` from observables x[d] and kernel
-// parameters p[k]
+// struct NAME { nout; eval(x, p, o); scale(k) } computing `outputs` from observables x[d] and
+// kernel parameters p[k].  scale(k) is the index, among the segment's FP64 hoisted terms, of
+// the factor output k's total is multiplied by (-1: none); see split_output_scale.
 template <typename kernelT, typename evalT>
 std::string emit_functor(const std::string& name, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& outputs,
                          const std::vector<std::string>& observables, const std::vector<std::string>& param_names,
-                         bool fp32, size_t* statements = nullptr) {
+                         bool fp32, size_t* statements = nullptr, const std::vector<int>* scale_index = nullptr) {
   KernelEmitter em(fp32);
   for (size_t d = 0; d < observables.size(); d++) em.bindings[observables[d]] = "x[" + std::to_string(d) + "]";
   for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
+  for (auto* g : outputs) em.count_products(*g);
   std::vector<std::string> res;
   for (auto* g : outputs) res.push_back(em.emit(*g));
   std::ostringstream s;
@@ -89,7 +91,18 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
     << "    (void)x; (void)p;\n"
     << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << res[k] << ";\n";
-  s << "  }\n};\n";
+  s << "  }\n  __device__ __forceinline__ static int scale(int k) {";
+  bool any = false;
+  if (scale_index)
+    for (int v : *scale_index) any |= v >= 0;
+  if (!any) {
+    s << " (void)k; return -1; }\n";
+  } else {
+    s << " const int s_[" << outputs.size() << "] = {";
+    for (size_t k = 0; k < outputs.size(); k++) s << (k ? ", " : "") << (k < scale_index->size() ? (*scale_index)[k] : -1);
+    s << "}; return s_[k]; }\n";
+  }
+  s << "};\n";
   if (statements) *statements += (size_t)em.statements;
   return s.str();
 }

@@ -58,7 +58,7 @@ inline std::string cpu_source(const ModuleSource& ms, const std::string& kernel_
   s << "#include <cmath>\n#include <cstddef>\n"
     << "#define __device__\n#define __forceinline__ inline\n"
     << "typedef " << (ms.fp32 ? "float" : "double") << " MF_T;\n";
-  const char* fns[] = {"exp", "log", "sqrt", "sin", "cos", "tan", "asin", "acos", "atan", "erf"};
+  const char* fns[] = {"exp", "log", "exp2", "log2", "sqrt", "sin", "cos", "tan", "asin", "acos", "atan", "erf"};
   for (auto* f : fns) {
     s << "static inline MF_T mf_" << f << "(MF_T x) { return std::" << f << "(x); }\n";
     s << "static inline double mfd_" << f << "(double x) { return std::" << f << "(x); }\n";

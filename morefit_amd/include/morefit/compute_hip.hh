@@ -450,12 +450,15 @@ struct LikelihoodGraphs {
   std::vector<std::string> obs, theta;
   std::vector<double> centres, lo, hi;
   std::vector<Ptr> lh, grad, hess;           // per kind outputs (grad/hess include value first)
+  std::vector<int> lh_scale, grad_scale, hess_scale;  // per output: hoisted term scaling its total, or -1
+  Ptr gen;                                   // log p for the generator, hoisted but otherwise as written
   std::vector<std::string> hoist_names;
   std::vector<Ptr> hoist_exprs;
   size_t nodes = 0;
 
+  // base2: rewrite exp/log into the hardware's exp2/log2 (FP32 kernels), see to_base2
   LikelihoodGraphs(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, bool centre, bool with_hessian,
-                   double hoist_threshold = 0.0) {
+                   bool base2 = false, double hoist_threshold = 0.0) {
     std::vector<std::string> fixed;
     std::vector<double> fixed_v;
     for (auto* p : params) {
@@ -480,15 +483,28 @@ struct LikelihoodGraphs {
     if (with_hessian)
       for (size_t i = 0; i < theta.size(); i++)
         for (size_t j = 0; j <= i; j++) d2.emplace_back(factor_common(*d1[i]->diff(theta[j])->simplify()));
-    auto hoist = [&](const Ptr& g) {
-      return g->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, "morefit_parambuffer_", (float)hoist_threshold)->simplify();
+    const std::string prefix = "morefit_parambuffer_";
+    // per output: (base-2 rewrite), parameter-only factor split off as an FP64 hoisted scale
+    // of the total, the rest hoisted as usual
+    auto output = [&](const Ptr& g0, std::vector<int>& scales) {
+      Ptr g = base2 ? to_base2(*g0)->simplify() : g0->copy();
+      Ptr sc;
+      Ptr ev = split_output_scale(*g, theta, sc);
+      int idx = -1;
+      if (sc) {
+        const std::string nm = ComputeGraphNode<kernelT, evalT>::hoist(sc->simplify(), hoist_names, hoist_exprs, prefix);
+        idx = (int)(std::find(hoist_names.begin(), hoist_names.end(), nm) - hoist_names.begin());
+      }
+      scales.push_back(idx);
+      return ev->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, (float)hoist_threshold)->simplify();
     };
-    lh.emplace_back(hoist(lp));
-    grad.emplace_back(hoist(lp));
-    for (auto& g : d1) grad.emplace_back(hoist(g));
-    hess.emplace_back(hoist(lp));
-    for (auto& g : d1) hess.emplace_back(hoist(g));
-    for (auto& g : d2) hess.emplace_back(hoist(g));
+    gen = lp->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, (float)hoist_threshold)->simplify();
+    lh.emplace_back(output(lp, lh_scale));
+    grad.emplace_back(output(lp, grad_scale));
+    for (auto& g : d1) grad.emplace_back(output(g, grad_scale));
+    hess.emplace_back(output(lp, hess_scale));
+    for (auto& g : d1) hess.emplace_back(output(g, hess_scale));
+    for (auto& g : d2) hess.emplace_back(output(g, hess_scale));
     for (auto& g : hess) nodes += g->node_count();
   }
   std::vector<std::string> param_names() const {
@@ -515,14 +531,15 @@ struct LikelihoodGraphs {
       ms.gen_width.push_back(hi[d] - lo[d]);
     }
     auto pn = param_names();
-    ms.functors = emit_functor("mf_f_lh", raw(lh), obs, pn, ms.fp32, &ms.statements) +
-                  emit_functor("mf_f_grad", raw(grad), obs, pn, ms.fp32, &ms.statements);
-    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), obs, pn, ms.fp32, &ms.statements);
+    ms.functors = emit_functor("mf_f_lh", raw(lh), obs, pn, ms.fp32, &ms.statements, &lh_scale) +
+                  emit_functor("mf_f_grad", raw(grad), obs, pn, ms.fp32, &ms.statements, &grad_scale);
+    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), obs, pn, ms.fp32, &ms.statements, &hess_scale);
+    ms.functors += emit_functor<kernelT, evalT>("mf_f_gen", {gen.get()}, obs, pn, ms.fp32, &ms.statements);
     ms.hoist = emit_hoist(raw(hoist_exprs), theta);
     const int u = std::max(1, o.hip_unroll);
     ms.evals = {{"mf_eval_lh", "mf_f_lh", u}, {"mf_eval_grad", "mf_f_grad", u}};
     if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u});
-    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_lh)"};
+    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)"};
     return ms;
   }
 };
@@ -553,7 +570,7 @@ class HipModel {
                                   std::vector<mf_kernel_resources>* res = nullptr, double* compile_ms = nullptr) {
     const bool f32 = std::is_same<kernelT, float>::value;
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> g(
-        new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian));
+        new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian, f32));
     ModuleSource ms = g->module(o, with_hessian);
     auto t0 = std::chrono::steady_clock::now();
     fit_register_budget(ms, o.hip_compile_options, res);
@@ -698,7 +715,7 @@ class HipBatch {
   int max_seg_chunks = 0;                // most chunks of any active segment (<= 1: no final pass)
   bool dirty = true;
   std::vector<mf_segment> segs_h;
-  DevBuf data, params_d, segs_d[4], chunks_d, partials_d;
+  DevBuf data, params_d, hoist64_d, segs_d[4], chunks_d, partials_d;
   // staging in mapped host memory, read and written by the kernels in place: theta and the
   // per-kind segment tables (modes) in, totals out -- a call's graph holds kernels only
   HostBuf theta_h, segs_hk[4], results_h[4];
@@ -735,6 +752,7 @@ class HipBatch {
     data.alloc(dev, std::max<long long>(1, elems) * sizeof(kernelT));
     theta_h.alloc_mapped(dev, std::max(1, S * m->ntheta) * sizeof(double));
     params_d.alloc(dev, std::max(1, S * m->nparam) * sizeof(kernelT));
+    hoist64_d.alloc(dev, (size_t)std::max(1, S) * std::max(1, m->nhoist) * sizeof(double));
     chunk_cap = (int)(elems / m->nobs / chunk) + 2 * S + 1;
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
     int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
@@ -856,7 +874,8 @@ class HipBatch {
   void prep(int kind = 0) {
     const mf_segment* src = kind > 0 ? (const mf_segment*)segs_hk[kind].dptr : nullptr;
     mf_segment* dst = kind > 0 ? segs_d[kind].template as<mf_segment>() : nullptr;
-    hip_check(mf_launch_prep(m->mod, (const double*)theta_h.dptr, params_d.ptr, nullptr, S, src, dst), "mf_launch_prep");
+    hip_check(mf_launch_prep(m->mod, (const double*)theta_h.dptr, params_d.ptr, hoist64_d.template as<double>(), nullptr, S, src, dst),
+              "mf_launch_prep");
   }
 
   // On-device accept-reject into the listed slots (all active slots when `slots` is empty).
@@ -934,6 +953,7 @@ class HipBatch {
     a.params = params_d.ptr;
     a.partials = partials_d.template as<double>();
     a.results = (double*)results_h[kind].dptr;
+    a.hoist64 = hoist64_d.template as<double>();
     if (max_seg_chunks <= 1) a.flags |= MF_EVAL_SKIP_FINAL;
     return a;
   }

@@ -46,7 +46,8 @@ namespace morefit {
 
 enum class NodeKind {
   Constant, Variable, Sum, Prod, Neg, Inv, Exp, Log, Pow, Sqrt,
-  Sin, Cos, Tan, Asin, Acos, Atan, Erf, Conditional
+  Sin, Cos, Tan, Asin, Acos, Atan, Erf, Conditional,
+  Exp2, Log2  // base-2 forms, introduced only by to_base2() for FP32 kernels
 };
 
 template <typename kernelT, typename evalT> class ComputeGraphNode;
@@ -195,6 +196,8 @@ class ComputeGraphNode {
       case NodeKind::Inv: return 1.0 / a;
       case NodeKind::Exp: return std::exp(a);
       case NodeKind::Log: return std::log(a);
+      case NodeKind::Exp2: return std::exp2(a);
+      case NodeKind::Log2: return std::log2(a);
       case NodeKind::Sqrt: return std::sqrt(a);
       case NodeKind::Sin: return std::sin(a);
       case NodeKind::Cos: return std::cos(a);
@@ -325,6 +328,10 @@ class ComputeGraphNode {
       }
       case NodeKind::Erf:  // 2/sqrt(pi) exp(-a^2) a'
         return P3(K(1.1283791670955126), U(NodeKind::Exp, U(NodeKind::Neg, B(NodeKind::Prod, a->copy(), a->copy()))), a->diff(variable));
+      case NodeKind::Exp2:  // ln2 2^a a'
+        return P3(K(0.6931471805599453), copy(), a->diff(variable));
+      case NodeKind::Log2:  // a' / (a ln2)
+        return P3(K(1.4426950408889634), a->diff(variable), U(NodeKind::Inv, a->copy()));
       case NodeKind::Pow: {
         const auto& e = children_[1];
         if (!e->variable_in_tree(variable)) {  // e * a^(e-1) * a'
@@ -387,20 +394,25 @@ class ComputeGraphNode {
         return make_node<kernelT, evalT>(NodeKind::Sum, std::move(out));
       }
       case NodeKind::Prod: {
-        std::vector<Ptr> flat;
-        for (auto& c : ch) {
-          if (c->kind() == NodeKind::Prod)
-            for (auto& g : c->children_) flat.emplace_back(std::move(g));
-          else
-            flat.emplace_back(std::move(c));
-        }
+        // flatten nested products, also those under a negation (the sign is collected)
         double cprod = 1.0;
         std::vector<Ptr> out;
         bool negate = false;
-        for (auto& c : flat) {
-          if (c->is_constant()) cprod *= c->value();
-          else if (c->kind() == NodeKind::Neg) { negate = !negate; out.emplace_back(std::move(c->children_[0])); }
-          else out.emplace_back(std::move(c));
+        std::vector<Ptr> work;
+        for (auto& c : ch) work.emplace_back(std::move(c));
+        for (size_t i = 0; i < work.size(); i++) {
+          Ptr c = std::move(work[i]);
+          while (c->kind() == NodeKind::Neg) {
+            negate = !negate;
+            c = std::move(c->children_[0]);
+          }
+          if (c->kind() == NodeKind::Prod) {
+            for (auto& g : c->children_) work.emplace_back(std::move(g));
+          } else if (c->is_constant()) {
+            cprod *= c->value();
+          } else {
+            out.emplace_back(std::move(c));
+          }
         }
         if (negate) cprod = -cprod;
         if (cprod == 0.0) return K(0.0);
@@ -526,6 +538,8 @@ class ComputeGraphNode {
       case NodeKind::Acos: return "acos";
       case NodeKind::Atan: return "atan";
       case NodeKind::Erf: return "erf";
+      case NodeKind::Exp2: return "exp2";
+      case NodeKind::Log2: return "log2";
       case NodeKind::Conditional: return "Conditional";
     }
     return "?";
@@ -605,6 +619,8 @@ MOREFIT_UNARY_NODE(AsinNode, Asin)
 MOREFIT_UNARY_NODE(AcosNode, Acos)
 MOREFIT_UNARY_NODE(AtanNode, Atan)
 MOREFIT_UNARY_NODE(ErfNode, Erf)
+MOREFIT_UNARY_NODE(Exp2Node, Exp2)
+MOREFIT_UNARY_NODE(Log2Node, Log2)
 #undef MOREFIT_UNARY_NODE
 
 template <typename kernelT, typename evalT>
@@ -673,6 +689,8 @@ GraphPtr<kernelT, evalT> make_node(NodeKind kind, std::vector<GraphPtr<kernelT, 
     case NodeKind::Acos: return std::make_unique<AcosNode<kernelT, evalT>>(kind, std::move(ch));
     case NodeKind::Atan: return std::make_unique<AtanNode<kernelT, evalT>>(kind, std::move(ch));
     case NodeKind::Erf: return std::make_unique<ErfNode<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Exp2: return std::make_unique<Exp2Node<kernelT, evalT>>(kind, std::move(ch));
+    case NodeKind::Log2: return std::make_unique<Log2Node<kernelT, evalT>>(kind, std::move(ch));
   }
   return nullptr;
 }
@@ -886,6 +904,59 @@ GraphPtr<kernelT, evalT> factor_common(const ComputeGraphNode<kernelT, evalT>& n
   return make_node<kernelT, evalT>(NodeKind::Prod, std::move(common))->simplify();
 }
 
+// ==================================================================== FP32 kernel rewrites
+// exp(a) -> exp2(log2(e) a) and log(a) -> ln(2) log2(a): the hardware evaluates exp2 and
+// log2, so the conversion constants are otherwise multiplied in per event.  After
+// simplify() they sit in products next to parameter-only factors, which hoisting folds into
+// one FP64-computed kernel parameter (inside exp2) or into an output scale (split_output_scale).
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> to_base2(const ComputeGraphNode<kernelT, evalT>& n) {
+  std::vector<GraphPtr<kernelT, evalT>> ch;
+  for (const auto& c : n.children_) ch.emplace_back(to_base2(*c));
+  if (n.kind() == NodeKind::Exp) {
+    std::vector<GraphPtr<kernelT, evalT>> p;
+    p.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 1.4426950408889634));
+    p.emplace_back(std::move(ch[0]));
+    std::vector<GraphPtr<kernelT, evalT>> e;
+    e.emplace_back(make_node<kernelT, evalT>(NodeKind::Prod, std::move(p)));
+    return make_node<kernelT, evalT>(NodeKind::Exp2, std::move(e));
+  }
+  if (n.kind() == NodeKind::Log) {
+    std::vector<GraphPtr<kernelT, evalT>> p;
+    p.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 0.6931471805599453));
+    p.emplace_back(make_node<kernelT, evalT>(NodeKind::Log2, std::move(ch)));
+    return make_node<kernelT, evalT>(NodeKind::Prod, std::move(p));
+  }
+  return make_node<kernelT, evalT>(n.kind(), std::move(ch), n.value(), n.name(), n.comparator());
+}
+
+// Splits an output (a per-event term whose sum over events is wanted) into a factor that
+// depends only on `params` and constants, and the event-dependent rest: the sum is taken of
+// the rest and multiplied by the factor once, in FP64, per segment.  Returns the rest and
+// sets `scale` (null when the output has no such factor or no event-dependent part).
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> split_output_scale(const ComputeGraphNode<kernelT, evalT>& g, const std::vector<std::string>& params,
+                                            GraphPtr<kernelT, evalT>& scale) {
+  scale.reset();
+  const ComputeGraphNode<kernelT, evalT>* n = &g;
+  bool neg = false;
+  if (n->kind() == NodeKind::Neg) {
+    neg = true;
+    n = n->children_[0].get();
+  }
+  if (!n->contains_other_variables(params)) return g.copy();
+  std::vector<GraphPtr<kernelT, evalT>> sc, ev;
+  if (n->kind() == NodeKind::Prod) {
+    for (const auto& c : n->children_) (c->contains_other_variables(params) ? ev : sc).emplace_back(c->copy());
+  } else {
+    ev.emplace_back(n->copy());
+  }
+  if (neg) sc.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, -1.0));
+  if (sc.empty()) return g.copy();
+  scale = sc.size() == 1 ? std::move(sc[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(sc))->simplify();
+  return ev.size() == 1 ? std::move(ev[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(ev));
+}
+
 // ==================================================================== code emission with CSE
 // Emits the outputs of several graphs as one straight-line block of SSA statements
 //   const T t17 = ...;
@@ -902,6 +973,11 @@ struct KernelEmitter {
   int statements = 0;
   std::string prefix = "mf_";  // math wrapper prefix ("mfd_" for the FP64 hoisted terms)
   std::string tmp = "t";       // SSA name prefix
+  // how often each operand (by structural hash) occurs in the products of the graphs about
+  // to be emitted; products are then emitted as chains in falling frequency, so operands
+  // shared by several products form common prefixes that are computed once
+  std::unordered_map<uint64_t, int> prod_freq;
+  std::unordered_map<const void*, uint64_t> hash_memo;
 
   explicit KernelEmitter(bool is_fp32) : T(is_fp32 ? "float" : "double"), fp32(is_fp32) {}
 
@@ -932,9 +1008,24 @@ struct KernelEmitter {
       case NodeKind::Acos: return prefix + "acos";
       case NodeKind::Atan: return prefix + "atan";
       case NodeKind::Erf: return prefix + "erf";
+      case NodeKind::Exp2: return prefix + "exp2";
+      case NodeKind::Log2: return prefix + "log2";
       case NodeKind::Inv: return prefix + "rcp";
       default: return "";
     }
+  }
+
+  template <typename kernelT, typename evalT>
+  uint64_t node_hash(const ComputeGraphNode<kernelT, evalT>& n) {
+    auto it = hash_memo.find(&n);
+    if (it != hash_memo.end()) return it->second;
+    return hash_memo[&n] = structural_hash(n);
+  }
+  template <typename kernelT, typename evalT>
+  void count_products(const ComputeGraphNode<kernelT, evalT>& n) {
+    if (n.kind() == NodeKind::Prod)
+      for (const auto& c : n.children_) prod_freq[node_hash(*c)]++;
+    for (const auto& c : n.children_) count_products(*c);
   }
 
   template <typename kernelT, typename evalT>
@@ -959,6 +1050,24 @@ struct KernelEmitter {
     }
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
+    if (n.kind() == NodeKind::Prod && !prod_freq.empty() && ops.size() > 2) {
+      std::vector<size_t> idx(ops.size());
+      std::vector<int> freq(ops.size());
+      for (size_t i = 0; i < ops.size(); i++) {
+        idx[i] = i;
+        auto f = prod_freq.find(node_hash(*n.children_[i]));
+        freq[i] = f == prod_freq.end() ? 0 : f->second;
+      }
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return freq[a] != freq[b] ? freq[a] > freq[b] : ops[a] < ops[b]; });
+      std::string acc = ops[idx[0]];
+      for (size_t k = 1; k < idx.size(); k++) {
+        const std::string pk = "P(" + acc + "," + ops[idx[k]] + ")";
+        auto pi = memo.find(pk);
+        acc = pi != memo.end() ? pi->second : fresh(pk, acc + " * " + ops[idx[k]]);
+      }
+      memo[key] = acc;
+      return acc;
+    }
     std::string e;
     switch (n.kind()) {
       case NodeKind::Sum:
