@@ -64,6 +64,46 @@ def measured_traffic(current_shape):
     return float(kv["traffic_bytes_per_launch"])
 
 
+def serve_workload(ctx, M, O, nreq, rate, seed=11, plain_subset=12):
+    """Fit requests arriving as a Poisson stream at `rate` per second: 70 % mass fits, 30 %
+    angular fits, event counts log-uniform in [1e3, 2e6], data generated on the device per
+    request (seeded), FP32 kernels, evaluations of concurrent fits in shared launches.
+    Compared with submitting the same kinds of fits one at a time through the plain FP64
+    per-fit path (a subset of the requests, data generated by the oracle's generator)."""
+    rs = np.random.default_rng(seed)
+    cls = (rs.random(nreq) < 0.3).astype(int)
+    n = np.exp(rs.uniform(np.log(1e3), np.log(2e6), nreq)).astype(np.int64)
+    seeds = rs.integers(1, 2**62, nreq)
+    arrival = np.cumsum(rs.exponential(1.0 / rate, nreq))
+    mass, kst = ctx.model(M.MASS, fp32=True), ctx.model(M.KSTAR, fp32=True)
+    truth = [M.MASS_TRUTH, M.KSTAR_TRUTH]
+    r = M.serve([mass, kst], truth, truth, cls, n, seeds, arrival, budget_bytes=8 << 30, slots=512)
+    lat = (r["done"] - r["arrival"]) * 1e3
+    ok = r["status"] == 0
+    span = float(r["done"].max() - r["arrival"].min())
+    t_plain = 0.0
+    for i in range(plain_subset):
+        d = O.generate(int(cls[i]), truth[cls[i]], int(n[i]), seed=int(seeds[i]) % (2**63), nstreams=64)
+        t0 = time.perf_counter()
+        ctx.fit_plain(int(cls[i]), d, truth[cls[i]], fp32=False)
+        t_plain += time.perf_counter() - t0
+    # the subset's mix of sizes and classes differs from the whole stream's: scale by events
+    ev_sub = float(np.sum(n[:plain_subset] * np.where(cls[:plain_subset] == 1, 3, 1)))
+    ev_all = float(np.sum(n * np.where(cls == 1, 3, 1)))
+    plain_rate = nreq / (t_plain * ev_all / ev_sub)
+    mass.close()
+    kst.close()
+    return {"requests": nreq, "arrival_rate_per_s": rate, "mass_fraction": float(np.mean(cls == 0)),
+            "events_log_uniform": [1000, 2000000], "converged": int(ok.sum()), "rejected": r["rejected"],
+            "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
+            "fits_per_s": nreq / span, "span_s": span, "ticks": r["ticks"], "launches": r["launches"],
+            "peak_live": r["peak_live"], "generate_ms": r["generate_ms"],
+            "plain_fp64_fits_per_s": plain_rate,
+            "plain_fp64_sample": f"{plain_subset} requests fitted one at a time (FP64 plain path), "
+                                 "time scaled to the whole stream by its event count (angular events weighted x3)",
+            "speedup_vs_plain": (nreq / span) / plain_rate}
+
+
 def profile_workload():
     """Headline kernels only, for hardware-counter passes: FP32 mf_eval_grad and mf_eval_lh at
     N = 10^7 (generated on the device), then the copy kernel over 40 MB (the counters' scale)."""
@@ -89,6 +129,9 @@ def main():
     ap.add_argument("--plain-toys", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -230,15 +273,23 @@ def main():
             tplain += time.perf_counter() - t1
         plain_rate = args.plain_toys / tplain
         toys = {"toys": args.toys, "mean_events": float(sizes.mean()), "generate_s": tgen, "fit_s": tfit,
-                "toy_fits_per_s": batched_rate, "plain_fp64_toy_fits_per_s": plain_rate,
+                "toy_fits_per_s": batched_rate, "study_fits_per_s_incl_generation": args.toys / (tgen + tfit),
+                "plain_fp64_toy_fits_per_s": plain_rate,
                 "speedup_vs_plain": batched_rate / plain_rate, "converged": int(ok.sum()),
                 "pull_mean": pulls.mean(axis=0).round(4).tolist(), "pull_width": pulls.std(axis=0).round(4).tolist(),
                 "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
-                "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms,
+                "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms, "host_threads": tst.host_threads,
                 "launch_shape": shape(mt)}
         log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain ({batched_rate / plain_rate:.1f}x)")
         bt.close()
         mt.close()
+
+    # ---------------------------------------------------------------- fit service
+    serving = None
+    if not args.no_serve:
+        serving = serve_workload(ctx, M, O, args.serve_requests, args.serve_rate)
+        log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
+            f"{serving['fits_per_s']:.0f} fits/s vs {serving['plain_fp64_fits_per_s']:.1f} one at a time")
 
     # ---------------------------------------------------------------- CPU baseline (oracle)
     cpu = None
@@ -283,6 +334,7 @@ def main():
         "copy_ceiling_GBps": copy_gbs,
         "streaming": stream,
         "toys": toys,
+        "serving": serving,
         "device": name,
     }
     for x in (b32, b64):

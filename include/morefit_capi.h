@@ -55,6 +55,8 @@ typedef struct mfh_batch_stats {
   int ticks;
   int launches_lh, launches_grad, launches_hess;
   double wall_ms, gpu_wait_ms, host_ms;
+  int host_threads;        /* host threads that ran the fits' MIGRAD code */
+  int pad;
 } mfh_batch_stats;
 
 const char* mfh_last_error(void);
@@ -87,6 +89,9 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
  * theta for events data (SoA nobs x n): out[i * nout + k].  Checks the code generator's
  * rewrites against the density's definition. */
 int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double* out);
+/* without a device: xoshiro128++ states of streams first .. first+count-1 of `seed` by the
+ * jump-ahead matrices the device seeding uses (out: 4 words per stream) */
+int mfh_stream_states(unsigned long long seed, long long first, int count, unsigned int* out);
 
 int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out);
 int mfh_batch_destroy(mfh_batch* b);

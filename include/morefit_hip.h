@@ -25,6 +25,7 @@
  *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
  *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
+ *   mf_launch_seed_streams          host seeding of per-work-item generator states (compute_opencl.hh:522-539), on the device
  *   mf_sync                         Finish (clFinish)
  *   mf_graph_*                      no counterpart (per-iteration launch sequence as a hipGraph)
  *   mf_timer_*, mf_bench_copy       no counterpart (measurement)
@@ -207,6 +208,12 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
 int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* hoist64, const int* seg_list, int nseg,
                    const mf_segment* seg_src, mf_segment* seg_dst);
 int mf_launch_generate(mf_module* mod, const mf_gen_args* args, int grid, int block);
+/* xoshiro128++ states of generator streams, all device pointers: stream g (of nstreams)
+ * gets J^k applied to seg_base[4 * streams[g].seg ...], k = streams[g].begin /
+ * events_per_stream, J the jump() map; powers holds J^(2^i), i < levels, as 128 columns of 4
+ * words each (the layout of random.hh xoshiro_jump_powers).  One thread per stream. */
+int mf_launch_seed_streams(mf_device* dev, const unsigned int* powers, int levels, const mf_gen_stream* streams, int nstreams,
+                           int events_per_stream, const unsigned int* seg_base, unsigned int* states);
 
 /* hipGraph capture of everything enqueued on the device stream between begin and end */
 int mf_graph_begin(mf_device* dev);

@@ -55,7 +55,7 @@ class FitOut(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("ticks", ctypes.c_int), ("launches_lh", ctypes.c_int), ("launches_grad", ctypes.c_int),
                 ("launches_hess", ctypes.c_int), ("wall_ms", ctypes.c_double), ("gpu_wait_ms", ctypes.c_double),
-                ("host_ms", ctypes.c_double)]
+                ("host_ms", ctypes.c_double), ("host_threads", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 _hip = None
@@ -67,11 +67,11 @@ HIP_SYMBOLS = [
     "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_module_load",
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
-    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped",
+    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped", "mf_launch_seed_streams",
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
-    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs",
+    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run",
@@ -102,6 +102,7 @@ def libs():
         L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
         L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
         L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, dp]
+        L.mfh_stream_states.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ip, ctypes.POINTER(ctypes.c_uint32)]
         L.mfh_batch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, pp]
         L.mfh_batch_destroy.argtypes = [vp]
         L.mfh_batch_upload.argtypes = [vp, ip, dp, ctypes.c_longlong]
@@ -163,6 +164,14 @@ def host_outputs(kind, fp32, which, theta, data):
     out = np.zeros(n * nout)
     _check(libs()[1].mfh_model_host_outputs(kind, int(fp32), which, _dp(_f64(theta)), _dp(data), n, _dp(out)))
     return out.reshape(n, nout)
+
+
+def stream_states(seed, first, count):
+    """xoshiro128++ states of generator streams first .. first+count-1 of `seed`, by the
+    jump-ahead matrices the device seeding uses (no device): array (count, 4) of uint32"""
+    out = (ctypes.c_uint32 * (4 * count))()
+    _check(libs()[1].mfh_stream_states(int(seed), int(first), int(count), out))
+    return np.frombuffer(out, dtype=np.uint32).reshape(count, 4).copy()
 
 
 def precompile(kind, fp32, with_hessian=True, **kw):

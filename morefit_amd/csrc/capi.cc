@@ -324,6 +324,18 @@ int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, c
   MFH_CATCH
 }
 
+int mfh_stream_states(unsigned long long seed, long long first, int count, unsigned int* out) {
+  MFH_TRY
+  if (first < 0 || count < 0 || (count > 0 && !out)) throw std::invalid_argument("bad stream range");
+  static const std::vector<uint32_t> powers = xoshiro_jump_powers(40);
+  Xoshiro128pp r;
+  r.setSeed((uint64_t)seed);
+  uint32_t base[4];
+  for (int w = 0; w < 4; w++) base[w] = r.getSeed(w);
+  for (int i = 0; i < count; i++) xoshiro_stream_state(powers, base, (unsigned long long)(first + i), out + 4 * (size_t)i);
+  MFH_CATCH
+}
+
 int mfh_batch_create(mfh_model* m, const long long* sizes, int nseg, mfh_batch** out) {
   MFH_TRY
   std::vector<long long> s(sizes, sizes + nseg);
@@ -457,6 +469,7 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
       stats->wall_ms = bf.stats.wall_ms;
       stats->gpu_wait_ms = bf.stats.gpu_wait_ms;
       stats->host_ms = bf.stats.host_ms;
+      stats->host_threads = bf.stats.host_threads;
     }
     return 0;
   });

@@ -341,6 +341,35 @@ __global__ void k_fill_zero_bytes(unsigned char* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
+// generator stream states by jump-ahead: state = J^k base, J^(2^i) given as column sets
+__global__ void __launch_bounds__(256) k_seed_streams(const unsigned int* __restrict__ powers, int levels,
+                                                      const mf_gen_stream* __restrict__ streams, int nstreams, int eps,
+                                                      const unsigned int* __restrict__ seg_base, unsigned int* __restrict__ states) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nstreams) return;
+  const mf_gen_stream st = streams[g];
+  unsigned long long k = (unsigned long long)(st.begin / eps);
+  unsigned int s[4];
+  for (int w = 0; w < 4; w++) s[w] = seg_base[4 * st.seg + w];
+  for (int i = 0; i < levels && (k >> i); i++) {
+    if (!((k >> i) & 1ull)) continue;
+    const unsigned int* cols = powers + (size_t)i * 512;
+    unsigned int a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int b = 0; b < 128; b++) {
+      const unsigned int m = 0u - ((s[b >> 5] >> (b & 31)) & 1u);
+      a0 ^= cols[4 * b] & m;
+      a1 ^= cols[4 * b + 1] & m;
+      a2 ^= cols[4 * b + 2] & m;
+      a3 ^= cols[4 * b + 3] & m;
+    }
+    s[0] = a0;
+    s[1] = a1;
+    s[2] = a2;
+    s[3] = a3;
+  }
+  for (int w = 0; w < 4; w++) states[4 * (size_t)g + w] = s[w];
+}
+
 typedef float mf_v4f __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_copy(const mf_v4f* __restrict__ src, mf_v4f* __restrict__ dst, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -686,6 +715,20 @@ int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* ho
   void* kp[] = {(void*)&theta, (void*)&params, (void*)&hoist64, (void*)&seg_list, (void*)&nseg, (void*)&seg_src, (void*)&seg_dst};
   int grid = (nseg + 63) / 64;
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep");
+  return MF_OK;
+}
+
+int mf_launch_seed_streams(mf_device* dev, const unsigned int* powers, int levels, const mf_gen_stream* streams, int nstreams,
+                           int events_per_stream, const unsigned int* seg_base, unsigned int* states) {
+  MF_INJECT("mf_launch_seed_streams");
+  if (!dev || nstreams < 0 || events_per_stream <= 0 || levels < 0 || levels > 63 ||
+      (nstreams > 0 && (!powers || !streams || !seg_base || !states)))
+    return set_error(MF_ERR_INVALID, "mf_launch_seed_streams: bad argument");
+  if (nstreams == 0) return MF_OK;
+  MF_HIP(hipSetDevice(dev->ordinal), "mf_launch_seed_streams");
+  hipLaunchKernelGGL(k_seed_streams, dim3((nstreams + 255) / 256), dim3(256), 0, dev->stream, powers, levels, streams, nstreams,
+                     events_per_stream, seg_base, states);
+  MF_HIP(hipGetLastError(), "mf_launch_seed_streams");
   return MF_OK;
 }
 

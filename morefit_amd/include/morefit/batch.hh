@@ -15,12 +15,15 @@
 
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
 #include <list>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "compute_hip.hh"
@@ -39,6 +42,7 @@ struct BatchFitStats {
   int ticks = 0;
   int launches[4] = {0, 0, 0, 0};
   double wall_ms = 0.0, gpu_wait_ms = 0.0, host_ms = 0.0;
+  int host_threads = 1;
 };
 
 template <typename kernelT, typename evalT = double>
@@ -49,10 +53,12 @@ class FitScheduler {
     FitScheduler* owner = nullptr;
     Batch* b = nullptr;
     int seg = 0;
+    int worker = 0;                 // host thread that runs this fit, for its whole life
     long long id = 0;
     std::vector<double> start;
-    std::vector<char> stack;
+    std::unique_ptr<char[]> stack;  // not zero-filled: pages are touched only as used
     ucontext_t ctx;
+    ucontext_t* ret = nullptr;      // context of the worker that resumed this fit
     int pending = 0;
     bool done = false;
     BatchFitResult result;
@@ -65,12 +71,17 @@ class FitScheduler {
   // MIGRAD trace level of every fit (>= 3: per-iteration lines on stderr); the environment
   // variable MOREFIT_PRINT_LEVEL sets it for fits driven through the C API
   int print_level = std::getenv("MOREFIT_PRINT_LEVEL") ? std::atoi(std::getenv("MOREFIT_PRINT_LEVEL")) : 0;
-  size_t stack_bytes = 256 * 1024;
+  size_t stack_bytes = 128 * 1024;
+  // host threads that run the fits' MIGRAD code between launches (0: one per hardware
+  // thread, at most 16); a fit stays on the thread it was assigned, so a fibre never moves
+  // between OS threads.  Device work is issued by the calling thread only.
+  int host_threads = 0;
   BatchFitStats stats;
 
   FitScheduler() {}
   FitScheduler(const FitScheduler&) = delete;
   FitScheduler& operator=(const FitScheduler&) = delete;
+  ~FitScheduler() { stop_workers(); }
 
   // a new fit of segment `seg` of batch `b` from `start` (ntheta values)
   Task* add(Batch* b, int seg, const double* start, long long id = 0) {
@@ -80,12 +91,13 @@ class FitScheduler {
     t.b = b;
     t.seg = seg;
     t.id = id;
+    t.worker = (int)(added_++ % (unsigned long long)nthreads());
     t.start.assign(start, start + b->m->ntheta);
-    t.stack.resize(stack_bytes);
+    t.stack.reset(new char[stack_bytes]);
     getcontext(&t.ctx);
-    t.ctx.uc_stack.ss_sp = t.stack.data();
-    t.ctx.uc_stack.ss_size = t.stack.size();
-    t.ctx.uc_link = &main_;
+    t.ctx.uc_stack.ss_sp = t.stack.get();
+    t.ctx.uc_stack.ss_size = stack_bytes;
+    t.ctx.uc_link = nullptr;  // a finished fit switches back explicitly (trampoline)
     uintptr_t p = (uintptr_t)&t;
     makecontext(&t.ctx, (void (*)())&FitScheduler::trampoline, 2, (unsigned int)(p & 0xffffffffu), (unsigned int)(p >> 32));
     batches_.insert(b);
@@ -93,15 +105,32 @@ class FitScheduler {
   }
   size_t live() const { return tasks_.size(); }
 
-  // Runs every task that is not waiting until it posts a request or finishes; finished
-  // tasks are handed to `on_done` and removed.  Returns the number still live.
+  // Runs every task that is not waiting until it posts a request or finishes (the host
+  // threads in parallel, each over its own tasks); finished tasks are handed to `on_done`
+  // and removed.  Returns the number still live.
   template <typename F>
   size_t step(F on_done) {
+    const int T = nthreads();
+    if (T > 1 && tasks_.size() > 1) {
+      start_workers(T);
+      groups_.assign(T, {});
+      for (auto& t : tasks_)
+        if (!t.pending && !t.done) groups_[t.worker].push_back(&t);
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        ++generation_;
+        running_ = T;
+        cv_go_.notify_all();
+        cv_done_.wait(lk, [&] { return running_ == 0; });
+      }
+    } else {
+      ucontext_t here;
+      for (auto& t : tasks_)
+        if (!t.pending && !t.done) resume(t, &here);
+    }
     for (auto it = tasks_.begin(); it != tasks_.end();) {
-      Task& t = *it;
-      if (!t.pending && !t.done) swapcontext(&main_, &t.ctx);
-      if (t.done) {
-        on_done(t);
+      if (it->done) {
+        on_done(*it);
         it = tasks_.erase(it);
       } else {
         ++it;
@@ -134,6 +163,12 @@ class FitScheduler {
     stats.gpu_wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
   }
 
+  int nthreads() const {
+    if (host_threads > 0) return host_threads;
+    static const int hw = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    return hw;
+  }
+
  private:
   struct Fcn : MinimizerFcn {
     Task* t;
@@ -143,7 +178,7 @@ class FitScheduler {
       const auto& ti = t->b->m->theta_index;
       for (size_t k = 0; k < ti.size(); k++) th[k] = x[ti[k]];
       t->pending = kind;
-      swapcontext(&t->ctx, &t->owner->main_);
+      swapcontext(&t->ctx, t->ret);
     }
     double value(const double* x) override {
       post(x, EVAL_LH);
@@ -180,13 +215,58 @@ class FitScheduler {
 
   std::list<Task> tasks_;  // stable addresses: contexts point into them
   std::set<Batch*> batches_;
-  ucontext_t main_;
+  unsigned long long added_ = 0;
+
+  // worker pool: each step, worker w resumes the tasks in groups_[w]
+  std::vector<std::thread> workers_;
+  std::vector<std::vector<Task*>> groups_;
+  std::mutex mu_;
+  std::condition_variable cv_go_, cv_done_;
+  unsigned long long generation_ = 0;
+  int running_ = 0;
+  bool stopping_ = false;
+
+  static void resume(Task& t, ucontext_t* here) {
+    t.ret = here;
+    swapcontext(here, &t.ctx);
+  }
+  void start_workers(int T) {
+    if ((int)workers_.size() == T) return;
+    stop_workers();
+    stopping_ = false;
+    for (int w = 0; w < T; w++)
+      workers_.emplace_back([this, w] {
+        unsigned long long seen = 0;
+        ucontext_t here;
+        for (;;) {
+          {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_go_.wait(lk, [&] { return stopping_ || generation_ != seen; });
+            if (stopping_) return;
+            seen = generation_;
+          }
+          for (Task* t : groups_[w]) resume(*t, &here);
+          std::lock_guard<std::mutex> lk(mu_);
+          if (--running_ == 0) cv_done_.notify_one();
+        }
+      });
+  }
+  void stop_workers() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stopping_ = true;
+      cv_go_.notify_all();
+    }
+    for (auto& th : workers_) th.join();
+    workers_.clear();
+  }
 
   static void trampoline(unsigned int lo, unsigned int hi) {
     Task* t = (Task*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
     t->owner->run_fit(*t);
     t->done = true;
     t->pending = 0;
+    setcontext(t->ret);  // back to the worker that resumed it; this context is not used again
   }
 
   void run_fit(Task& t) {
@@ -251,6 +331,7 @@ class BatchFitter {
     std::vector<BatchFitResult> out(segs.size());
     while (sch.step([&](auto& t) { out[t.id] = t.result; }) > 0) sch.launch();
     stats = sch.stats;
+    stats.host_threads = segs.size() > 1 ? sch.nthreads() : 1;
     stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats.host_ms = stats.wall_ms - stats.gpu_wait_ms;
     return out;

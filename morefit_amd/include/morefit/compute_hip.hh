@@ -716,6 +716,10 @@ class HipBatch {
   bool dirty = true;
   std::vector<mf_segment> segs_h;
   DevBuf data, params_d, hoist64_d, segs_d[4], chunks_d, partials_d;
+  // generator scratch, kept between calls (grows as needed)
+  DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_;
+  int gen_levels_ = 0;
+  std::vector<mf_gen_stream> gen_streams_h_;
   // staging in mapped host memory, read and written by the kernels in place: theta and the
   // per-kind segment tables (modes) in, totals out -- a call's graph holds kernels only
   HostBuf theta_h, segs_hk[4], results_h[4];
@@ -879,8 +883,10 @@ class HipBatch {
   }
 
   // On-device accept-reject into the listed slots (all active slots when `slots` is empty).
-  // theta_gen: ntheta generation values per listed slot; seeds: one per listed slot (its
-  // streams are separated by jump()).  Returns the total number of proposals.
+  // theta_gen: ntheta generation values per listed slot; seeds: one per listed slot.  Slot
+  // s is split into streams of events_per_stream events; stream k starts from the seed's
+  // xoshiro128++ state advanced by k jump()s, computed on the device by jump-ahead
+  // (random.hh xoshiro_jump_powers).  Returns the total number of proposals.
   unsigned long long generate(const std::vector<double>& theta_gen, const std::vector<uint64_t>& seeds, int events_per_stream = 64,
                               std::vector<int> slots = {}) {
     rebuild();
@@ -890,43 +896,60 @@ class HipBatch {
     const int L = (int)slots.size();
     for (int i = 0; i < L; i++) std::memcpy(theta(slots[i]), &theta_gen[(size_t)i * m->ntheta], m->ntheta * sizeof(double));
     prep();
-    std::vector<mf_gen_stream> st;
-    std::vector<unsigned int> states;
+    std::vector<mf_gen_stream>& st = gen_streams_h_;
+    st.clear();
+    std::vector<unsigned int> base(4 * (size_t)S, 0u);
     std::vector<double> mp(S, 1.0);
+    long long max_streams = 1;
     for (int i = 0; i < L; i++) {
       const int s = slots[i];
       mp[s] = m->maxprob(&theta_gen[(size_t)i * m->ntheta]);
-      int ns = (int)((sizes[s] + events_per_stream - 1) / events_per_stream);
-      size_t base = states.size();
-      states.resize(base + 4 * (size_t)ns);
-      xoshiro_streams(seeds[i], ns, states.data() + base);
-      for (int k = 0; k < ns; k++)
-        st.push_back({s, (int)((long long)k * events_per_stream), (int)std::min<long long>(sizes[s], (long long)(k + 1) * events_per_stream), 0});
+      Xoshiro128pp r;
+      r.setSeed(seeds[i]);
+      for (int w = 0; w < 4; w++) base[4 * (size_t)s + w] = r.getSeed(w);
+      const long long ns = (sizes[s] + events_per_stream - 1) / events_per_stream;
+      max_streams = std::max(max_streams, ns);
+      for (long long k = 0; k < ns; k++)
+        st.push_back({s, (int)(k * events_per_stream), (int)std::min<long long>(sizes[s], (k + 1) * events_per_stream), 0});
     }
     if (st.empty()) return 0;
-    DevBuf st_d(dev, std::max<size_t>(16, st.size() * sizeof(mf_gen_stream)));
-    DevBuf states_d(dev, std::max<size_t>(16, states.size() * 4));
-    DevBuf mp_d(dev, S * sizeof(double));
-    DevBuf seg_d(dev, S * sizeof(mf_segment));
-    DevBuf props_d(dev, S * sizeof(unsigned long long));
-    hip_check(mf_memcpy_h2d(dev, st_d.ptr, st.data(), st.size() * sizeof(mf_gen_stream)), "generate streams");
-    hip_check(mf_memcpy_h2d(dev, states_d.ptr, states.data(), states.size() * 4), "generate states");
-    hip_check(mf_memcpy_h2d(dev, mp_d.ptr, mp.data(), S * sizeof(double)), "generate maxprob");
-    hip_check(mf_memcpy_h2d(dev, seg_d.ptr, segs_h.data(), S * sizeof(mf_segment)), "generate segs");
-    hip_check(mf_fill_zero(dev, props_d.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    int levels = 1;
+    while ((1ll << levels) < max_streams) levels++;
+    if (gen_levels_ < levels) {
+      const int lv = std::max(levels, 24);
+      auto pw = xoshiro_jump_powers(lv);
+      gen_powers_.alloc(dev, pw.size() * sizeof(uint32_t));
+      hip_check(mf_memcpy_h2d(dev, gen_powers_.ptr, pw.data(), pw.size() * sizeof(uint32_t)), "generate jump powers");
+      gen_levels_ = lv;
+    }
+    gen_streams_.alloc(dev, st.size() * sizeof(mf_gen_stream));
+    gen_states_.alloc(dev, st.size() * 4 * sizeof(unsigned int));
+    gen_base_.alloc(dev, base.size() * sizeof(unsigned int));
+    gen_maxprob_.alloc(dev, S * sizeof(double));
+    gen_segs_.alloc(dev, S * sizeof(mf_segment));
+    gen_props_.alloc(dev, S * sizeof(unsigned long long));
+    hip_check(mf_memcpy_h2d(dev, gen_streams_.ptr, st.data(), st.size() * sizeof(mf_gen_stream)), "generate streams");
+    hip_check(mf_memcpy_h2d(dev, gen_base_.ptr, base.data(), base.size() * sizeof(unsigned int)), "generate seeds");
+    hip_check(mf_memcpy_h2d(dev, gen_maxprob_.ptr, mp.data(), S * sizeof(double)), "generate maxprob");
+    hip_check(mf_memcpy_h2d(dev, gen_segs_.ptr, segs_h.data(), S * sizeof(mf_segment)), "generate segs");
+    hip_check(mf_fill_zero(dev, gen_props_.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    hip_check(mf_launch_seed_streams(dev, gen_powers_.template as<unsigned int>(), levels, gen_streams_.template as<mf_gen_stream>(),
+                                     (int)st.size(), events_per_stream, gen_base_.template as<unsigned int>(),
+                                     gen_states_.template as<unsigned int>()),
+              "mf_launch_seed_streams");
     mf_gen_args g{};
     g.data = data.ptr;
-    g.segs = seg_d.template as<mf_segment>();
-    g.streams = st_d.template as<mf_gen_stream>();
+    g.segs = gen_segs_.template as<mf_segment>();
+    g.streams = gen_streams_.template as<mf_gen_stream>();
     g.nstreams = (int)st.size();
-    g.states = states_d.template as<unsigned int>();
+    g.states = gen_states_.template as<unsigned int>();
     g.params = params_d.ptr;
-    g.maxprob = mp_d.template as<double>();
-    g.proposals = props_d.template as<unsigned long long>();
+    g.maxprob = gen_maxprob_.template as<double>();
+    g.proposals = gen_props_.template as<unsigned long long>();
     const int block = 64;
     hip_check(mf_launch_generate(m->mod, &g, (int)((st.size() + block - 1) / block), block), "mf_launch_generate");
     std::vector<unsigned long long> props(S);
-    hip_check(mf_memcpy_d2h(dev, props.data(), props_d.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    hip_check(mf_memcpy_d2h(dev, props.data(), gen_props_.ptr, S * sizeof(unsigned long long)), "generate proposals");
     hip_check(mf_sync(dev), "generate");
     unsigned long long tot = 0;
     for (auto p : props) tot += p;

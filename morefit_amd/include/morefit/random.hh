@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <vector>
 #include <limits>
 #include <stdexcept>
 
@@ -133,6 +134,42 @@ class Xoshiro128pp : public RandomGenerator {
     s_[3] = d;
   }
 };
+
+// Jump-ahead of xoshiro128++ streams in O(log k).  The state transition, and hence jump(), is
+// linear over GF(2) on the 128 state bits, so the state of stream k (k jumps from a seeded
+// state s) is J^k s.  xoshiro_jump_powers(L) returns J^(2^i) for i < L, each as 128 columns
+// of 4 words: column b is the image of state bit b (word b / 32, bit b % 32).
+inline void xoshiro_apply_matrix(const uint32_t* cols, uint32_t s[4]) {
+  uint32_t a[4] = {0, 0, 0, 0};
+  for (int b = 0; b < 128; b++)
+    if ((s[b >> 5] >> (b & 31)) & 1u)
+      for (int w = 0; w < 4; w++) a[w] ^= cols[4 * b + w];
+  for (int w = 0; w < 4; w++) s[w] = a[w];
+}
+inline std::vector<uint32_t> xoshiro_jump_powers(int levels) {
+  std::vector<uint32_t> m((size_t)levels * 512);
+  for (int b = 0; b < 128; b++) {
+    uint32_t e[4] = {0, 0, 0, 0};
+    e[b >> 5] = 1u << (b & 31);
+    Xoshiro128pp r(e);
+    r.jump();
+    for (int w = 0; w < 4; w++) m[4 * b + w] = r.getSeed(w);
+  }
+  for (int i = 1; i < levels; i++)
+    for (int b = 0; b < 128; b++) {
+      uint32_t c[4];
+      for (int w = 0; w < 4; w++) c[w] = m[(size_t)(i - 1) * 512 + 4 * b + w];
+      xoshiro_apply_matrix(&m[(size_t)(i - 1) * 512], c);
+      for (int w = 0; w < 4; w++) m[(size_t)i * 512 + 4 * b + w] = c[w];
+    }
+  return m;
+}
+// state of stream k of `base` (k < 2^levels)
+inline void xoshiro_stream_state(const std::vector<uint32_t>& powers, const uint32_t base[4], unsigned long long k, uint32_t out[4]) {
+  for (int w = 0; w < 4; w++) out[w] = base[w];
+  for (int i = 0; k >> i; i++)
+    if ((k >> i) & 1ull) xoshiro_apply_matrix(&powers[(size_t)i * 512], out);
+}
 
 }  // namespace morefit
 #endif

@@ -90,7 +90,9 @@ class FitServer {
     while (finished < reqs.size()) {
       const double t = now();
       while (next < reqs.size() && reqs[next].arrival_s <= t) queue.push_back(next++);
-      // admission in arrival order, bounded by slots and the HBM budget
+      // admission in arrival order, bounded by slots and the HBM budget; the data of all
+      // requests admitted in this tick are generated in one launch per class
+      std::vector<std::vector<size_t>> admitted(classes.size());
       while (!queue.empty()) {
         size_t i = queue.front();
         const ServeRequest& r = reqs[i];
@@ -111,11 +113,26 @@ class FitServer {
         used += need;
         stats.peak_bytes = std::max(stats.peak_bytes, used);
         slot_of[i] = slot;
-        auto tg = std::chrono::steady_clock::now();
-        c.pool->generate(c.theta_gen, {r.seed}, 64, {slot});
-        stats.generate_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
-        res[i].admit_s = now();
-        sch.add(c.pool.get(), slot, c.start.data(), (long long)i);
+        admitted[r.cls].push_back(i);
+      }
+      for (size_t ci = 0; ci < classes.size(); ci++) {
+        if (admitted[ci].empty()) continue;
+        Class& c = classes[ci];
+        std::vector<double> tg;
+        std::vector<uint64_t> seeds;
+        std::vector<int> slots;
+        for (size_t i : admitted[ci]) {
+          tg.insert(tg.end(), c.theta_gen.begin(), c.theta_gen.end());
+          seeds.push_back(reqs[i].seed);
+          slots.push_back(slot_of[i]);
+        }
+        auto tgen = std::chrono::steady_clock::now();
+        c.pool->generate(tg, seeds, 64, slots);
+        stats.generate_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tgen).count();
+        for (size_t i : admitted[ci]) {
+          res[i].admit_s = now();
+          sch.add(c.pool.get(), slot_of[i], c.start.data(), (long long)i);
+        }
       }
       if (sch.live() == 0) {
         if (next < reqs.size() && queue.empty()) {

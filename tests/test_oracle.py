@@ -195,3 +195,28 @@ def test_generator_streams_independent_of_partition():
     assert np.array_equal(a, b)
     c = O.generate(O.MASS, [5.28, 0.02, 0.3, -1.5], 1000, seed=6, events_per_stream=100)
     assert not np.array_equal(a, c)
+
+
+def test_stream_jump_ahead_matches_sequential_jumps():
+    """generator streams are seeded on the device by jump-ahead (J^k from precomputed powers
+    J^(2^i)); the matrix method equals k sequential jump()s of the oracle, for small k and for
+    k with high bits set"""
+    import ctypes
+    import morefit_amd as M
+    for seed in (1, 229387429, 2**61 + 12345):
+        s = O.xoshiro_seed(seed)
+        seq = []
+        for k in range(70):
+            seq.append([s[j] for j in range(4)])
+            O.lib().orc_xoshiro128pp_jump(s)
+        got = M.stream_states(seed, 0, 70)
+        assert np.array_equal(got, np.array(seq, dtype=np.uint32)), seed
+    # far streams: k = 1000..1002 from sequential jumps
+    s = O.xoshiro_seed(7)
+    for k in range(1000):
+        O.lib().orc_xoshiro128pp_jump(s)
+    far = []
+    for k in range(3):
+        far.append([s[j] for j in range(4)])
+        O.lib().orc_xoshiro128pp_jump(s)
+    assert np.array_equal(M.stream_states(7, 1000, 3), np.array(far, dtype=np.uint32))
