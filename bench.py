@@ -35,14 +35,25 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def shape(model):
-    """launch shape of a model's evaluation kernels (options of its context, fitted unrolls)"""
+def shape(model, tuned):
+    """launch shape of a model's evaluation kernels: the autotuned (block, events per thread,
+    requested unroll) of its context, the unrolls fitted per kernel and their VGPRs"""
     import morefit_amd as M
     o = M.options()
-    return {"block": o.block_size, "events_per_thread": o.events_per_thread, "accumulate": o.accumulate,
+    return {"block": tuned[0], "events_per_thread": tuned[1], "requested_unroll": tuned[2], "accumulate": o.accumulate,
             "fast_math": bool(o.fast_math), "unroll": {"lh": model.unroll[M.EVAL_LH], "grad": model.unroll[M.EVAL_GRAD],
                                                        "hess": model.unroll[M.EVAL_HESS]},
             "vgprs": {"lh": model.vgprs[M.EVAL_LH], "grad": model.vgprs[M.EVAL_GRAD], "hess": model.vgprs[M.EVAL_HESS]}}
+
+
+def tuned_context(M, kind, fp32, sizes, label):
+    """a context whose launch shape the autotuner picked for this workload"""
+    ctx = M.Context(device=0)
+    t0 = time.perf_counter()
+    r = ctx.autotune(kind, fp32, sizes, eval_kind=M.EVAL_GRAD)
+    log(f"autotune {label}: block {r['best'][0]}, {r['best'][1]} events/thread, unroll {r['best'][2]} "
+        f"({min(u for _, u in r['table']):.2f} us; {len(r['table'])} shapes in {time.perf_counter() - t0:.1f} s)")
+    return ctx, r
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp32_mass_1e7_traffic.txt")
@@ -64,7 +75,17 @@ def measured_traffic(current_shape):
     return float(kv["traffic_bytes_per_launch"])
 
 
-def serve_workload(ctx, M, O, nreq, rate, seed=11, plain_subset=12):
+def plain_context(M, O):
+    """a default-shape context for the plain per-fit FP64 path, its modules compiled by one
+    untimed fit per density (the plain path builds its kernels at the first fit)"""
+    cp = M.Context(device=0)
+    for kind in (M.MASS, M.KSTAR):
+        d = O.generate(kind, M.TRUTH[kind], 2000, seed=1, nstreams=4)
+        cp.fit_plain(kind, d, M.TRUTH[kind], fp32=False)
+    return cp
+
+
+def serve_workload(ctx, cplain, M, O, nreq, rate, seed=11, plain_subset=12):
     """Fit requests arriving as a Poisson stream at `rate` per second: 70 % mass fits, 30 %
     angular fits, event counts log-uniform in [1e3, 2e6], data generated on the device per
     request (seeded), FP32 kernels, evaluations of concurrent fits in shared launches.
@@ -85,7 +106,7 @@ def serve_workload(ctx, M, O, nreq, rate, seed=11, plain_subset=12):
     for i in range(plain_subset):
         d = O.generate(int(cls[i]), truth[cls[i]], int(n[i]), seed=int(seeds[i]) % (2**63), nstreams=64)
         t0 = time.perf_counter()
-        ctx.fit_plain(int(cls[i]), d, truth[cls[i]], fp32=False)
+        cplain.fit_plain(int(cls[i]), d, truth[cls[i]], fp32=False)
         t_plain += time.perf_counter() - t0
     # the subset's mix of sizes and classes differs from the whole stream's: scale by events
     ev_sub = float(np.sum(n[:plain_subset] * np.where(cls[:plain_subset] == 1, 3, 1)))
@@ -106,10 +127,14 @@ def serve_workload(ctx, M, O, nreq, rate, seed=11, plain_subset=12):
 
 def profile_workload():
     """Headline kernels only, for hardware-counter passes: FP32 mf_eval_grad and mf_eval_lh at
-    N = 10^7 (generated on the device), then the copy kernel over 40 MB (the counters' scale)."""
+    N = 10^7 (generated on the device) in the autotuned launch shape, then the copy kernel
+    over 40 MB (the counters' scale)."""
     import morefit_amd as M
-    ctx = M.Context(device=0)
+    ctx, tune = tuned_context(M, M.MASS, True, [N_EVENTS], "mass N=1e7 FP32")
     m32 = ctx.model(M.MASS, fp32=True)
+    s = shape(m32, tune["best"])
+    log("profile launch_shape: block=%d ept=%d acc=%d unroll=%d" % (s["block"], s["events_per_thread"], s["accumulate"],
+                                                                    s["unroll"]["grad"]))
     b32 = m32.batch([N_EVENTS])
     b32.generate(M.MASS_TRUTH, [229387429])
     log("profile: grad", b32.time_kernel(M.EVAL_GRAD, iters=20) * 1e3, "us")
@@ -145,14 +170,16 @@ def main():
     import morefit_amd as M
     from oracle import oracle as O
 
-    ctx = M.Context(device=0)
-    name, cus = ctx.device_name()
-    log("device", name, cus, "CUs")
     n = N_EVENTS
     start = M.MASS_START.copy()
+    # launch shapes picked by the autotuner for each workload
+    ctx, tune32 = tuned_context(M, M.MASS, True, [n], "mass N=1e7 FP32")
+    ctx64, tune64 = tuned_context(M, M.MASS, False, [n], "mass N=1e7 FP64")
+    name, cus = ctx.device_name()
+    log("device", name, cus, "CUs")
 
     # ---------------------------------------------------------------- data (on device)
-    m64 = ctx.model(M.MASS, fp32=False)
+    m64 = ctx64.model(M.MASS, fp32=False)
     b64 = m64.batch([n])
     t0 = time.perf_counter()
     b64.generate(M.MASS_TRUTH, [229387429])
@@ -182,7 +209,7 @@ def main():
     achieved = bytes32 / (k32_ms * 1e-3) / 1e9
     log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s (with final pass {k32_pair_ms * 1e3:.2f} us); "
         f"lh kernel {k32_lh_ms * 1e3:.2f} us")
-    traffic = measured_traffic(shape(m32))
+    traffic = measured_traffic(shape(m32, tune32["best"]))
 
     # ---------------------------------------------------------------- FP64 path
     for _ in range(max(5, args.warmup // 2)):
@@ -192,7 +219,7 @@ def main():
     for _ in range(steps64):
         r64 = b64.eval(M.EVAL_GRAD, start)
     v64 = steps64 / (time.perf_counter() - t0)
-    k64_ms = b64.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
+    k64_ms = b64.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True)
     log(f"fp64 -2lnL+grad: {v64:.1f} evals/s, kernel {k64_ms * 1e3:.2f} us")
 
     # ---------------------------------------------------------------- accuracy vs oracle
@@ -247,12 +274,16 @@ def main():
         bs.close()
         ms.close()
 
+    # ---------------------------------------------------------------- plain per-fit FP64 path
+    cplain = plain_context(M, O)
+
     # ---------------------------------------------------------------- toy study
     toys = None
     if not args.no_toys:
         rs = np.random.default_rng(229387429)
         sizes = rs.poisson(5000, args.toys)
-        mt = ctx.model(M.MASS, fp32=True)
+        ctxt, tunet = tuned_context(M, M.MASS, True, sizes, f"{args.toys} toys FP32")
+        mt = ctxt.model(M.MASS, fp32=True)
         bt = mt.batch(sizes)
         t0 = time.perf_counter()
         bt.generate(M.MASS_TRUTH, list(range(1, args.toys + 1)))
@@ -269,7 +300,7 @@ def main():
         for i in range(args.plain_toys):
             d = bt.download(i)
             t1 = time.perf_counter()
-            ctx.fit_plain(M.MASS, d, starts[i], fp32=False)
+            cplain.fit_plain(M.MASS, d, starts[i], fp32=False)
             tplain += time.perf_counter() - t1
         plain_rate = args.plain_toys / tplain
         toys = {"toys": args.toys, "mean_events": float(sizes.mean()), "generate_s": tgen, "fit_s": tfit,
@@ -279,15 +310,17 @@ def main():
                 "pull_mean": pulls.mean(axis=0).round(4).tolist(), "pull_width": pulls.std(axis=0).round(4).tolist(),
                 "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
                 "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms, "host_threads": tst.host_threads,
-                "launch_shape": shape(mt)}
+                "launch_shape": shape(mt, tunet["best"]),
+                "autotune_table_us": [[list(c), round(u, 2)] for c, u in tunet["table"]]}
         log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain ({batched_rate / plain_rate:.1f}x)")
         bt.close()
         mt.close()
+        ctxt.close()
 
     # ---------------------------------------------------------------- fit service
     serving = None
     if not args.no_serve:
-        serving = serve_workload(ctx, M, O, args.serve_requests, args.serve_rate)
+        serving = serve_workload(ctx, cplain, M, O, args.serve_requests, args.serve_rate)
         log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
             f"{serving['fits_per_s']:.0f} fits/s vs {serving['plain_fp64_fits_per_s']:.1f} one at a time")
 
@@ -323,12 +356,14 @@ def main():
         "data": "synthetic: seeded accept-reject sample of the density, generated on the device",
         "config": {"workload": "mass fit f*Gauss+(1-f)*Exp on [5.0,5.6], N=1e7 events, one -2lnL+gradient evaluation per step",
                    "events": n, "precision": "FP32 per-event arithmetic, FP32 sums of 8-event groups added in FP64 per lane, FP64 totals",
-                   "launch_shape": shape(m32)},
+                   "launch_shape": shape(m32, tune32["best"]),
+                   "autotune_table_us": [[list(c), round(u, 2)] for c, u in tune32["table"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "kernel": "mf_eval_grad (fp32, per-chunk kernel)", "kernel_ms": k32_ms,
                      "algorithmic_bytes": bytes32, "with_final_pass_ms": k32_pair_ms},
         "cpu_baseline": cpu,
-        "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9},
+        "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9,
+                 "launch_shape": shape(m64, tune64["best"])},
         "lh_kernel_ms_fp32": k32_lh_ms,
         "accuracy": acc,
         "copy_ceiling_GBps": copy_gbs,
@@ -342,6 +377,8 @@ def main():
     m32.close()
     m64.close()
     ctx.close()
+    ctx64.close()
+    cplain.close()
     print(json.dumps(out), flush=True)
 
 

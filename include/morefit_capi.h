@@ -89,6 +89,15 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
  * theta for events data (SoA nobs x n): out[i * nout + k].  Checks the code generator's
  * rewrites against the density's definition. */
 int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double* out);
+/* launch-shape autotuner: for each candidate (block, events_per_thread, unroll) triple in
+ * cand[3 * ncand] (NULL: 16 built-in shapes, block 256/512 x 8..64 events per thread x unroll
+ * 4/8) times `eval_kind` evaluations (per-chunk kernel + final pass, mean of `iters`
+ * back-to-back launches) of the configuration density on a batch of nseg segments of the
+ * given sizes generated on the device at its generating values; sets the context's launch
+ * options to the fastest (used by models created afterwards), returns it in best[3] and the
+ * per-candidate us in us_per_launch[ncand] (may be NULL).  No counterpart in the reference. */
+int mfh_autotune(mfh_context* ctx, int kind, int fp32, const long long* sizes, int nseg, int eval_kind, const int* cand, int ncand,
+                 int iters, int* best, double* us_per_launch);
 /* without a device: xoshiro128++ states of streams first .. first+count-1 of `seed` by the
  * jump-ahead matrices the device seeding uses (out: 4 words per stream) */
 int mfh_stream_states(unsigned long long seed, long long first, int count, unsigned int* out);

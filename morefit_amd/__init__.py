@@ -71,7 +71,7 @@ HIP_SYMBOLS = [
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
-    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states",
+    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states", "mfh_autotune",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run",
@@ -103,6 +103,8 @@ def libs():
         L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
         L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, dp]
         L.mfh_stream_states.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ip, ctypes.POINTER(ctypes.c_uint32)]
+        L.mfh_autotune.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip, ip, ctypes.POINTER(ctypes.c_int), ip, ip,
+                                   ctypes.POINTER(ctypes.c_int), dp]
         L.mfh_batch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip, pp]
         L.mfh_batch_destroy.argtypes = [vp]
         L.mfh_batch_upload.argtypes = [vp, ip, dp, ctypes.c_longlong]
@@ -203,6 +205,19 @@ class Context:
         g = ctypes.c_double(0)
         _check(libs()[1].mfh_copy_ceiling(self.h, float(nbytes), iters, ctypes.byref(g)))
         return g.value
+
+    def autotune(self, kind, fp32, sizes, eval_kind=2, candidates=None, iters=30):
+        """Pick the launch shape (block, events per thread, unroll) of this context's evaluation
+        kernels by timing `eval_kind` evaluations of the configuration density on a batch of
+        the given segment sizes (generated on the device).  Models created afterwards use it.
+        Returns {"best": (block, ept, unroll), "table": [((block, ept, unroll), us), ...]}."""
+        cands = candidates or [(b, e, u) for b in (256, 512) for e in (8, 16, 32, 64) for u in (4, 8)]
+        flat = (ctypes.c_int * (3 * len(cands)))(*[int(v) for c in cands for v in c])
+        sz = (ctypes.c_longlong * len(sizes))(*[int(s) for s in sizes])
+        best = (ctypes.c_int * 3)()
+        us = np.zeros(len(cands))
+        _check(libs()[1].mfh_autotune(self.h, kind, int(fp32), sz, len(sizes), eval_kind, flat, len(cands), iters, best, _dp(us)))
+        return {"best": tuple(best), "table": [(tuple(c), float(u)) for c, u in zip(cands, us)]}
 
     def model(self, kind, fp32=False, rng=None, with_hessian=True):
         return Model(self, kind, fp32, rng, with_hessian)

@@ -1,5 +1,9 @@
 // capi.cc -- libmorefit_capi.so: the host C ABI of include/morefit_capi.h over the morefit
 // C++ headers (graph, densities, fitter, HIP and CPU backends, batched fits).
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <fstream>
+#include <sstream>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -8,6 +12,7 @@
 #include <vector>
 
 #include "morefit.hh"
+#include "morefit/autotune.hh"
 #include "morefit/batch.hh"
 #include "morefit/serve.hh"
 #include "morefit_capi.h"
@@ -321,6 +326,122 @@ int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, c
     ModelDef<double> d(kind, nullptr);
     run(d);
   }
+  MFH_CATCH
+}
+
+}  // extern "C"
+
+namespace {
+// Autotuner results persist in <kernel cache>/autotune.txt (MOREFIT_KCACHE, else kcache/ next
+// to the library directory), one line per workload key, so later processes on the same
+// machine reuse the same launch shape instead of timing again.
+std::string autotune_path() {
+  const char* env = getenv("MOREFIT_KCACHE");
+  if (env && *env) return std::string(env) + "/autotune.txt";
+  Dl_info info;
+  std::string dir = ".";
+  if (dladdr((void*)&mfh_last_error, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    size_t k = p.rfind('/');
+    if (k != std::string::npos) dir = p.substr(0, k);
+  }
+  return dir + "/../kcache/autotune.txt";
+}
+uint64_t fnv64(const std::string& s) {
+  uint64_t h = 1469598103934665603ULL;
+  for (unsigned char ch : s) {
+    h ^= ch;
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+bool autotune_lookup(const std::string& key, std::vector<double>& us, int* best) {
+  std::ifstream f(autotune_path());
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string k;
+    int n = 0;
+    if (!(is >> k >> best[0] >> best[1] >> best[2] >> n) || k != key || n != (int)us.size()) continue;
+    for (int i = 0; i < n; i++) is >> us[i];
+    if (is) return true;
+  }
+  return false;
+}
+void autotune_store(const std::string& key, const std::vector<double>& us, const int* best) {
+  std::string path = autotune_path();
+  mkdir(path.substr(0, path.rfind('/')).c_str(), 0755);
+  std::ofstream f(path, std::ios::app);
+  if (!f) return;
+  f << key << " " << best[0] << " " << best[1] << " " << best[2] << " " << us.size();
+  for (double u : us) f << " " << u;
+  f << "\n";
+}
+}  // namespace
+
+extern "C" {
+
+int mfh_autotune(mfh_context* ctx, int kind, int fp32, const long long* sizes, int nseg, int eval_kind, const int* cand, int ncand,
+                 int iters, int* best, double* us_per_launch) {
+  MFH_TRY
+  if (!ctx || !sizes || nseg <= 0 || eval_kind < 1 || eval_kind > 3 || iters <= 0 || !best)
+    throw std::invalid_argument("mfh_autotune: bad argument");
+  std::vector<LaunchShape> cs;
+  if (cand && ncand > 0)
+    for (int i = 0; i < ncand; i++) cs.push_back({cand[3 * i], cand[3 * i + 1], cand[3 * i + 2]});
+  else
+    cs = default_launch_candidates();
+  for (auto& s : cs)
+    if (s.block < 64 || s.block > 1024 || s.block % 64 || s.events_per_thread < 1 || s.unroll < 1)
+      throw std::invalid_argument("mfh_autotune: bad candidate launch shape");
+  std::vector<long long> sz(sizes, sizes + nseg);
+  LaunchShape win;
+  std::vector<double> us(cs.size(), NAN);
+  auto run = [&](auto& d) {
+    typedef typename std::remove_reference<decltype(d)>::type::kernel_type K;
+    // key: device, workload, candidates and the generated module (so a code change re-tunes)
+    std::ostringstream k;
+    k << ctx->be->props().name << "|" << ctx->be->props().compute_units << "|" << kind << "|" << fp32 << "|" << eval_kind << "|"
+      << iters << "|" << nseg;
+    long long tot = 0, mx = 0;
+    for (auto v : sz) {
+      tot += v;
+      mx = std::max(mx, v);
+    }
+    k << "|" << tot << "|" << mx;
+    for (auto& s : cs) k << "|" << s.block << "," << s.events_per_thread << "," << s.unroll;
+    k << "|" << HipModel<K>::make_source(d.pdf, d.params, ctx->co, eval_kind == EVAL_HESS).device_source();
+    char key[32];
+    std::snprintf(key, sizeof(key), "%016llx", (unsigned long long)fnv64(k.str()));
+    int b3[3];
+    if (autotune_lookup(key, us, b3)) {
+      win = {b3[0], b3[1], b3[2]};
+      ctx->co.hip_block_size = win.block;
+      ctx->co.hip_events_per_thread = win.events_per_thread;
+      ctx->co.hip_unroll = win.unroll;
+      return;
+    }
+    std::vector<double> tg;
+    for (auto* p : d.params)
+      if (!p->is_constant()) tg.push_back(p->get_value());
+    us = autotune_launch_shape(ctx->be.get(), d.pdf, d.params, sz, eval_kind, tg, cs, iters, &win);
+    b3[0] = win.block;
+    b3[1] = win.events_per_thread;
+    b3[2] = win.unroll;
+    autotune_store(key, us, b3);
+  };
+  if (fp32) {
+    ModelDef<float> d(kind, nullptr);
+    run(d);
+  } else {
+    ModelDef<double> d(kind, nullptr);
+    run(d);
+  }
+  best[0] = win.block;
+  best[1] = win.events_per_thread;
+  best[2] = win.unroll;
+  if (us_per_launch)
+    for (size_t i = 0; i < us.size(); i++) us_per_launch[i] = us[i];
   MFH_CATCH
 }
 

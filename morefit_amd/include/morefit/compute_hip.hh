@@ -180,6 +180,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     long long nevents = 0, nevents_padded = 0;
     int nstreams = 0;
     double maxprob_value = 1.0;
+    int block = 256;              // work-group size the module was compiled for (MF_BLOCK)
     std::vector<double> centres;  // observables stored as x - centre (FP32 path), empty: raw
     ~State() {
       if (mod) mf_module_unload(mod);
@@ -201,7 +202,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     sg.out_off = 0;
     sg.mode = 1;
     std::vector<mf_chunk> ch;
-    append_chunks(ch, sg, 0, s.nevents, opts().hip_block_size * opts().hip_events_per_thread);
+    append_chunks(ch, sg, 0, s.nevents, s.block * opts().hip_events_per_thread);
     s.segs.alloc(dev(), sizeof(mf_segment));
     s.chunks.alloc(dev(), ch.size() * sizeof(mf_chunk));
     hip_check(mf_memcpy_h2d(dev(), s.segs.ptr, &sg, sizeof(sg)), "HIPBlock segs");
@@ -220,6 +221,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     if (s.mod) mf_module_unload(s.mod);
     s.mod = nullptr;
     fit_register_budget(ms, opts().hip_compile_options);
+    s.block = ms.block;
     std::string src = ms.device_source();
     if (opts().print_kernel) std::cout << "KERNEL SOURCE\n" << src << "END KERNEL SOURCE" << std::endl;
     hip_check(mf_module_load(dev(), src.c_str(), opts().hip_compile_options.c_str(), &s.mod), "HIPBlock: mf_module_load");
@@ -429,7 +431,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
       a.store_stride = s.nevents_padded;
     }
     const int grid = opts().hip_grid > 0 ? std::min(opts().hip_grid, nchunks_) : nchunks_;
-    hip_check(mf_launch_eval(s.mod, s.kernel.c_str(), &a, grid, opts().hip_block_size), "mf_launch_eval");
+    hip_check(mf_launch_eval(s.mod, s.kernel.c_str(), &a, grid, s.block), "mf_launch_eval");
     if (s.kahan) hip_check(mf_memcpy_d2h(dev(), s.results_h.ptr, s.results.ptr, s.nout * sizeof(double)), "kahan results");
     return true;
   }
@@ -563,6 +565,8 @@ class HipModel {
   mf_module* mod = nullptr;
   std::vector<std::pair<std::string, int>> unrolls;  // evaluation kernel -> unroll
   std::vector<mf_kernel_resources> resources;        // every kernel of the module
+  int block = 256;                                   // work-group size compiled into the module
+  int grid_opt = 0;                                  // hip_grid at construction (0: one group per chunk)
 
   // generated module with the unroll of each evaluation kernel fitted to the register budget
   static ModuleSource make_source(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, const compute_options& o,
@@ -598,6 +602,8 @@ class HipModel {
     ModuleSource ms = make_source(p, ps, *be->opts_, hessian, &g, &resources, &fit_ms);
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> keep(g);
     for (auto& e : ms.evals) unrolls.push_back({e.name, e.unroll});
+    block = ms.block;
+    grid_opt = be->opts_->hip_grid;
     for (size_t i = 0; i < ps.size(); i++)
       if (!ps[i]->is_constant()) theta_index.push_back((int)i);
     obs = g->obs;
@@ -746,7 +752,7 @@ class HipBatch {
   void init(int capacity, long long elems) {
     const compute_options& o = *m->be->opts_;
     S = capacity;
-    chunk = o.hip_block_size * o.hip_events_per_thread;
+    chunk = m->block * o.hip_events_per_thread;
     arena.reset(elems);
     total_elems = elems;
     sizes.assign(S, 0);
@@ -963,7 +969,7 @@ class HipBatch {
     return kind == EVAL_LH ? "mf_eval_lh" : (kind == EVAL_GRAD ? "mf_eval_grad" : "mf_eval_hess");
   }
   int grid() const {
-    const int g = m->be->opts_->hip_grid;
+    const int g = m->grid_opt;
     return g > 0 ? std::min(g, nchunks) : nchunks;
   }
   mf_eval_args eval_args(int kind) const {
@@ -986,7 +992,7 @@ class HipBatch {
     if (nchunks == 0) return;
     mf_eval_args a = eval_args(kind);
     if (main_only) a.flags |= MF_EVAL_SKIP_FINAL;
-    hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->be->opts_->hip_block_size), "mf_launch_eval");
+    hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->block), "mf_launch_eval");
   }
   // the full per-call sequence of one kind: prep (theta and segment modes from mapped host
   // memory) -> per-chunk kernel -> final pass (totals into mapped host memory)

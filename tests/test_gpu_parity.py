@@ -272,3 +272,29 @@ def test_full_size_properties(ctx):
         x.close()
     m64.close()
     m32.close()
+
+
+def test_autotuner_picks_a_candidate_and_results_hold():
+    """the autotuner times every candidate, leaves the context at the fastest, and a model
+    built afterwards (that launch shape) still matches the oracle"""
+    n = 300_000
+    cands = [(256, 8, 4), (256, 32, 8), (512, 16, 8)]
+    cx = M.Context()
+    say("autotune", cands)
+    r = cx.autotune(M.MASS, True, [n, 7000, 1], eval_kind=M.EVAL_GRAD, candidates=cands, iters=5)
+    us = [u for _, u in r["table"]]
+    assert all(np.isfinite(u) and u > 0 for u in us)
+    assert r["best"] == r["table"][int(np.argmin(us))][0]
+    data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=5, nstreams=16)
+    theta = np.array(M.MASS_START)
+    mdl = cx.model(M.MASS, fp32=True)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    out = b.eval(M.EVAL_GRAD, theta)[0]
+    stored = _stored(M.MASS, data, True)
+    ref = O.nll_sums(M.MASS, stored, theta, mode=1, nthreads=4)
+    absref = O.nll_sums(M.MASS, stored, theta, mode=11, nthreads=4)
+    assert np.all(np.abs(out - ref) <= 2e-6 * absref)
+    b.close()
+    mdl.close()
+    cx.close()
