@@ -86,22 +86,37 @@ def plain_context(M, O):
 
 
 def serve_workload(ctx, cplain, M, O, nreq, rate, seed=11, plain_subset=12):
-    """Fit requests arriving as a Poisson stream at `rate` per second: 70 % mass fits, 30 %
-    angular fits, event counts log-uniform in [1e3, 2e6], data generated on the device per
-    request (seeded), FP32 kernels, evaluations of concurrent fits in shared launches.
-    Compared with submitting the same kinds of fits one at a time through the plain FP64
-    per-fit path (a subset of the requests, data generated by the oracle's generator)."""
+    """Fit requests: 70 % mass fits, 30 % angular fits, event counts log-uniform in [1e3, 2e6],
+    data generated on the device per request (seeded), FP32 kernels, evaluations of concurrent
+    fits in shared launches, admission bounded by a 64 GB HBM budget.  First all requests at
+    once (the sustainable throughput), then Poisson arrivals at 50 %, 80 % and 95 % of it and at
+    `rate` per second, each with p50/p99 latency from arrival to result.  Compared with
+    submitting the same kinds of fits one at a time through the plain FP64 per-fit path (a
+    subset of the requests, data generated by the oracle's generator)."""
     rs = np.random.default_rng(seed)
     cls = (rs.random(nreq) < 0.3).astype(int)
     n = np.exp(rs.uniform(np.log(1e3), np.log(2e6), nreq)).astype(np.int64)
     seeds = rs.integers(1, 2**62, nreq)
-    arrival = np.cumsum(rs.exponential(1.0 / rate, nreq))
+    gaps = rs.exponential(1.0, nreq)
     mass, kst = ctx.model(M.MASS, fp32=True), ctx.model(M.KSTAR, fp32=True)
     truth = [M.MASS_TRUTH, M.KSTAR_TRUTH]
-    r = M.serve([mass, kst], truth, truth, cls, n, seeds, arrival, budget_bytes=8 << 30, slots=512)
-    lat = (r["done"] - r["arrival"]) * 1e3
-    ok = r["status"] == 0
-    span = float(r["done"].max() - r["arrival"].min())
+
+    def run(arrival):
+        r = M.serve([mass, kst], truth, truth, cls, n, seeds, arrival, budget_bytes=64 << 30, slots=512)
+        lat = (r["done"] - r["arrival"]) * 1e3
+        span = float(r["done"].max() - r["arrival"].min())
+        return {"fits_per_s": nreq / span, "p50_latency_ms": float(np.percentile(lat, 50)),
+                "p99_latency_ms": float(np.percentile(lat, 99)), "converged": int(np.sum(r["status"] == 0)),
+                "rejected": r["rejected"], "ticks": r["ticks"], "launches": r["launches"], "peak_live": r["peak_live"],
+                "generate_ms": r["generate_ms"], "span_s": span}
+
+    sat = run(np.zeros(nreq))
+    sustained = sat["fits_per_s"]
+    rows = []
+    for label, lam in [("50%", 0.5 * sustained), ("80%", 0.8 * sustained), ("95%", 0.95 * sustained), ("fixed", rate)]:
+        row = run(np.cumsum(gaps / lam))
+        row.update({"arrival_rate_per_s": lam, "load": label})
+        rows.append(row)
     t_plain = 0.0
     for i in range(plain_subset):
         d = O.generate(int(cls[i]), truth[cls[i]], int(n[i]), seed=int(seeds[i]) % (2**63), nstreams=64)
@@ -114,15 +129,65 @@ def serve_workload(ctx, cplain, M, O, nreq, rate, seed=11, plain_subset=12):
     plain_rate = nreq / (t_plain * ev_all / ev_sub)
     mass.close()
     kst.close()
-    return {"requests": nreq, "arrival_rate_per_s": rate, "mass_fraction": float(np.mean(cls == 0)),
-            "events_log_uniform": [1000, 2000000], "converged": int(ok.sum()), "rejected": r["rejected"],
-            "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
-            "fits_per_s": nreq / span, "span_s": span, "ticks": r["ticks"], "launches": r["launches"],
-            "peak_live": r["peak_live"], "generate_ms": r["generate_ms"],
+    fixed = rows[-1]
+    return {"requests": nreq, "mass_fraction": float(np.mean(cls == 0)), "events_log_uniform": [1000, 2000000],
+            "hbm_budget_bytes": 64 << 30, "saturation": sat, "sustained_fits_per_s": sustained, "rates": rows,
+            "p50_latency_ms": fixed["p50_latency_ms"], "p99_latency_ms": fixed["p99_latency_ms"],
+            "fits_per_s": fixed["fits_per_s"], "arrival_rate_per_s": rate,
             "plain_fp64_fits_per_s": plain_rate,
             "plain_fp64_sample": f"{plain_subset} requests fitted one at a time (FP64 plain path), "
                                  "time scaled to the whole stream by its event count (angular events weighted x3)",
-            "speedup_vs_plain": (nreq / span) / plain_rate}
+            "speedup_vs_plain": sustained / plain_rate}
+
+
+def angular_workload(M, O):
+    """The angular configuration: K*mumu P-wave density in (cos theta_l, cos theta_K, phi), 8 free
+    parameters, N = 10^6 events generated on the device (FP64 generator), FP32 kernels in the
+    autotuned shape and FP64 kernels: -2lnL+gradient per call and per kernel, accuracy against
+    the oracle (FP32 on the same stored observables), fits of both from a 1 sigma smeared start."""
+    n = 1_000_000
+    ctx, tune = tuned_context(M, M.KSTAR, True, [n], "angular N=1e6 FP32")
+    ctx64 = M.Context(device=0)  # FP64 at the default launch shape
+    m64 = ctx64.model(M.KSTAR, fp32=False)
+    b64 = m64.batch([n])
+    b64.generate(M.KSTAR_TRUTH, [20240607])
+    data = b64.download(0)
+    m32 = ctx.model(M.KSTAR, fp32=True)
+    b32 = m32.batch([n])
+    b32.upload(0, data)
+    theta = np.array(M.KSTAR_START)
+    out = {"events": n, "launch_shape": shape(m32, tune["best"])}
+    for label, b in (("fp32", b32), ("fp64", b64)):
+        us = b.time_eval(M.EVAL_GRAD, iters=100)
+        k = b.time_kernel(M.EVAL_GRAD, iters=100, main_only=True)
+        out[label] = {"evals_per_s": 1e6 / us, "us_per_eval": us, "kernel_us": k * 1e3,
+                      "GBps": n * 3 * (4 if label == "fp32" else 8) / (k * 1e-3) / 1e9}
+    ref = O.nll_sums(M.KSTAR, data, theta, mode=1, nthreads=0)
+    stored = data.astype(np.float32).astype(np.float64)  # the angular ranges are centred on 0
+    ref32 = O.nll_sums(M.KSTAR, stored, theta, mode=1, nthreads=0)
+    absref = O.nll_sums(M.KSTAR, data, theta, mode=11, nthreads=8)
+    r64 = b64.eval(M.EVAL_GRAD, theta)[0]
+    r32 = b32.eval(M.EVAL_GRAD, theta)[0]
+    out["fp64"]["rel_err_m2lnL"] = float(abs(r64[0] - ref[0]) / abs(ref[0]))
+    out["fp64"]["max_grad_err_scaled"] = float(np.max(np.abs(r64[1:] - ref[1:]) / absref[1:]))
+    out["fp32"]["rel_err_m2lnL_same_stored_data"] = float(abs(r32[0] - ref32[0]) / abs(ref32[0]))
+    out["fp32"]["max_grad_err_scaled"] = float(np.max(np.abs(r32[1:] - ref32[1:]) / absref[1:]))
+    start = M.toy_starts(M.KSTAR, [n], seed=3)[0]
+    v64, e64, o64, s64 = b64.fit(start)
+    v32, e32, o32, s32 = b32.fit(start)
+    out["fit"] = {"status": [o32[0].status, o64[0].status], "fp32_vs_fp64_max_dtheta_over_sigma":
+                  float(np.max(np.abs(v32[0] - v64[0]) / e64[0])), "fit_ms": [s32.wall_ms, s64.wall_ms],
+                  "fp64_values": v64[0].tolist(), "fp64_errors": e64[0].tolist()}
+    log(f"angular N=1e6: fp32 {out['fp32']['evals_per_s']:.0f} evals/s (kernel {out['fp32']['kernel_us']:.1f} us), "
+        f"fp64 {out['fp64']['evals_per_s']:.0f} evals/s (kernel {out['fp64']['kernel_us']:.1f} us); "
+        f"fits fp32/fp64 dtheta/sigma {out['fit']['fp32_vs_fp64_max_dtheta_over_sigma']:.2e}")
+    for x in (b32, b64):
+        x.close()
+    m32.close()
+    m64.close()
+    ctx.close()
+    ctx64.close()
+    return out
 
 
 def profile_workload():
@@ -155,6 +220,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-angular", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     args = ap.parse_args()
@@ -274,6 +340,9 @@ def main():
         bs.close()
         ms.close()
 
+    # ---------------------------------------------------------------- angular configuration
+    angular = None if args.no_angular else angular_workload(M, O)
+
     # ---------------------------------------------------------------- plain per-fit FP64 path
     cplain = plain_context(M, O)
 
@@ -321,25 +390,33 @@ def main():
     serving = None
     if not args.no_serve:
         serving = serve_workload(ctx, cplain, M, O, args.serve_requests, args.serve_rate)
-        log(f"serving: p50 {serving['p50_latency_ms']:.1f} ms, p99 {serving['p99_latency_ms']:.1f} ms, "
-            f"{serving['fits_per_s']:.0f} fits/s vs {serving['plain_fp64_fits_per_s']:.1f} one at a time")
+        log(f"serving: sustained {serving['sustained_fits_per_s']:.0f} fits/s vs {serving['plain_fp64_fits_per_s']:.1f} "
+            f"one at a time; " + "; ".join(f"{r['load']} ({r['arrival_rate_per_s']:.0f}/s): p50 {r['p50_latency_ms']:.1f} ms "
+                                           f"p99 {r['p99_latency_ms']:.1f} ms" for r in serving["rates"]))
 
     # ---------------------------------------------------------------- CPU baseline (oracle)
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         sample = 2_000_000
         d = data[:, :sample]
-        t0 = time.perf_counter()
-        reps = 0
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            O.nll_sums(O.MASS, d, start, mode=1, nthreads=1)
-            reps += 1
-        el = time.perf_counter() - t0
-        per_eval_full = el / reps * (n / sample)
-        cpu = {"value": 1.0 / per_eval_full, "unit": "evals/s", "cores": 1, "kind": "port",
-               "sample": f"{reps} evaluations of -2lnL+grad over the first {sample} of the {n} events "
-                         f"(FP64, per-output Kahan, compute_llvm.hh semantics), scaled to N={n}"}
-        log(f"cpu baseline: {cpu['value']:.3f} evals/s (1 thread)")
+
+        def cpu_rate(threads, seconds):
+            t0 = time.perf_counter()
+            reps = 0
+            while time.perf_counter() - t0 < seconds:
+                O.nll_sums(O.MASS, d, start, mode=1, nthreads=threads)
+                reps += 1
+            el = time.perf_counter() - t0
+            return reps, 1.0 / (el / reps * (n / sample))
+
+        cores = max(1, min(16, len(os.sched_getaffinity(0))))
+        reps1, rate1 = cpu_rate(1, args.cpu_seconds / 2)
+        repsn, raten = cpu_rate(cores, args.cpu_seconds / 2)
+        what = (f"evaluations of -2lnL+grad over the first {sample} of the {n} events (FP64, per-output Kahan, "
+                f"compute_llvm.hh semantics: contiguous slices per thread), scaled to N={n}")
+        cpu = {"value": raten, "unit": "evals/s", "cores": cores, "kind": "port", "sample": f"{repsn} {what}"}
+        cpu1 = {"value": rate1, "unit": "evals/s", "cores": 1, "kind": "port", "sample": f"{reps1} {what}"}
+        log(f"cpu baseline: {raten:.3f} evals/s ({cores} threads), {rate1:.3f} evals/s (1 thread)")
 
     out = {
         "metric": "-2lnL+grad evals/s; toy fits/s vs plain GPU path; p50/p99 fit latency",
@@ -362,6 +439,7 @@ def main():
                      "traffic": traffic, "kernel": "mf_eval_grad (fp32, per-chunk kernel)", "kernel_ms": k32_ms,
                      "algorithmic_bytes": bytes32, "with_final_pass_ms": k32_pair_ms},
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9,
                  "launch_shape": shape(m64, tune64["best"])},
         "lh_kernel_ms_fp32": k32_lh_ms,
@@ -370,6 +448,7 @@ def main():
         "streaming": stream,
         "toys": toys,
         "serving": serving,
+        "angular": angular,
         "device": name,
     }
     for x in (b32, b64):

@@ -13,6 +13,7 @@
  *   mfh_batch_generate   generator::generate on the accelerator (generator.hh:71-175)
  *   mfh_fit_plain        fitter::fit with the drop-in HIPBlock (FP64 or FP32), one launch per call
  *   mfh_fit_cpu          fitter::fit with the host-compiled CPU backend (reference LLVM semantics)
+ *   mfh_fit_weighted     fitter::fit of weighted events with correct_weighted_fit (fitter.hh:894-947, 1327-1432)
  *   mfh_batch_fit        toystudy::toy's fit loop (toystudy.hh:55-111) as one batched study
  *   mfh_serve_run        no counterpart: continuous-batching fit service (serve.hh)
  * Status: 0 on success, else nonzero with mfh_last_error().
@@ -141,6 +142,11 @@ int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long
  * kernels (FP32-rounded parameters, centred FP32 observables) as the device's FP32 path does */
 int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, double* values,
                 double* errors, mfh_fit_out* out, double* ms);
+/* weighted fit (log p of event i multiplied by weights[i]) with the reference's covariance
+ * correction -2 H^-1 F H^-1 (Fisher matrix F from analytic kernels): through the drop-in
+ * HIPBlock when ctx is set, else through the CPU backend with nthreads threads */
+int mfh_fit_weighted(mfh_context* ctx, int kind, int fp32, const double* data, const double* weights, long long n, const double* start,
+                     int nthreads, double* values, double* errors, mfh_fit_out* out, double* ms);
 /* -2lnL, gradient (-2 scale) and Hessian through the CPU backend at theta (no device needed) */
 int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess);
 

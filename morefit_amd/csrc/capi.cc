@@ -660,16 +660,25 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
 namespace {
 template <typename K, typename Backend, typename Block>
 void run_fit(Backend* be, int kind, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
-             double* ms) {
+             double* ms, const double* weights = nullptr) {
   ModelDef<K> def(kind, nullptr);
-  EventVector<double, double> ev(def.dimv, (unsigned long)n);
+  // weights ride along as an extra column named "w" that the fitter multiplies log p by
+  dimension<double> wdim("w", -1e300, 1e300);
+  std::vector<dimension<double>*> dims = def.dimv;
+  if (weights) dims.push_back(&wdim);
+  EventVector<double, double> ev(dims, (unsigned long)n);
   for (size_t d = 0; d < def.dimv.size(); d++)
     for (long long i = 0; i < n; i++) ev(i, (unsigned)d) = data[d * n + i];
+  if (weights) {
+    for (long long i = 0; i < n; i++) ev(i, (unsigned)def.dimv.size()) = weights[i];
+    ev.set_event_weight("w");
+  }
   for (size_t k = 0; k < def.params.size(); k++) {
     def.params[k]->set_value(start[k]);
     def.params[k]->set_start_value(start[k]);
   }
   fitter_options fo;
+  fo.analytic_fisher = weights != nullptr;
   fitter<K, double, Backend, Block> f(&fo, be);
   f.fit(def.pdf, def.params, &ev);
   for (size_t k = 0; k < def.params.size(); k++) {
@@ -706,6 +715,23 @@ int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const doubl
   CPUBackend be(&co);
   if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms);
   else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms);
+  MFH_CATCH
+}
+
+int mfh_fit_weighted(mfh_context* ctx, int kind, int fp32, const double* data, const double* weights, long long n, const double* start,
+                     int nthreads, double* values, double* errors, mfh_fit_out* out, double* ms) {
+  MFH_TRY
+  if (!weights) throw std::invalid_argument("mfh_fit_weighted: no weights");
+  if (ctx) {
+    if (fp32) run_fit<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms, weights);
+    else run_fit<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms, weights);
+  } else {
+    compute_options co;
+    co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
+    CPUBackend be(&co);
+    if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms, weights);
+    else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms, weights);
+  }
   MFH_CATCH
 }
 

@@ -82,11 +82,11 @@ class fitter {
   typedef GraphPtr<kernelT, evalT> Ptr;
   fitter_options* opts_;
   backendT* backend_;
-  computeT block_, grad_block_, hessian_block_;
+  computeT block_, grad_block_, hessian_block_, fisher_block_;
   std::vector<parameter<evalT>*> params_;
   PDF<kernelT, evalT>* pdf_ = nullptr;
-  std::vector<Ptr> buffer_expressions_, grad_buffer_expressions_, hessian_buffer_expressions_;
-  std::vector<std::string> buffer_names_, grad_buffer_names_, hessian_buffer_names_;
+  std::vector<Ptr> buffer_expressions_, grad_buffer_expressions_, hessian_buffer_expressions_, fisher_buffer_expressions_;
+  std::vector<std::string> buffer_names_, grad_buffer_names_, hessian_buffer_names_, fisher_buffer_names_;
   std::vector<std::string> floating_names_;
   std::vector<double> centres_;
   size_t nevents_ = 0;
@@ -99,7 +99,7 @@ class fitter {
   double fit_ms_ = 0.0, kernel_ms_ = 0.0;
 
   fitter(fitter_options* options, backendT* backend)
-      : opts_(options), backend_(backend), block_(backend), grad_block_(backend), hessian_block_(backend) {}
+      : opts_(options), backend_(backend), block_(backend), grad_block_(backend), hessian_block_(backend), fisher_block_(backend) {}
 
   bool centring() const {
     return std::is_same<kernelT, float>::value && has_set_centres<computeT>::value && backend_->opts_->hip_centre_observables;
@@ -170,6 +170,16 @@ class fitter {
       std::vector<dimension<evalT>> out(k.size(), res);
       hessian_block_.MakeComputeKernel("lh_hessian_kernel", nevents_, in_sig, out, param_sig(hessian_buffer_names_), k, true);
     }
+    // Fisher information (fitter.hh:471-497): per event (w d_i log p)(w d_j log p), i >= j, from
+    // the derivatives of the weighted log-density
+    if (opts_->analytic_fisher) {
+      std::vector<Ptr> g;
+      for (size_t i = 0; i < d1.size(); i++)
+        for (size_t j = 0; j <= i; j++) g.emplace_back((d1[i]->copy() * d1[j]->copy())->simplify());
+      auto k = hoisted(g, fisher_buffer_names_, fisher_buffer_expressions_, "morefit_parambuffer_");
+      std::vector<dimension<evalT>> out(k.size(), res);
+      fisher_block_.MakeComputeKernel("lh_fisher_kernel", nevents_, in_sig, out, param_sig(fisher_buffer_names_), k, true);
+    }
     return true;
   }
 
@@ -199,6 +209,12 @@ class fitter {
       hessian_block_.SetupInputBuffer(&block_, true);
       hessian_block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
       hessian_block_.SetupParameterBuffer((floating_names_.size() + hessian_buffer_names_.size() + 1) * sizeof(kernelT));
+    }
+    if (opts_->analytic_fisher) {
+      apply_centres(fisher_block_, centres_);
+      fisher_block_.SetupInputBuffer(&block_, true);
+      fisher_block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
+      fisher_block_.SetupParameterBuffer((floating_names_.size() + fisher_buffer_names_.size() + 1) * sizeof(kernelT));
     }
   }
 
@@ -301,6 +317,62 @@ class fitter {
     return true;
   }
 
+  // Fisher matrix of the (weighted) log-likelihood in -2 lnL units, n x n at full-matrix
+  // positions (fitter.hh:1327-1432): -2 sum (w d_i log p)(w d_j log p), plus 2N/nu^2 for pairs of
+  // yields of an extended density
+  bool fisher(int npar, double* f) {
+    if (!opts_->analytic_fisher) throw std::runtime_error("fitter::fisher: analytic_fisher is off");
+    auto s = run(fisher_block_, fisher_buffer_expressions_);
+    const size_t N = params_.size();
+    for (size_t i = 0; i < N * N; i++) f[i] = 0.0;
+    std::vector<int> fl;
+    for (size_t i = 0; i < N; i++)
+      if (!params_[i]->is_constant()) fl.push_back((int)i);
+    size_t idx = 0;
+    for (size_t i = 0; i < fl.size(); i++)
+      for (size_t j = 0; j <= i; j++) {
+        double v = -2.0 * s.at(idx++);
+        f[fl[i] + N * fl[j]] = v;
+        f[fl[j] + N * fl[i]] = v;
+      }
+    if (pdf_->is_extended()) {
+      double nu = total_yield();
+      for (size_t i = 0; i < fl.size(); i++)
+        for (size_t j = 0; j < fl.size(); j++)
+          if (is_yield(params_[fl[i]]) && is_yield(params_[fl[j]])) f[fl[i] + N * fl[j]] += 2.0 * nevents_ / (nu * nu);
+    }
+    return true;
+  }
+
+  // covariance of a weighted fit (fitter.hh:894-947): -2 H^-1 F H^-1 over the floating
+  // parameters, H the Hessian and F the Fisher matrix of -2 lnL at the minimum
+  std::vector<double> weighted_covariance() {
+    const size_t N = params_.size();
+    std::vector<double> h(N * N), f(N * N);
+    hessian((int)N, h.data());
+    fisher((int)N, f.data());
+    std::vector<int> fl;
+    for (size_t i = 0; i < N; i++)
+      if (!params_[i]->is_constant()) fl.push_back((int)i);
+    const int n = (int)fl.size();
+    std::vector<double> hi(n * n), fm(n * n), t(n * n, 0.0), c(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        hi[i * n + j] = h[fl[i] + N * fl[j]];
+        fm[i * n + j] = f[fl[i] + N * fl[j]];
+      }
+    if (!la::spd_inverse(hi, n)) throw std::runtime_error("weighted fit correction: Hessian not positive-definite");
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++) t[i * n + j] += hi[i * n + k] * fm[k * n + j];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        for (int k = 0; k < n; k++) c[i * n + j] += t[i * n + k] * hi[k * n + j];
+        c[i * n + j] *= -2.0;
+      }
+    return c;
+  }
+
   void update_parameters(const double* x) {
     for (size_t i = 0; i < params_.size(); i++) params_[i]->set_value(x[i]);
   }
@@ -363,6 +435,18 @@ class fitter {
     m.minimize(fcn);
     update_parameters(m.x.data());
     if (opts_->postrun_hesse && opts_->analytic_hessian) m.hesse(fcn);
+    if (weighted_ && opts_->correct_weighted_fit) {
+      if (!(opts_->analytic_fisher && opts_->analytic_hessian))
+        throw std::runtime_error("correction for weighted fit only supported with analytic hessian and fisher matrix");
+      update_parameters(m.x.data());
+      m.cov = weighted_covariance();
+      int k = 0;
+      for (size_t i = 0; i < params_.size(); i++)
+        if (!params_[i]->is_constant()) {
+          m.errors[i] = std::sqrt(m.cov[(size_t)k * (m.nfree()) + k]);
+          k++;
+        }
+    }
     for (size_t i = 0; i < params_.size(); i++)
       if (!params_[i]->is_constant()) {
         params_[i]->set_value(m.x[i]);

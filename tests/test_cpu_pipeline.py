@@ -128,3 +128,31 @@ def test_generated_outputs_match_oracle_per_event(model, fp32):
         for w, k in ((1, 1), (2, 1 + P), (3, len(ref))):
             got = outs[w][e]
             assert np.all(np.abs(got - ref[:k]) <= 1e-10 * scale[:k]), (w, e, got - ref[:k])
+
+
+def test_weighted_fit_covariance_correction_matches_oracle_sandwich():
+    """weighted fit (w * log p) with the reference's correction (fitter.hh:894-947): the fitted
+    values zero the oracle's weighted gradient, and the errors equal sqrt(diag(-2 H^-1 F H^-1))
+    with H = -2 sum w d2 log p and F = -2 sum w^2 (d log p)(d log p)^T from the oracle's
+    per-event jets at the fitted values"""
+    n = 3000
+    data = O.generate(O.MASS, MASS_TRUE, n, seed=77, nstreams=4)
+    w = np.random.default_rng(5).uniform(0.4, 1.6, n)
+    v, e, out, _ = M.fit_weighted(O.MASS, data, w, M.MASS_START, nthreads=2)
+    assert out.status == 0
+    P = 4
+    g = np.zeros(P)
+    H = np.zeros((P, P))
+    F = np.zeros((P, P))
+    for i in range(n):
+        lp, gi, hi = O.event_jet(O.MASS, data[:, i], v)
+        g += w[i] * gi
+        H += -2.0 * w[i] * hi
+        F += -2.0 * w[i] ** 2 * np.outer(gi, gi)
+    # stationary point of the weighted likelihood (scale: the per-event gradient size)
+    assert np.all(np.abs(g) * np.sqrt(np.diag(np.linalg.inv(H / 2))) < 1e-3)
+    Hi = np.linalg.inv(H)
+    C = -2.0 * Hi @ F @ Hi
+    np.testing.assert_allclose(e, np.sqrt(np.diag(C)), rtol=1e-6)
+    # and it differs from the uncorrected HESSE errors (the correction is live)
+    assert np.max(np.abs(e / np.sqrt(np.diag(2 * Hi)) - 1)) > 0.01

@@ -298,3 +298,19 @@ def test_autotuner_picks_a_candidate_and_results_hold():
     b.close()
     mdl.close()
     cx.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_weighted_fit_gpu_equals_cpu_backend(ctx, fp32):
+    """weighted fit with the covariance correction through the drop-in HIP block equals the CPU
+    backend's (FP64: 1e-4 sigma, errors 1e-6; FP32: 1e-3 sigma, errors 1e-3)"""
+    n = 50_000
+    data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=91, nstreams=16)
+    w = np.random.default_rng(6).uniform(0.4, 1.6, n)
+    say("weighted fit", "fp32", fp32)
+    v, e, out, _ = M.fit_weighted(M.MASS, data, w, M.MASS_START, ctx=ctx, fp32=fp32)
+    assert out.status == 0
+    vc, ec, oc, _ = M.fit_weighted(M.MASS, data, w, M.MASS_START, nthreads=4)
+    assert oc.status == 0
+    assert np.max(np.abs(v - vc) / ec) < (1e-3 if fp32 else 1e-4)
+    np.testing.assert_allclose(e, ec, rtol=1e-3 if fp32 else 1e-6)
