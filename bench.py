@@ -43,7 +43,8 @@ def shape(model, tuned):
     return {"block": tuned[0], "events_per_thread": tuned[1], "requested_unroll": tuned[2], "accumulate": o.accumulate,
             "fast_math": bool(o.fast_math), "unroll": {"lh": model.unroll[M.EVAL_LH], "grad": model.unroll[M.EVAL_GRAD],
                                                        "hess": model.unroll[M.EVAL_HESS]},
-            "vgprs": {"lh": model.vgprs[M.EVAL_LH], "grad": model.vgprs[M.EVAL_GRAD], "hess": model.vgprs[M.EVAL_HESS]}}
+            "vgprs": {"lh": model.vgprs[M.EVAL_LH], "grad": model.vgprs[M.EVAL_GRAD], "hess": model.vgprs[M.EVAL_HESS]},
+            "segment_reduction": "in-kernel" if model.fused[M.EVAL_GRAD] else "final kernel"}
 
 
 def tuned_context(M, kind, fp32, sizes, label):
@@ -68,8 +69,9 @@ def measured_traffic(current_shape):
     except OSError:
         return None
     kv = dict(l.split(":", 1) for l in lines if l.startswith(("traffic_bytes_per_launch:", "launch_shape:")))
-    want = "block=%d ept=%d acc=%d unroll=%d" % (current_shape["block"], current_shape["events_per_thread"],
-                                                current_shape["accumulate"], current_shape["unroll"]["grad"])
+    want = "block=%d ept=%d acc=%d unroll=%d reduction=%s" % (
+        current_shape["block"], current_shape["events_per_thread"], current_shape["accumulate"], current_shape["unroll"]["grad"],
+        current_shape["segment_reduction"].replace(" ", "-"))
     if kv.get("launch_shape", "").strip() != want:
         return None
     return float(kv["traffic_bytes_per_launch"])
@@ -159,7 +161,7 @@ def angular_workload(M, O):
     out = {"events": n, "launch_shape": shape(m32, tune["best"])}
     for label, b in (("fp32", b32), ("fp64", b64)):
         us = b.time_eval(M.EVAL_GRAD, iters=100)
-        k = b.time_kernel(M.EVAL_GRAD, iters=100, main_only=True)
+        k = b.time_kernel(M.EVAL_GRAD, iters=100, main_only=not m32.fused[M.EVAL_GRAD])
         out[label] = {"evals_per_s": 1e6 / us, "us_per_eval": us, "kernel_us": k * 1e3,
                       "GBps": n * 3 * (4 if label == "fp32" else 8) / (k * 1e-3) / 1e9}
     ref = O.nll_sums(M.KSTAR, data, theta, mode=1, nthreads=0)
@@ -198,8 +200,8 @@ def profile_workload():
     ctx, tune = tuned_context(M, M.MASS, True, [N_EVENTS], "mass N=1e7 FP32")
     m32 = ctx.model(M.MASS, fp32=True)
     s = shape(m32, tune["best"])
-    log("profile launch_shape: block=%d ept=%d acc=%d unroll=%d" % (s["block"], s["events_per_thread"], s["accumulate"],
-                                                                    s["unroll"]["grad"]))
+    log("profile launch_shape: block=%d ept=%d acc=%d unroll=%d reduction=%s" % (
+        s["block"], s["events_per_thread"], s["accumulate"], s["unroll"]["grad"], s["segment_reduction"].replace(" ", "-")))
     b32 = m32.batch([N_EVENTS])
     b32.generate(M.MASS_TRUTH, [229387429])
     log("profile: grad", b32.time_kernel(M.EVAL_GRAD, iters=20) * 1e3, "us")
@@ -266,16 +268,26 @@ def main():
     value = args.steps / dt
     log(f"fp32 -2lnL+grad: {value:.1f} evals/s ({ms_per_step:.4f} ms/eval)")
 
-    # device time of the per-chunk kernel alone and with its final pass (HIP events on the
-    # library's stream, around back-to-back launches)
-    k32_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True)
+    # device time (HIP events on the library's stream, around back-to-back launches) of the
+    # dominant kernel, mf_eval_grad -- the per-chunk kernel alone when a final kernel reduces the
+    # segments, the whole kernel when it reduces them itself -- and of a call's kernels together
+    fused32 = m32.fused[M.EVAL_GRAD]
     k32_pair_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
-    k32_lh_ms = b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps), main_only=True)
+    k32_ms = k32_pair_ms if fused32 else b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True)
+    k32_lh_ms = b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps), main_only=not fused32)
     bytes32 = n * 4
     achieved = bytes32 / (k32_ms * 1e-3) / 1e9
-    log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s (with final pass {k32_pair_ms * 1e3:.2f} us); "
+    log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s (with the segment reduction {k32_pair_ms * 1e3:.2f} us); "
         f"lh kernel {k32_lh_ms * 1e3:.2f} us")
     traffic = measured_traffic(shape(m32, tune32["best"]))
+    # where a call's time goes: C++ loop of full calls (launch + sync), device time of the call
+    # graph replayed back to back, and the same through the direct-launch path
+    step = {"python_loop_us": ms_per_step * 1e3,
+            "cpp_loop_direct_launch_us": b32.time_eval(M.EVAL_GRAD, iters=max(50, args.steps)),
+            "cpp_loop_graph_replay_us": b32.time_eval(M.EVAL_GRAD, iters=max(50, args.steps), use_graph=True),
+            "device_graph_us": b32.time_graph(M.EVAL_GRAD, iters=max(50, args.steps)),
+            "kernel_us": k32_ms * 1e3, "kernels_of_a_call_us": k32_pair_ms * 1e3}
+    log("call breakdown (us): " + ", ".join(f"{k} {v:.1f}" for k, v in step.items()))
 
     # ---------------------------------------------------------------- FP64 path
     for _ in range(max(5, args.warmup // 2)):
@@ -285,7 +297,7 @@ def main():
     for _ in range(steps64):
         r64 = b64.eval(M.EVAL_GRAD, start)
     v64 = steps64 / (time.perf_counter() - t0)
-    k64_ms = b64.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True)
+    k64_ms = b64.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=not m64.fused[M.EVAL_GRAD])
     log(f"fp64 -2lnL+grad: {v64:.1f} evals/s, kernel {k64_ms * 1e3:.2f} us")
 
     # ---------------------------------------------------------------- accuracy vs oracle
@@ -436,13 +448,15 @@ def main():
                    "launch_shape": shape(m32, tune32["best"]),
                    "autotune_table_us": [[list(c), round(u, 2)] for c, u in tune32["table"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "kernel": "mf_eval_grad (fp32, per-chunk kernel)", "kernel_ms": k32_ms,
-                     "algorithmic_bytes": bytes32, "with_final_pass_ms": k32_pair_ms},
+                     "traffic": traffic,
+                     "kernel": "mf_eval_grad (fp32, " + ("event loop and segment reduction)" if fused32 else "per-chunk kernel; mf_eval_grad_final reduces the segment)"),
+                     "kernel_ms": k32_ms, "algorithmic_bytes": bytes32, "kernels_of_a_call_ms": k32_pair_ms},
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
         "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9,
                  "launch_shape": shape(m64, tune64["best"])},
         "lh_kernel_ms_fp32": k32_lh_ms,
+        "call_breakdown": step,
         "accuracy": acc,
         "copy_ceiling_GBps": copy_gbs,
         "streaming": stream,

@@ -41,6 +41,9 @@ typedef struct mfh_options {
   int accumulate;          /* FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group then FP64 */
   int centre;              /* FP32: observables stored as offsets from the range centre */
   int unroll;              /* events per thread whose loads are issued together */
+  int balance;             /* large segments: chunk count a multiple of the resident work-groups */
+  int base2;               /* exp/log as exp2/log2: 0 never, 1 FP32 kernels, 2 FP32 and FP64 */
+  int fused_final;         /* 1: segment totals in the work-group finishing a segment; 0: final kernel */
 } mfh_options;
 
 typedef struct mfh_fit_out {
@@ -73,7 +76,8 @@ int mfh_model_create(mfh_context* ctx, int kind, int fp32, const double* range, 
 int mfh_model_destroy(mfh_model* m);
 /* info[0..7] = ntheta, nhoist, nobs, nout_lh, nout_grad, nout_hess, statements, graph_nodes;
  * info[8..10] = unroll of the value, gradient and Hessian kernels (fitted to the register
- * budget, 0 when not compiled), info[11..13] = their VGPR counts;
+ * budget, 0 when not compiled), info[11..13] = their VGPR counts, info[14..16] = 1 where the
+ * kernel reduces multi-chunk segments itself (0: a final kernel follows it);
  * ms[0..1] = codegen ms, hipRTC compile + load ms (small when served from the cache) */
 int mfh_model_info(mfh_model* m, int* info, double* ms);
 int mfh_model_source(mfh_model* m, char* buf, int len);
@@ -81,8 +85,9 @@ int mfh_model_source(mfh_model* m, char* buf, int len);
 int mfh_model_precompile(int kind, int fp32, int with_hessian, const mfh_options* opts, double* compile_ms);
 /* without a device: compile a configuration module and report, for the value, gradient and
  * Hessian kernels, info[0..2] = unroll, info[3..5] = VGPRs, info[6..8] = AGPRs, info[9..11] =
- * scratch bytes per lane.  fit = 1 fits each kernel's unroll to the register budget (as a
- * model does), fit = 0 compiles every kernel at opts->unroll as given. */
+ * scratch bytes per lane, info[12..14] = 1 where the kernel reduces multi-chunk segments
+ * itself.  fit = 1 fits each kernel's unroll to the register budget (as a model does), fit = 0
+ * compiles every kernel at opts->unroll as given. */
 int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* opts, int fit, int* info);
 /* without a device: the per-event outputs of the value (which = 1), gradient (2) or Hessian (3)
  * kernel of a configuration module exactly as generated (observables centred for fp32, base-2
@@ -118,6 +123,9 @@ int mfh_batch_eval(mfh_batch* b, int kind, const double* theta, int use_graph, d
 int mfh_batch_time_kernel(mfh_batch* b, int kind, int iters, int main_only, double* ms_per_launch);
 /* mean wall time of the full per-call sequence (upload, prep, kernel, totals, sync) */
 int mfh_batch_time_eval(mfh_batch* b, int kind, int iters, int use_graph, double* us_per_eval);
+/* mean device time per call of the captured call graph (prep, kernel, final pass) replayed
+ * `iters` times back to back with no host synchronisation in between (HIP events) */
+int mfh_batch_time_graph(mfh_batch* b, int kind, int iters, double* us_per_eval);
 long long mfh_batch_bytes(mfh_batch* b);
 /* fits of the listed segments (NULL = all) from start values (nfit x ntheta) */
 int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, int use_graph, double* values, double* errors,

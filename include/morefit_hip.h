@@ -22,6 +22,7 @@
  *   mf_memcpy_d2h                   CopyFromOutputBuffer / CopyFromKahanBuffer (compute_opencl.hh:401-424, 977-986)
  *   mf_module_load                  MakeComputeKernel / MakeGenerateKernel build step (compute_opencl.hh:425-804)
  *   mf_module_resources             no counterpart (register budget of each compiled kernel)
+ *   mf_module_occupancy             no counterpart (resident work-groups per CU, for balanced chunk tables)
  *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
  *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
@@ -97,6 +98,10 @@ typedef struct mf_eval_args {
   long long store_stride;     /* column stride of `store` */
   int flags;                  /* MF_EVAL_* bits */
   int pad;
+  unsigned int* seg_done;     /* [nseg] chunk counters, zero between launches (the kernel that
+                                 completes a segment sets its counter back to zero).  Given, the
+                                 work-group that finishes a segment's last chunk reduces its totals
+                                 and no final kernel is launched; NULL: the `_final` kernel does. */
 } mf_eval_args;
 
 /* mf_eval_args.flags: launch only the per-chunk kernel.  Correct when no active segment has
@@ -196,6 +201,8 @@ typedef struct mf_kernel_resources {
 int mf_module_resources(const char* generated_src, const char* options, mf_kernel_resources* out, int cap, int* count);
 int mf_module_unload(mf_module* mod);
 int mf_module_has_kernel(mf_module* mod, const char* kernel);
+/* work-groups of `block` threads of `kernel` that one compute unit holds at once */
+int mf_module_occupancy(mf_module* mod, const char* kernel, int block, int* blocks_per_cu);
 double mf_module_compile_ms(mf_module* mod);  /* 0 when served from a cache */
 
 /* launches (on the device stream) */

@@ -44,7 +44,8 @@ class MorefitError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("block_size", ctypes.c_int), ("events_per_thread", ctypes.c_int),
                 ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int),
-                ("unroll", ctypes.c_int)]
+                ("unroll", ctypes.c_int), ("balance", ctypes.c_int), ("base2", ctypes.c_int),
+                ("fused_final", ctypes.c_int)]
 
 
 class FitOut(ctypes.Structure):
@@ -67,13 +68,13 @@ HIP_SYMBOLS = [
     "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_module_load",
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
-    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped", "mf_launch_seed_streams",
+    "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped", "mf_launch_seed_streams", "mf_module_occupancy",
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
     "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states", "mfh_autotune",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
-    "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_bytes", "mfh_batch_fit",
+    "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted",
 ]
 
@@ -112,6 +113,7 @@ def libs():
         L.mfh_batch_generate.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_ulonglong), ip, ctypes.POINTER(ctypes.c_ulonglong)]
         L.mfh_batch_eval.argtypes = [vp, ip, dp, ip, dp]
         L.mfh_batch_time_kernel.argtypes = [vp, ip, ip, ip, dp]
+        L.mfh_batch_time_graph.argtypes = [vp, ip, ip, dp]
         L.mfh_batch_time_eval.argtypes = [vp, ip, ip, ip, dp]
         L.mfh_batch_bytes.argtypes = [vp]
         L.mfh_batch_bytes.restype = ctypes.c_longlong
@@ -152,9 +154,9 @@ def kernel_resources(kind, fp32, with_hessian=True, fit=True, **kw):
     """Without a device: per evaluation kind {EVAL_LH, EVAL_GRAD, EVAL_HESS} the kernel's
     unroll, VGPRs, AGPRs and scratch bytes, fitted to the register budget (fit=True, as a
     model is built) or at the requested unroll as given (fit=False)."""
-    info = (ctypes.c_int * 12)()
+    info = (ctypes.c_int * 15)()
     _check(libs()[1].mfh_model_kernels(kind, int(fp32), int(with_hessian), ctypes.byref(options(**kw)), int(fit), info))
-    return {k: {"unroll": info[j], "vgpr": info[3 + j], "agpr": info[6 + j], "scratch": info[9 + j]}
+    return {k: {"unroll": info[j], "vgpr": info[3 + j], "agpr": info[6 + j], "scratch": info[9 + j], "fused": bool(info[12 + j])}
             for j, k in enumerate((EVAL_LH, EVAL_GRAD, EVAL_HESS))}
 
 
@@ -245,7 +247,7 @@ class Model:
         if self._rng is not None:
             r = _dp(self._rng)
         _check(libs()[1].mfh_model_create(ctx.h, kind, int(fp32), r, int(with_hessian), ctypes.byref(self.h)))
-        info = (ctypes.c_int * 14)()
+        info = (ctypes.c_int * 17)()
         ms = (ctypes.c_double * 2)()
         _check(libs()[1].mfh_model_info(self.h, info, ms))
         self.ntheta, self.nhoist, self.nobs = info[0], info[1], info[2]
@@ -253,6 +255,7 @@ class Model:
         self.statements, self.graph_nodes = info[6], info[7]
         self.unroll = {EVAL_LH: info[8], EVAL_GRAD: info[9], EVAL_HESS: info[10]}
         self.vgprs = {EVAL_LH: info[11], EVAL_GRAD: info[12], EVAL_HESS: info[13]}
+        self.fused = {EVAL_LH: bool(info[14]), EVAL_GRAD: bool(info[15]), EVAL_HESS: bool(info[16])}
         self.codegen_ms, self.compile_ms = ms[0], ms[1]
 
     def source(self):
@@ -305,7 +308,7 @@ class Batch:
         _check(libs()[1].mfh_batch_generate(self.h, _dp(th), sd, events_per_stream, ctypes.byref(props)))
         return props.value
 
-    def eval(self, kind, theta, use_graph=True):
+    def eval(self, kind, theta, use_graph=False):
         th = _f64(np.broadcast_to(np.asarray(theta, dtype=np.float64), (self.S, self.model.ntheta)))
         out = np.zeros((self.S, self.model.nout[kind]))
         _check(libs()[1].mfh_batch_eval(self.h, kind, _dp(th), int(use_graph), _dp(out)))
@@ -318,12 +321,18 @@ class Batch:
         _check(libs()[1].mfh_batch_time_kernel(self.h, kind, iters, int(main_only), ctypes.byref(ms)))
         return ms.value
 
-    def time_eval(self, kind, iters=50, use_graph=True):
+    def time_graph(self, kind, iters=50):
+        """us of device time per call: the call graph replayed back to back, no host sync between"""
+        us = ctypes.c_double(0)
+        _check(libs()[1].mfh_batch_time_graph(self.h, kind, iters, ctypes.byref(us)))
+        return us.value
+
+    def time_eval(self, kind, iters=50, use_graph=False):
         us = ctypes.c_double(0)
         _check(libs()[1].mfh_batch_time_eval(self.h, kind, iters, int(use_graph), ctypes.byref(us)))
         return us.value
 
-    def fit(self, start, segs=None, use_graph=True):
+    def fit(self, start, segs=None, use_graph=False):
         P = self.model.ntheta
         lst = list(range(self.S)) if segs is None else list(segs)
         n = len(lst)

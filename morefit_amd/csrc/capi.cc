@@ -97,6 +97,9 @@ void apply_options(compute_options& co, const mfh_options* o) {
   co.hip_accumulate = o->accumulate;
   co.hip_centre_observables = o->centre != 0;
   co.hip_unroll = o->unroll > 0 ? o->unroll : 8;
+  co.hip_balance_chunks = o->balance != 0;
+  co.hip_base2 = o->base2;
+  co.hip_fused_final = o->fused_final != 0;
 }
 }  // namespace
 
@@ -142,6 +145,9 @@ void mfh_default_options(mfh_options* o) {
   o->accumulate = 2;
   o->centre = 1;
   o->unroll = 8;
+  o->balance = 0;
+  o->base2 = 1;
+  o->fused_final = 0;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {
@@ -214,6 +220,7 @@ int mfh_model_info(mfh_model* m, int* info, double* ms) {
     for (int k = 0; k < 3; k++) {
       info[8 + k] = hm.kernel_unroll(names[k]);
       info[11 + k] = hm.kernel_vgprs(names[k]);
+      info[14 + k] = hm.kernel_fused(names[k]) ? 1 : 0;
     }
     if (ms) {
       ms[0] = hm.codegen_ms;
@@ -263,7 +270,9 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
     if (fit) {
       ms = HipModel<K>::make_source(d.pdf, d.params, co, with_hessian != 0, nullptr, &res);
     } else {
-      LikelihoodGraphs<K> g(d.pdf, d.params, std::is_same<K, float>::value && co.hip_centre_observables, with_hessian != 0);
+      const bool f32 = std::is_same<K, float>::value;
+      LikelihoodGraphs<K> g(d.pdf, d.params, f32 && co.hip_centre_observables, with_hessian != 0,
+                            co.hip_base2 >= 2 || (f32 && co.hip_base2 == 1));
       ms = g.module(co, with_hessian != 0);
       mf_kernel_resources r[64];
       int n = 0;
@@ -271,10 +280,13 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
       res.assign(r, r + std::min(n, 64));
     }
     const char* names[3] = {"mf_eval_lh", "mf_eval_grad", "mf_eval_hess"};
-    for (int k = 0; k < 12; k++) info[k] = 0;
+    for (int k = 0; k < 15; k++) info[k] = 0;
     for (int k = 0; k < 3; k++) {
       for (auto& e : ms.evals)
-        if (e.name == names[k]) info[k] = e.unroll;
+        if (e.name == names[k]) {
+          info[k] = e.unroll;
+          info[12 + k] = e.fused ? 1 : 0;
+        }
       for (auto& r : res)
         if (std::string(r.name) == names[k]) {
           info[3 + k] = r.vgpr_count;
@@ -543,6 +555,20 @@ int mfh_batch_time_eval(mfh_batch* b, int kind, int iters, int use_graph, double
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < iters; i++) bb.run(kind, true, use_graph != 0);
     *us_per_eval = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / iters;
+    return 0;
+  });
+  MFH_CATCH
+}
+
+int mfh_batch_time_graph(mfh_batch* b, int kind, int iters, double* us_per_eval) {
+  MFH_TRY
+  b->with([&](auto& bb) {
+    bb.run(kind, true, true);  // capture
+    hip_check(mf_timer_begin(bb.dev), "timer");
+    for (int i = 0; i < iters; i++) hip_check(mf_graph_launch(bb.graphs[kind]), "mf_graph_launch");
+    float ms = 0.0f;
+    hip_check(mf_timer_end(bb.dev, &ms), "timer");
+    *us_per_eval = 1e3 * (double)ms / iters;
     return 0;
   });
   MFH_CATCH

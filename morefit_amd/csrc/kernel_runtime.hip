@@ -21,12 +21,15 @@
 // and its derivatives; per output the compensated sum over the events of each segment.
 // Differences by design: the sum is fused into the per-event kernel (no N x outputs
 // round trip through memory, no ceil(log32 N) extra passes); partial sums are indexed by
-// (segment, chunk) and a second small kernel reduces each segment in a fixed order, so a
-// result is bitwise the same whatever else shares the launch, with no fences or atomics.
+// (segment, chunk) and each segment is reduced in a fixed order, so a result is bitwise the
+// same whatever else shares the launch.  The reduction runs either in the work-group that
+// completes a segment's last chunk (a per-segment counter tells it, mf_eval_args.seg_done) or
+// in a second small kernel launched after the first (no fences or atomics).
 
 #define MF_NPARAM (MF_NTHETA + MF_NHOIST)
 #define MF_GEN_MAX_TRIES (1u << 20)
 #define MF_NWAVE (MF_BLOCK / 64)
+#define MF_CONST __attribute__((address_space(4)))  // constant memory: read-only for the kernel's lifetime
 
 // ------------------------------------------------------------------ math
 #if MF_FP32
@@ -151,23 +154,103 @@ __device__ __forceinline__ double mf_output_scale(const mf_eval_args& a, int seg
   return (si >= 0 && a.hoist64) ? a.hoist64[(long long)seg * MF_NHOIST + si] : 1.0;
 }
 
+// ------------------------------------------------------------------ segment totals
+// Totals of segment s from its chunk partials part[j][k] (j < nch chunks, k < NO outputs), by
+// the NT threads of one work-group.  Outputs go in passes of KP (NO rounded up to a power of
+// two, at most NT); thread t = r * KP + q takes output k0 + q and sums chunks r, r + R, r + 2R,
+// ... (R = NT / KP) with compensated (two-sum) FP64 additions -- consecutive threads read
+// consecutive addresses, and each thread holds one sum.  The R sums of an output are then
+// combined in a fixed tree: the wave64 butterfly over the lanes of one output, and the waves in
+// wave order through LDS.  Every addition's order depends only on the segment's chunk count
+// and NT, so a segment's totals are bitwise the same whatever else shares the launch.
+#define MF_FINAL_BLOCK 256
+// threadIdx.x through a volatile register move: what the segment reduction derives from the
+// thread index is then computed where it is used, not hoisted above the event loop of the
+// evaluation kernel, where it would hold registers for the whole loop
+__device__ __forceinline__ int mf_thread_index() {
+  int t;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+  return t;
+}
+__host__ __device__ constexpr int mf_pow2_ceil(int v) { return v <= 1 ? 1 : 2 * mf_pow2_ceil((v + 1) / 2); }
+template <class F, int NT>
+__device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, const mf_segment& sg) {
+  constexpr int NO = F::nout;
+  constexpr int KP = mf_pow2_ceil(NO) < NT ? mf_pow2_ceil(NO) : NT;
+  constexpr int R = NT / KP;
+  constexpr int NW = NT / 64;
+  __shared__ double s_fin[NT];
+  const int nch = sg.chunk_end - sg.chunk_begin;
+  const int t = mf_thread_index(), q = t % KP, r = t / KP, lane = t & 63, wave = t >> 6;
+  const double* __restrict__ part = a.partials + (long long)sg.chunk_begin * NO;
+  for (int k0 = 0; k0 < NO; k0 += KP) {
+    const int k = k0 + q;
+    double acc = 0.0, err = 0.0;
+    if (k < NO) {
+#pragma unroll 8
+      for (int j = r; j < nch; j += R) {
+        double u, e;
+        mf_two_sum(acc, part[(long long)j * NO + k], u, e);
+        acc = u;
+        err += e;
+      }
+    }
+    double v = acc + err;
+    if constexpr (KP < 64) {
+      // lanes of one output differ by multiples of KP: butterfly over those offsets
+#pragma unroll
+      for (int off = 32; off >= KP; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane < KP) s_fin[wave * KP + lane] = v;
+      __syncthreads();
+      if (t < KP && k < NO) {
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) tot += s_fin[w * KP + t];
+        a.results[sg.out_off + k] = tot * mf_output_scale<F>(a, s, k);
+      }
+    } else {
+      // each lane of a wave is its own output: combine the R threads of output q in order
+      s_fin[t] = v;
+      __syncthreads();
+      if (t < KP && k < NO) {
+        double tot = 0.0;
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) tot += s_fin[rr * KP + t];
+        a.results[sg.out_off + k] = tot * mf_output_scale<F>(a, s, k);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ fused evaluation
 // Grid-stride over the chunk table (a persistent grid when gridDim < nchunks).  Per chunk:
 // coalesced SoA loads, per-event outputs from the generated functor F, per-lane
 // accumulation, wave64 butterfly + LDS reduction to one FP64 partial per output, written to
-// partials[chunk][k].  No fences or atomics: the segment totals come from the separate
-// final kernel below, launched right after on the same stream.
-template <class F, int U>
+// partials[chunk][k].  A segment of one chunk gets its totals right there.  For longer
+// segments, with a.seg_done each work-group publishes its partials (device-scope release
+// fence) and counts its chunk on the segment's counter; the one that counts last (acquire
+// fence) reduces the segment and sets the counter back to zero for the next launch.  Nothing
+// waits on another work-group.  Without a.seg_done the final kernel below does the reduction.
+// The chunk and segment tables and the kernel parameters are read through constant-address-
+// space pointers (this kernel never writes them): uniform loads then go through the scalar
+// cache into SGPRs, which the compiler cannot otherwise prove safe once the loop holds the
+// counter atomics and fences.
+template <class F, int U, bool FUSED>
 __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
   constexpr int NO = F::nout;
   __shared__ double s_red[MF_NWAVE][NO];
   const MF_T* __restrict__ data = (const MF_T*)a.data;
+  const MF_CONST mf_chunk* chunks = (const MF_CONST mf_chunk*)a.chunks;
+  const MF_CONST mf_segment* segs = (const MF_CONST mf_segment*)a.segs;
+  const MF_CONST MF_T* params = (const MF_CONST MF_T*)a.params;
+  unsigned int* seg_done = a.seg_done;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
-    const mf_chunk ch = a.chunks[c];
-    const mf_segment sg = a.segs[ch.seg];
+    const mf_chunk ch = chunks[c];
+    const mf_segment sg = segs[ch.seg];
     if (sg.mode == 0) continue;
-    const MF_T* __restrict__ P = (const MF_T*)a.params + sg.param_off;
+    const MF_T* __restrict__ P = (const MF_T*)(params + sg.param_off);
     const MF_T* __restrict__ col = data + sg.data_off;
     mf_acc<NO> acc;
     acc.zero();
@@ -224,58 +307,34 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       if (sg.chunk_end - sg.chunk_begin == 1) a.results[sg.out_off + threadIdx.x] = t * mf_output_scale<F>(a, ch.seg, threadIdx.x);
     }
     __syncthreads();
+    const int nch = sg.chunk_end - sg.chunk_begin;
+    if (FUSED && nch > 1 && seg_done) {
+      __shared__ int s_last;
+      if (threadIdx.x == 0) {
+        __threadfence();  // release: this work-group's partials before its count
+        const unsigned int prev = atomicAdd(seg_done + ch.seg, 1u);
+        __threadfence();  // acquire: every other chunk's partials after the last count
+        s_last = prev == (unsigned int)(nch - 1);
+      }
+      __syncthreads();
+      if (s_last) {
+        __threadfence();
+        mf_segment_totals<F, MF_BLOCK>(a, ch.seg, sg);
+        if (threadIdx.x == 0) seg_done[ch.seg] = 0u;
+      }
+    }
   }
 }
 
-// Segment totals of segments with more than one chunk: one work-group of MF_FINAL_BLOCK
-// threads per segment.  Thread t sums chunks t, t + MF_FINAL_BLOCK, ... with compensated
-// (two-sum) FP64 additions, MF_FINAL_KB outputs at a time; the lanes are combined by the fixed
-// wave64 butterfly and the waves in wave order through LDS.  The order of every addition
-// depends only on the segment's own chunk count, so a segment's totals are bitwise the same
-// whatever else shares the launch.
-#define MF_FINAL_BLOCK 256
-#define MF_FINAL_KB 8
+// Segment totals of segments with more than one chunk as a kernel of its own (used when
+// mf_eval_args.seg_done is NULL): one work-group of MF_FINAL_BLOCK threads per segment.
 template <class F>
 __device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
-  constexpr int NO = F::nout;
-  __shared__ double s_red[MF_FINAL_BLOCK / 64][MF_FINAL_KB];
   const int s = blockIdx.x;
   if (s >= a.nseg) return;
   const mf_segment sg = a.segs[s];
-  const int nch = sg.chunk_end - sg.chunk_begin;
-  if (sg.mode == 0 || nch <= 1) return;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const double* __restrict__ part = a.partials + (long long)sg.chunk_begin * NO;
-  for (int k0 = 0; k0 < NO; k0 += MF_FINAL_KB) {
-    double acc[MF_FINAL_KB], err[MF_FINAL_KB];
-#pragma unroll
-    for (int kk = 0; kk < MF_FINAL_KB; kk++) acc[kk] = err[kk] = 0.0;
-#pragma unroll 4
-    for (int j = t; j < nch; j += MF_FINAL_BLOCK) {
-#pragma unroll
-      for (int kk = 0; kk < MF_FINAL_KB; kk++) {
-        if (k0 + kk < NO) {
-          double u, e;
-          mf_two_sum(acc[kk], part[(long long)j * NO + k0 + kk], u, e);
-          acc[kk] = u;
-          err[kk] += e;
-        }
-      }
-    }
-#pragma unroll
-    for (int kk = 0; kk < MF_FINAL_KB; kk++) {
-      const double w = mf_wave_sum(acc[kk] + err[kk]);
-      if (lane == 0) s_red[wave][kk] = w;
-    }
-    __syncthreads();
-    if (t < MF_FINAL_KB && k0 + t < NO) {
-      double tot = 0.0;
-#pragma unroll
-      for (int w = 0; w < MF_FINAL_BLOCK / 64; w++) tot += s_red[w][t];
-      a.results[sg.out_off + k0 + t] = tot * mf_output_scale<F>(a, s, k0 + t);
-    }
-    __syncthreads();
-  }
+  if (sg.mode == 0 || sg.chunk_end - sg.chunk_begin <= 1) return;
+  mf_segment_totals<F, MF_FINAL_BLOCK>(a, s, sg);
 }
 
 // per-event outputs written to memory (the reference's kahan_on_accelerator == false path
@@ -303,8 +362,8 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
   }
 }
 
-#define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL)                                                                    \
-  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL>(a); } \
+#define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL, FUSED)                                                                    \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL, FUSED>(a); } \
   extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR>(a); }
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }

@@ -676,6 +676,17 @@ static int get_fn(mf_module* mod, const char* kernel, hipFunction_t* fn) {
   return MF_OK;
 }
 
+int mf_module_occupancy(mf_module* mod, const char* kernel, int block, int* blocks_per_cu) {
+  if (!mod || !kernel || !blocks_per_cu || block <= 0) return set_error(MF_ERR_INVALID, "mf_module_occupancy: bad argument");
+  hipFunction_t fn;
+  int rc = get_fn(mod, kernel, &fn);
+  if (rc != MF_OK) return rc;
+  int nb = 0;
+  MF_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0), "mf_module_occupancy");
+  *blocks_per_cu = nb;
+  return MF_OK;
+}
+
 int mf_module_has_kernel(mf_module* mod, const char* kernel) {
   if (!mod || !kernel) return 0;
   hipFunction_t fn;
@@ -698,7 +709,7 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval");
   std::string fin = std::string(kernel) + "_final";
   hipFunction_t ff;
-  if (args->nseg > 0 && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff))
+  if (args->nseg > 0 && !args->seg_done && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff))
     MF_HIP(hipModuleLaunchKernel(ff, args->nseg, 1, 1, 256, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval final");
   return MF_OK;
 }

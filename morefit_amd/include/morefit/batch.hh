@@ -64,7 +64,7 @@ class FitScheduler {
     BatchFitResult result;
   };
 
-  bool use_graph = true;
+  bool use_graph = false;  // direct launches: on this stack cheaper than a graph replay
   bool run_hesse = true;
   int strategy = 2;
   double tolerance = 0.01;
@@ -316,7 +316,7 @@ template <typename kernelT, typename evalT = double>
 class BatchFitter {
  public:
   HipBatch<kernelT, evalT>& b;
-  bool use_graph = true;
+  bool use_graph = false;
   bool run_hesse = true;
   BatchFitStats stats;
   explicit BatchFitter(HipBatch<kernelT, evalT>& batch) : b(batch) {}

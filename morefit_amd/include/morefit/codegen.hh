@@ -20,10 +20,12 @@
 
 namespace morefit {
 
-// one MF_EVAL_KERNEL instantiation: kernel name, functor, events per thread per load batch
+// one MF_EVAL_KERNEL instantiation: kernel name, functor, events per thread per load batch,
+// and whether the kernel reduces multi-chunk segments itself (else its _final kernel does)
 struct EvalKernelSpec {
   std::string name, functor;
   int unroll = 1;
+  bool fused = true;
 };
 
 struct ModuleSource {
@@ -66,7 +68,8 @@ struct ModuleSource {
   std::string device_source() const {
     std::ostringstream s;
     s << "// generated by morefit codegen.hh\n" << prelude() << "//@MF_RUNTIME@\n" << functors << "\n" << hoist << "\n";
-    for (const auto& e : evals) s << "MF_EVAL_KERNEL(" << e.name << ", " << e.functor << ", " << e.unroll << ")\n";
+    for (const auto& e : evals)
+      s << "MF_EVAL_KERNEL(" << e.name << ", " << e.functor << ", " << e.unroll << ", " << (e.fused ? 1 : 0) << ")\n";
     for (const auto& k : kernels) s << k << "\n";
     return s.str();
   }

@@ -48,6 +48,13 @@ struct compute_options {
   int hip_events_per_thread = 32;    // chunk = block * events_per_thread events
   int hip_grid = 0;                  // 0: one work-group per chunk, >0: persistent grid of that many
   int hip_unroll = 8;                // events per thread whose loads are issued together
+  int hip_base2 = 1;                 // exp/log -> exp2/log2 rewrite: 0 never, 1 FP32 kernels, 2 FP32 and FP64
+  bool hip_balance_chunks = false;
+  // segment totals in the work-group that completes a segment (no final kernel).  Off: each
+  // work-group's device-scope release fence writes back its XCD's L2, and on MI355X those
+  // write-backs cost more than the final kernel's launch (measured 36.7 vs 19.0 us per
+  // gradient call at N = 1e7)
+  bool hip_fused_final = false;   // large segments: chunk count a multiple of the resident work-groups (measured: no gain)
   bool hip_fast_math = true;         // FP32: hardware exp2/log2/rcp (error measured against the oracle)
   int hip_accumulate = 2;            // FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group + FP64
   bool hip_centre_observables = true;// FP32: store observables as offsets from the range centre

@@ -62,6 +62,9 @@ inline void fit_register_budget(ModuleSource& ms, const std::string& options, st
         if (e.unroll > 1) {
           e.unroll /= 2;
           changed = true;
+        } else if (e.fused) {
+          e.fused = false;  // the segment reduction moves to the kernel's _final kernel
+          changed = true;
         } else if (r[k].agpr_count > 0) {
           throw std::runtime_error("kernel " + e.name + " needs " + std::to_string(r[k].vgpr_count) + " VGPRs + " +
                                    std::to_string(r[k].agpr_count) + " AGPRs at unroll 1; more than the 256 architectural "
@@ -155,8 +158,20 @@ struct HostBuf {
 };
 
 // one segment's chunk table [chunk_begin, chunk_end) for n events at `chunk` events per chunk
-inline void append_chunks(std::vector<mf_chunk>& chunks, mf_segment& sg, int seg, long long n, int chunk) {
+// Splits segment `seg` of n events into chunks of about `chunk` events.  With `resident` (the
+// work-groups the whole device holds at once) a segment of at least half that many chunks is
+// split into a multiple of `resident` equal chunks instead, so the last round of work-groups
+// is full (e.g. 10^7 events at 8192 per chunk: 1221 chunks on a 1024-group device would run
+// 1.19 rounds; 1024 chunks of 9766 run one).
+inline void append_chunks(std::vector<mf_chunk>& chunks, mf_segment& sg, int seg, long long n, int chunk, int resident = 0) {
   sg.chunk_begin = (int)chunks.size();
+  if (resident > 0 && n > chunk) {
+    const long long nch0 = (n + chunk - 1) / chunk;
+    if (2 * nch0 >= resident) {
+      const long long rounds = std::max(1ll, std::llround((double)nch0 / resident));
+      chunk = (int)((n + rounds * resident - 1) / (rounds * resident));
+    }
+  }
   if (n <= 0) {
     chunks.push_back({seg, 0, 0, 0});  // an empty chunk still completes the segment (totals 0)
   } else {
@@ -171,12 +186,13 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
   struct State {
     HIPBackend* be = nullptr;
     std::shared_ptr<DevBuf> input, output;
-    DevBuf params, seeds_states, streams, maxprob, segs, chunks, partials, results;
+    DevBuf params, seeds_states, streams, maxprob, segs, chunks, partials, results, seg_done;
     HostBuf results_h;
     mf_module* mod = nullptr;
     std::string kernel;
     int nout = 0;
     bool kahan = false, generate = false, tables_ready = false;
+    bool fused = false;           // the evaluation kernel reduces the segment itself (mf_eval_args.seg_done)
     long long nevents = 0, nevents_padded = 0;
     int nstreams = 0;
     double maxprob_value = 1.0;
@@ -210,6 +226,8 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     s.partials.alloc(dev(), ch.size() * std::max(1, s.nout) * sizeof(double));
     s.results.alloc(dev(), std::max(1, s.nout) * sizeof(double));
     s.results_h.alloc(dev(), std::max(1, s.nout) * sizeof(double));
+    s.seg_done.alloc(dev(), sizeof(unsigned int));
+    hip_check(mf_fill_zero(dev(), s.seg_done.ptr, s.seg_done.bytes), "HIPBlock counters");
     hip_check(mf_sync(dev()), "HIPBlock tables");
     nchunks_ = (int)ch.size();
     s.tables_ready = true;
@@ -222,6 +240,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     s.mod = nullptr;
     fit_register_budget(ms, opts().hip_compile_options);
     s.block = ms.block;
+    s.fused = !ms.evals.empty() && ms.evals[0].fused;
     std::string src = ms.device_source();
     if (opts().print_kernel) std::cout << "KERNEL SOURCE\n" << src << "END KERNEL SOURCE" << std::endl;
     hip_check(mf_module_load(dev(), src.c_str(), opts().hip_compile_options.c_str(), &s.mod), "HIPBlock: mf_module_load");
@@ -345,7 +364,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
     ms.hoist = "__device__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
     s.kernel = name;
-    if (kahan_summation) ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll)});
+    if (kahan_summation) ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final});
     else ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
     compile(ms);
     s.nout = (int)graphs.size();
@@ -425,6 +444,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     a.partials = s.partials.template as<double>();
     a.results = s.results.template as<double>();
     if (nchunks_ <= 1) a.flags |= MF_EVAL_SKIP_FINAL;  // one chunk: the per-chunk kernel writes the totals
+    else if (s.kahan && s.fused) a.seg_done = s.seg_done.template as<unsigned int>();
     if (!s.kahan) {
       if (!s.output) throw std::runtime_error("HIPBlock::SubmitKernel: store kernel needs an output buffer");
       a.store = s.output->ptr;
@@ -539,8 +559,9 @@ struct LikelihoodGraphs {
     ms.functors += emit_functor<kernelT, evalT>("mf_f_gen", {gen.get()}, obs, pn, ms.fp32, &ms.statements);
     ms.hoist = emit_hoist(raw(hoist_exprs), theta);
     const int u = std::max(1, o.hip_unroll);
-    ms.evals = {{"mf_eval_lh", "mf_f_lh", u}, {"mf_eval_grad", "mf_f_grad", u}};
-    if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u});
+    const bool f = o.hip_fused_final;
+    ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f}, {"mf_eval_grad", "mf_f_grad", u, f}};
+    if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u, f});
     ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)"};
     return ms;
   }
@@ -564,9 +585,11 @@ class HipModel {
   double codegen_ms = 0.0, compile_ms = 0.0;
   mf_module* mod = nullptr;
   std::vector<std::pair<std::string, int>> unrolls;  // evaluation kernel -> unroll
+  std::vector<std::string> fused;                    // evaluation kernels that reduce multi-chunk segments themselves
   std::vector<mf_kernel_resources> resources;        // every kernel of the module
   int block = 256;                                   // work-group size compiled into the module
   int grid_opt = 0;                                  // hip_grid at construction (0: one group per chunk)
+  int resident = 0;                                  // gradient work-groups the device holds at once (0: no balancing)
 
   // generated module with the unroll of each evaluation kernel fitted to the register budget
   static ModuleSource make_source(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, const compute_options& o,
@@ -574,7 +597,8 @@ class HipModel {
                                   std::vector<mf_kernel_resources>* res = nullptr, double* compile_ms = nullptr) {
     const bool f32 = std::is_same<kernelT, float>::value;
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> g(
-        new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian, f32));
+        new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian,
+                                             o.hip_base2 >= 2 || (f32 && o.hip_base2 == 1)));
     ModuleSource ms = g->module(o, with_hessian);
     auto t0 = std::chrono::steady_clock::now();
     fit_register_budget(ms, o.hip_compile_options, res);
@@ -587,6 +611,11 @@ class HipModel {
     for (auto& e : unrolls)
       if (e.first == name) return e.second;
     return 0;
+  }
+  bool kernel_fused(const std::string& name) const {
+    for (auto& e : fused)
+      if (e == name) return true;
+    return false;
   }
   int kernel_vgprs(const std::string& name) const {
     for (auto& r : resources)
@@ -601,7 +630,10 @@ class HipModel {
     double fit_ms = 0.0;
     ModuleSource ms = make_source(p, ps, *be->opts_, hessian, &g, &resources, &fit_ms);
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> keep(g);
-    for (auto& e : ms.evals) unrolls.push_back({e.name, e.unroll});
+    for (auto& e : ms.evals) {
+      unrolls.push_back({e.name, e.unroll});
+      if (e.fused) fused.push_back(e.name);
+    }
     block = ms.block;
     grid_opt = be->opts_->hip_grid;
     for (size_t i = 0; i < ps.size(); i++)
@@ -626,6 +658,11 @@ class HipModel {
     if (be->opts_->print_kernel) std::cout << source << std::endl;
     auto t1 = std::chrono::steady_clock::now();
     hip_check(mf_module_load(be->device(), source.c_str(), be->opts_->hip_compile_options.c_str(), &mod), "HipModel: mf_module_load");
+    if (be->opts_->hip_balance_chunks && grid_opt == 0) {
+      int per_cu = 0;
+      hip_check(mf_module_occupancy(mod, "mf_eval_grad", block, &per_cu), "HipModel: mf_module_occupancy");
+      resident = per_cu * be->props().compute_units;
+    }
     compile_ms = fit_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
   }
   ~HipModel() { mf_module_unload(mod); }
@@ -719,9 +756,10 @@ class HipBatch {
   long long total_elems = 0;             // arena capacity in elements
   int chunk = 0, nchunks = 0, chunk_cap = 0;
   int max_seg_chunks = 0;                // most chunks of any active segment (<= 1: no final pass)
+  bool fused_[4] = {false, false, false, false};  // per kind: the kernel reduces multi-chunk segments itself
   bool dirty = true;
   std::vector<mf_segment> segs_h;
-  DevBuf data, params_d, hoist64_d, segs_d[4], chunks_d, partials_d;
+  DevBuf data, params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d;
   // generator scratch, kept between calls (grows as needed)
   DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_;
   int gen_levels_ = 0;
@@ -763,10 +801,14 @@ class HipBatch {
     theta_h.alloc_mapped(dev, std::max(1, S * m->ntheta) * sizeof(double));
     params_d.alloc(dev, std::max(1, S * m->nparam) * sizeof(kernelT));
     hoist64_d.alloc(dev, (size_t)std::max(1, S) * std::max(1, m->nhoist) * sizeof(double));
-    chunk_cap = (int)(elems / m->nobs / chunk) + 2 * S + 1;
+    // ceil per segment adds at most one chunk each; balancing (append_chunks) rounds a segment
+    // of nch0 >= resident/2 chunks to the nearest multiple of resident, at most doubling it
+    chunk_cap = (int)(2 * (elems / m->nobs / chunk)) + 2 * S + 1 + m->resident;
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
     int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
     partials_d.alloc(dev, (size_t)chunk_cap * nmax * sizeof(double));
+    seg_done_d.alloc(dev, (size_t)std::max(1, S) * sizeof(unsigned int));
+    hip_check(mf_fill_zero(dev, seg_done_d.ptr, seg_done_d.bytes), "HipBatch counters");
     for (int k = 1; k <= 3; k++) {
       if (m->nout[k] == 0) continue;
       segs_d[k].alloc(dev, std::max(1, S) * sizeof(mf_segment));
@@ -776,6 +818,7 @@ class HipBatch {
     }
     std::memset(theta_h.ptr, 0, theta_h.bytes);
     request_.assign(S, 0);
+    for (int k = 1; k <= 3; k++) fused_[k] = m->nout[k] > 0 && m->kernel_fused(kernel_name(k));
     hip_check(mf_sync(dev), "HipBatch init");
   }
 
@@ -831,7 +874,7 @@ class HipBatch {
       sg.n = (int)sizes[s];
       sg.param_off = s * m->nparam;
       sg.mode = 0;
-      if (active[s]) append_chunks(ch, sg, s, sizes[s], chunk);
+      if (active[s]) append_chunks(ch, sg, s, sizes[s], chunk, m->resident);
       max_seg_chunks = std::max(max_seg_chunks, sg.chunk_end - sg.chunk_begin);
       segs_h[s] = sg;
     }
@@ -984,14 +1027,19 @@ class HipBatch {
     a.results = (double*)results_h[kind].dptr;
     a.hoist64 = hoist64_d.template as<double>();
     if (max_seg_chunks <= 1) a.flags |= MF_EVAL_SKIP_FINAL;
+    else if (fused_[kind]) a.seg_done = seg_done_d.template as<unsigned int>();
     return a;
   }
   // just the evaluation kernels of `kind` (segment table and parameters already on device);
-  // main_only launches the per-chunk kernel without the final pass (for timing it alone)
+  // main_only launches the per-chunk kernel without any reduction of multi-chunk segments (for
+  // timing the event loop alone)
   void launch_kernel(int kind, bool main_only = false) {
     if (nchunks == 0) return;
     mf_eval_args a = eval_args(kind);
-    if (main_only) a.flags |= MF_EVAL_SKIP_FINAL;
+    if (main_only) {
+      a.flags |= MF_EVAL_SKIP_FINAL;
+      a.seg_done = nullptr;
+    }
     hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->block), "mf_launch_eval");
   }
   // the full per-call sequence of one kind: prep (theta and segment modes from mapped host

@@ -75,6 +75,29 @@ def test_eval_matches_oracle(ctx, model, fp32):
 
 
 @pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
+def test_fp64_base2_matches_oracle(model):
+    """FP64 kernels with exp/log rewritten as exp2/log2 (option base2=2) keep the FP64 bar, 1e-12."""
+    n = 100000
+    data = O.generate(model, M.TRUTH[model], n, seed=4321, nstreams=16)
+    theta = np.array(M.START[model])
+    say("fp64 base2 parity", "model", model, "n", n)
+    cx = M.Context(base2=2)
+    mdl = cx.model(model, fp32=False)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    ref = O.nll_sums(model, data, theta, mode=2, nthreads=0)
+    absref = O.nll_sums(model, data, theta, mode=12, nthreads=4)
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        out = b.eval(kind, theta)[0]
+        k = len(out)
+        assert abs(out[0] - ref[0]) <= 1e-12 * abs(ref[0])
+        assert np.all(np.abs(out - ref[:k]) <= 1e-12 * absref[:k]), (kind, out - ref[:k])
+    b.close()
+    mdl.close()
+    cx.close()
+
+
+@pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
 @pytest.mark.parametrize("accumulate", [0, 1, 2])
 def test_fp32_accumulation_modes(model, accumulate):
     """FP32 kernels with each per-lane accumulation (0: FP64 per event, 1: Kahan FP32, 2: FP32
@@ -117,6 +140,42 @@ def test_graph_replays_match_eager(ctx):
             b.eval(M.EVAL_LH, t, use_graph=False)
     b.close()
     mdl.close()
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_segment_reduction_in_kernel_and_final_pass(fp32, fused):
+    """Multi-chunk segments reduced by the work-group that completes them (fused=1: per-segment
+    counters that the kernel sets back to zero) or by the separate final kernel (fused=0):
+    three evaluations in a row at different parameters, each kind, a batch of multi-chunk,
+    exactly-one-chunk, tiny and empty segments -- every total within the oracle bars (a counter
+    left non-zero would leave the previous call's totals in place and fail the later rounds)."""
+    sizes = [1_000_003, 0, 77, 8192, 8193, 300_001]
+    datas = [O.generate(M.MASS, M.MASS_TRUTH, n, seed=500 + i, nstreams=8) if n else None for i, n in enumerate(sizes)]
+    say("segment reduction fused", fused, "fp32", fp32, "sizes", sizes)
+    cx = M.Context(fused_final=fused)
+    mdl = cx.model(M.MASS, fp32=fp32)
+    b = mdl.batch(sizes)
+    for i, d in enumerate(datas):
+        if d is not None:
+            b.upload(i, d)
+    tol = 2e-6 if fp32 else 1e-12
+    for rep in range(3):
+        theta = np.array(M.MASS_START) * (1.0 + 1e-3 * rep)
+        for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+            out = b.eval(kind, theta)
+            for i, d in enumerate(datas):
+                if d is None:
+                    assert np.all(out[i] == 0)
+                    continue
+                st = _stored(M.MASS, d, fp32)
+                k = out.shape[1]
+                ref = O.nll_sums(M.MASS, st, theta, mode=2, nthreads=8)[:k]
+                absref = O.nll_sums(M.MASS, st, theta, mode=12, nthreads=8)[:k]
+                assert np.all(np.abs(out[i] - ref) <= tol * absref), (rep, kind, i, out[i] - ref)
+    b.close()
+    mdl.close()
+    cx.close()
 
 
 @pytest.mark.parametrize("fp32", [False, True])
