@@ -25,6 +25,7 @@
  *   mf_module_occupancy             no counterpart (resident work-groups per CU, for balanced chunk tables)
  *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
  *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
+ *   mf_launch_prep_inline           the same with theta passed by value (no counterpart: latency of one call)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
  *   mf_launch_seed_streams          host seeding of per-work-item generator states (compute_opencl.hh:522-539), on the device
  *   mf_sync                         Finish (clFinish)
@@ -102,12 +103,34 @@ typedef struct mf_eval_args {
                                  completes a segment sets its counter back to zero).  Given, the
                                  work-group that finishes a segment's last chunk reduces its totals
                                  and no final kernel is launched; NULL: the `_final` kernel does. */
+  int* done_flag;             /* mapped host memory or NULL: once every total is written, the final
+                                 kernel stores done_value there (after a system-scope fence), so the
+                                 host can poll it instead of waiting on the stream.  Only with one
+                                 segment whose totals the final kernel reduces (nseg == 1, neither
+                                 MF_EVAL_SKIP_FINAL nor seg_done). */
+  int done_value;
+  int pad2;
 } mf_eval_args;
 
 /* mf_eval_args.flags: launch only the per-chunk kernel.  Correct when no active segment has
  * more than one chunk (the per-chunk kernel then writes the totals itself); otherwise the
  * totals are left unreduced (used to time the per-chunk kernel alone). */
 #define MF_EVAL_SKIP_FINAL 1
+
+/* argument block of mf_prep_inline (mf_launch_prep_inline) */
+#define MF_PREP_INLINE_THETA 64
+typedef struct mf_prep_inline_args {
+  double theta[MF_PREP_INLINE_THETA]; /* [segment][ntheta] */
+  void* params;
+  double* hoist64;
+  const mf_segment* seg_src;          /* NULL: the device segment table is left as it is */
+  mf_segment* seg_dst;
+  int nseg;
+  int pad;
+} mf_prep_inline_args;
+
+/* work-group size of the `_final` kernels (one work-group per segment) */
+#define MF_FINAL_THREADS 1024
 
 /* argument block of the generated generator kernel (mf_generate):
  * stream k owns events [begin, end) of segment seg and the xoshiro128++ state
@@ -214,6 +237,12 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
  * into device memory in the same launch). */
 int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* hoist64, const int* seg_list, int nseg,
                    const mf_segment* seg_src, mf_segment* seg_dst);
+/* the same for segments 0..nseg-1 with theta (host memory, nseg * ntheta values, ntheta the
+ * module's floating parameters) copied into the kernel arguments at the call: the kernel reads
+ * nothing from host memory.  nseg * ntheta <= MF_PREP_INLINE_THETA.  Not for graph capture
+ * (a captured launch keeps the theta of the capture). */
+int mf_launch_prep_inline(mf_module* mod, const double* theta, int ntheta, int nseg, void* params, double* hoist64,
+                          const mf_segment* seg_src, mf_segment* seg_dst);
 int mf_launch_generate(mf_module* mod, const mf_gen_args* args, int grid, int block);
 /* xoshiro128++ states of generator streams, all device pointers: stream g (of nstreams)
  * gets J^k applied to seg_base[4 * streams[g].seg ...], k = streams[g].begin /

@@ -163,7 +163,7 @@ __device__ __forceinline__ double mf_output_scale(const mf_eval_args& a, int seg
 // combined in a fixed tree: the wave64 butterfly over the lanes of one output, and the waves in
 // wave order through LDS.  Every addition's order depends only on the segment's chunk count
 // and NT, so a segment's totals are bitwise the same whatever else shares the launch.
-#define MF_FINAL_BLOCK 256
+#define MF_FINAL_BLOCK MF_FINAL_THREADS
 // threadIdx.x through a volatile register move: what the segment reduction derives from the
 // thread index is then computed where it is used, not hoisted above the event loop of the
 // evaluation kernel, where it would hold registers for the whole loop
@@ -173,7 +173,7 @@ __device__ __forceinline__ int mf_thread_index() {
   return t;
 }
 __host__ __device__ constexpr int mf_pow2_ceil(int v) { return v <= 1 ? 1 : 2 * mf_pow2_ceil((v + 1) / 2); }
-template <class F, int NT>
+template <class F, int NT, int UNR>
 __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, const mf_segment& sg) {
   constexpr int NO = F::nout;
   constexpr int KP = mf_pow2_ceil(NO) < NT ? mf_pow2_ceil(NO) : NT;
@@ -187,7 +187,7 @@ __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, 
     const int k = k0 + q;
     double acc = 0.0, err = 0.0;
     if (k < NO) {
-#pragma unroll 8
+#pragma unroll UNR
       for (int j = r; j < nch; j += R) {
         double u, e;
         mf_two_sum(acc, part[(long long)j * NO + k], u, e);
@@ -319,7 +319,7 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       __syncthreads();
       if (s_last) {
         __threadfence();
-        mf_segment_totals<F, MF_BLOCK>(a, ch.seg, sg);
+        mf_segment_totals<F, MF_BLOCK, 8>(a, ch.seg, sg);
         if (threadIdx.x == 0) seg_done[ch.seg] = 0u;
       }
     }
@@ -331,10 +331,19 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
 template <class F>
 __device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
   const int s = blockIdx.x;
-  if (s >= a.nseg) return;
-  const mf_segment sg = a.segs[s];
-  if (sg.mode == 0 || sg.chunk_end - sg.chunk_begin <= 1) return;
-  mf_segment_totals<F, MF_FINAL_BLOCK>(a, s, sg);
+  if (s < a.nseg) {
+    const mf_segment sg = a.segs[s];
+    if (sg.mode != 0 && sg.chunk_end - sg.chunk_begin > 1) mf_segment_totals<F, MF_FINAL_BLOCK, 16>(a, s, sg);
+  }
+  // one segment (checked by the host): this work-group wrote every total; publish them to the
+  // host, then the flag it polls
+  if (a.done_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence_system();
+      __hip_atomic_store(a.done_flag, a.done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
 }
 
 // per-event outputs written to memory (the reference's kahan_on_accelerator == false path
@@ -374,24 +383,37 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
 // theta may live in mapped host memory (read once, before anything is stored); with seg_src
 // the segment's table entry is staged from there into the device table the evaluation kernels
 // read; with hoist64 the hoisted terms are also kept in FP64 (output scales, mf_eval_args).
-__device__ void mf_hoist(const double* th, double* buf);
+// hoisted parameter-only terms in FP64 from the floating parameters (generated, after this runtime)
+__device__ __forceinline__ void mf_hoist(const double* th, double* buf);
 
-#define MF_PREP_KERNEL                                                                                  \
-  extern "C" __global__ void mf_prep(const double* theta, MF_T* params, double* hoist64, const int* seg_list, \
-                                     int nseg, const mf_segment* seg_src, mf_segment* seg_dst) {        \
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;                                                \
-    if (i >= nseg) return;                                                                              \
-    const int s = seg_list ? seg_list[i] : i;                                                           \
-    double th[MF_NTHETA > 0 ? MF_NTHETA : 1];                                                           \
-    for (int k = 0; k < MF_NTHETA; k++) th[k] = theta[(long long)s * MF_NTHETA + k];                    \
-    if (seg_src) seg_dst[s] = seg_src[s];                                                               \
-    MF_T* out = params + (long long)s * MF_NPARAM;                                                      \
-    double hb[MF_NHOIST > 0 ? MF_NHOIST : 1];                                                           \
-    mf_hoist(th, hb);                                                                                   \
-    for (int k = 0; k < MF_NTHETA; k++) out[k] = (MF_T)th[k];                                           \
-    for (int j = 0; j < MF_NHOIST; j++) out[MF_NTHETA + j] = (MF_T)hb[j];                               \
-    if (hoist64)                                                                                        \
-      for (int j = 0; j < MF_NHOIST; j++) hoist64[(long long)s * MF_NHOIST + j] = hb[j];                \
+// one segment: FP64 theta th -> hoisted terms -> [theta, hoisted] in MF_T at params, FP64 hoisted
+// terms at hoist64; theta is read in full before anything is stored
+__device__ __forceinline__ void mf_prep_one(const double* th_src, int s, MF_T* params, double* hoist64, const mf_segment* seg_src,
+                                            mf_segment* seg_dst) {
+  double th[MF_NTHETA > 0 ? MF_NTHETA : 1];
+  for (int k = 0; k < MF_NTHETA; k++) th[k] = th_src[k];
+  if (seg_src) seg_dst[s] = seg_src[s];
+  MF_T* out = params + (long long)s * MF_NPARAM;
+  double hb[MF_NHOIST > 0 ? MF_NHOIST : 1];
+  mf_hoist(th, hb);
+  for (int k = 0; k < MF_NTHETA; k++) out[k] = (MF_T)th[k];
+  for (int j = 0; j < MF_NHOIST; j++) out[MF_NTHETA + j] = (MF_T)hb[j];
+  if (hoist64)
+    for (int j = 0; j < MF_NHOIST; j++) hoist64[(long long)s * MF_NHOIST + j] = hb[j];
+}
+
+#define MF_PREP_KERNEL                                                                                             \
+  extern "C" __global__ void mf_prep(const double* theta, MF_T* params, double* hoist64, const int* seg_list, int nseg, \
+                                     const mf_segment* seg_src, mf_segment* seg_dst) {                             \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;                                                           \
+    if (i >= nseg) return;                                                                                         \
+    const int s = seg_list ? seg_list[i] : i;                                                                      \
+    mf_prep_one(theta + (long long)s * MF_NTHETA, s, params, hoist64, seg_src, seg_dst);                           \
+  }                                                                                                                \
+  extern "C" __global__ void mf_prep_inline(mf_prep_inline_args g) {                                               \
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;                                                           \
+    if (s >= g.nseg) return;                                                                                       \
+    mf_prep_one(g.theta + s * MF_NTHETA, s, (MF_T*)g.params, g.hoist64, g.seg_src, g.seg_dst);                     \
   }
 
 // ------------------------------------------------------------------ generator

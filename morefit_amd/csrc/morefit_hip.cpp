@@ -700,6 +700,11 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
     return set_error(MF_ERR_INVALID, "mf_launch_eval: bad launch shape %d x %d", grid, block);
   if (args->nchunks < 0 || args->nseg < 0 || !args->segs || !args->chunks || !args->params || !args->data)
     return set_error(MF_ERR_INVALID, "mf_launch_eval: incomplete argument block");
+  std::string fin = std::string(kernel) + "_final";
+  hipFunction_t ff = nullptr;
+  const bool final = args->nseg > 0 && !args->seg_done && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff);
+  if (args->done_flag && (args->nseg != 1 || !final || args->nchunks == 0))
+    return set_error(MF_ERR_INVALID, "mf_launch_eval: a done flag needs the final kernel over exactly one segment");
   if (args->nchunks == 0) return MF_OK;
   hipFunction_t fn;
   int rc = get_fn(mod, kernel, &fn);
@@ -707,10 +712,8 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
   mf_eval_args a = *args;
   void* kp[] = {(void*)&a};
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval");
-  std::string fin = std::string(kernel) + "_final";
-  hipFunction_t ff;
-  if (args->nseg > 0 && !args->seg_done && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff))
-    MF_HIP(hipModuleLaunchKernel(ff, args->nseg, 1, 1, 256, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval final");
+  if (final)
+    MF_HIP(hipModuleLaunchKernel(ff, args->nseg, 1, 1, MF_FINAL_THREADS, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_eval final");
   return MF_OK;
 }
 
@@ -726,6 +729,30 @@ int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* ho
   void* kp[] = {(void*)&theta, (void*)&params, (void*)&hoist64, (void*)&seg_list, (void*)&nseg, (void*)&seg_src, (void*)&seg_dst};
   int grid = (nseg + 63) / 64;
   MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep");
+  return MF_OK;
+}
+
+int mf_launch_prep_inline(mf_module* mod, const double* theta, int ntheta, int nseg, void* params, double* hoist64,
+                          const mf_segment* seg_src, mf_segment* seg_dst) {
+  MF_INJECT("mf_launch_prep_inline");
+  if (!mod || (!theta && ntheta > 0) || !params || nseg < 0 || ntheta < 0 || (seg_src && !seg_dst))
+    return set_error(MF_ERR_INVALID, "mf_launch_prep_inline: bad argument");
+  if ((long long)nseg * ntheta > MF_PREP_INLINE_THETA)
+    return set_error(MF_ERR_INVALID, "mf_launch_prep_inline: %d x %d parameters exceed the %d passed by value", nseg, ntheta,
+                     MF_PREP_INLINE_THETA);
+  if (nseg == 0) return MF_OK;
+  hipFunction_t fn;
+  int rc = get_fn(mod, "mf_prep_inline", &fn);
+  if (rc != MF_OK) return rc;
+  mf_prep_inline_args g{};
+  std::memcpy(g.theta, theta, sizeof(double) * (size_t)nseg * ntheta);
+  g.params = params;
+  g.hoist64 = hoist64;
+  g.seg_src = seg_src;
+  g.seg_dst = seg_dst;
+  g.nseg = nseg;
+  void* kp[] = {(void*)&g};
+  MF_HIP(hipModuleLaunchKernel(fn, (nseg + 63) / 64, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep_inline");
   return MF_OK;
 }
 

@@ -120,7 +120,7 @@ std::string emit_hoist(const std::vector<const ComputeGraphNode<kernelT, evalT>*
   std::vector<std::string> res;
   for (auto* g : exprs) res.push_back(em.emit(*g));
   std::ostringstream s;
-  s << "__device__ void mf_hoist(const double* th, double* buf) {\n  (void)th; (void)buf;\n" << em.body.str();
+  s << "__device__ __forceinline__ void mf_hoist(const double* th, double* buf) {\n  (void)th; (void)buf;\n" << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "  buf[" << k << "] = " << res[k] << ";\n";
   s << "}\n";
   return s.str();

@@ -362,7 +362,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
     for (auto& g : graphs) outs.push_back(g.get());
     ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
-    ms.hoist = "__device__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
+    ms.hoist = "__device__ __forceinline__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
     s.kernel = name;
     if (kahan_summation) ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final});
     else ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
@@ -392,7 +392,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     }
     auto lp = Log<kernelT, evalT>(graphs.at(0)->copy())->simplify();
     ms.functors = emit_functor<kernelT, evalT>("mf_f_gen", {lp.get()}, obs, params, ms.fp32, &ms.statements);
-    ms.hoist = "__device__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
+    ms.hoist = "__device__ __forceinline__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
     ms.kernels.push_back("MF_GENERATE_KERNEL(mf_f_gen)");
     compile(ms);
     s.generate = true;
@@ -757,6 +757,7 @@ class HipBatch {
   int chunk = 0, nchunks = 0, chunk_cap = 0;
   int max_seg_chunks = 0;                // most chunks of any active segment (<= 1: no final pass)
   bool fused_[4] = {false, false, false, false};  // per kind: the kernel reduces multi-chunk segments itself
+  bool segs_stale_[4] = {true, true, true, true};  // per kind: device segment table differs from the mapped one
   bool dirty = true;
   std::vector<mf_segment> segs_h;
   DevBuf data, params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d;
@@ -766,7 +767,8 @@ class HipBatch {
   std::vector<mf_gen_stream> gen_streams_h_;
   // staging in mapped host memory, read and written by the kernels in place: theta and the
   // per-kind segment tables (modes) in, totals out -- a call's graph holds kernels only
-  HostBuf theta_h, segs_hk[4], results_h[4];
+  HostBuf theta_h, segs_hk[4], results_h[4], done_h;
+  int done_seq_ = 0;
   mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> request_;
 
@@ -817,6 +819,8 @@ class HipBatch {
       std::memset(results_h[k].ptr, 0, results_h[k].bytes);
     }
     std::memset(theta_h.ptr, 0, theta_h.bytes);
+    done_h.alloc_mapped(dev, 64);
+    std::memset(done_h.ptr, 0, done_h.bytes);
     request_.assign(S, 0);
     for (int k = 1; k <= 3; k++) fused_[k] = m->nout[k] > 0 && m->kernel_fused(kernel_name(k));
     hip_check(mf_sync(dev), "HipBatch init");
@@ -892,6 +896,7 @@ class HipBatch {
     }
     hip_check(mf_sync(dev), "HipBatch rebuild");
     drop_graphs();
+    for (int k = 0; k < 4; k++) segs_stale_[k] = true;
     dirty = false;
   }
 
@@ -924,11 +929,22 @@ class HipBatch {
 
   // kernel parameters of every slot from the staged theta; with kind > 0 the same launch
   // stages that kind's segment table (modes) into device memory
-  void prep(int kind = 0) {
-    const mf_segment* src = kind > 0 ? (const mf_segment*)segs_hk[kind].dptr : nullptr;
-    mf_segment* dst = kind > 0 ? segs_d[kind].template as<mf_segment>() : nullptr;
-    hip_check(mf_launch_prep(m->mod, (const double*)theta_h.dptr, params_d.ptr, hoist64_d.template as<double>(), nullptr, S, src, dst),
-              "mf_launch_prep");
+  // Kernel parameters from theta, and the segment table of `kind` staged to the device.  Direct
+  // launches pass a small theta by value and copy the table only when its modes changed since
+  // the last copy, so the kernel reads nothing from host memory; a launch for graph capture
+  // reads both from mapped host memory at each replay.
+  void prep(int kind = 0, bool direct = true) {
+    const bool copy = kind > 0 && (!direct || segs_stale_[kind]);
+    const mf_segment* src = copy ? (const mf_segment*)segs_hk[kind].dptr : nullptr;
+    mf_segment* dst = copy ? segs_d[kind].template as<mf_segment>() : nullptr;
+    if (direct && (long long)S * m->ntheta <= MF_PREP_INLINE_THETA)
+      hip_check(mf_launch_prep_inline(m->mod, theta_h.template as<double>(), m->ntheta, S, params_d.ptr,
+                                      hoist64_d.template as<double>(), src, dst),
+                "mf_launch_prep_inline");
+    else
+      hip_check(mf_launch_prep(m->mod, (const double*)theta_h.dptr, params_d.ptr, hoist64_d.template as<double>(), nullptr, S, src, dst),
+                "mf_launch_prep");
+    if (copy) segs_stale_[kind] = false;
   }
 
   // On-device accept-reject into the listed slots (all active slots when `slots` is empty).
@@ -1033,24 +1049,50 @@ class HipBatch {
   // just the evaluation kernels of `kind` (segment table and parameters already on device);
   // main_only launches the per-chunk kernel without any reduction of multi-chunk segments (for
   // timing the event loop alone)
-  void launch_kernel(int kind, bool main_only = false) {
+  void launch_kernel(int kind, bool main_only = false, int done_value = 0) {
     if (nchunks == 0) return;
     mf_eval_args a = eval_args(kind);
     if (main_only) {
       a.flags |= MF_EVAL_SKIP_FINAL;
       a.seg_done = nullptr;
     }
+    if (done_value) {
+      a.done_flag = (int*)done_h.dptr;
+      a.done_value = done_value;
+    }
     hip_check(mf_launch_eval(m->mod, kernel_name(kind), &a, grid(), m->block), "mf_launch_eval");
   }
   // the full per-call sequence of one kind: prep (theta and segment modes from mapped host
   // memory) -> per-chunk kernel -> final pass (totals into mapped host memory)
-  void enqueue(int kind) {
-    prep(kind);
-    launch_kernel(kind);
+  void enqueue(int kind, bool direct = true, int done_value = 0) {
+    prep(kind, direct);
+    launch_kernel(kind, false, done_value);
   }
+  // a one-segment batch whose totals the final kernel reduces: that kernel signals completion
+  // through a flag in mapped host memory, which the host polls (a stream synchronisation costs
+  // several microseconds more)
+  bool can_poll(int kind) const { return S == 1 && nchunks > 0 && max_seg_chunks > 1 && !fused_[kind]; }
+  // wait for done_value in the flag: a bounded spin, then the stream (which reports any error)
+  void wait_done(int done_value) {
+    const int* f = done_h.template as<int>();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 1;; spin++) {
+      if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == done_value) return;
+      if ((spin & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kPollMicros)) break;
+    }
+    hip_check(mf_sync(dev), "HipBatch run");
+    if (__atomic_load_n(f, __ATOMIC_ACQUIRE) != done_value) throw std::runtime_error("HipBatch: completion flag not set after the stream drained");
+  }
+  static constexpr int kPollMicros = 200;
   void set_modes(int kind, bool all) {
     mf_segment* t = segs_hk[kind].template as<mf_segment>();
-    for (int s = 0; s < S; s++) t[s].mode = active[s] && (all || request_[s] == kind) ? 1 : 0;
+    for (int s = 0; s < S; s++) {
+      const int mode = active[s] && (all || request_[s] == kind) ? 1 : 0;
+      if (t[s].mode != mode) {
+        t[s].mode = mode;
+        segs_stale_[kind] = true;
+      }
+    }
   }
   // evaluate `kind` on the requested (or all active) segments; as a replayed hipGraph when use_graph
   void run(int kind, bool all, bool use_graph) {
@@ -1061,10 +1103,15 @@ class HipBatch {
       if (!graphs[kind]) {
         hip_check(mf_sync(dev), "graph capture");
         hip_check(mf_graph_begin(dev), "mf_graph_begin");
-        enqueue(kind);
+        enqueue(kind, false);
         hip_check(mf_graph_end(dev, &graphs[kind]), "mf_graph_end");
       }
       hip_check(mf_graph_launch(graphs[kind]), "mf_graph_launch");
+    } else if (can_poll(kind)) {
+      const int v = ++done_seq_ > 0 ? done_seq_ : (done_seq_ = 1);
+      enqueue(kind, true, v);
+      wait_done(v);
+      return;
     } else {
       enqueue(kind);
     }

@@ -142,6 +142,29 @@ def test_graph_replays_match_eager(ctx):
     mdl.close()
 
 
+@pytest.mark.parametrize("fp32", [True, False])
+def test_polled_completion_returns_complete_totals(ctx, fp32):
+    """A one-segment multi-chunk batch: the final kernel publishes a completion flag that the host
+    polls instead of waiting on the stream.  200 calls alternating two parameter points, all
+    kinds: each result bitwise equal to the stream-synchronised result at that point (a flag
+    published before the totals would show the other point's totals)."""
+    n = 1_000_003
+    data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=11, nstreams=16)
+    mdl = ctx.model(M.MASS, fp32=fp32)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    thetas = [np.array(M.MASS_START), np.array([5.27, 0.021, 0.31, -1.4])]
+    say("polled completion fp32", fp32)
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        want = [b.eval(kind, t, use_graph=True) for t in thetas]
+        assert not np.array_equal(want[0], want[1])
+        for i in range(200):
+            got = b.eval(kind, thetas[i % 2])
+            assert np.array_equal(got, want[i % 2]), (kind, i)
+    b.close()
+    mdl.close()
+
+
 @pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("fp32", [False, True])
 def test_segment_reduction_in_kernel_and_final_pass(fp32, fused):
@@ -181,7 +204,8 @@ def test_segment_reduction_in_kernel_and_final_pass(fp32, fused):
 @pytest.mark.parametrize("fp32", [False, True])
 def test_segments_alone_equal_mixed(ctx, fp32):
     """ragged segments (0, 1, 63, 4096, 4097, 100000 events): each segment's sums are bitwise
-    identical evaluated alone and together with the others, and match the oracle."""
+    identical evaluated alone (first and later calls) and together with the others, and match
+    the oracle."""
     sizes = [0, 1, 63, 4096, 4097, 100000]
     rs = np.random.default_rng(9)
     datas = [O.generate(M.MASS, M.MASS_TRUTH, s, seed=100 + i, nstreams=2) if s else np.zeros((1, 0)) for i, s in enumerate(sizes)]
@@ -199,6 +223,11 @@ def test_segments_alone_equal_mixed(ctx, fp32):
             alone.upload(0, d)
         r = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
         assert np.array_equal(r[0], both[i]), (i, r[0] - both[i])
+        # from its second call on, a one-segment batch passes theta to mf_prep by value and
+        # leaves the (unchanged) device segment table alone: still the same bits
+        for _ in range(2):
+            r2 = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
+            assert np.array_equal(r2[0], both[i]), (i, r2[0] - both[i])
         if sizes[i] == 0:
             assert np.all(r[0] == 0)
         else:
