@@ -44,6 +44,8 @@ typedef struct mfh_options {
   int balance;             /* large segments: chunk count a multiple of the resident work-groups */
   int base2;               /* exp/log as exp2/log2: 0 never, 1 FP32 kernels, 2 FP32 and FP64 */
   int fused_final;         /* 1: segment totals in the work-group finishing a segment; 0: final kernel */
+  int event_columns;       /* 1: event-only terms precomputed into data columns (optimize_dimensions) */
+  double event_cost;       /* cost above which an event-only term becomes a column (graph cost units) */
 } mfh_options;
 
 typedef struct mfh_fit_out {
@@ -61,6 +63,8 @@ typedef struct mfh_batch_stats {
   double wall_ms, gpu_wait_ms, host_ms;
   int host_threads;        /* host threads that ran the fits' MIGRAD code */
   int pad;
+  double setup_ms;         /* creating the fits (fibres and their stacks) */
+  double step_ms;          /* running the fits' host code between launches */
 } mfh_batch_stats;
 
 const char* mfh_last_error(void);
@@ -77,7 +81,8 @@ int mfh_model_destroy(mfh_model* m);
 /* info[0..7] = ntheta, nhoist, nobs, nout_lh, nout_grad, nout_hess, statements, graph_nodes;
  * info[8..10] = unroll of the value, gradient and Hessian kernels (fitted to the register
  * budget, 0 when not compiled), info[11..13] = their VGPR counts, info[14..16] = 1 where the
- * kernel reduces multi-chunk segments itself (0: a final kernel follows it);
+ * kernel reduces multi-chunk segments itself (0: a final kernel follows it), info[17] = data
+ * columns per event (observables, then precomputed event terms);
  * ms[0..1] = codegen ms, hipRTC compile + load ms (small when served from the cache) */
 int mfh_model_info(mfh_model* m, int* info, double* ms);
 int mfh_model_source(mfh_model* m, char* buf, int len);
@@ -92,9 +97,11 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
 /* without a device: the per-event outputs of the value (which = 1), gradient (2) or Hessian (3)
  * kernel of a configuration module exactly as generated (observables centred for fp32, base-2
  * transcendentals, hoisted terms, output scales applied), evaluated on the host in FP64 at
- * theta for events data (SoA nobs x n): out[i * nout + k].  Checks the code generator's
- * rewrites against the density's definition. */
-int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double* out);
+ * theta for events data (SoA nobs x n): out[i * nout + k].  event_cost >= 0: event-only terms
+ * costing more become columns computed from the observables first (event_columns).  Checks the
+ * code generator's rewrites against the density's definition. */
+int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double event_cost,
+                           double* out);
 /* launch-shape autotuner: for each candidate (block, events_per_thread, unroll) triple in
  * cand[3 * ncand] (NULL: 16 built-in shapes, block 256/512 x 8..64 events per thread x unroll
  * 4/8) times `eval_kind` evaluations (per-chunk kernel + final pass, mean of `iters`
@@ -143,13 +150,16 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
                   const long long* n, const unsigned long long* seeds, const double* arrival_s, int nreq, double budget_bytes,
                   int slots, double* timings, double* values, double* errors, int* status, double* stats);
 
-/* one fit through the drop-in HIPBlock (reference fitter flow); data SoA nobs x n */
+/* one fit through the drop-in HIPBlock (reference fitter flow); data SoA nobs x n.  The
+ * context's event_columns option turns on the fitter's optimize_dimensions (a store kernel
+ * precomputes event-only terms into extra columns, fitter.hh:339-375). */
 int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, double* values,
                   double* errors, mfh_fit_out* out, double* ms);
 /* one fit through the host-compiled CPU backend (no device needed); fp32 = 1 runs the FP32
- * kernels (FP32-rounded parameters, centred FP32 observables) as the device's FP32 path does */
-int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, double* values,
-                double* errors, mfh_fit_out* out, double* ms);
+ * kernels (FP32-rounded parameters, centred FP32 observables) as the device's FP32 path does;
+ * optimize_dimensions as the fitter option of that name */
+int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, int optimize_dimensions,
+                double* values, double* errors, mfh_fit_out* out, double* ms);
 /* weighted fit (log p of event i multiplied by weights[i]) with the reference's covariance
  * correction -2 H^-1 F H^-1 (Fisher matrix F from analytic kernels): through the drop-in
  * HIPBlock when ctx is set, else through the CPU backend with nthreads threads */

@@ -45,7 +45,7 @@ class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("block_size", ctypes.c_int), ("events_per_thread", ctypes.c_int),
                 ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int),
                 ("unroll", ctypes.c_int), ("balance", ctypes.c_int), ("base2", ctypes.c_int),
-                ("fused_final", ctypes.c_int)]
+                ("fused_final", ctypes.c_int), ("event_columns", ctypes.c_int), ("event_cost", ctypes.c_double)]
 
 
 class FitOut(ctypes.Structure):
@@ -56,7 +56,8 @@ class FitOut(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("ticks", ctypes.c_int), ("launches_lh", ctypes.c_int), ("launches_grad", ctypes.c_int),
                 ("launches_hess", ctypes.c_int), ("wall_ms", ctypes.c_double), ("gpu_wait_ms", ctypes.c_double),
-                ("host_ms", ctypes.c_double), ("host_threads", ctypes.c_int), ("pad", ctypes.c_int)]
+                ("host_ms", ctypes.c_double), ("host_threads", ctypes.c_int), ("pad", ctypes.c_int),
+                ("setup_ms", ctypes.c_double), ("step_ms", ctypes.c_double)]
 
 
 _hip = None
@@ -102,7 +103,7 @@ def libs():
         L.mfh_model_source.argtypes = [vp, ctypes.c_char_p, ip]
         L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
         L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
-        L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, dp]
+        L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, ctypes.c_double, dp]
         L.mfh_stream_states.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ip, ctypes.POINTER(ctypes.c_uint32)]
         L.mfh_autotune.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip, ip, ctypes.POINTER(ctypes.c_int), ip, ip,
                                    ctypes.POINTER(ctypes.c_int), dp]
@@ -119,7 +120,7 @@ def libs():
         L.mfh_batch_bytes.restype = ctypes.c_longlong
         L.mfh_batch_fit.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ip, dp, ip, dp, dp, ctypes.POINTER(FitOut), ctypes.POINTER(BatchStats)]
         L.mfh_fit_plain.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, dp, dp, ctypes.POINTER(FitOut), dp]
-        L.mfh_fit_cpu.argtypes = [ip, ip, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
+        L.mfh_fit_cpu.argtypes = [ip, ip, dp, ctypes.c_longlong, dp, ip, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_fit_weighted.argtypes = [vp, ip, ip, dp, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_eval_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
         L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
@@ -146,7 +147,7 @@ def options(**kw):
     o = Options()
     libs()[1].mfh_default_options(ctypes.byref(o))
     for k, v in kw.items():
-        setattr(o, k, int(v))
+        setattr(o, k, float(v) if k == "event_cost" else int(v))
     return o
 
 
@@ -160,14 +161,15 @@ def kernel_resources(kind, fp32, with_hessian=True, fit=True, **kw):
             for j, k in enumerate((EVAL_LH, EVAL_GRAD, EVAL_HESS))}
 
 
-def host_outputs(kind, fp32, which, theta, data):
+def host_outputs(kind, fp32, which, theta, data, event_cost=-1.0):
     """Per-event outputs of a configuration module's value/gradient/Hessian kernel (which = 1/2/3)
-    as generated, evaluated on the host in FP64 (no device): array (n, nout)."""
+    as generated, evaluated on the host in FP64 (no device): array (n, nout).  event_cost >= 0:
+    with the event-only terms above that cost taken from precomputed columns."""
     data = _f64(np.atleast_2d(data))
     n = data.shape[1]
     nout = {1: 1, 2: 1 + NTHETA[kind], 3: 1 + NTHETA[kind] + NTHETA[kind] * (NTHETA[kind] + 1) // 2}[which]
     out = np.zeros(n * nout)
-    _check(libs()[1].mfh_model_host_outputs(kind, int(fp32), which, _dp(_f64(theta)), _dp(data), n, _dp(out)))
+    _check(libs()[1].mfh_model_host_outputs(kind, int(fp32), which, _dp(_f64(theta)), _dp(data), n, float(event_cost), _dp(out)))
     return out.reshape(n, nout)
 
 
@@ -247,7 +249,7 @@ class Model:
         if self._rng is not None:
             r = _dp(self._rng)
         _check(libs()[1].mfh_model_create(ctx.h, kind, int(fp32), r, int(with_hessian), ctypes.byref(self.h)))
-        info = (ctypes.c_int * 17)()
+        info = (ctypes.c_int * 18)()
         ms = (ctypes.c_double * 2)()
         _check(libs()[1].mfh_model_info(self.h, info, ms))
         self.ntheta, self.nhoist, self.nobs = info[0], info[1], info[2]
@@ -256,6 +258,7 @@ class Model:
         self.unroll = {EVAL_LH: info[8], EVAL_GRAD: info[9], EVAL_HESS: info[10]}
         self.vgprs = {EVAL_LH: info[11], EVAL_GRAD: info[12], EVAL_HESS: info[13]}
         self.fused = {EVAL_LH: bool(info[14]), EVAL_GRAD: bool(info[15]), EVAL_HESS: bool(info[16])}
+        self.ncol = info[17]
         self.codegen_ms, self.compile_ms = ms[0], ms[1]
 
     def source(self):
@@ -374,15 +377,16 @@ def serve(models, theta_gen, start, cls, n, seeds, arrival_s, budget_bytes, slot
             "rejected": int(stats[3]), "wall_s": stats[4], "peak_bytes": stats[5], "generate_ms": stats[6]}
 
 
-def fit_cpu(kind, data, start, nthreads=1, fp32=False):
+def fit_cpu(kind, data, start, nthreads=1, fp32=False, optimize_dimensions=False):
     """One fit through the host-compiled CPU backend (reference LLVM-backend semantics);
-    fp32=True runs the FP32 kernels the way the device's FP32 path does."""
+    fp32=True runs the FP32 kernels the way the device's FP32 path does; optimize_dimensions
+    precomputes event-only terms into extra data columns first (fitter option of that name)."""
     data = _f64(np.atleast_2d(data))
     P = NTHETA[kind]
     vals, errs = np.zeros(P), np.zeros(P)
     out, ms = FitOut(), ctypes.c_double(0)
-    _check(libs()[1].mfh_fit_cpu(kind, int(fp32), _dp(data), data.shape[1], _dp(_f64(start)), nthreads, _dp(vals), _dp(errs),
-                                 ctypes.byref(out), ctypes.byref(ms)))
+    _check(libs()[1].mfh_fit_cpu(kind, int(fp32), _dp(data), data.shape[1], _dp(_f64(start)), nthreads, int(optimize_dimensions),
+                                 _dp(vals), _dp(errs), ctypes.byref(out), ctypes.byref(ms)))
     return vals, errs, out, ms.value
 
 

@@ -100,6 +100,8 @@ void apply_options(compute_options& co, const mfh_options* o) {
   co.hip_balance_chunks = o->balance != 0;
   co.hip_base2 = o->base2;
   co.hip_fused_final = o->fused_final != 0;
+  co.hip_event_columns = o->event_columns != 0;
+  co.hip_event_cost_threshold = o->event_cost;
 }
 }  // namespace
 
@@ -148,6 +150,8 @@ void mfh_default_options(mfh_options* o) {
   o->balance = 0;
   o->base2 = 1;
   o->fused_final = 0;
+  o->event_columns = 0;
+  o->event_cost = 10.0;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {
@@ -222,6 +226,7 @@ int mfh_model_info(mfh_model* m, int* info, double* ms) {
       info[11 + k] = hm.kernel_vgprs(names[k]);
       info[14 + k] = hm.kernel_fused(names[k]) ? 1 : 0;
     }
+    info[17] = hm.ncol;
     if (ms) {
       ms[0] = hm.codegen_ms;
       ms[1] = hm.compile_ms;
@@ -272,7 +277,8 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
     } else {
       const bool f32 = std::is_same<K, float>::value;
       LikelihoodGraphs<K> g(d.pdf, d.params, f32 && co.hip_centre_observables, with_hessian != 0,
-                            co.hip_base2 >= 2 || (f32 && co.hip_base2 == 1));
+                            co.hip_base2 >= 2 || (f32 && co.hip_base2 == 1), 0.0,
+                            co.hip_event_columns ? co.hip_event_cost_threshold : -1.0);
       ms = g.module(co, with_hessian != 0);
       mf_kernel_resources r[64];
       int n = 0;
@@ -305,12 +311,13 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
   MFH_CATCH
 }
 
-int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double* out) {
+int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double event_cost,
+                           double* out) {
   MFH_TRY
   if (which < 1 || which > 3) throw std::invalid_argument("which must be 1, 2 or 3");
   auto run = [&](auto& d) {
     typedef typename std::remove_reference<decltype(d)>::type::kernel_type K;
-    LikelihoodGraphs<K> g(d.pdf, d.params, fp32 != 0, true, fp32 != 0);
+    LikelihoodGraphs<K> g(d.pdf, d.params, fp32 != 0, true, fp32 != 0, 0.0, event_cost);
     std::vector<std::string> names(g.theta);
     std::vector<double> vals(theta, theta + g.theta.size());
     const size_t nt = names.size();
@@ -321,12 +328,17 @@ int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, c
     vals.insert(vals.end(), hv.begin(), hv.end());
     const size_t off = names.size();
     names.insert(names.end(), g.obs.begin(), g.obs.end());
+    const size_t eoff = names.size();
+    names.insert(names.end(), g.ev_names.begin(), g.ev_names.end());
     vals.resize(names.size());
+    const std::vector<std::string> obs_names(g.obs);
+    std::vector<double> xv(g.obs.size());
     const auto& outs = which == 1 ? g.lh : (which == 2 ? g.grad : g.hess);
     const auto& sc = which == 1 ? g.lh_scale : (which == 2 ? g.grad_scale : g.hess_scale);
     const size_t no = outs.size();
     for (long long i = 0; i < n; i++) {
-      for (size_t dd = 0; dd < g.obs.size(); dd++) vals[off + dd] = data[dd * n + i] - g.centres[dd];
+      for (size_t dd = 0; dd < g.obs.size(); dd++) xv[dd] = vals[off + dd] = data[dd * n + i] - g.centres[dd];
+      for (size_t e = 0; e < g.ev_exprs.size(); e++) vals[eoff + e] = g.ev_exprs[e]->eval(obs_names, xv);
       for (size_t k = 0; k < no; k++)
         out[i * no + k] = (double)outs[k]->eval(names, vals) * (sc[k] >= 0 ? hv[sc[k]] : 1.0);
     }
@@ -617,6 +629,8 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
       stats->gpu_wait_ms = bf.stats.gpu_wait_ms;
       stats->host_ms = bf.stats.host_ms;
       stats->host_threads = bf.stats.host_threads;
+      stats->setup_ms = bf.stats.setup_ms;
+      stats->step_ms = bf.stats.step_ms;
     }
     return 0;
   });
@@ -686,7 +700,7 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
 namespace {
 template <typename K, typename Backend, typename Block>
 void run_fit(Backend* be, int kind, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
-             double* ms, const double* weights = nullptr) {
+             double* ms, const double* weights = nullptr, bool optimize_dimensions = false) {
   ModelDef<K> def(kind, nullptr);
   // weights ride along as an extra column named "w" that the fitter multiplies log p by
   dimension<double> wdim("w", -1e300, 1e300);
@@ -705,6 +719,7 @@ void run_fit(Backend* be, int kind, const double* data, long long n, const doubl
   }
   fitter_options fo;
   fo.analytic_fisher = weights != nullptr;
+  fo.optimize_dimensions = optimize_dimensions;
   fitter<K, double, Backend, Block> f(&fo, be);
   f.fit(def.pdf, def.params, &ev);
   for (size_t k = 0; k < def.params.size(); k++) {
@@ -728,19 +743,22 @@ extern "C" {
 int mfh_fit_plain(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, double* values,
                   double* errors, mfh_fit_out* out, double* ms) {
   MFH_TRY
-  if (fp32) run_fit<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms);
-  else run_fit<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms);
+  // the context's event_columns option selects the fitter's optimize_dimensions
+  const bool od = ctx->be->opts_->hip_event_columns;
+  if (fp32) run_fit<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms, nullptr, od);
+  else run_fit<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms, nullptr, od);
   MFH_CATCH
 }
 
-int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, double* values,
-                double* errors, mfh_fit_out* out, double* ms) {
+int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const double* start, int nthreads, int optimize_dimensions,
+                double* values, double* errors, mfh_fit_out* out, double* ms) {
   MFH_TRY
   compute_options co;
   co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
   CPUBackend be(&co);
-  if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms);
-  else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms);
+  const bool od = optimize_dimensions != 0;
+  if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms, nullptr, od);
+  else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms, nullptr, od);
   MFH_CATCH
 }
 

@@ -4,7 +4,9 @@
 // The generated source defines, before this runtime:
 //   MF_T            float or double (storage and per-event arithmetic type)
 //   MF_FP32         1 for float kernels
-//   MF_NOBS         observables per event (SoA columns)
+//   MF_NOBS         observables per event (SoA columns the generator writes and uploads fill)
+//   MF_NCOL         columns the evaluation kernels read: the observables, then event-only terms
+//                   the precompute kernel fills (MF_NCOL > MF_NOBS; mf_event_terms)
 //   MF_NTHETA       floating parameters (FP64 on the host, first kernel-parameter slots)
 //   MF_NHOIST       hoisted parameter-only terms (FP64 on the device prep kernel)
 //   MF_BLOCK        work-group size of the evaluation kernels (multiple of 64)
@@ -26,6 +28,9 @@
 // completes a segment's last chunk (a per-segment counter tells it, mf_eval_args.seg_done) or
 // in a second small kernel launched after the first (no fences or atomics).
 
+#ifndef MF_NCOL
+#define MF_NCOL MF_NOBS
+#endif
 #define MF_NPARAM (MF_NTHETA + MF_NHOIST)
 #define MF_GEN_MAX_TRIES (1u << 20)
 #define MF_NWAVE (MF_BLOCK / 64)
@@ -257,11 +262,11 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
     int i = ch.begin + (int)threadIdx.x;
     // U events per thread per step: their loads are issued together, then computed
     for (; i + (U - 1) * MF_BLOCK < ch.end; i += U * MF_BLOCK) {
-      MF_T x[U][MF_NOBS];
+      MF_T x[U][MF_NCOL];
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int d = 0; d < MF_NOBS; d++) x[u][d] = __builtin_nontemporal_load(col + d * sg.stride + i + u * MF_BLOCK);
+        for (int d = 0; d < MF_NCOL; d++) x[u][d] = __builtin_nontemporal_load(col + d * sg.stride + i + u * MF_BLOCK);
 #if MF_FP32 && MF_ACC == 2
       MF_T g[NO];
       F::eval(x[0], P, g);
@@ -283,9 +288,9 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
 #endif
     }
     for (; i < ch.end; i += MF_BLOCK) {
-      MF_T x[MF_NOBS];
+      MF_T x[MF_NCOL];
 #pragma unroll
-      for (int d = 0; d < MF_NOBS; d++) x[d] = __builtin_nontemporal_load(col + d * sg.stride + i);
+      for (int d = 0; d < MF_NCOL; d++) x[d] = __builtin_nontemporal_load(col + d * sg.stride + i);
       MF_T o[NO];
       F::eval(x, P, o);
       acc.add(o);
@@ -360,9 +365,9 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
     const MF_T* __restrict__ P = (const MF_T*)a.params + sg.param_off;
     const MF_T* __restrict__ col = data + sg.data_off;
     for (int i = ch.begin + (int)threadIdx.x; i < ch.end; i += MF_BLOCK) {
-      MF_T x[MF_NOBS];
+      MF_T x[MF_NCOL];
 #pragma unroll
-      for (int d = 0; d < MF_NOBS; d++) x[d] = col[d * sg.stride + i];
+      for (int d = 0; d < MF_NCOL; d++) x[d] = col[d * sg.stride + i];
       MF_T o[NO];
       F::eval(x, P, o);
 #pragma unroll
@@ -370,6 +375,35 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
     }
   }
 }
+
+// ------------------------------------------------------------------ event columns
+// Event-only terms of the density (generated mf_event_terms: FP64 from the stored observables)
+// into columns MF_NOBS..MF_NCOL-1 of every segment whose mode is set, rounded once to MF_T.
+// Runs once per upload or generation; the evaluation kernels then read the terms instead of
+// recomputing them at every call (the reference's optimize_dimensions, fitter.hh:339-375).
+#if MF_NCOL > MF_NOBS
+__device__ __forceinline__ void mf_event_terms(const double* x, double* buf);
+__device__ __forceinline__ void mf_precompute_impl(const mf_eval_args& a) {
+  MF_T* __restrict__ data = (MF_T*)a.data;  // this kernel writes the event columns of the data buffer
+  for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+    const mf_chunk ch = a.chunks[c];
+    const mf_segment sg = a.segs[ch.seg];
+    if (sg.mode == 0) continue;
+    MF_T* __restrict__ col = data + sg.data_off;
+    for (int i = ch.begin + (int)threadIdx.x; i < ch.end; i += MF_BLOCK) {
+      double xd[MF_NOBS];
+#pragma unroll
+      for (int d = 0; d < MF_NOBS; d++) xd[d] = (double)col[d * sg.stride + i];
+      double o[MF_NCOL - MF_NOBS];
+      mf_event_terms(xd, o);
+#pragma unroll
+      for (int k = 0; k < MF_NCOL - MF_NOBS; k++) col[(MF_NOBS + k) * sg.stride + i] = (MF_T)o[k];
+    }
+  }
+}
+#define MF_PRECOMPUTE_KERNEL \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) mf_precompute(mf_eval_args a) { mf_precompute_impl(a); }
+#endif
 
 #define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL, FUSED)                                                                    \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL, FUSED>(a); } \

@@ -1,7 +1,7 @@
 /**
  * @file batch.hh
  * Many fits at once.  Each fit runs the ordinary blocking MIGRAD/HESSE code in its own fibre
- * (ucontext); when a fit asks for a value, a gradient or a Hessian it posts its parameters
+ * (fiber.hh); when a fit asks for a value, a gradient or a Hessian it posts its parameters
  * into its segment's slot and yields.  Once every live fibre has yielded, the scheduler
  * issues one shared launch per (batch, kind) over all requesting segments, and resumes the
  * fibres with their totals.  Converged fits leave, new fits may join between ticks (the fit
@@ -10,8 +10,6 @@
  */
 #ifndef MOREFIT_BATCH_HH
 #define MOREFIT_BATCH_HH
-
-#include <ucontext.h>
 
 #include <chrono>
 #include <cmath>
@@ -27,6 +25,7 @@
 #include <vector>
 
 #include "compute_hip.hh"
+#include "fiber.hh"
 #include "minimizer.hh"
 
 namespace morefit {
@@ -43,6 +42,8 @@ struct BatchFitStats {
   int launches[4] = {0, 0, 0, 0};
   double wall_ms = 0.0, gpu_wait_ms = 0.0, host_ms = 0.0;
   int host_threads = 1;
+  double setup_ms = 0.0;  // creating the fits (fibres, stacks)
+  double step_ms = 0.0;   // running the fibres between launches (all host threads)
 };
 
 template <typename kernelT, typename evalT = double>
@@ -56,9 +57,9 @@ class FitScheduler {
     int worker = 0;                 // host thread that runs this fit, for its whole life
     long long id = 0;
     std::vector<double> start;
-    std::unique_ptr<char[]> stack;  // not zero-filled: pages are touched only as used
-    ucontext_t ctx;
-    ucontext_t* ret = nullptr;      // context of the worker that resumed this fit
+    char* stack = nullptr;          // from FiberStacks, returned when the task leaves
+    void* sp = nullptr;             // saved stack pointer of this fibre while it is suspended
+    void** ret = nullptr;           // where the worker that resumed it saved its own
     int pending = 0;
     bool done = false;
     BatchFitResult result;
@@ -71,7 +72,6 @@ class FitScheduler {
   // MIGRAD trace level of every fit (>= 3: per-iteration lines on stderr); the environment
   // variable MOREFIT_PRINT_LEVEL sets it for fits driven through the C API
   int print_level = std::getenv("MOREFIT_PRINT_LEVEL") ? std::atoi(std::getenv("MOREFIT_PRINT_LEVEL")) : 0;
-  size_t stack_bytes = 128 * 1024;
   // host threads that run the fits' MIGRAD code between launches (0: one per hardware
   // thread, at most 16); a fit stays on the thread it was assigned, so a fibre never moves
   // between OS threads.  Device work is issued by the calling thread only.
@@ -81,7 +81,10 @@ class FitScheduler {
   FitScheduler() {}
   FitScheduler(const FitScheduler&) = delete;
   FitScheduler& operator=(const FitScheduler&) = delete;
-  ~FitScheduler() { stop_workers(); }
+  // fits still suspended are abandoned (their fibres never resume); their stacks go back
+  ~FitScheduler() {
+    for (auto& t : tasks_) FiberStacks::get().release(t.stack);
+  }
 
   // a new fit of segment `seg` of batch `b` from `start` (ntheta values)
   Task* add(Batch* b, int seg, const double* start, long long id = 0) {
@@ -93,13 +96,8 @@ class FitScheduler {
     t.id = id;
     t.worker = (int)(added_++ % (unsigned long long)nthreads());
     t.start.assign(start, start + b->m->ntheta);
-    t.stack.reset(new char[stack_bytes]);
-    getcontext(&t.ctx);
-    t.ctx.uc_stack.ss_sp = t.stack.get();
-    t.ctx.uc_stack.ss_size = stack_bytes;
-    t.ctx.uc_link = nullptr;  // a finished fit switches back explicitly (trampoline)
-    uintptr_t p = (uintptr_t)&t;
-    makecontext(&t.ctx, (void (*)())&FitScheduler::trampoline, 2, (unsigned int)(p & 0xffffffffu), (unsigned int)(p >> 32));
+    t.stack = FiberStacks::get().acquire();
+    t.sp = fiber_prepare(t.stack, FiberStacks::kStackBytes, &FitScheduler::fiber_main, &t);
     batches_.insert(b);
     return &t;
   }
@@ -110,32 +108,31 @@ class FitScheduler {
   // and removed.  Returns the number still live.
   template <typename F>
   size_t step(F on_done) {
+    const auto ts = std::chrono::steady_clock::now();
     const int T = nthreads();
     if (T > 1 && tasks_.size() > 1) {
-      start_workers(T);
       groups_.assign(T, {});
       for (auto& t : tasks_)
         if (!t.pending && !t.done) groups_[t.worker].push_back(&t);
-      {
-        std::unique_lock<std::mutex> lk(mu_);
-        ++generation_;
-        running_ = T;
-        cv_go_.notify_all();
-        cv_done_.wait(lk, [&] { return running_ == 0; });
-      }
+      HostThreads::get().run(T, [this](int w) {
+        void* here = nullptr;
+        for (Task* t : groups_[w]) resume(*t, &here);
+      });
     } else {
-      ucontext_t here;
+      void* here = nullptr;
       for (auto& t : tasks_)
         if (!t.pending && !t.done) resume(t, &here);
     }
     for (auto it = tasks_.begin(); it != tasks_.end();) {
       if (it->done) {
         on_done(*it);
+        FiberStacks::get().release(it->stack);
         it = tasks_.erase(it);
       } else {
         ++it;
       }
     }
+    stats.step_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
     return tasks_.size();
   }
   // one shared launch per (batch, kind) over every pending request, then wakes the tasks
@@ -178,7 +175,7 @@ class FitScheduler {
       const auto& ti = t->b->m->theta_index;
       for (size_t k = 0; k < ti.size(); k++) th[k] = x[ti[k]];
       t->pending = kind;
-      swapcontext(&t->ctx, t->ret);
+      mf_fiber_switch(&t->sp, *t->ret);
     }
     double value(const double* x) override {
       post(x, EVAL_LH);
@@ -217,56 +214,25 @@ class FitScheduler {
   std::set<Batch*> batches_;
   unsigned long long added_ = 0;
 
-  // worker pool: each step, worker w resumes the tasks in groups_[w]
-  std::vector<std::thread> workers_;
+  // per step: the tasks worker w resumes (its own, in order)
   std::vector<std::vector<Task*>> groups_;
-  std::mutex mu_;
-  std::condition_variable cv_go_, cv_done_;
-  unsigned long long generation_ = 0;
-  int running_ = 0;
-  bool stopping_ = false;
 
-  static void resume(Task& t, ucontext_t* here) {
+  static void resume(Task& t, void** here) {
     t.ret = here;
-    swapcontext(here, &t.ctx);
+    mf_fiber_switch(here, t.sp);
   }
-  void start_workers(int T) {
-    if ((int)workers_.size() == T) return;
-    stop_workers();
-    stopping_ = false;
-    for (int w = 0; w < T; w++)
-      workers_.emplace_back([this, w] {
-        unsigned long long seen = 0;
-        ucontext_t here;
-        for (;;) {
-          {
-            std::unique_lock<std::mutex> lk(mu_);
-            cv_go_.wait(lk, [&] { return stopping_ || generation_ != seen; });
-            if (stopping_) return;
-            seen = generation_;
-          }
-          for (Task* t : groups_[w]) resume(*t, &here);
-          std::lock_guard<std::mutex> lk(mu_);
-          if (--running_ == 0) cv_done_.notify_one();
-        }
-      });
-  }
-  void stop_workers() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stopping_ = true;
-      cv_go_.notify_all();
+  // body of every fibre: the fit, then back to the worker for good
+  static void fiber_main(void* arg) {
+    Task* t = (Task*)arg;
+    try {
+      t->owner->run_fit(*t);
+    } catch (...) {
+      t->result.status = 4;
     }
-    for (auto& th : workers_) th.join();
-    workers_.clear();
-  }
-
-  static void trampoline(unsigned int lo, unsigned int hi) {
-    Task* t = (Task*)(((uintptr_t)hi << 32) | (uintptr_t)lo);
-    t->owner->run_fit(*t);
     t->done = true;
     t->pending = 0;
-    setcontext(t->ret);  // back to the worker that resumed it; this context is not used again
+    mf_fiber_switch(&t->sp, *t->ret);
+    __builtin_trap();  // never resumed
   }
 
   void run_fit(Task& t) {
@@ -328,9 +294,11 @@ class BatchFitter {
     sch.run_hesse = run_hesse;
     const int P = b.m->ntheta;
     for (size_t i = 0; i < segs.size(); i++) sch.add(&b, segs[i], &start[i * P], (long long)i);
+    const double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::vector<BatchFitResult> out(segs.size());
     while (sch.step([&](auto& t) { out[t.id] = t.result; }) > 0) sch.launch();
     stats = sch.stats;
+    stats.setup_ms = setup_ms;
     stats.host_threads = segs.size() > 1 ? sch.nthreads() : 1;
     stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats.host_ms = stats.wall_ms - stats.gpu_wait_ms;

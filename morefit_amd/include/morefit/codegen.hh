@@ -30,7 +30,9 @@ struct EvalKernelSpec {
 
 struct ModuleSource {
   bool fp32 = false;
-  int nobs = 1;
+  int nobs = 1;                            // observables (generator output, uploaded columns)
+  int ncol = 0;                            // columns the evaluation kernels read: observables, then
+                                           // precomputed event terms (0: same as nobs)
   int ntheta = 0;
   int nhoist = 0;
   int block = 256;
@@ -39,6 +41,7 @@ struct ModuleSource {
   std::vector<double> gen_lo, gen_width;  // proposal box per observable, in stored coordinates
   std::string functors;                    // functor structs
   std::string hoist;                       // mf_hoist definition
+  std::string event_terms;                 // mf_event_terms definition (ncol > nobs)
   std::vector<EvalKernelSpec> evals;       // evaluation kernels (unroll chosen per kernel)
   std::vector<std::string> kernels;        // other instantiation lines
   size_t statements = 0;                   // SSA statements emitted (all functors)
@@ -48,6 +51,7 @@ struct ModuleSource {
     s << "#define MF_T " << (fp32 ? "float" : "double") << "\n"
       << "#define MF_FP32 " << (fp32 ? 1 : 0) << "\n"
       << "#define MF_NOBS " << nobs << "\n"
+      << "#define MF_NCOL " << (ncol > nobs ? ncol : nobs) << "\n"
       << "#define MF_NTHETA " << ntheta << "\n"
       << "#define MF_NHOIST " << nhoist << "\n"
       << "#define MF_BLOCK " << block << "\n"
@@ -67,7 +71,7 @@ struct ModuleSource {
   // generated module text: prelude, the runtime marker, then functors/hoist/kernels
   std::string device_source() const {
     std::ostringstream s;
-    s << "// generated by morefit codegen.hh\n" << prelude() << "//@MF_RUNTIME@\n" << functors << "\n" << hoist << "\n";
+    s << "// generated by morefit codegen.hh\n" << prelude() << "//@MF_RUNTIME@\n" << functors << "\n" << hoist << "\n" << event_terms << "\n";
     for (const auto& e : evals)
       s << "MF_EVAL_KERNEL(" << e.name << ", " << e.functor << ", " << e.unroll << ", " << (e.fused ? 1 : 0) << ")\n";
     for (const auto& k : kernels) s << k << "\n";
@@ -110,20 +114,28 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
   return s.str();
 }
 
-// void mf_hoist(const double* th, double* buf): FP64 values of the hoisted expressions
+// void NAME(const double* ARG, double* buf): FP64 values of `exprs`, whose variables are
+// var_names bound to ARG[k]
 template <typename kernelT, typename evalT>
-std::string emit_hoist(const std::vector<const ComputeGraphNode<kernelT, evalT>*>& exprs, const std::vector<std::string>& theta_names) {
+std::string emit_fp64_function(const std::string& name, const std::string& arg, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& exprs,
+                               const std::vector<std::string>& var_names) {
   KernelEmitter em(false);
   em.prefix = "mfd_";
   em.tmp = "h";
-  for (size_t k = 0; k < theta_names.size(); k++) em.bindings[theta_names[k]] = "th[" + std::to_string(k) + "]";
+  for (size_t k = 0; k < var_names.size(); k++) em.bindings[var_names[k]] = arg + "[" + std::to_string(k) + "]";
   std::vector<std::string> res;
   for (auto* g : exprs) res.push_back(em.emit(*g));
   std::ostringstream s;
-  s << "__device__ __forceinline__ void mf_hoist(const double* th, double* buf) {\n  (void)th; (void)buf;\n" << em.body.str();
+  s << "__device__ __forceinline__ void " << name << "(const double* " << arg << ", double* buf) {\n  (void)" << arg << "; (void)buf;\n"
+    << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "  buf[" << k << "] = " << res[k] << ";\n";
   s << "}\n";
   return s.str();
+}
+// void mf_hoist(const double* th, double* buf): FP64 values of the hoisted parameter-only terms
+template <typename kernelT, typename evalT>
+std::string emit_hoist(const std::vector<const ComputeGraphNode<kernelT, evalT>*>& exprs, const std::vector<std::string>& theta_names) {
+  return emit_fp64_function("mf_hoist", "th", exprs, theta_names);
 }
 
 }  // namespace morefit

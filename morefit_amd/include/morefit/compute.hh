@@ -50,6 +50,11 @@ struct compute_options {
   int hip_unroll = 8;                // events per thread whose loads are issued together
   int hip_base2 = 1;                 // exp/log -> exp2/log2 rewrite: 0 never, 1 FP32 kernels, 2 FP32 and FP64
   bool hip_balance_chunks = false;
+  // event-only subexpressions costing more than hip_event_cost_threshold (cost units of the
+  // graph: 1 per add/multiply, 20 per transcendental) become data columns computed once per
+  // upload (the reference's optimize_dimensions); each column adds sizeof(T) bytes per event
+  bool hip_event_columns = false;
+  double hip_event_cost_threshold = 10.0;
   // segment totals in the work-group that completes a segment (no final kernel).  Off: each
   // work-group's device-scope release fence writes back its XCD's L2, and on MI355X those
   // write-backs cost more than the final kernel's launch (measured 36.7 vs 19.0 us per

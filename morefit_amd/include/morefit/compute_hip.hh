@@ -476,11 +476,17 @@ struct LikelihoodGraphs {
   Ptr gen;                                   // log p for the generator, hoisted but otherwise as written
   std::vector<std::string> hoist_names;
   std::vector<Ptr> hoist_exprs;
+  // event-only terms kept as extra data columns (in terms of the stored observables)
+  std::vector<std::string> ev_names;
+  std::vector<Ptr> ev_exprs;
   size_t nodes = 0;
 
-  // base2: rewrite exp/log into the hardware's exp2/log2 (FP32 kernels), see to_base2
+  // base2: rewrite exp/log into the hardware's exp2/log2 (FP32 kernels), see to_base2;
+  // event_threshold >= 0: subexpressions of the kernel outputs that depend on the observables
+  // only and cost more than it become precomputed data columns (the reference's
+  // optimize_dimensions, fitter.hh:339-375); the generator's log density keeps its form
   LikelihoodGraphs(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, bool centre, bool with_hessian,
-                   bool base2 = false, double hoist_threshold = 0.0) {
+                   bool base2 = false, double hoist_threshold = 0.0, double event_threshold = -1.0) {
     std::vector<std::string> fixed;
     std::vector<double> fixed_v;
     for (auto* p : params) {
@@ -499,8 +505,13 @@ struct LikelihoodGraphs {
     }
     Ptr lp = pdf->logprob_normalised()->substitute(fixed, fixed_v)->simplify();
     if (centre) lp = centre_observables(*lp, obs, centres);
+    // event columns come out of log p itself, before differentiation: the derivatives then
+    // treat them as opaque variables (products of columns stay products, not new columns)
+    Ptr lpe = event_threshold >= 0.0
+                  ? lp->optimize_buffering_constant_terms(ev_names, ev_exprs, obs, "morefit_eventbuffer_", (float)event_threshold, false)->simplify()
+                  : lp->copy();
     std::vector<Ptr> d1;
-    for (auto& t : theta) d1.emplace_back(factor_common(*lp->diff(t)->simplify()));
+    for (auto& t : theta) d1.emplace_back(factor_common(*lpe->diff(t)->simplify()));
     std::vector<Ptr> d2;
     if (with_hessian)
       for (size_t i = 0; i < theta.size(); i++)
@@ -521,13 +532,18 @@ struct LikelihoodGraphs {
       return ev->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, (float)hoist_threshold)->simplify();
     };
     gen = lp->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, (float)hoist_threshold)->simplify();
-    lh.emplace_back(output(lp, lh_scale));
-    grad.emplace_back(output(lp, grad_scale));
+    lh.emplace_back(output(lpe, lh_scale));
+    grad.emplace_back(output(lpe, grad_scale));
     for (auto& g : d1) grad.emplace_back(output(g, grad_scale));
-    hess.emplace_back(output(lp, hess_scale));
+    hess.emplace_back(output(lpe, hess_scale));
     for (auto& g : d1) hess.emplace_back(output(g, hess_scale));
     for (auto& g : d2) hess.emplace_back(output(g, hess_scale));
     for (auto& g : hess) nodes += g->node_count();
+  }
+  std::vector<std::string> columns() const {
+    std::vector<std::string> r(obs);
+    r.insert(r.end(), ev_names.begin(), ev_names.end());
+    return r;
   }
   std::vector<std::string> param_names() const {
     std::vector<std::string> r(theta);
@@ -553,16 +569,20 @@ struct LikelihoodGraphs {
       ms.gen_width.push_back(hi[d] - lo[d]);
     }
     auto pn = param_names();
-    ms.functors = emit_functor("mf_f_lh", raw(lh), obs, pn, ms.fp32, &ms.statements, &lh_scale) +
-                  emit_functor("mf_f_grad", raw(grad), obs, pn, ms.fp32, &ms.statements, &grad_scale);
-    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), obs, pn, ms.fp32, &ms.statements, &hess_scale);
+    const auto cols = columns();
+    ms.ncol = (int)cols.size();
+    ms.functors = emit_functor("mf_f_lh", raw(lh), cols, pn, ms.fp32, &ms.statements, &lh_scale) +
+                  emit_functor("mf_f_grad", raw(grad), cols, pn, ms.fp32, &ms.statements, &grad_scale);
+    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), cols, pn, ms.fp32, &ms.statements, &hess_scale);
     ms.functors += emit_functor<kernelT, evalT>("mf_f_gen", {gen.get()}, obs, pn, ms.fp32, &ms.statements);
     ms.hoist = emit_hoist(raw(hoist_exprs), theta);
+    if (!ev_exprs.empty()) ms.event_terms = emit_fp64_function("mf_event_terms", "x", raw(ev_exprs), obs);
     const int u = std::max(1, o.hip_unroll);
     const bool f = o.hip_fused_final;
     ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f}, {"mf_eval_grad", "mf_f_grad", u, f}};
     if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u, f});
     ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)"};
+    if (!ev_exprs.empty()) ms.kernels.push_back("MF_PRECOMPUTE_KERNEL");
     return ms;
   }
 };
@@ -577,6 +597,7 @@ class HipModel {
   std::vector<std::string> obs;
   std::vector<double> centres, lo, hi;
   int nobs = 0, ntheta = 0, nhoist = 0, nparam = 0;
+  int ncol = 0;  // data columns per event: observables, then precomputed event terms
   int nout[4] = {0, 0, 0, 0};
   bool fp32 = std::is_same<kernelT, float>::value;
   bool with_hessian = true;
@@ -598,7 +619,8 @@ class HipModel {
     const bool f32 = std::is_same<kernelT, float>::value;
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> g(
         new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian,
-                                             o.hip_base2 >= 2 || (f32 && o.hip_base2 == 1)));
+                                             o.hip_base2 >= 2 || (f32 && o.hip_base2 == 1), 0.0,
+                                             o.hip_event_columns ? o.hip_event_cost_threshold : -1.0));
     ModuleSource ms = g->module(o, with_hessian);
     auto t0 = std::chrono::steady_clock::now();
     fit_register_budget(ms, o.hip_compile_options, res);
@@ -643,6 +665,7 @@ class HipModel {
     lo = g->lo;
     hi = g->hi;
     nobs = ms.nobs;
+    ncol = std::max(ms.ncol, ms.nobs);
     ntheta = ms.ntheta;
     nhoist = ms.nhoist;
     nparam = ntheta + nhoist;
@@ -767,7 +790,7 @@ class HipBatch {
   std::vector<mf_gen_stream> gen_streams_h_;
   // staging in mapped host memory, read and written by the kernels in place: theta and the
   // per-kind segment tables (modes) in, totals out -- a call's graph holds kernels only
-  HostBuf theta_h, segs_hk[4], results_h[4], done_h;
+  HostBuf theta_h, segs_hk[4], results_h[4], done_h, segs_pre_h;
   int done_seq_ = 0;
   mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> request_;
@@ -775,7 +798,7 @@ class HipBatch {
   // fixed segments, packed back to back
   HipBatch(HipModel<kernelT, evalT>* model, const std::vector<long long>& seg_sizes) : m(model), dev(model->be->device()) {
     long long tot = 0;
-    for (auto n : seg_sizes) tot += ArenaAllocator::round(n * m->nobs);
+    for (auto n : seg_sizes) tot += ArenaAllocator::round(n * m->ncol);
     init((int)seg_sizes.size(), tot);
     for (size_t s = 0; s < seg_sizes.size(); s++)
       if (add_segment(seg_sizes[s]) != (int)s) throw std::runtime_error("HipBatch: packing failed");
@@ -805,7 +828,7 @@ class HipBatch {
     hoist64_d.alloc(dev, (size_t)std::max(1, S) * std::max(1, m->nhoist) * sizeof(double));
     // ceil per segment adds at most one chunk each; balancing (append_chunks) rounds a segment
     // of nch0 >= resident/2 chunks to the nearest multiple of resident, at most doubling it
-    chunk_cap = (int)(2 * (elems / m->nobs / chunk)) + 2 * S + 1 + m->resident;
+    chunk_cap = (int)(2 * (elems / m->ncol / chunk)) + 2 * S + 1 + m->resident;
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
     int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
     partials_d.alloc(dev, (size_t)chunk_cap * nmax * sizeof(double));
@@ -820,6 +843,7 @@ class HipBatch {
     }
     std::memset(theta_h.ptr, 0, theta_h.bytes);
     done_h.alloc_mapped(dev, 64);
+    if (m->ncol > m->nobs) segs_pre_h.alloc_mapped(dev, std::max(1, S) * sizeof(mf_segment));
     std::memset(done_h.ptr, 0, done_h.bytes);
     request_.assign(S, 0);
     for (int k = 1; k <= 3; k++) fused_[k] = m->nout[k] > 0 && m->kernel_fused(kernel_name(k));
@@ -833,7 +857,7 @@ class HipBatch {
     for (int s = 0; s < S && slot < 0; s++)
       if (!active[s]) slot = s;
     if (slot < 0) return -1;
-    long long off = arena.alloc(n * m->nobs);
+    long long off = arena.alloc(n * m->ncol);
     if (off < 0) return -1;
     active[slot] = 1;
     sizes[slot] = n;
@@ -843,7 +867,7 @@ class HipBatch {
   }
   void remove_segment(int s) {
     if (s < 0 || s >= S || !active[s]) return;
-    arena.release(offsets[s], sizes[s] * m->nobs);
+    arena.release(offsets[s], sizes[s] * m->ncol);
     active[s] = 0;
     sizes[s] = 0;
     request_[s] = 0;
@@ -852,7 +876,7 @@ class HipBatch {
   bool fits(long long n) const {
     bool slot = false;
     for (int s = 0; s < S && !slot; s++) slot = !active[s];
-    return slot && arena.largest_free() >= ArenaAllocator::round(n * m->nobs);
+    return slot && arena.largest_free() >= ArenaAllocator::round(n * m->ncol);
   }
   int active_count() const {
     int c = 0;
@@ -904,7 +928,7 @@ class HipBatch {
   const double* result(int s, int kind) const { return results_h[kind].template as<double>() + (size_t)s * m->nout[kind]; }
   long long data_bytes() const {
     long long b = 0;
-    for (int s = 0; s < S; s++) b += active[s] ? sizes[s] * m->nobs : 0;
+    for (int s = 0; s < S; s++) b += active[s] ? sizes[s] * m->ncol : 0;
     return b * (long long)sizeof(kernelT);
   }
 
@@ -915,7 +939,32 @@ class HipBatch {
     for (int d = 0; d < m->nobs; d++)
       for (long long i = 0; i < n; i++) tmp[d * n + i] = (kernelT)(cols[d * col_stride + i] - m->centres[d]);
     hip_check(mf_memcpy_h2d(dev, data.template as<kernelT>() + offsets[s], tmp.data(), tmp.size() * sizeof(kernelT)), "HipBatch upload");
+    precompute({s});
     hip_check(mf_sync(dev), "HipBatch upload");
+  }
+  // event columns (precomputed event-only terms) of the listed slots from their observables;
+  // the segment table of this launch is read from mapped host memory
+  void precompute(const std::vector<int>& slots) {
+    if (m->ncol == m->nobs || slots.empty()) return;
+    rebuild();
+    if (nchunks == 0) return;
+    mf_segment* t = segs_pre_h.template as<mf_segment>();
+    for (int s = 0; s < S; s++) {
+      t[s] = segs_h[s];
+      t[s].mode = 0;
+    }
+    for (int s : slots) t[s].mode = active[s] ? 1 : 0;
+    mf_eval_args a{};
+    a.data = data.ptr;
+    a.segs = (const mf_segment*)segs_pre_h.dptr;
+    a.chunks = chunks_d.template as<mf_chunk>();
+    a.nchunks = nchunks;
+    a.nseg = S;
+    a.params = params_d.ptr;
+    a.partials = partials_d.template as<double>();
+    a.flags = MF_EVAL_SKIP_FINAL;
+    hip_check(mf_launch_eval(m->mod, "mf_precompute", &a, grid(), m->block), "mf_precompute");
+    hip_check(mf_sync(dev), "HipBatch precompute");
   }
   // stored columns of segment s back as FP64 observables (centre added back)
   void download(int s, double* cols) {
@@ -927,8 +976,6 @@ class HipBatch {
       for (long long i = 0; i < n; i++) cols[d * n + i] = (double)tmp[d * n + i] + m->centres[d];
   }
 
-  // kernel parameters of every slot from the staged theta; with kind > 0 the same launch
-  // stages that kind's segment table (modes) into device memory
   // Kernel parameters from theta, and the segment table of `kind` staged to the device.  Direct
   // launches pass a small theta by value and copy the table only when its modes changed since
   // the last copy, so the kernel reads nothing from host memory; a launch for graph capture
@@ -1016,6 +1063,7 @@ class HipBatch {
     std::vector<unsigned long long> props(S);
     hip_check(mf_memcpy_d2h(dev, props.data(), gen_props_.ptr, S * sizeof(unsigned long long)), "generate proposals");
     hip_check(mf_sync(dev), "generate");
+    precompute(slots);
     unsigned long long tot = 0;
     for (auto p : props) tot += p;
     return tot;

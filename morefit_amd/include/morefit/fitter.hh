@@ -60,6 +60,9 @@ struct fitter_options {
   double minuit2_tolerance = 0.01;
   double minuit_errordef = 1.0;
   float buffering_cost_threshold = 0.0f;  // hoist every parameter-only operation
+  // optimize_dimensions: observable-only subexpressions costing more than this become extra
+  // data columns (a column is read at every call, so cheap terms stay in the kernel)
+  float event_cost_threshold = 10.0f;
   bool postrun_hesse = true;
   void print() const {
     const int w = 40;
@@ -82,7 +85,10 @@ class fitter {
   typedef GraphPtr<kernelT, evalT> Ptr;
   fitter_options* opts_;
   backendT* backend_;
-  computeT block_, grad_block_, hessian_block_, fisher_block_;
+  computeT block_, grad_block_, hessian_block_, fisher_block_, precompute_block_;
+  // optimize_dimensions: columns of the precomputed data buffer (observables, then event terms)
+  std::vector<std::string> event_buffer_names_;
+  std::vector<Ptr> event_buffer_expressions_;
   std::vector<parameter<evalT>*> params_;
   PDF<kernelT, evalT>* pdf_ = nullptr;
   std::vector<Ptr> buffer_expressions_, grad_buffer_expressions_, hessian_buffer_expressions_, fisher_buffer_expressions_;
@@ -99,7 +105,13 @@ class fitter {
   double fit_ms_ = 0.0, kernel_ms_ = 0.0;
 
   fitter(fitter_options* options, backendT* backend)
-      : opts_(options), backend_(backend), block_(backend), grad_block_(backend), hessian_block_(backend), fisher_block_(backend) {}
+      : opts_(options),
+        backend_(backend),
+        block_(backend),
+        grad_block_(backend),
+        hessian_block_(backend),
+        fisher_block_(backend),
+        precompute_block_(backend) {}
 
   bool centring() const {
     return std::is_same<kernelT, float>::value && has_set_centres<computeT>::value && backend_->opts_->hip_centre_observables;
@@ -128,6 +140,25 @@ class fitter {
     Ptr lp = pdf->logprob_normalised()->substitute(fixed, fixed_v)->simplify();
     if (!weight_name.empty()) lp = (Variable<kernelT, evalT>(weight_name) * std::move(lp))->simplify();
     if (centring()) lp = centre_observables(*lp, obs, centres_);
+    std::vector<dimension<evalT>> in_sig;
+    for (auto* d : dims) in_sig.push_back(*d);
+    // event-only terms as extra data columns, filled once per dataset by a store kernel
+    // (fitter.hh:339-375): the column list starts with the observables themselves
+    event_buffer_names_.clear();
+    event_buffer_expressions_.clear();
+    if (opts_->optimize_dimensions) {
+      for (auto& o : obs) {
+        event_buffer_names_.push_back(o);
+        event_buffer_expressions_.emplace_back(Variable<kernelT, evalT>(o));
+      }
+      lp = lp->optimize_buffering_constant_terms(event_buffer_names_, event_buffer_expressions_, obs, "morefit_eventbuffer_",
+                                                 opts_->event_cost_threshold, false)
+               ->simplify();
+      std::vector<dimension<evalT>> out_sig(in_sig);
+      for (size_t i = obs.size(); i < event_buffer_names_.size(); i++) out_sig.push_back(dimension<evalT>(event_buffer_names_[i], 0.0, 1.0));
+      precompute_block_.MakeComputeKernel("precompute_kernel", nevents_, in_sig, out_sig, {}, event_buffer_expressions_, false);
+      in_sig = out_sig;
+    }
     std::vector<Ptr> d1;
     for (auto& t : floating_names_) d1.emplace_back(factor_common(*lp->diff(t)->simplify()));
     auto hoisted = [&](std::vector<Ptr>& graphs, std::vector<std::string>& names, std::vector<Ptr>& exprs, const std::string& prefix) {
@@ -140,8 +171,6 @@ class fitter {
                              : g->copy());
       return out;
     };
-    std::vector<dimension<evalT>> in_sig;
-    for (auto* d : dims) in_sig.push_back(*d);
     auto param_sig = [&](const std::vector<std::string>& names) {
       std::vector<std::string> r(floating_names_);
       r.insert(r.end(), names.begin(), names.end());
@@ -193,11 +222,22 @@ class fitter {
   void bind_data(EventVector<dataT, evalT>* data) {
     nevents_ = data->nevents();
     apply_centres(block_, centres_);
-    block_.SetupInputBuffer(data->nevents_padded() * data->ndimensions() * sizeof(kernelT));
+    if (opts_->optimize_dimensions) {
+      // observables in, observables and event columns out; the kernels read that buffer
+      apply_centres(precompute_block_, centres_);
+      precompute_block_.SetupInputBuffer(data->nevents_padded() * data->ndimensions() * sizeof(kernelT));
+      precompute_block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
+      precompute_block_.SetupOutputBuffer(data->nevents_padded() * event_buffer_names_.size() * sizeof(kernelT));
+      precompute_block_.CopyToInputBuffer(*data);
+      precompute_block_.SubmitKernel();
+      precompute_block_.Finish();
+      block_.SetupInputBuffer(&precompute_block_, false);
+    } else {
+      block_.SetupInputBuffer(data->nevents_padded() * data->ndimensions() * sizeof(kernelT));
+      block_.CopyToInputBuffer(*data);
+    }
     block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
     block_.SetupParameterBuffer((floating_names_.size() + buffer_names_.size() + 1) * sizeof(kernelT));
-    if constexpr (std::is_same<dataT, kernelT>::value) block_.CopyToInputBuffer(*data);
-    else block_.CopyToInputBuffer(*data);
     if (opts_->analytic_gradient) {
       apply_centres(grad_block_, centres_);
       grad_block_.SetupInputBuffer(&block_, true);

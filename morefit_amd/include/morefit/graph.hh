@@ -484,10 +484,13 @@ class ComputeGraphNode {
     exprs.emplace_back(std::move(expr));
     return n;
   }
+  // group_always: a group of such operands is hoisted even below the threshold when it has
+  // the kind of its parent (free for per-segment parameter terms; event columns pass false,
+  // since every column costs memory bandwidth per event)
   virtual Ptr optimize_buffering_constant_terms(std::vector<std::string>& buffernames, std::vector<Ptr>& bufferexpressions,
                                                 const std::vector<std::string>& variables,
                                                 const std::string& prefix = "morefit_buffer_",
-                                                float buffering_cost_threshold = 2.0) {
+                                                float buffering_cost_threshold = 2.0, bool group_always = true) {
     if (kind_ == NodeKind::Constant || kind_ == NodeKind::Variable) return copy();
     if (!contains_other_variables(variables)) {
       if (cost() > buffering_cost_threshold)
@@ -502,11 +505,11 @@ class ComputeGraphNode {
       std::vector<Ptr> group;
       for (auto& c : children_) {
         if (has_param && !c->contains_other_variables(variables)) group.emplace_back(c->copy());
-        else ch.emplace_back(c->optimize_buffering_constant_terms(buffernames, bufferexpressions, variables, prefix, buffering_cost_threshold));
+        else ch.emplace_back(c->optimize_buffering_constant_terms(buffernames, bufferexpressions, variables, prefix, buffering_cost_threshold, group_always));
       }
       if (!group.empty()) {
         Ptr g = group.size() == 1 ? std::move(group[0]) : make_node<kernelT, evalT>(kind_, std::move(group));
-        if (g->cost() > buffering_cost_threshold || g->kind() == kind_)
+        if (g->cost() > buffering_cost_threshold || (group_always && g->kind() == kind_))
           ch.emplace_back(make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, hoist(std::move(g), buffernames, bufferexpressions, prefix)));
         else
           ch.emplace_back(std::move(g));
@@ -514,7 +517,7 @@ class ComputeGraphNode {
       return make_node<kernelT, evalT>(kind_, std::move(ch));
     }
     for (auto& c : children_)
-      ch.emplace_back(c->optimize_buffering_constant_terms(buffernames, bufferexpressions, variables, prefix, buffering_cost_threshold));
+      ch.emplace_back(c->optimize_buffering_constant_terms(buffernames, bufferexpressions, variables, prefix, buffering_cost_threshold, group_always));
     return make_node<kernelT, evalT>(kind_, std::move(ch), value_, name_, cmp_);
   }
 

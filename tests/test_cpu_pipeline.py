@@ -109,17 +109,19 @@ def test_cpu_fit_empty_and_tiny_datasets_fail_cleanly_or_converge():
     assert out.status in (0, 1, 2, 3, 4)
 
 
+@pytest.mark.parametrize("event_cost", [-1.0, 10.0])
 @pytest.mark.parametrize("model", [O.MASS, O.KSTAR])
 @pytest.mark.parametrize("fp32", [False, True])
-def test_generated_outputs_match_oracle_per_event(model, fp32):
+def test_generated_outputs_match_oracle_per_event(model, fp32, event_cost):
     """the code generator's rewrites (centred observables, base-2 exp/log for FP32, hoisted
-    parameter terms, parameter-only factors split off as FP64 output scales) leave every
-    per-event output equal to the oracle's log p, gradient and Hessian: evaluated on the host
-    in FP64 with the bindings the kernels get, within 1e-10 of the per-event scale"""
+    parameter terms, parameter-only factors split off as FP64 output scales, and with
+    event_cost >= 0 event-only terms taken from precomputed columns) leave every per-event
+    output equal to the oracle's log p, gradient and Hessian: evaluated on the host in FP64
+    with the bindings the kernels get, within 1e-10 of the per-event scale"""
     P = O.NPAR[model]
     data = O.generate(model, M.TRUTH[model], 200, seed=31, nstreams=4)
     theta = np.array(M.START[model])
-    outs = {w: M.host_outputs(model, fp32, w, theta, data) for w in (1, 2, 3)}
+    outs = {w: M.host_outputs(model, fp32, w, theta, data, event_cost=event_cost) for w in (1, 2, 3)}
     iu = [(i, j) for i in range(P) for j in range(i + 1)]
     for e in range(data.shape[1]):
         lp, g, H = O.event_jet(model, data[:, e], theta)
@@ -128,6 +130,20 @@ def test_generated_outputs_match_oracle_per_event(model, fp32):
         for w, k in ((1, 1), (2, 1 + P), (3, len(ref))):
             got = outs[w][e]
             assert np.all(np.abs(got - ref[:k]) <= 1e-10 * scale[:k]), (w, e, got - ref[:k])
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_optimize_dimensions_fit_equals_plain_fit(fp32):
+    """fitter option optimize_dimensions (fitter.hh:339-375): a store kernel writes the observables
+    and the event-only terms of the angular density into an expanded data buffer that the
+    likelihood kernels read; the fit equals the one without it (FP64: 1e-9 sigma, FP32: 1e-4
+    sigma -- the FP32 event terms are rounded once instead of evaluated in FP32 per call)"""
+    data = O.generate(O.KSTAR, M.TRUTH[O.KSTAR], 20000, seed=5, nstreams=4)
+    a = M.fit_cpu(O.KSTAR, data, M.START[O.KSTAR], nthreads=2, fp32=fp32)
+    b = M.fit_cpu(O.KSTAR, data, M.START[O.KSTAR], nthreads=2, fp32=fp32, optimize_dimensions=True)
+    assert a[2].status == 0 and b[2].status == 0
+    assert np.max(np.abs(a[0] - b[0]) / a[1]) < (1e-4 if fp32 else 1e-9)
+    assert abs(a[2].fval - b[2].fval) < (1e-3 if fp32 else 1e-7)
 
 
 def test_weighted_fit_covariance_correction_matches_oracle_sandwich():

@@ -74,6 +74,63 @@ def test_eval_matches_oracle(ctx, model, fp32):
     mdl.close()
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_event_columns_match_oracle(fp32):
+    """Angular density with event-only terms precomputed into data columns (event_columns=1):
+    after an upload, every sum within the bars of test_eval_matches_oracle; after generation
+    on the device, the same sums as a batch without columns holding the same stored events
+    (within the FP32 bar for both precisions: the columns are computed in FP64 and rounded once
+    in place of per-call FP32 arithmetic)."""
+    model, n = M.KSTAR, 200_000
+    data = O.generate(model, M.TRUTH[model], n, seed=4242, nstreams=16)
+    theta = np.array(M.START[model])
+    say("event columns fp32", fp32)
+    cx = M.Context(event_columns=1)
+    mdl = cx.model(model, fp32=fp32)
+    assert mdl.ncol > mdl.nobs
+    b = mdl.batch([n])
+    b.upload(0, data)
+    stored = _stored(model, data, fp32)
+    ref = O.nll_sums(model, stored, theta, mode=2, nthreads=8)
+    absref = O.nll_sums(model, stored, theta, mode=12, nthreads=8)
+    tol = 2e-6 if fp32 else 1e-12
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        out = b.eval(kind, theta)[0]
+        k = len(out)
+        assert np.all(np.abs(out - ref[:k]) <= tol * absref[:k]), (kind, (out - ref[:k]) / absref[:k])
+    # generated on the device: columns filled by the precompute kernel after generation
+    g = mdl.batch([n])
+    g.generate(M.TRUTH[model], [77])
+    ev = g.download(0)
+    plain = ctx_plain = M.Context()
+    pm = plain.model(model, fp32=fp32)
+    pb = pm.batch([n])
+    pb.upload(0, ev)
+    gabs = O.nll_sums(model, _stored(model, ev, fp32), theta, mode=12, nthreads=8)
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        a = g.eval(kind, theta)[0]
+        r = pb.eval(kind, theta)[0]
+        assert np.all(np.abs(a - r) <= 2e-6 * gabs[:len(a)]), (kind, (a - r) / gabs[:len(a)])
+    for h in (pb, pm, ctx_plain, g, b, mdl, cx):
+        h.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_fit_plain_optimize_dimensions(fp32):
+    """The drop-in fitter flow with optimize_dimensions (HIPBlock store kernel filling the
+    expanded data buffer, fitter.hh:339-375): the fit equals the one without it (FP64: 1e-9
+    sigma, FP32: 1e-4 sigma)"""
+    data = O.generate(M.KSTAR, M.TRUTH[M.KSTAR], 50000, seed=6, nstreams=8)
+    say("fit_plain optimize_dimensions fp32", fp32)
+    cp, ce = M.Context(), M.Context(event_columns=1)
+    a = cp.fit_plain(M.KSTAR, data, M.START[M.KSTAR], fp32=fp32)
+    b = ce.fit_plain(M.KSTAR, data, M.START[M.KSTAR], fp32=fp32)
+    assert a[2].status == 0 and b[2].status == 0
+    assert np.max(np.abs(a[0] - b[0]) / a[1]) < (1e-4 if fp32 else 1e-9)
+    cp.close()
+    ce.close()
+
+
 @pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
 def test_fp64_base2_matches_oracle(model):
     """FP64 kernels with exp/log rewritten as exp2/log2 (option base2=2) keep the FP64 bar, 1e-12."""

@@ -82,3 +82,15 @@ def test_fp32_fits_match_fp64_on_cpu_backend():
         assert o32.status == 0, (i, o32.status)
         assert np.max(np.abs(v32 - v64) / e64) < 1e-3, (i, (v32 - v64) / e64)
         np.testing.assert_allclose(e32, e64, rtol=1e-3)
+
+
+def test_host_fibres_switch_and_reuse_stacks(tmp_path):
+    """the batched fits' host fibres (morefit/fiber.hh): 3000 fibres on 8 pinned pool threads,
+    ~22 switches each, rounding modes kept per fibre, exceptions and deep recursion inside
+    fibres, stacks reused by a second round (built at -O0 and -O2)"""
+    for opt in ("-O0", "-O2"):
+        exe = str(tmp_path / ("fiber_check" + opt))
+        subprocess.check_call(["g++", opt, "-std=c++17", "-I", os.path.join(ROOT, "morefit_amd", "include"),
+                               os.path.join(ROOT, "tests", "cpp", "fiber_check.cc"), "-lpthread", "-o", exe])
+        out = subprocess.check_output([exe]).decode().split()
+        assert out[0] == "ok", out
