@@ -278,6 +278,9 @@ class Model:
 class Batch:
     """Resident datasets (segments) of one model."""
 
+    # mfh_batch_eval with plain addresses: the per-call path of eval() avoids ctypes pointer objects
+    _EVAL_PROTO = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
+
     def __init__(self, model, sizes):
         self.model = model
         self.sizes = np.asarray(sizes, dtype=np.int64)
@@ -285,11 +288,14 @@ class Batch:
         self.h = ctypes.c_void_p()
         arr = (ctypes.c_longlong * self.S)(*[int(s) for s in self.sizes])
         _check(libs()[1].mfh_batch_create(model.h, arr, self.S, ctypes.byref(self.h)))
+        self._eval = Batch._EVAL_PROTO(ctypes.cast(libs()[1].mfh_batch_eval, ctypes.c_void_p).value)
+        self._hv = self.h.value
 
     def close(self):
         if self.h:
             libs()[1].mfh_batch_destroy(self.h)
             self.h = ctypes.c_void_p()
+            self._hv = None
 
     def nbytes(self):
         return libs()[1].mfh_batch_bytes(self.h)
@@ -312,9 +318,17 @@ class Batch:
         return props.value
 
     def eval(self, kind, theta, use_graph=False):
-        th = _f64(np.broadcast_to(np.asarray(theta, dtype=np.float64), (self.S, self.model.ntheta)))
-        out = np.zeros((self.S, self.model.nout[kind]))
-        _check(libs()[1].mfh_batch_eval(self.h, kind, _dp(th), int(use_graph), _dp(out)))
+        """sums of `kind` for every segment at theta (one row per segment, or one row for all):
+        array (S, nout)"""
+        if not self._hv:
+            raise MorefitError("batch is closed")
+        th = np.asarray(theta, dtype=np.float64)
+        if th.size != self.S * self.model.ntheta or not th.flags.c_contiguous:
+            th = np.ascontiguousarray(np.broadcast_to(th, (self.S, self.model.ntheta)))
+        out = np.empty((self.S, self.model.nout[kind]))
+        rc = self._eval(self._hv, kind, th.__array_interface__["data"][0], int(use_graph), out.__array_interface__["data"][0])
+        if rc:
+            _check(rc)
         return out
 
     def time_kernel(self, kind, iters=50, main_only=False):
