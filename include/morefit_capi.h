@@ -46,6 +46,8 @@ typedef struct mfh_options {
   int fused_final;         /* 1: segment totals in the work-group finishing a segment; 0: final kernel */
   int event_columns;       /* 1: event-only terms precomputed into data columns (optimize_dimensions) */
   double event_cost;       /* cost above which an event-only term becomes a column (graph cost units) */
+  int linear;              /* outputs as linear forms over shared monomials: 0 never, 1 when shorter, 2 always */
+  int pad;
 } mfh_options;
 
 typedef struct mfh_fit_out {
@@ -98,10 +100,11 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
  * kernel of a configuration module exactly as generated (observables centred for fp32, base-2
  * transcendentals, hoisted terms, output scales applied), evaluated on the host in FP64 at
  * theta for events data (SoA nobs x n): out[i * nout + k].  event_cost >= 0: event-only terms
- * costing more become columns computed from the observables first (event_columns).  Checks the
- * code generator's rewrites against the density's definition. */
+ * costing more become columns computed from the observables first (event_columns); linear as
+ * mfh_options.linear (the accumulated terms combined per event as the kernels combine their
+ * sums).  Checks the code generator's rewrites against the density's definition. */
 int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double event_cost,
-                           double* out);
+                           int linear, double* out);
 /* launch-shape autotuner: for each candidate (block, events_per_thread, unroll) triple in
  * cand[3 * ncand] (NULL: 16 built-in shapes, block 256/512 x 8..64 events per thread x unroll
  * 4/8) times `eval_kind` evaluations (per-chunk kernel + final pass, mean of `iters`

@@ -45,7 +45,8 @@ class Options(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("block_size", ctypes.c_int), ("events_per_thread", ctypes.c_int),
                 ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int),
                 ("unroll", ctypes.c_int), ("balance", ctypes.c_int), ("base2", ctypes.c_int),
-                ("fused_final", ctypes.c_int), ("event_columns", ctypes.c_int), ("event_cost", ctypes.c_double)]
+                ("fused_final", ctypes.c_int), ("event_columns", ctypes.c_int), ("event_cost", ctypes.c_double),
+                ("linear", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class FitOut(ctypes.Structure):
@@ -103,7 +104,7 @@ def libs():
         L.mfh_model_source.argtypes = [vp, ctypes.c_char_p, ip]
         L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
         L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
-        L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, ctypes.c_double, dp]
+        L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, ctypes.c_double, ip, dp]
         L.mfh_stream_states.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ip, ctypes.POINTER(ctypes.c_uint32)]
         L.mfh_autotune.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip, ip, ctypes.POINTER(ctypes.c_int), ip, ip,
                                    ctypes.POINTER(ctypes.c_int), dp]
@@ -161,7 +162,7 @@ def kernel_resources(kind, fp32, with_hessian=True, fit=True, **kw):
             for j, k in enumerate((EVAL_LH, EVAL_GRAD, EVAL_HESS))}
 
 
-def host_outputs(kind, fp32, which, theta, data, event_cost=-1.0):
+def host_outputs(kind, fp32, which, theta, data, event_cost=-1.0, linear=1):
     """Per-event outputs of a configuration module's value/gradient/Hessian kernel (which = 1/2/3)
     as generated, evaluated on the host in FP64 (no device): array (n, nout).  event_cost >= 0:
     with the event-only terms above that cost taken from precomputed columns."""
@@ -169,7 +170,7 @@ def host_outputs(kind, fp32, which, theta, data, event_cost=-1.0):
     n = data.shape[1]
     nout = {1: 1, 2: 1 + NTHETA[kind], 3: 1 + NTHETA[kind] + NTHETA[kind] * (NTHETA[kind] + 1) // 2}[which]
     out = np.zeros(n * nout)
-    _check(libs()[1].mfh_model_host_outputs(kind, int(fp32), which, _dp(_f64(theta)), _dp(data), n, float(event_cost), _dp(out)))
+    _check(libs()[1].mfh_model_host_outputs(kind, int(fp32), which, _dp(_f64(theta)), _dp(data), n, float(event_cost), int(linear), _dp(out)))
     return out.reshape(n, nout)
 
 

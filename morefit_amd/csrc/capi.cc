@@ -102,6 +102,7 @@ void apply_options(compute_options& co, const mfh_options* o) {
   co.hip_fused_final = o->fused_final != 0;
   co.hip_event_columns = o->event_columns != 0;
   co.hip_event_cost_threshold = o->event_cost;
+  co.hip_linear_outputs = o->linear;
 }
 }  // namespace
 
@@ -152,6 +153,7 @@ void mfh_default_options(mfh_options* o) {
   o->fused_final = 0;
   o->event_columns = 0;
   o->event_cost = 10.0;
+  o->linear = 1;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {
@@ -278,7 +280,7 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
       const bool f32 = std::is_same<K, float>::value;
       LikelihoodGraphs<K> g(d.pdf, d.params, f32 && co.hip_centre_observables, with_hessian != 0,
                             co.hip_base2 >= 2 || (f32 && co.hip_base2 == 1), 0.0,
-                            co.hip_event_columns ? co.hip_event_cost_threshold : -1.0);
+                            co.hip_event_columns ? co.hip_event_cost_threshold : -1.0, co.hip_linear_outputs);
       ms = g.module(co, with_hessian != 0);
       mf_kernel_resources r[64];
       int n = 0;
@@ -312,12 +314,12 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
 }
 
 int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double event_cost,
-                           double* out) {
+                           int linear, double* out) {
   MFH_TRY
   if (which < 1 || which > 3) throw std::invalid_argument("which must be 1, 2 or 3");
   auto run = [&](auto& d) {
     typedef typename std::remove_reference<decltype(d)>::type::kernel_type K;
-    LikelihoodGraphs<K> g(d.pdf, d.params, fp32 != 0, true, fp32 != 0, 0.0, event_cost);
+    LikelihoodGraphs<K> g(d.pdf, d.params, fp32 != 0, true, fp32 != 0, 0.0, event_cost, linear);
     std::vector<std::string> names(g.theta);
     std::vector<double> vals(theta, theta + g.theta.size());
     const size_t nt = names.size();
@@ -333,14 +335,19 @@ int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, c
     vals.resize(names.size());
     const std::vector<std::string> obs_names(g.obs);
     std::vector<double> xv(g.obs.size());
-    const auto& outs = which == 1 ? g.lh : (which == 2 ? g.grad : g.hess);
-    const auto& sc = which == 1 ? g.lh_scale : (which == 2 ? g.grad_scale : g.hess_scale);
-    const size_t no = outs.size();
+    const auto& kf = which == 1 ? g.lh : (which == 2 ? g.grad : g.hess);
+    const size_t na = kf.acc.size(), no = kf.form.size();
+    std::vector<double> av(na);
     for (long long i = 0; i < n; i++) {
       for (size_t dd = 0; dd < g.obs.size(); dd++) xv[dd] = vals[off + dd] = data[dd * n + i] - g.centres[dd];
       for (size_t e = 0; e < g.ev_exprs.size(); e++) vals[eoff + e] = g.ev_exprs[e]->eval(obs_names, xv);
-      for (size_t k = 0; k < no; k++)
-        out[i * no + k] = (double)outs[k]->eval(names, vals) * (sc[k] >= 0 ? hv[sc[k]] : 1.0);
+      for (size_t j = 0; j < na; j++) av[j] = (double)kf.acc[j]->eval(names, vals);
+      // the kernels' FP64 combination of the accumulated terms, for one event (n = 1)
+      for (size_t k = 0; k < no; k++) {
+        double v = 0.0;
+        for (const auto& t : kf.form[k]) v += t.mult * (t.hoist >= 0 ? hv[t.hoist] : 1.0) * (t.acc >= 0 ? av[t.acc] : 1.0);
+        out[i * no + k] = v;
+      }
     }
   };
   if (fp32) {

@@ -151,12 +151,15 @@ __device__ __forceinline__ void mf_two_sum(double a, double b, double& s, double
   e = (a - (s - bb)) + (b - bb);
 }
 
-// FP64 factor of output k's total for segment `seg`: the parameter-only factor the code
-// generator split off the per-event term (functor scale(k) >= 0), else 1
+// Totals of segment s (its F::nout outputs) from the segment sums S of the F::nacc per-event
+// terms, formed by the generated F::combine in FP64 with the segment's hoisted terms
 template <class F>
-__device__ __forceinline__ double mf_output_scale(const mf_eval_args& a, int seg, int k) {
-  const int si = F::scale(k);
-  return (si >= 0 && a.hoist64) ? a.hoist64[(long long)seg * MF_NHOIST + si] : 1.0;
+__device__ __forceinline__ void mf_write_totals(const mf_eval_args& a, int s, const mf_segment& sg, const double* S) {
+  double out[F::nout];
+  const double* h = a.hoist64 ? a.hoist64 + (long long)s * MF_NHOIST : nullptr;
+  F::combine(S, h, (double)sg.n, out);
+#pragma unroll
+  for (int k = 0; k < F::nout; k++) a.results[sg.out_off + k] = out[k];
 }
 
 // ------------------------------------------------------------------ segment totals
@@ -180,7 +183,8 @@ __device__ __forceinline__ int mf_thread_index() {
 __host__ __device__ constexpr int mf_pow2_ceil(int v) { return v <= 1 ? 1 : 2 * mf_pow2_ceil((v + 1) / 2); }
 template <class F, int NT, int UNR>
 __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, const mf_segment& sg) {
-  constexpr int NO = F::nout;
+  constexpr int NO = F::nacc;
+  __shared__ double s_tot[NO];
   constexpr int KP = mf_pow2_ceil(NO) < NT ? mf_pow2_ceil(NO) : NT;
   constexpr int R = NT / KP;
   constexpr int NW = NT / 64;
@@ -211,7 +215,7 @@ __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, 
         double tot = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; w++) tot += s_fin[w * KP + t];
-        a.results[sg.out_off + k] = tot * mf_output_scale<F>(a, s, k);
+        s_tot[k] = tot;
       }
     } else {
       // each lane of a wave is its own output: combine the R threads of output q in order
@@ -221,11 +225,13 @@ __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, 
         double tot = 0.0;
 #pragma unroll
         for (int rr = 0; rr < R; rr++) tot += s_fin[rr * KP + t];
-        a.results[sg.out_off + k] = tot * mf_output_scale<F>(a, s, k);
+        s_tot[k] = tot;
       }
     }
     __syncthreads();
   }
+  if (t == 0) mf_write_totals<F>(a, s, sg, s_tot);
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ fused evaluation
@@ -243,8 +249,9 @@ __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, 
 // counter atomics and fences.
 template <class F, int U, bool FUSED>
 __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
-  constexpr int NO = F::nout;
+  constexpr int NO = F::nacc;
   __shared__ double s_red[MF_NWAVE][NO];
+  __shared__ double s_tot[NO];
   const MF_T* __restrict__ data = (const MF_T*)a.data;
   const MF_CONST mf_chunk* chunks = (const MF_CONST mf_chunk*)a.chunks;
   const MF_CONST mf_segment* segs = (const MF_CONST mf_segment*)a.segs;
@@ -308,10 +315,11 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
 #pragma unroll
       for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
       a.partials[(long long)c * NO + threadIdx.x] = t;
-      // a segment of one chunk is complete here: its totals need no second pass
-      if (sg.chunk_end - sg.chunk_begin == 1) a.results[sg.out_off + threadIdx.x] = t * mf_output_scale<F>(a, ch.seg, threadIdx.x);
+      s_tot[threadIdx.x] = t;
     }
     __syncthreads();
+    // a segment of one chunk is complete here: its totals need no second pass
+    if (threadIdx.x == 0 && sg.chunk_end - sg.chunk_begin == 1) mf_write_totals<F>(a, ch.seg, sg, s_tot);
     const int nch = sg.chunk_end - sg.chunk_begin;
     if (FUSED && nch > 1 && seg_done) {
       __shared__ int s_last;
@@ -355,7 +363,7 @@ __device__ __forceinline__ void mf_final_impl(const mf_eval_args& a) {
 // and its per-event precompute kernels): store[k * store_stride + data_off + i]
 template <class F>
 __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
-  constexpr int NO = F::nout;
+  constexpr int NO = F::nacc;
   const MF_T* __restrict__ data = (const MF_T*)a.data;
   MF_T* __restrict__ out = (MF_T*)a.store;
   for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {

@@ -12,6 +12,7 @@
 #ifndef MOREFIT_CODEGEN_HH
 #define MOREFIT_CODEGEN_HH
 
+#include <cstdio>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -45,6 +46,7 @@ struct ModuleSource {
   std::vector<EvalKernelSpec> evals;       // evaluation kernels (unroll chosen per kernel)
   std::vector<std::string> kernels;        // other instantiation lines
   size_t statements = 0;                   // SSA statements emitted (all functors)
+  int nacc[4] = {0, 0, 0, 0};              // per evaluation kind: per-event terms accumulated
 
   std::string prelude() const {
     std::ostringstream s;
@@ -79,37 +81,58 @@ struct ModuleSource {
   }
 };
 
-// struct NAME { nout; eval(x, p, o); scale(k) } computing `outputs` from observables x[d] and
-// kernel parameters p[k].  scale(k) is the index, among the segment's FP64 hoisted terms, of
-// the factor output k's total is multiplied by (-1: none); see split_output_scale.
+// One term of an output's combination from the segment sums of a kind's accumulated per-event
+// terms: mult * (hoist >= 0 ? h[hoist] : 1) * (acc >= 0 ? S[acc] : n), h the segment's FP64
+// hoisted terms, n its event count.
+struct CombineTerm {
+  int acc = -1;
+  int hoist = -1;
+  double mult = 1.0;
+};
+typedef std::vector<std::vector<CombineTerm>> CombineForm;  // per output
+
+// FP64 statement of out[k] from CombineForm terms
+inline std::string emit_combine_terms(const std::vector<CombineTerm>& terms) {
+  if (terms.empty()) return "0.0";
+  std::ostringstream s;
+  for (size_t i = 0; i < terms.size(); i++) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%a", terms[i].mult);
+    s << (i ? " + " : "") << buf;
+    if (terms[i].hoist >= 0) s << " * h[" << terms[i].hoist << "]";
+    s << " * " << (terms[i].acc >= 0 ? "S[" + std::to_string(terms[i].acc) + "]" : std::string("n"));
+  }
+  return s.str();
+}
+
+// struct NAME { nacc; nout; eval(x, p, o); combine(S, h, n, out) }: eval computes the nacc
+// per-event terms `acc` (their sums over a segment's events are accumulated) from the columns
+// x[d] and kernel parameters p[k]; combine forms the nout outputs from those sums S, the
+// segment's FP64 hoisted terms h and its event count n, in FP64 (form; null: out[k] = S[k]).
 template <typename kernelT, typename evalT>
-std::string emit_functor(const std::string& name, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& outputs,
+std::string emit_functor(const std::string& name, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& acc,
                          const std::vector<std::string>& observables, const std::vector<std::string>& param_names,
-                         bool fp32, size_t* statements = nullptr, const std::vector<int>* scale_index = nullptr) {
+                         bool fp32, size_t* statements = nullptr, const CombineForm* form = nullptr) {
   KernelEmitter em(fp32);
   for (size_t d = 0; d < observables.size(); d++) em.bindings[observables[d]] = "x[" + std::to_string(d) + "]";
   for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
-  for (auto* g : outputs) em.count_products(*g);
+  for (auto* g : acc) em.count_products(*g);
   std::vector<std::string> res;
-  for (auto* g : outputs) res.push_back(em.emit(*g));
+  for (auto* g : acc) res.push_back(em.emit(*g));
+  const size_t nout = form ? form->size() : acc.size();
   std::ostringstream s;
-  s << "struct " << name << " {\n  static constexpr int nout = " << outputs.size() << ";\n"
+  s << "struct " << name << " {\n  static constexpr int nacc = " << acc.size() << ";\n  static constexpr int nout = " << nout << ";\n"
     << "  __device__ __forceinline__ static void eval(const MF_T* __restrict__ x, const MF_T* __restrict__ p, MF_T* __restrict__ o) {\n"
     << "    (void)x; (void)p;\n"
     << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << res[k] << ";\n";
-  s << "  }\n  __device__ __forceinline__ static int scale(int k) {";
-  bool any = false;
-  if (scale_index)
-    for (int v : *scale_index) any |= v >= 0;
-  if (!any) {
-    s << " (void)k; return -1; }\n";
-  } else {
-    s << " const int s_[" << outputs.size() << "] = {";
-    for (size_t k = 0; k < outputs.size(); k++) s << (k ? ", " : "") << (k < scale_index->size() ? (*scale_index)[k] : -1);
-    s << "}; return s_[k]; }\n";
+  s << "  }\n  __device__ __forceinline__ static void combine(const double* S, const double* h, double n, double* out) {\n"
+    << "    (void)S; (void)h; (void)n;\n";
+  for (size_t k = 0; k < nout; k++) {
+    if (form) s << "    out[" << k << "] = " << emit_combine_terms((*form)[k]) << ";\n";
+    else s << "    out[" << k << "] = S[" << k << "];\n";
   }
-  s << "};\n";
+  s << "  }\n};\n";
   if (statements) *statements += (size_t)em.statements;
   return s.str();
 }

@@ -54,6 +54,9 @@ struct compute_options {
   // graph: 1 per add/multiply, 20 per transcendental) become data columns computed once per
   // upload (the reference's optimize_dimensions); each column adds sizeof(T) bytes per event
   bool hip_event_columns = false;
+  // a kind's outputs as linear forms over shared per-event monomials, combined per segment in
+  // FP64: 0 never, 1 when the generated code is shorter, 2 always (where the expansion exists)
+  int hip_linear_outputs = 1;
   double hip_event_cost_threshold = 10.0;
   // segment totals in the work-group that completes a segment (no final kernel).  Off: each
   // work-group's device-scope release fence writes back its XCD's L2, and on MI355X those

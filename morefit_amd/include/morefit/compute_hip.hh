@@ -471,8 +471,14 @@ struct LikelihoodGraphs {
   typedef GraphPtr<kernelT, evalT> Ptr;
   std::vector<std::string> obs, theta;
   std::vector<double> centres, lo, hi;
-  std::vector<Ptr> lh, grad, hess;           // per kind outputs (grad/hess include value first)
-  std::vector<int> lh_scale, grad_scale, hess_scale;  // per output: hoisted term scaling its total, or -1
+  // A kernel kind (value; value + gradient; value + gradient + Hessian): the per-event terms the
+  // kernel accumulates, and how the kind's outputs follow from their segment sums (FP64)
+  struct KindForm {
+    std::vector<Ptr> acc;
+    CombineForm form;
+    bool linear = false;  // acc are the distinct monomials of the outputs' linear forms
+  };
+  KindForm lh, grad, hess;
   Ptr gen;                                   // log p for the generator, hoisted but otherwise as written
   std::vector<std::string> hoist_names;
   std::vector<Ptr> hoist_exprs;
@@ -485,8 +491,11 @@ struct LikelihoodGraphs {
   // event_threshold >= 0: subexpressions of the kernel outputs that depend on the observables
   // only and cost more than it become precomputed data columns (the reference's
   // optimize_dimensions, fitter.hh:339-375); the generator's log density keeps its form
+  // linear: 0 every output summed as it is (its parameter-only factor split off as an FP64
+  // scale), 1 a kind's outputs as linear forms over shared monomials when that emits fewer
+  // operations (linear_expand), 2 always linear forms where they exist
   LikelihoodGraphs(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, bool centre, bool with_hessian,
-                   bool base2 = false, double hoist_threshold = 0.0, double event_threshold = -1.0) {
+                   bool base2 = false, double hoist_threshold = 0.0, double event_threshold = -1.0, int linear = 1) {
     std::vector<std::string> fixed;
     std::vector<double> fixed_v;
     for (auto* p : params) {
@@ -517,28 +526,142 @@ struct LikelihoodGraphs {
       for (size_t i = 0; i < theta.size(); i++)
         for (size_t j = 0; j <= i; j++) d2.emplace_back(factor_common(*d1[i]->diff(theta[j])->simplify()));
     const std::string prefix = "morefit_parambuffer_";
-    // per output: (base-2 rewrite), parameter-only factor split off as an FP64 hoisted scale
-    // of the total, the rest hoisted as usual
-    auto output = [&](const Ptr& g0, std::vector<int>& scales) {
-      Ptr g = base2 ? to_base2(*g0)->simplify() : g0->copy();
-      Ptr sc;
-      Ptr ev = split_output_scale(*g, theta, sc);
-      int idx = -1;
-      if (sc) {
-        const std::string nm = ComputeGraphNode<kernelT, evalT>::hoist(sc->simplify(), hoist_names, hoist_exprs, prefix);
-        idx = (int)(std::find(hoist_names.begin(), hoist_names.end(), nm) - hoist_names.begin());
-      }
-      scales.push_back(idx);
-      return ev->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, (float)hoist_threshold)->simplify();
+    const float thr = (float)hoist_threshold;
+    auto hoist_index = [&](Ptr e) {
+      const std::string nm = ComputeGraphNode<kernelT, evalT>::hoist(std::move(e), hoist_names, hoist_exprs, prefix);
+      return (int)(std::find(hoist_names.begin(), hoist_names.end(), nm) - hoist_names.begin());
     };
-    gen = lp->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, (float)hoist_threshold)->simplify();
-    lh.emplace_back(output(lpe, lh_scale));
-    grad.emplace_back(output(lpe, grad_scale));
-    for (auto& g : d1) grad.emplace_back(output(g, grad_scale));
-    hess.emplace_back(output(lpe, hess_scale));
-    for (auto& g : d1) hess.emplace_back(output(g, hess_scale));
-    for (auto& g : d2) hess.emplace_back(output(g, hess_scale));
-    for (auto& g : hess) nodes += g->node_count();
+    // a combine term's parameter-only coefficient: constants stay in mult, the rest is hoisted
+    auto coefficient = [&](Ptr c, CombineTerm& t) {
+      c = c->simplify();
+      if (c->is_constant()) {
+        t.mult = c->value();
+        return;
+      }
+      if (c->kind() == NodeKind::Prod) {
+        std::vector<Ptr> rest;
+        for (auto& f : c->children_) {
+          if (f->is_constant()) t.mult *= f->value();
+          else rest.emplace_back(f->copy());
+        }
+        if (rest.empty()) return;
+        c = rest.size() == 1 ? std::move(rest[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(rest));
+      }
+      t.hoist = hoist_index(std::move(c));
+    };
+    // every output summed as it is, its parameter-only factor applied to the total
+    auto build_scaled = [&](const std::vector<Ptr>& outs) {
+      KindForm kf;
+      for (size_t k = 0; k < outs.size(); k++) {
+        Ptr sc;
+        Ptr ev = split_output_scale(*outs[k], theta, sc);
+        CombineTerm t;
+        t.acc = (int)k;
+        if (sc) coefficient(std::move(sc), t);
+        kf.form.push_back({t});
+        kf.acc.emplace_back(ev->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify());
+      }
+      return kf;
+    };
+    // outputs as linear forms over the kind's distinct monomials (false: expansion too large)
+    auto build_linear = [&](const std::vector<Ptr>& outs, KindForm& kf) {
+      std::map<std::string, int> index;
+      std::vector<Ptr> monos;
+      for (size_t k = 0; k < outs.size(); k++) {
+        std::vector<LinearTerm<kernelT, evalT>> terms;
+        if (!linear_expand(*outs[k], theta, terms, 64)) return false;
+        std::vector<std::string> keys;
+        std::map<std::string, std::vector<Ptr>> coefs;
+        for (auto& t : terms) {
+          std::string key;
+          Ptr mono = linear_monomial(t.factors, key);
+          if (!key.empty() && !index.count(key)) {
+            index[key] = (int)monos.size();
+            monos.emplace_back(std::move(mono));
+          }
+          if (!coefs.count(key)) keys.push_back(key);
+          coefs[key].emplace_back(t.coef ? std::move(t.coef) : make_node<kernelT, evalT>(NodeKind::Constant, {}, 1.0));
+        }
+        std::vector<CombineTerm> ft;
+        for (const auto& key : keys) {
+          auto& cl = coefs[key];
+          Ptr c = cl.size() == 1 ? std::move(cl[0]) : make_node<kernelT, evalT>(NodeKind::Sum, std::move(cl));
+          CombineTerm t;
+          t.acc = key.empty() ? -1 : index[key];
+          coefficient(std::move(c), t);
+          if (t.hoist < 0 && t.mult == 0.0) continue;
+          ft.push_back(t);
+        }
+        kf.form.push_back(ft);
+      }
+      for (auto& m : monos)
+        kf.acc.emplace_back(m->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify());
+      kf.linear = true;
+      return true;
+    };
+    // statements the code generator emits for a kind's per-event terms (after CSE)
+    auto statements = [&](const std::vector<Ptr>& acc) {
+      KernelEmitter em(std::is_same<kernelT, float>::value);
+      std::vector<std::string> cols(obs);
+      cols.insert(cols.end(), ev_names.begin(), ev_names.end());
+      for (size_t d = 0; d < cols.size(); d++) em.bindings[cols[d]] = "x[" + std::to_string(d) + "]";
+      std::vector<std::string> pn(theta);
+      pn.insert(pn.end(), hoist_names.begin(), hoist_names.end());
+      for (size_t k = 0; k < pn.size(); k++) em.bindings[pn[k]] = "p[" + std::to_string(k) + "]";
+      for (auto& g : acc) em.count_products(*g);
+      for (auto& g : acc) em.emit(*g);
+      return (double)em.statements;
+    };
+    auto make_kind = [&](std::vector<Ptr> outs) {
+      if (base2)
+        for (auto& g : outs) g = to_base2(*g)->simplify();
+      if (linear == 0) return build_scaled(outs);
+      std::vector<std::string> names0(hoist_names);
+      std::vector<Ptr> exprs0;
+      for (auto& e : hoist_exprs) exprs0.emplace_back(e->copy());
+      auto restore = [&] {
+        hoist_names = names0;
+        hoist_exprs.clear();
+        for (auto& e : exprs0) hoist_exprs.emplace_back(e->copy());
+      };
+      KindForm b;
+      if (!build_linear(outs, b)) {
+        restore();
+        return build_scaled(outs);
+      }
+      // accumulating a term costs about 1.25 operations per event (an FP32 add, and an FP64
+      // convert + add per unrolled group)
+      const double cost_b = statements(b.acc) + 1.25 * b.acc.size();
+      if (linear == 2) return b;
+      restore();
+      KindForm a = build_scaled(outs);
+      const double cost_a = statements(a.acc) + 1.25 * a.acc.size();
+      if (cost_a <= cost_b) return a;
+      restore();
+      KindForm b2;
+      build_linear(outs, b2);
+      return b2;
+    };
+    gen = lp->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
+    {
+      std::vector<Ptr> o;
+      o.emplace_back(lpe->copy());
+      lh = make_kind(std::move(o));
+    }
+    {
+      std::vector<Ptr> o;
+      o.emplace_back(lpe->copy());
+      for (auto& g : d1) o.emplace_back(g->copy());
+      grad = make_kind(std::move(o));
+    }
+    if (with_hessian) {
+      std::vector<Ptr> o;
+      o.emplace_back(lpe->copy());
+      for (auto& g : d1) o.emplace_back(g->copy());
+      for (auto& g : d2) o.emplace_back(g->copy());
+      hess = make_kind(std::move(o));
+    }
+    for (auto& g : hess.acc) nodes += g->node_count();
   }
   std::vector<std::string> columns() const {
     std::vector<std::string> r(obs);
@@ -571,9 +694,12 @@ struct LikelihoodGraphs {
     auto pn = param_names();
     const auto cols = columns();
     ms.ncol = (int)cols.size();
-    ms.functors = emit_functor("mf_f_lh", raw(lh), cols, pn, ms.fp32, &ms.statements, &lh_scale) +
-                  emit_functor("mf_f_grad", raw(grad), cols, pn, ms.fp32, &ms.statements, &grad_scale);
-    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess), cols, pn, ms.fp32, &ms.statements, &hess_scale);
+    ms.functors = emit_functor("mf_f_lh", raw(lh.acc), cols, pn, ms.fp32, &ms.statements, &lh.form) +
+                  emit_functor("mf_f_grad", raw(grad.acc), cols, pn, ms.fp32, &ms.statements, &grad.form);
+    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess.acc), cols, pn, ms.fp32, &ms.statements, &hess.form);
+    ms.nacc[EVAL_LH] = (int)lh.acc.size();
+    ms.nacc[EVAL_GRAD] = (int)grad.acc.size();
+    ms.nacc[EVAL_HESS] = with_hessian ? (int)hess.acc.size() : 0;
     ms.functors += emit_functor<kernelT, evalT>("mf_f_gen", {gen.get()}, obs, pn, ms.fp32, &ms.statements);
     ms.hoist = emit_hoist(raw(hoist_exprs), theta);
     if (!ev_exprs.empty()) ms.event_terms = emit_fp64_function("mf_event_terms", "x", raw(ev_exprs), obs);
@@ -599,6 +725,7 @@ class HipModel {
   int nobs = 0, ntheta = 0, nhoist = 0, nparam = 0;
   int ncol = 0;  // data columns per event: observables, then precomputed event terms
   int nout[4] = {0, 0, 0, 0};
+  int nacc[4] = {0, 0, 0, 0};  // per-event terms each kind's kernel accumulates (partials per chunk)
   bool fp32 = std::is_same<kernelT, float>::value;
   bool with_hessian = true;
   std::string source;
@@ -620,7 +747,7 @@ class HipModel {
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> g(
         new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian,
                                              o.hip_base2 >= 2 || (f32 && o.hip_base2 == 1), 0.0,
-                                             o.hip_event_columns ? o.hip_event_cost_threshold : -1.0));
+                                             o.hip_event_columns ? o.hip_event_cost_threshold : -1.0, o.hip_linear_outputs));
     ModuleSource ms = g->module(o, with_hessian);
     auto t0 = std::chrono::steady_clock::now();
     fit_register_budget(ms, o.hip_compile_options, res);
@@ -672,6 +799,7 @@ class HipModel {
     nout[EVAL_LH] = 1;
     nout[EVAL_GRAD] = 1 + ntheta;
     nout[EVAL_HESS] = hessian ? 1 + ntheta + ntheta * (ntheta + 1) / 2 : 0;
+    for (int k = 1; k <= 3; k++) nacc[k] = nout[k] ? ms.nacc[k] : 0;
     statements = ms.statements;
     graph_nodes = g->nodes;
     source = ms.device_source();
@@ -830,7 +958,7 @@ class HipBatch {
     // of nch0 >= resident/2 chunks to the nearest multiple of resident, at most doubling it
     chunk_cap = (int)(2 * (elems / m->ncol / chunk)) + 2 * S + 1 + m->resident;
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
-    int nmax = std::max(m->nout[EVAL_GRAD], m->nout[EVAL_HESS]);
+    const int nmax = std::max(1, std::max(m->nacc[EVAL_LH], std::max(m->nacc[EVAL_GRAD], m->nacc[EVAL_HESS])));
     partials_d.alloc(dev, (size_t)chunk_cap * nmax * sizeof(double));
     seg_done_d.alloc(dev, (size_t)std::max(1, S) * sizeof(unsigned int));
     hip_check(mf_fill_zero(dev, seg_done_d.ptr, seg_done_d.bytes), "HipBatch counters");

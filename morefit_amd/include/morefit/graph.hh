@@ -40,6 +40,7 @@
 #include <string>
 #include <type_traits>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 namespace morefit {
@@ -960,6 +961,125 @@ GraphPtr<kernelT, evalT> split_output_scale(const ComputeGraphNode<kernelT, eval
   return ev.size() == 1 ? std::move(ev[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(ev));
 }
 
+// ==================================================================== linear form of an output
+// An output (a per-event term whose sum over events is wanted) written as
+//   sum_j c_j * m_j
+// with every c_j free of the event variables (a function of `params` and constants) and every
+// m_j a product of event-dependent factors (an empty product: the constant 1, whose sum is the
+// event count).  Products are distributed over event-dependent sums and differences; terms with
+// the same monomial (its factors sorted by their emitted text) are merged.  The kernels then sum
+// the distinct monomials of all outputs of a kind and form the outputs from those sums once per
+// segment in FP64 -- for the mass gradient, (G r)(a (m - mu) + b) and its siblings become
+// G r, G r (m - mu), ... : fewer operations per event.
+template <typename kernelT, typename evalT>
+struct LinearTerm {
+  GraphPtr<kernelT, evalT> coef;                   // parameter-only (null: 1)
+  std::vector<GraphPtr<kernelT, evalT>> factors;   // event-dependent (empty: 1)
+};
+
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> linear_mul(GraphPtr<kernelT, evalT> a, GraphPtr<kernelT, evalT> b) {
+  if (!a) return b;
+  if (!b) return a;
+  std::vector<GraphPtr<kernelT, evalT>> v;
+  v.emplace_back(std::move(a));
+  v.emplace_back(std::move(b));
+  return make_node<kernelT, evalT>(NodeKind::Prod, std::move(v));
+}
+
+// A shifted event variable, x + c(params): kept as one factor (the centred differences the
+// graphs are written in), not split into x and c -- that would trade one add for extra
+// monomials (x^2, x, 1 for (x + c)^2) and undo the centring that keeps FP32 sums accurate
+template <typename kernelT, typename evalT>
+bool linear_shift(const ComputeGraphNode<kernelT, evalT>& n, const std::vector<std::string>& params) {
+  if (n.kind() != NodeKind::Sum) return false;
+  int ev = 0;
+  for (const auto& c : n.children_) {
+    if (!c->contains_other_variables(params)) continue;
+    if (c->kind() != NodeKind::Variable) return false;
+    ev++;
+  }
+  return ev == 1;
+}
+
+// false when the expansion would exceed max_terms terms
+template <typename kernelT, typename evalT>
+bool linear_expand(const ComputeGraphNode<kernelT, evalT>& n, const std::vector<std::string>& params,
+                   std::vector<LinearTerm<kernelT, evalT>>& out, size_t max_terms) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  if (!n.contains_other_variables(params)) {
+    out.push_back({n.copy(), {}});
+    return out.size() <= max_terms;
+  }
+  switch (n.kind()) {
+    case NodeKind::Sum:
+      for (const auto& c : n.children_)
+        if (!linear_expand(*c, params, out, max_terms)) return false;
+      return true;
+    case NodeKind::Neg: {
+      std::vector<LinearTerm<kernelT, evalT>> t;
+      if (!linear_expand(*n.children_[0], params, t, max_terms)) return false;
+      for (auto& x : t) {
+        x.coef = linear_mul<kernelT, evalT>(make_node<kernelT, evalT>(NodeKind::Constant, {}, -1.0), std::move(x.coef));
+        out.push_back(std::move(x));
+      }
+      return out.size() <= max_terms;
+    }
+    case NodeKind::Prod: {
+      std::vector<LinearTerm<kernelT, evalT>> acc;
+      acc.push_back({nullptr, {}});
+      for (const auto& c : n.children_) {
+        std::vector<LinearTerm<kernelT, evalT>> ft;
+        if (!c->contains_other_variables(params)) {
+          ft.push_back({c->copy(), {}});
+        } else if ((c->kind() == NodeKind::Sum && !linear_shift(*c, params)) || c->kind() == NodeKind::Neg ||
+                   c->kind() == NodeKind::Prod) {
+          if (!linear_expand(*c, params, ft, max_terms)) return false;
+        } else {
+          LinearTerm<kernelT, evalT> t;
+          t.factors.push_back(c->copy());
+          ft.push_back(std::move(t));
+        }
+        std::vector<LinearTerm<kernelT, evalT>> nx;
+        for (const auto& a : acc)
+          for (const auto& b : ft) {
+            LinearTerm<kernelT, evalT> t;
+            t.coef = linear_mul<kernelT, evalT>(a.coef ? a.coef->copy() : Ptr(), b.coef ? b.coef->copy() : Ptr());
+            for (const auto& f : a.factors) t.factors.push_back(f->copy());
+            for (const auto& f : b.factors) t.factors.push_back(f->copy());
+            nx.push_back(std::move(t));
+            if (nx.size() > max_terms) return false;
+          }
+        acc = std::move(nx);
+      }
+      for (auto& t : acc) out.push_back(std::move(t));
+      return out.size() <= max_terms;
+    }
+    default: {
+      LinearTerm<kernelT, evalT> t;
+      t.factors.push_back(n.copy());
+      out.push_back(std::move(t));
+      return out.size() <= max_terms;
+    }
+  }
+}
+
+// Canonical monomial of a term: factors sorted by their emitted text (key "" for the constant 1)
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> linear_monomial(std::vector<GraphPtr<kernelT, evalT>>& factors, std::string& key) {
+  std::vector<std::pair<std::string, size_t>> order;
+  for (size_t i = 0; i < factors.size(); i++) order.push_back({factors[i]->get_kernel(), i});
+  std::sort(order.begin(), order.end());
+  key.clear();
+  if (factors.empty()) return nullptr;
+  std::vector<GraphPtr<kernelT, evalT>> f;
+  for (auto& o : order) {
+    key += o.first + "*";
+    f.push_back(std::move(factors[o.second]));
+  }
+  return f.size() == 1 ? std::move(f[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(f));
+}
+
 // ==================================================================== code emission with CSE
 // Emits the outputs of several graphs as one straight-line block of SSA statements
 //   const T t17 = ...;
@@ -1024,12 +1144,21 @@ struct KernelEmitter {
     if (it != hash_memo.end()) return it->second;
     return hash_memo[&n] = structural_hash(n);
   }
+  // counts each distinct subtree once (the graphs hold many copies of shared subtrees, e.g.
+  // the Gaussian factor in every monomial of a linear form)
+  std::unordered_set<uint64_t> counted;
   template <typename kernelT, typename evalT>
   void count_products(const ComputeGraphNode<kernelT, evalT>& n) {
-    if (n.kind() == NodeKind::Prod)
-      for (const auto& c : n.children_) prod_freq[node_hash(*c)]++;
+    if (n.children_.empty() || !counted.insert(node_hash(n)).second) return;
+    if (n.kind() == NodeKind::Prod) {
+      // products that contain a factor (a repeated factor counts once per product)
+      std::unordered_set<uint64_t> seen;
+      for (const auto& c : n.children_)
+        if (seen.insert(node_hash(*c)).second) prod_freq[node_hash(*c)]++;
+    }
     for (const auto& c : n.children_) count_products(*c);
   }
+
 
   template <typename kernelT, typename evalT>
   std::string emit(const ComputeGraphNode<kernelT, evalT>& n) {
@@ -1064,9 +1193,12 @@ struct KernelEmitter {
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return freq[a] != freq[b] ? freq[a] > freq[b] : ops[a] < ops[b]; });
       std::string acc = ops[idx[0]];
       for (size_t k = 1; k < idx.size(); k++) {
-        const std::string pk = "P(" + acc + "," + ops[idx[k]] + ")";
+        // each step keyed like a two-factor product node (operands sorted), so chains and
+        // binary products share their common prefixes
+        const std::string& b = ops[idx[k]];
+        const std::string pk = std::to_string((int)NodeKind::Prod) + ":0(" + std::min(acc, b) + "," + std::max(acc, b) + ",)";
         auto pi = memo.find(pk);
-        acc = pi != memo.end() ? pi->second : fresh(pk, acc + " * " + ops[idx[k]]);
+        acc = pi != memo.end() ? pi->second : fresh(pk, acc + " * " + b);
       }
       memo[key] = acc;
       return acc;
