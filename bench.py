@@ -61,20 +61,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp32_mass_1e7_traffi
 
 
 def measured_traffic(current_shape):
-    """HBM bytes per launch of the FP32 mf_eval_grad at N = 10^7, from the committed counter
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy kernel's reading), when that
-    summary was taken with the current launch shape; else None"""
+    """HBM bytes per launch of the FP32 mf_eval_grad at N = 10^7 from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy kernel's reading) and the launch
+    shape it was taken at (the kernel reads each event once at every shape, so the figure
+    carries over; the shape is reported beside it); (None, None) without a summary"""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
-        return None
+        return None, None
     kv = dict(l.split(":", 1) for l in lines if l.startswith(("traffic_bytes_per_launch:", "launch_shape:")))
-    want = "block=%d ept=%d acc=%d unroll=%d reduction=%s" % (
-        current_shape["block"], current_shape["events_per_thread"], current_shape["accumulate"], current_shape["unroll"]["grad"],
-        current_shape["segment_reduction"].replace(" ", "-"))
-    if kv.get("launch_shape", "").strip() != want:
-        return None
-    return float(kv["traffic_bytes_per_launch"])
+    if "traffic_bytes_per_launch" not in kv:
+        return None, None
+    return float(kv["traffic_bytes_per_launch"]), kv.get("launch_shape", "").strip()
 
 
 def plain_context(M, O):
@@ -306,7 +304,7 @@ def main():
     achieved = bytes32 / (k32_ms * 1e-3) / 1e9
     log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s (with the segment reduction {k32_pair_ms * 1e3:.2f} us); "
         f"lh kernel {k32_lh_ms * 1e3:.2f} us")
-    traffic = measured_traffic(shape(m32, tune32["best"]))
+    traffic, traffic_shape = measured_traffic(shape(m32, tune32["best"]))
     # where a call's time goes: C++ loop of full calls (launch + sync), device time of the call
     # graph replayed back to back, and the same through the direct-launch path
     step = {"python_loop_us": ms_per_step * 1e3,
@@ -478,7 +476,7 @@ def main():
                    "launch_shape": shape(m32, tune32["best"]),
                    "autotune_table_us": [[list(c), round(u, 2)] for c, u in tune32["table"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
+                     "traffic": traffic, "traffic_launch_shape": traffic_shape,
                      "kernel": "mf_eval_grad (fp32, " + ("event loop and segment reduction)" if fused32 else "per-chunk kernel; mf_eval_grad_final reduces the segment)"),
                      "kernel_ms": k32_ms, "algorithmic_bytes": bytes32, "kernels_of_a_call_ms": k32_pair_ms},
         "cpu_baseline": cpu,

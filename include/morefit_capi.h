@@ -39,7 +39,7 @@ typedef struct mfh_options {
   int grid;                /* 0 = one work-group per chunk, else persistent grid */
   int fast_math;           /* FP32 hardware exp/log/rcp */
   int accumulate;          /* FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group then FP64 */
-  int centre;              /* FP32: observables stored as offsets from the range centre */
+  int centre;              /* observables as offsets from the range centre (FP32 stored so, FP64 computed per event) */
   int unroll;              /* events per thread whose loads are issued together */
   int balance;             /* large segments: chunk count a multiple of the resident work-groups */
   int base2;               /* exp/log as exp2/log2: 0 never, 1 FP32 kernels, 2 FP32 and FP64 */
@@ -47,7 +47,7 @@ typedef struct mfh_options {
   int event_columns;       /* 1: event-only terms precomputed into data columns (optimize_dimensions) */
   double event_cost;       /* cost above which an event-only term becomes a column (graph cost units) */
   int linear;              /* outputs as linear forms over shared monomials: 0 never, 1 when shorter, 2 always */
-  int pad;
+  int exp_pivot;           /* linear forms: a sum of exponential terms evaluated around one of them */
 } mfh_options;
 
 typedef struct mfh_fit_out {

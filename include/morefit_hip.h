@@ -129,6 +129,10 @@ typedef struct mf_prep_inline_args {
   int pad;
 } mf_prep_inline_args;
 
+/* prep kernels (mf_prep, mf_prep_inline): work-groups of 64 segments along y, and MF_PREP_SPLIT
+ * along x that share a segment's hoisted terms (term k in work-group k % MF_PREP_SPLIT) */
+#define MF_PREP_SPLIT 16
+
 /* work-group size of the `_final` kernels (one work-group per segment) */
 #define MF_FINAL_THREADS 1024
 

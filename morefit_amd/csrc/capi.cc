@@ -103,6 +103,7 @@ void apply_options(compute_options& co, const mfh_options* o) {
   co.hip_event_columns = o->event_columns != 0;
   co.hip_event_cost_threshold = o->event_cost;
   co.hip_linear_outputs = o->linear;
+  co.hip_exp_pivot = o->exp_pivot != 0;
 }
 }  // namespace
 
@@ -154,6 +155,7 @@ void mfh_default_options(mfh_options* o) {
   o->event_columns = 0;
   o->event_cost = 10.0;
   o->linear = 1;
+  o->exp_pivot = 1;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {
@@ -278,9 +280,9 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
       ms = HipModel<K>::make_source(d.pdf, d.params, co, with_hessian != 0, nullptr, &res);
     } else {
       const bool f32 = std::is_same<K, float>::value;
-      LikelihoodGraphs<K> g(d.pdf, d.params, f32 && co.hip_centre_observables, with_hessian != 0,
+      LikelihoodGraphs<K> g(d.pdf, d.params, co.hip_centre_observables, with_hessian != 0,
                             co.hip_base2 >= 2 || (f32 && co.hip_base2 == 1), 0.0,
-                            co.hip_event_columns ? co.hip_event_cost_threshold : -1.0, co.hip_linear_outputs);
+                            co.hip_event_columns ? co.hip_event_cost_threshold : -1.0, co.hip_linear_outputs, co.hip_exp_pivot, !f32);
       ms = g.module(co, with_hessian != 0);
       mf_kernel_resources r[64];
       int n = 0;
@@ -319,7 +321,7 @@ int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, c
   if (which < 1 || which > 3) throw std::invalid_argument("which must be 1, 2 or 3");
   auto run = [&](auto& d) {
     typedef typename std::remove_reference<decltype(d)>::type::kernel_type K;
-    LikelihoodGraphs<K> g(d.pdf, d.params, fp32 != 0, true, fp32 != 0, 0.0, event_cost, linear);
+    LikelihoodGraphs<K> g(d.pdf, d.params, true, true, fp32 != 0, 0.0, event_cost, linear, true, fp32 == 0);
     std::vector<std::string> names(g.theta);
     std::vector<double> vals(theta, theta + g.theta.size());
     const size_t nt = names.size();

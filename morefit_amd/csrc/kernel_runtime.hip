@@ -14,7 +14,7 @@
 //   MF_FAST_MATH    1: hardware exp2/log2/rcp approximations in FP32
 //   MF_GEN_LO<d>, MF_GEN_W<d>  generator proposal box per observable (stored coordinates)
 // and, after it, functor structs { static constexpr int nout; static void eval(x, p, o); },
-// mf_hoist(theta, buf) and the kernel instantiations (MF_EVAL_KERNEL etc. below).  The
+// mf_hoist_term(k, theta) and the kernel instantiations (MF_EVAL_KERNEL etc. below).  The
 // unroll of each evaluation kernel (events per thread whose loads are issued together) is an
 // argument of MF_EVAL_KERNEL, chosen per kernel by the host so that the kernel's registers
 // stay in the 256 architectural VGPRs (see mf_module_load).
@@ -420,42 +420,46 @@ __device__ __forceinline__ void mf_precompute_impl(const mf_eval_args& a) {
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }
 
 // ------------------------------------------------------------------ parameter preparation
-// One thread per segment: hoisted parameter-only terms in FP64 from the segment's floating
-// parameters, then [theta, hoisted] rounded once to MF_T into the kernel parameter block.
-// theta may live in mapped host memory (read once, before anything is stored); with seg_src
-// the segment's table entry is staged from there into the device table the evaluation kernels
-// read; with hoist64 the hoisted terms are also kept in FP64 (output scales, mf_eval_args).
-// hoisted parameter-only terms in FP64 from the floating parameters (generated, after this runtime)
-__device__ __forceinline__ void mf_hoist(const double* th, double* buf);
+// Hoisted parameter-only terms in FP64 from a segment's floating parameters, then [theta,
+// hoisted] rounded once to MF_T into the kernel parameter block.  Term-parallel: work-group
+// (j, b) of 64 lanes handles segments b * 64 + lane and hoisted terms j, j + MF_PREP_SPLIT, ...
+// (the term index is uniform over the work-group: no divergence), so a segment's terms --
+// erf, exp and reciprocals with long FP64 dependency chains -- are computed side by side
+// rather than one after another.  Work-group j == 0 also stores theta and stages the segment's
+// table entry (seg_src -> seg_dst) for the evaluation kernels.  theta may live in mapped host
+// memory (read in full before anything is stored); with hoist64 the hoisted terms are also
+// kept in FP64 (output scales, mf_eval_args).
+// hoisted term k in FP64 from the floating parameters (generated, after this runtime)
+__device__ double mf_hoist_term(int k, const double* th);
 
-// one segment: FP64 theta th -> hoisted terms -> [theta, hoisted] in MF_T at params, FP64 hoisted
-// terms at hoist64; theta is read in full before anything is stored
-__device__ __forceinline__ void mf_prep_one(const double* th_src, int s, MF_T* params, double* hoist64, const mf_segment* seg_src,
-                                            mf_segment* seg_dst) {
+__device__ __forceinline__ void mf_prep_part(const double* th_src, int s, int j, MF_T* params, double* hoist64, const mf_segment* seg_src,
+                                             mf_segment* seg_dst) {
   double th[MF_NTHETA > 0 ? MF_NTHETA : 1];
   for (int k = 0; k < MF_NTHETA; k++) th[k] = th_src[k];
-  if (seg_src) seg_dst[s] = seg_src[s];
   MF_T* out = params + (long long)s * MF_NPARAM;
-  double hb[MF_NHOIST > 0 ? MF_NHOIST : 1];
-  mf_hoist(th, hb);
-  for (int k = 0; k < MF_NTHETA; k++) out[k] = (MF_T)th[k];
-  for (int j = 0; j < MF_NHOIST; j++) out[MF_NTHETA + j] = (MF_T)hb[j];
-  if (hoist64)
-    for (int j = 0; j < MF_NHOIST; j++) hoist64[(long long)s * MF_NHOIST + j] = hb[j];
+  if (j == 0) {
+    if (seg_src) seg_dst[s] = seg_src[s];
+    for (int k = 0; k < MF_NTHETA; k++) out[k] = (MF_T)th[k];
+  }
+  for (int k = j; k < MF_NHOIST; k += MF_PREP_SPLIT) {
+    const double h = mf_hoist_term(k, th);
+    out[MF_NTHETA + k] = (MF_T)h;
+    if (hoist64) hoist64[(long long)s * MF_NHOIST + k] = h;
+  }
 }
 
 #define MF_PREP_KERNEL                                                                                             \
   extern "C" __global__ void mf_prep(const double* theta, MF_T* params, double* hoist64, const int* seg_list, int nseg, \
                                      const mf_segment* seg_src, mf_segment* seg_dst) {                             \
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;                                                           \
+    const int i = blockIdx.y * blockDim.x + threadIdx.x;                                                           \
     if (i >= nseg) return;                                                                                         \
     const int s = seg_list ? seg_list[i] : i;                                                                      \
-    mf_prep_one(theta + (long long)s * MF_NTHETA, s, params, hoist64, seg_src, seg_dst);                           \
+    mf_prep_part(theta + (long long)s * MF_NTHETA, s, blockIdx.x, params, hoist64, seg_src, seg_dst);              \
   }                                                                                                                \
   extern "C" __global__ void mf_prep_inline(mf_prep_inline_args g) {                                               \
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;                                                           \
+    const int s = blockIdx.y * blockDim.x + threadIdx.x;                                                           \
     if (s >= g.nseg) return;                                                                                       \
-    mf_prep_one(g.theta + s * MF_NTHETA, s, (MF_T*)g.params, g.hoist64, g.seg_src, g.seg_dst);                     \
+    mf_prep_part(g.theta + s * MF_NTHETA, s, blockIdx.x, (MF_T*)g.params, g.hoist64, g.seg_src, g.seg_dst);        \
   }
 
 // ------------------------------------------------------------------ generator

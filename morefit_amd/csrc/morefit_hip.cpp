@@ -727,8 +727,7 @@ int mf_launch_prep(mf_module* mod, const double* theta, void* params, double* ho
   int rc = get_fn(mod, "mf_prep", &fn);
   if (rc != MF_OK) return rc;
   void* kp[] = {(void*)&theta, (void*)&params, (void*)&hoist64, (void*)&seg_list, (void*)&nseg, (void*)&seg_src, (void*)&seg_dst};
-  int grid = (nseg + 63) / 64;
-  MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep");
+  MF_HIP(hipModuleLaunchKernel(fn, MF_PREP_SPLIT, (nseg + 63) / 64, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep");
   return MF_OK;
 }
 
@@ -752,7 +751,7 @@ int mf_launch_prep_inline(mf_module* mod, const double* theta, int ntheta, int n
   g.seg_dst = seg_dst;
   g.nseg = nseg;
   void* kp[] = {(void*)&g};
-  MF_HIP(hipModuleLaunchKernel(fn, (nseg + 63) / 64, 1, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep_inline");
+  MF_HIP(hipModuleLaunchKernel(fn, MF_PREP_SPLIT, (nseg + 63) / 64, 1, 64, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_prep_inline");
   return MF_OK;
 }
 

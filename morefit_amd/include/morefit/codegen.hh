@@ -2,7 +2,7 @@
  * @file codegen.hh
  * Turns graphs into the source of one likelihood module: functor structs (one per kernel
  * kind, all outputs of a kind emitted together with CSE), the FP64 hoisted-term function
- * mf_hoist, the generator's proposal box, and the kernel instantiation lines.  The same
+ * mf_hoist_term, the generator's proposal box, and the kernel instantiation lines.  The same
  * functor text compiles for gfx950 through hipRTC (with the device runtime prepended by
  * libmorefit_hip) and for the host CPU backend (with host definitions of the mf_* wrappers).
  *
@@ -41,7 +41,7 @@ struct ModuleSource {
   bool fast_math = false;
   std::vector<double> gen_lo, gen_width;  // proposal box per observable, in stored coordinates
   std::string functors;                    // functor structs
-  std::string hoist;                       // mf_hoist definition
+  std::string hoist;                       // mf_hoist_term definition
   std::string event_terms;                 // mf_event_terms definition (ncol > nobs)
   std::vector<EvalKernelSpec> evals;       // evaluation kernels (unroll chosen per kernel)
   std::vector<std::string> kernels;        // other instantiation lines
@@ -109,12 +109,33 @@ inline std::string emit_combine_terms(const std::vector<CombineTerm>& terms) {
 // per-event terms `acc` (their sums over a segment's events are accumulated) from the columns
 // x[d] and kernel parameters p[k]; combine forms the nout outputs from those sums S, the
 // segment's FP64 hoisted terms h and its event count n, in FP64 (form; null: out[k] = S[k]).
+// shift (optional, one entry per observable): the graphs' variable d is x[d] + shift[d],
+// computed once per event (offsets from the range centre for data stored as given).
+inline std::string emit_shift_prologue(KernelEmitter& em, const std::vector<std::string>& observables, const std::vector<double>* shift,
+                                       const std::string& type) {
+  std::string pro;
+  for (size_t d = 0; d < observables.size(); d++) {
+    const std::string xd = "x[" + std::to_string(d) + "]";
+    if (shift && d < shift->size() && (*shift)[d] != 0.0) {
+      const std::string nm = "xs" + std::to_string(d);
+      pro += "    const " + type + " " + nm + " = " + xd + " + " + em.literal((*shift)[d]) + ";\n";
+      em.bindings[observables[d]] = nm;
+      em.statements++;
+      em.cost += 1.0;
+    } else {
+      em.bindings[observables[d]] = xd;
+    }
+  }
+  return pro;
+}
+
 template <typename kernelT, typename evalT>
 std::string emit_functor(const std::string& name, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& acc,
                          const std::vector<std::string>& observables, const std::vector<std::string>& param_names,
-                         bool fp32, size_t* statements = nullptr, const CombineForm* form = nullptr) {
+                         bool fp32, size_t* statements = nullptr, const CombineForm* form = nullptr,
+                         const std::vector<double>* shift = nullptr) {
   KernelEmitter em(fp32);
-  for (size_t d = 0; d < observables.size(); d++) em.bindings[observables[d]] = "x[" + std::to_string(d) + "]";
+  const std::string pro = emit_shift_prologue(em, observables, shift, fp32 ? "float" : "double");
   for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
   for (auto* g : acc) em.count_products(*g);
   std::vector<std::string> res;
@@ -124,7 +145,7 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
   s << "struct " << name << " {\n  static constexpr int nacc = " << acc.size() << ";\n  static constexpr int nout = " << nout << ";\n"
     << "  __device__ __forceinline__ static void eval(const MF_T* __restrict__ x, const MF_T* __restrict__ p, MF_T* __restrict__ o) {\n"
     << "    (void)x; (void)p;\n"
-    << em.body.str();
+    << pro << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << res[k] << ";\n";
   s << "  }\n  __device__ __forceinline__ static void combine(const double* S, const double* h, double n, double* out) {\n"
     << "    (void)S; (void)h; (void)n;\n";
@@ -138,28 +159,52 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
 }
 
 // void NAME(const double* ARG, double* buf): FP64 values of `exprs`, whose variables are
-// var_names bound to ARG[k]
+// var_names bound to ARG[k] (+ shift[k], when given)
 template <typename kernelT, typename evalT>
 std::string emit_fp64_function(const std::string& name, const std::string& arg, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& exprs,
-                               const std::vector<std::string>& var_names) {
+                               const std::vector<std::string>& var_names, const std::vector<double>* shift = nullptr) {
   KernelEmitter em(false);
   em.prefix = "mfd_";
   em.tmp = "h";
-  for (size_t k = 0; k < var_names.size(); k++) em.bindings[var_names[k]] = arg + "[" + std::to_string(k) + "]";
+  std::string pro;
+  for (size_t k = 0; k < var_names.size(); k++) {
+    const std::string a = arg + "[" + std::to_string(k) + "]";
+    if (shift && k < shift->size() && (*shift)[k] != 0.0) {
+      const std::string nm = arg + "s" + std::to_string(k);
+      pro += "  const double " + nm + " = " + a + " + " + em.literal((*shift)[k]) + ";\n";
+      em.bindings[var_names[k]] = nm;
+    } else {
+      em.bindings[var_names[k]] = a;
+    }
+  }
   std::vector<std::string> res;
   for (auto* g : exprs) res.push_back(em.emit(*g));
   std::ostringstream s;
   s << "__device__ __forceinline__ void " << name << "(const double* " << arg << ", double* buf) {\n  (void)" << arg << "; (void)buf;\n"
-    << em.body.str();
+    << pro << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "  buf[" << k << "] = " << res[k] << ";\n";
   s << "}\n";
   return s.str();
 }
-// void mf_hoist(const double* th, double* buf): FP64 values of the hoisted parameter-only terms
+// double mf_hoist_term(int k, const double* th): FP64 value of hoisted parameter-only term k,
+// each term computed on its own (the prep kernels evaluate the terms side by side)
 template <typename kernelT, typename evalT>
 std::string emit_hoist(const std::vector<const ComputeGraphNode<kernelT, evalT>*>& exprs, const std::vector<std::string>& theta_names) {
-  return emit_fp64_function("mf_hoist", "th", exprs, theta_names);
+  std::ostringstream s;
+  s << "__device__ double mf_hoist_term(int k, const double* th) {\n  (void)th;\n  switch (k) {\n";
+  for (size_t k = 0; k < exprs.size(); k++) {
+    KernelEmitter em(false);
+    em.prefix = "mfd_";
+    em.tmp = "h";
+    for (size_t j = 0; j < theta_names.size(); j++) em.bindings[theta_names[j]] = "th[" + std::to_string(j) + "]";
+    const std::string r = em.emit(*exprs[k]);
+    s << "    case " << k << ": {\n" << em.body.str() << "      return " << r << ";\n    }\n";
+  }
+  s << "  }\n  return 0.0;\n}\n";
+  return s.str();
 }
+// the hoist of a module without hoisted terms
+inline std::string empty_hoist() { return "__device__ double mf_hoist_term(int k, const double* th) { (void)k; (void)th; return 0.0; }\n"; }
 
 }  // namespace morefit
 #endif

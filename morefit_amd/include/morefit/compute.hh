@@ -57,6 +57,10 @@ struct compute_options {
   // a kind's outputs as linear forms over shared per-event monomials, combined per segment in
   // FP64: 0 never, 1 when the generated code is shorter, 2 always (where the expansion exists)
   int hip_linear_outputs = 1;
+  // in the linear forms, a density that is a sum of exponential terms is evaluated around the
+  // term whose exponent bounds allow it: exp, log and rcp per event instead of two exp, log and
+  // rcp for f Gauss + (1 - f) Exp
+  bool hip_exp_pivot = true;
   double hip_event_cost_threshold = 10.0;
   // segment totals in the work-group that completes a segment (no final kernel).  Off: each
   // work-group's device-scope release fence writes back its XCD's L2, and on MI355X those
@@ -65,7 +69,9 @@ struct compute_options {
   bool hip_fused_final = false;   // large segments: chunk count a multiple of the resident work-groups (measured: no gain)
   bool hip_fast_math = true;         // FP32: hardware exp2/log2/rcp (error measured against the oracle)
   int hip_accumulate = 2;            // FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group + FP64
-  bool hip_centre_observables = true;// FP32: store observables as offsets from the range centre
+  // observables as offsets from the range centre: FP32 kernels store them so, FP64 kernels
+  // compute them per event from the data as given
+  bool hip_centre_observables = true;
   int hip_generator_events_per_stream = 64;
   std::string hip_compile_options;   // extra hipRTC options
   // host-compiled CPU backend

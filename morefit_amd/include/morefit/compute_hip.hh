@@ -362,7 +362,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
     for (auto& g : graphs) outs.push_back(g.get());
     ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
-    ms.hoist = "__device__ __forceinline__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
+    ms.hoist = empty_hoist();
     s.kernel = name;
     if (kahan_summation) ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final});
     else ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
@@ -392,7 +392,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     }
     auto lp = Log<kernelT, evalT>(graphs.at(0)->copy())->simplify();
     ms.functors = emit_functor<kernelT, evalT>("mf_f_gen", {lp.get()}, obs, params, ms.fp32, &ms.statements);
-    ms.hoist = "__device__ __forceinline__ void mf_hoist(const double* th, double* buf) { (void)th; (void)buf; }\n";
+    ms.hoist = empty_hoist();
     ms.kernels.push_back("MF_GENERATE_KERNEL(mf_f_gen)");
     compile(ms);
     s.generate = true;
@@ -470,13 +470,17 @@ template <typename kernelT, typename evalT = double>
 struct LikelihoodGraphs {
   typedef GraphPtr<kernelT, evalT> Ptr;
   std::vector<std::string> obs, theta;
-  std::vector<double> centres, lo, hi;
+  // the graphs' observables are offsets from `centres`; stored data are such offsets (FP32), or
+  // are stored as given and the kernels compute the offsets per event (shift = -centres, FP64)
+  std::vector<double> centres, lo, hi, shift;
+  std::vector<double> stored_centres() const { return shift.empty() ? centres : std::vector<double>(centres.size(), 0.0); }
   // A kernel kind (value; value + gradient; value + gradient + Hessian): the per-event terms the
   // kernel accumulates, and how the kind's outputs follow from their segment sums (FP64)
   struct KindForm {
     std::vector<Ptr> acc;
     CombineForm form;
     bool linear = false;  // acc are the distinct monomials of the outputs' linear forms
+    bool pivot = false;   // the monomials evaluate the density around a pivot exponential (exp_pivot)
   };
   KindForm lh, grad, hess;
   Ptr gen;                                   // log p for the generator, hoisted but otherwise as written
@@ -493,17 +497,22 @@ struct LikelihoodGraphs {
   // optimize_dimensions, fitter.hh:339-375); the generator's log density keeps its form
   // linear: 0 every output summed as it is (its parameter-only factor split off as an FP64
   // scale), 1 a kind's outputs as linear forms over shared monomials when that emits fewer
-  // operations (linear_expand), 2 always linear forms where they exist
+  // operations (linear_expand), 2 always linear forms where they exist; pivot: a density that
+  // is a sum of exponential terms is evaluated around one of them in the linear forms (one
+  // exponential fewer per event, exp_pivot)
   LikelihoodGraphs(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, bool centre, bool with_hessian,
-                   bool base2 = false, double hoist_threshold = 0.0, double event_threshold = -1.0, int linear = 1) {
+                   bool base2 = false, double hoist_threshold = 0.0, double event_threshold = -1.0, int linear = 1,
+                   bool pivot = true, bool centre_in_kernel = false) {
     std::vector<std::string> fixed;
     std::vector<double> fixed_v;
+    std::map<std::string, Interval> box;  // observable ranges (as stored) and parameter limits
     for (auto* p : params) {
       if (p->is_constant()) {
         fixed.push_back(p->get_name());
         fixed_v.push_back(p->get_value());
       } else {
         theta.push_back(p->get_name());
+        box[p->get_name()] = Interval{(double)p->get_min(), (double)p->get_max()};
       }
     }
     for (auto* d : pdf->dimensions()) {
@@ -511,6 +520,8 @@ struct LikelihoodGraphs {
       lo.push_back(d->get_min());
       hi.push_back(d->get_max());
       centres.push_back(centre ? d->get_centre() : 0.0);
+      box[obs.back()] = Interval{lo.back() - centres.back(), hi.back() - centres.back()};
+      if (centre && centre_in_kernel) shift.push_back(-centres.back());
     }
     Ptr lp = pdf->logprob_normalised()->substitute(fixed, fixed_v)->simplify();
     if (centre) lp = centre_observables(*lp, obs, centres);
@@ -594,12 +605,23 @@ struct LikelihoodGraphs {
         }
         kf.form.push_back(ft);
       }
-      for (auto& m : monos)
-        kf.acc.emplace_back(m->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify());
+      const std::string pv = "morefit_pivot";
+      Ptr pe;
+      // E(a_i - a_k) up to 2^40 in FP32 (a Hessian monomial's product of two stays below
+      // 2^127), 2^400 in FP64
+      if (pivot) kf.pivot = exp_pivot(monos, theta, box, pv, pe, std::is_same<kernelT, float>::value ? 40.0 : 400.0);
+      if (kf.pivot) pe = pe->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
+      for (auto& m : monos) {
+        Ptr h = m->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
+        // the pivot exponent substituted after simplification: a_i - a_k stays a difference
+        // of the two exponents as computed, sharing a_k with log p = a_k + log q
+        kf.acc.emplace_back(kf.pivot ? h->replace_variable(pv, *pe) : std::move(h));
+      }
       kf.linear = true;
       return true;
     };
-    // statements the code generator emits for a kind's per-event terms (after CSE)
+    // issue cost of the statements the code generator emits for a kind's per-event terms
+    // (after CSE), in full-rate operations
     auto statements = [&](const std::vector<Ptr>& acc) {
       KernelEmitter em(std::is_same<kernelT, float>::value);
       std::vector<std::string> cols(obs);
@@ -610,12 +632,12 @@ struct LikelihoodGraphs {
       for (size_t k = 0; k < pn.size(); k++) em.bindings[pn[k]] = "p[" + std::to_string(k) + "]";
       for (auto& g : acc) em.count_products(*g);
       for (auto& g : acc) em.emit(*g);
-      return (double)em.statements;
+      return em.cost;
     };
-    auto make_kind = [&](std::vector<Ptr> outs) {
+    auto make_kind = [&](std::vector<Ptr> outs, bool allow_linear) {
       if (base2)
         for (auto& g : outs) g = to_base2(*g)->simplify();
-      if (linear == 0) return build_scaled(outs);
+      if (linear == 0 || !allow_linear) return build_scaled(outs);
       std::vector<std::string> names0(hoist_names);
       std::vector<Ptr> exprs0;
       for (auto& e : hoist_exprs) exprs0.emplace_back(e->copy());
@@ -646,20 +668,24 @@ struct LikelihoodGraphs {
     {
       std::vector<Ptr> o;
       o.emplace_back(lpe->copy());
-      lh = make_kind(std::move(o));
+      lh = make_kind(std::move(o), true);
     }
     {
       std::vector<Ptr> o;
       o.emplace_back(lpe->copy());
       for (auto& g : d1) o.emplace_back(g->copy());
-      grad = make_kind(std::move(o));
+      grad = make_kind(std::move(o), true);
     }
     if (with_hessian) {
       std::vector<Ptr> o;
       o.emplace_back(lpe->copy());
       for (auto& g : d1) o.emplace_back(g->copy());
       for (auto& g : d2) o.emplace_back(g->copy());
-      hess = make_kind(std::move(o));
+      // FP64 Hessians keep per-event outputs: second derivatives cancel within each event
+      // (E''/p - (E'/p)^2 for a dominant component), and a linear form moves that cancellation
+      // to segment sums of 10^6 terms, costing FP64 digits (1e-12 of the reference) where f or
+      // lambda sit near their limits; the Hessian is evaluated once per fit
+      hess = make_kind(std::move(o), std::is_same<kernelT, float>::value);
     }
     for (auto& g : hess.acc) nodes += g->node_count();
   }
@@ -687,22 +713,24 @@ struct LikelihoodGraphs {
     ms.block = o.hip_block_size;
     ms.acc = o.hip_accumulate;
     ms.fast_math = o.hip_fast_math;
+    const auto sc = stored_centres();
     for (size_t d = 0; d < obs.size(); d++) {
-      ms.gen_lo.push_back(lo[d] - centres[d]);
+      ms.gen_lo.push_back(lo[d] - sc[d]);
       ms.gen_width.push_back(hi[d] - lo[d]);
     }
     auto pn = param_names();
     const auto cols = columns();
     ms.ncol = (int)cols.size();
-    ms.functors = emit_functor("mf_f_lh", raw(lh.acc), cols, pn, ms.fp32, &ms.statements, &lh.form) +
-                  emit_functor("mf_f_grad", raw(grad.acc), cols, pn, ms.fp32, &ms.statements, &grad.form);
-    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess.acc), cols, pn, ms.fp32, &ms.statements, &hess.form);
+    const std::vector<double>* sh = shift.empty() ? nullptr : &shift;
+    ms.functors = emit_functor("mf_f_lh", raw(lh.acc), cols, pn, ms.fp32, &ms.statements, &lh.form, sh) +
+                  emit_functor("mf_f_grad", raw(grad.acc), cols, pn, ms.fp32, &ms.statements, &grad.form, sh);
+    if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess.acc), cols, pn, ms.fp32, &ms.statements, &hess.form, sh);
     ms.nacc[EVAL_LH] = (int)lh.acc.size();
     ms.nacc[EVAL_GRAD] = (int)grad.acc.size();
     ms.nacc[EVAL_HESS] = with_hessian ? (int)hess.acc.size() : 0;
-    ms.functors += emit_functor<kernelT, evalT>("mf_f_gen", {gen.get()}, obs, pn, ms.fp32, &ms.statements);
+    ms.functors += emit_functor<kernelT, evalT>("mf_f_gen", {gen.get()}, obs, pn, ms.fp32, &ms.statements, nullptr, sh);
     ms.hoist = emit_hoist(raw(hoist_exprs), theta);
-    if (!ev_exprs.empty()) ms.event_terms = emit_fp64_function("mf_event_terms", "x", raw(ev_exprs), obs);
+    if (!ev_exprs.empty()) ms.event_terms = emit_fp64_function("mf_event_terms", "x", raw(ev_exprs), obs, sh);
     const int u = std::max(1, o.hip_unroll);
     const bool f = o.hip_fused_final;
     ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f}, {"mf_eval_grad", "mf_f_grad", u, f}};
@@ -745,9 +773,10 @@ class HipModel {
                                   std::vector<mf_kernel_resources>* res = nullptr, double* compile_ms = nullptr) {
     const bool f32 = std::is_same<kernelT, float>::value;
     std::unique_ptr<LikelihoodGraphs<kernelT, evalT>> g(
-        new LikelihoodGraphs<kernelT, evalT>(pdf, params, f32 && o.hip_centre_observables, with_hessian,
+        new LikelihoodGraphs<kernelT, evalT>(pdf, params, o.hip_centre_observables, with_hessian,
                                              o.hip_base2 >= 2 || (f32 && o.hip_base2 == 1), 0.0,
-                                             o.hip_event_columns ? o.hip_event_cost_threshold : -1.0, o.hip_linear_outputs));
+                                             o.hip_event_columns ? o.hip_event_cost_threshold : -1.0, o.hip_linear_outputs,
+                                             o.hip_exp_pivot, !f32));
     ModuleSource ms = g->module(o, with_hessian);
     auto t0 = std::chrono::steady_clock::now();
     fit_register_budget(ms, o.hip_compile_options, res);
@@ -788,7 +817,7 @@ class HipModel {
     for (size_t i = 0; i < ps.size(); i++)
       if (!ps[i]->is_constant()) theta_index.push_back((int)i);
     obs = g->obs;
-    centres = g->centres;
+    centres = g->stored_centres();
     lo = g->lo;
     hi = g->hi;
     nobs = ms.nobs;

@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
 #include <map>
 #include <memory>
@@ -1080,6 +1081,261 @@ GraphPtr<kernelT, evalT> linear_monomial(std::vector<GraphPtr<kernelT, evalT>>& 
   return f.size() == 1 ? std::move(f[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(f));
 }
 
+// ==================================================================== interval bounds
+// Bounds of an expression over a box of its variables (observable ranges, parameter limits):
+// plain interval arithmetic, so the bounds hold but may be loose where a variable occurs more
+// than once.  Unbounded ([-inf, inf]) wherever a variable is outside the box or a node has no
+// rule here.
+struct Interval {
+  double lo = -INFINITY, hi = INFINITY;
+  bool bounded() const { return std::isfinite(lo) && std::isfinite(hi); }
+};
+
+inline Interval interval_mul(Interval a, Interval b) {
+  const double p[4] = {a.lo * b.lo, a.lo * b.hi, a.hi * b.lo, a.hi * b.hi};
+  Interval r{p[0], p[0]};
+  for (double v : p) {
+    if (std::isnan(v)) return Interval();
+    r.lo = std::min(r.lo, v);
+    r.hi = std::max(r.hi, v);
+  }
+  return r;
+}
+
+// b^e for an integer e (|e| <= 64); an even power is non-negative
+inline Interval interval_ipow(Interval b, int e) {
+  Interval r{1.0, 1.0};
+  for (int k = 0; k < std::abs(e); k++) r = interval_mul(r, b);
+  if (e % 2 == 0 && b.lo < 0.0 && b.hi > 0.0) r = Interval{0.0, std::max(std::pow(b.lo, std::abs(e)), std::pow(b.hi, std::abs(e)))};
+  if (e >= 0) return r;
+  if (!(r.lo > 0.0 || r.hi < 0.0)) return Interval();
+  return Interval{1.0 / r.hi, 1.0 / r.lo};
+}
+
+template <typename kernelT, typename evalT>
+Interval interval_of(const ComputeGraphNode<kernelT, evalT>& n, const std::map<std::string, Interval>& box) {
+  auto child = [&](size_t i) { return interval_of(*n.children_[i], box); };
+  auto monotone = [](Interval a, double (*f)(double)) { return Interval{f(a.lo), f(a.hi)}; };
+  switch (n.kind()) {
+    case NodeKind::Constant: return Interval{n.value(), n.value()};
+    case NodeKind::Variable: {
+      auto it = box.find(n.name());
+      return it == box.end() ? Interval() : it->second;
+    }
+    case NodeKind::Sum: {
+      Interval r{0.0, 0.0};
+      for (size_t i = 0; i < n.children_.size(); i++) {
+        Interval c = child(i);
+        r.lo += c.lo;
+        r.hi += c.hi;
+      }
+      if (std::isnan(r.lo) || std::isnan(r.hi)) return Interval();
+      return r;
+    }
+    case NodeKind::Prod: {
+      // equal factors bounded as one power: (x - mu) (x - mu) is non-negative
+      Interval r{1.0, 1.0};
+      std::vector<bool> used(n.children_.size(), false);
+      for (size_t i = 0; i < n.children_.size(); i++) {
+        if (used[i]) continue;
+        int e = 1;
+        for (size_t j = i + 1; j < n.children_.size(); j++)
+          if (!used[j] && n.children_[j]->equals(*n.children_[i])) {
+            used[j] = true;
+            e++;
+          }
+        r = interval_mul(r, e == 1 ? child(i) : interval_ipow(child(i), e));
+      }
+      return r;
+    }
+    case NodeKind::Neg: {
+      Interval c = child(0);
+      return Interval{-c.hi, -c.lo};
+    }
+    case NodeKind::Inv: {
+      Interval c = child(0);
+      if (!(c.lo > 0.0 || c.hi < 0.0)) return Interval();
+      return Interval{1.0 / c.hi, 1.0 / c.lo};
+    }
+    case NodeKind::Pow: {
+      if (!n.children_[1]->is_constant()) return Interval();
+      const double e = n.children_[1]->value();
+      Interval b = child(0);
+      if (e == std::floor(e) && std::fabs(e) <= 64) return interval_ipow(b, (int)e);
+      if (!(b.lo > 0.0)) return Interval();
+      Interval r{std::pow(b.lo, e), std::pow(b.hi, e)};
+      if (r.lo > r.hi) std::swap(r.lo, r.hi);
+      return r;
+    }
+    case NodeKind::Exp: return monotone(child(0), [](double v) { return std::exp(v); });
+    case NodeKind::Exp2: return monotone(child(0), [](double v) { return std::exp2(v); });
+    case NodeKind::Log:
+    case NodeKind::Log2:
+    case NodeKind::Sqrt: {
+      Interval c = child(0);
+      if (!(c.lo > 0.0)) return Interval();
+      if (n.kind() == NodeKind::Sqrt) return monotone(c, [](double v) { return std::sqrt(v); });
+      return n.kind() == NodeKind::Log ? monotone(c, [](double v) { return std::log(v); })
+                                       : monotone(c, [](double v) { return std::log2(v); });
+    }
+    case NodeKind::Sin:
+    case NodeKind::Cos: return Interval{-1.0, 1.0};
+    case NodeKind::Conditional: {
+      Interval a = child(1), b = child(2);
+      return Interval{std::min(a.lo, b.lo), std::max(a.hi, b.hi)};
+    }
+    default: return Interval();
+  }
+}
+
+// ==================================================================== exponential pivot
+// A density that is a sum of exponential terms, p = sum_i c_i E(a_i) (E = exp or exp2, c_i free
+// of the event variables), enters the per-event monomials as log p and as products
+// E(a_i1)...E(a_im) / p^m (its derivatives).  With one term k as pivot and
+//   q = sum_i c_i E(a_i - a_k)        (the pivot's own term is c_k)
+// these are  log p = a_k + log q  and  E(a_i1)...E(a_im) / p^m = E(a_i1 - a_k)...E(a_im - a_k) / q^m,
+// so every event evaluates one exponential fewer (for f Gauss + (1 - f) Exp: exp2, log2 and rcp
+// instead of two exp2, log2 and rcp).  The pivot is the term whose exponent bounds (over the
+// observable ranges and parameter limits in `box`) keep every E(a_i - a_k) at or below
+// 2^max_log2 -- no overflow of it or of the products of two of them in the Hessian monomials,
+// where p itself is finite -- and nothing is rewritten when no term
+// qualifies, when a monomial has another number of exponentials than powers of 1/p, or when p
+// occurs in another form.  Terms whose E(a_i - a_k) underflows are negligible beside c_k, as
+// they are in p.  The pivot exponent a_k enters the rewritten monomials as the variable
+// `pivot_var` (set to a_k in `pivot`), so that once substituted it is computed once per event,
+// not once more inside each a_i - a_k.  Returns whether the monomials were rewritten.
+template <typename kernelT, typename evalT>
+bool exp_pivot(std::vector<GraphPtr<kernelT, evalT>>& monos, const std::vector<std::string>& params,
+               const std::map<std::string, Interval>& box, const std::string& pivot_var, GraphPtr<kernelT, evalT>& pivot,
+               double max_log2 = 40.0) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  typedef ComputeGraphNode<kernelT, evalT> Node;
+  auto factors_of = [](const Node& m) {
+    std::vector<const Node*> f;
+    if (m.kind() == NodeKind::Prod)
+      for (const auto& c : m.children_) f.push_back(c.get());
+    else
+      f.push_back(&m);
+    return f;
+  };
+  // the density: the argument of the log factor
+  const Node* P = nullptr;
+  NodeKind logk = NodeKind::Log;
+  for (const auto& m : monos)
+    for (const Node* f : factors_of(*m))
+      if ((f->kind() == NodeKind::Log || f->kind() == NodeKind::Log2) && f->children_[0]->contains_other_variables(params) && !P) {
+        P = f->children_[0].get();
+        logk = f->kind();
+      }
+  if (!P || P->kind() != NodeKind::Sum || P->children_.size() < 2) return false;
+  const NodeKind expk = logk == NodeKind::Log2 ? NodeKind::Exp2 : NodeKind::Exp;
+  const size_t nt = P->children_.size();
+  std::vector<std::vector<const Node*>> coef(nt);
+  std::vector<const Node*> ex(nt, nullptr);
+  for (size_t i = 0; i < nt; i++) {
+    for (const Node* f : factors_of(*P->children_[i])) {
+      if (!f->contains_other_variables(params)) coef[i].push_back(f);
+      else if (f->kind() == expk && !ex[i]) ex[i] = f;
+      else return false;
+    }
+    if (!ex[i]) return false;
+  }
+  // pivot: the smallest bound on max_i (a_i - a_k)
+  std::vector<Interval> iv(nt);
+  for (size_t i = 0; i < nt; i++) iv[i] = interval_of(*ex[i]->children_[0], box);
+  const double to_log2 = expk == NodeKind::Exp2 ? 1.0 : 1.4426950408889634;
+  int k = -1;
+  double best = INFINITY;
+  for (size_t j = 0; j < nt; j++) {
+    double b = -INFINITY;
+    for (size_t i = 0; i < nt; i++)
+      if (i != j) b = std::max(b, (iv[i].hi - iv[j].lo) * to_log2);
+    if (std::isfinite(b) && b < best) {
+      best = b;
+      k = (int)j;
+    }
+  }
+  if (k < 0 || !(best <= max_log2)) return false;
+  const Node& ak = *ex[k]->children_[0];
+  std::vector<Ptr> r(nt);
+  std::vector<Ptr> qt;
+  for (size_t i = 0; i < nt; i++) {
+    std::vector<Ptr> t;
+    for (const Node* c : coef[i]) t.emplace_back(c->copy());
+    if ((int)i != k) {
+      std::vector<Ptr> nk, d;
+      nk.emplace_back(make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, pivot_var));
+      d.emplace_back(ex[i]->children_[0]->copy());
+      d.emplace_back(make_node<kernelT, evalT>(NodeKind::Neg, std::move(nk)));
+      std::vector<Ptr> e;
+      e.emplace_back(make_node<kernelT, evalT>(NodeKind::Sum, std::move(d))->simplify());
+      r[i] = make_node<kernelT, evalT>(expk, std::move(e));
+      t.emplace_back(r[i]->copy());
+    }
+    if (t.empty()) t.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 1.0));
+    qt.emplace_back(t.size() == 1 ? std::move(t[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(t)));
+  }
+  Ptr Q = make_node<kernelT, evalT>(NodeKind::Sum, std::move(qt))->simplify();
+  // p or one of its exponentials anywhere but as a rewritable factor: leave everything as it is
+  std::function<bool(const Node&)> mentions = [&](const Node& n) {
+    if (n.equals(*P)) return true;
+    for (size_t i = 0; i < nt; i++)
+      if (n.equals(*ex[i])) return true;
+    for (const auto& c : n.children_)
+      if (mentions(*c)) return true;
+    return false;
+  };
+  std::vector<Ptr> out;
+  for (const auto& m : monos) {
+    std::vector<Ptr> nf;
+    int ninv = 0, nexp = 0;
+    bool logp = false;
+    auto fs = factors_of(*m);
+    for (const Node* f : fs) {
+      if (f->kind() == logk && f->children_[0]->equals(*P)) {
+        logp = true;
+        continue;
+      }
+      if (f->kind() == NodeKind::Inv && f->children_[0]->equals(*P)) {
+        ninv++;
+        std::vector<Ptr> v;
+        v.emplace_back(Q->copy());
+        nf.emplace_back(make_node<kernelT, evalT>(NodeKind::Inv, std::move(v)));
+        continue;
+      }
+      int hit = -1;
+      for (size_t i = 0; i < nt && hit < 0; i++)
+        if (f->equals(*ex[i])) hit = (int)i;
+      if (hit >= 0) {
+        nexp++;
+        if (hit != k) nf.emplace_back(r[hit]->copy());
+        continue;
+      }
+      if (mentions(*f)) return false;
+      nf.emplace_back(f->copy());
+    }
+    if (logp) {
+      if (fs.size() != 1) return false;
+      std::vector<Ptr> v;
+      v.emplace_back(Q->copy());
+      std::vector<Ptr> s;
+      s.emplace_back(make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, pivot_var));
+      s.emplace_back(make_node<kernelT, evalT>(logk, std::move(v)));
+      out.emplace_back(make_node<kernelT, evalT>(NodeKind::Sum, std::move(s)));
+      continue;
+    }
+    if (ninv != nexp) return false;
+    if (ninv == 0) {
+      out.emplace_back(m->copy());
+      continue;
+    }
+    out.emplace_back(nf.size() == 1 ? std::move(nf[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(nf)));
+  }
+  pivot = ak.copy();  // before the old monomials (which hold p) go
+  monos = std::move(out);
+  return true;
+}
+
 // ==================================================================== code emission with CSE
 // Emits the outputs of several graphs as one straight-line block of SSA statements
 //   const T t17 = ...;
@@ -1094,6 +1350,10 @@ struct KernelEmitter {
   std::map<std::string, std::string> bindings;
   int next = 0;
   int statements = 0;
+  // issue cost of the emitted statements in full-rate VALU operations: a transcendental that the
+  // hardware evaluates (FP32 exp2, log2, rcp, sqrt, sin, cos) is quarter rate; FP64 ones and the
+  // rest are library sequences
+  double cost = 0.0;
   std::string prefix = "mf_";  // math wrapper prefix ("mfd_" for the FP64 hoisted terms)
   std::string tmp = "t";       // SSA name prefix
   // how often each operand (by structural hash) occurs in the products of the graphs about
@@ -1112,10 +1372,18 @@ struct KernelEmitter {
     else std::snprintf(buf, sizeof(buf), "%a", v);
     return buf;
   }
+  double op_cost(const std::string& expr) const {
+    if (expr.compare(0, prefix.size(), prefix) != 0) return 1.0;
+    const std::string f = expr.substr(prefix.size(), expr.find('(') - prefix.size());
+    const bool hw = f == "exp2" || f == "log2" || f == "rcp" || f == "sqrt" || f == "sin" || f == "cos";
+    if (fp32) return hw ? 4.0 : (f == "exp" || f == "log") ? 5.0 : 20.0;
+    return (f == "rcp" || f == "sqrt") ? 8.0 : 20.0;
+  }
   std::string fresh(const std::string& key, const std::string& expr) {
     std::string n = tmp + std::to_string(next++);
     body << "  const " << T << " " << n << " = " << expr << ";\n";
     statements++;
+    cost += op_cost(expr);
     memo[key] = n;
     return n;
   }

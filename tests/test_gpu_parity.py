@@ -154,6 +154,45 @@ def test_fp64_base2_matches_oracle(model):
     cx.close()
 
 
+@pytest.mark.parametrize("pivot", [1, 0])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_exponential_pivot_matches_oracle(fp32, pivot):
+    """the mass density evaluated around its exponential term (exp_pivot=1: exp, log and rcp per
+    event) and as written (exp_pivot=0: two exp, log and rcp), at the start point and at points
+    on the parameter limits: FP64 within 1e-12, FP32 within 2e-6 of the sums of absolute values.
+    At the limit points the FP64 Hessian is held to 1e-11: there its lambda-lambda entry is
+    ill-conditioned, and the oracle and the kernels' graphs are each ~7e-13 off the 50-digit value
+    (test_cpu_pipeline.py::test_hessian_conditioning_at_parameter_limits).  FP32 is held to 1e-5
+    there: with sigma = 0.005 and mu at the range edge, the FP32 shift 5.3 - mu (absolute
+    rounding ~1.5e-8) resolves the distances m - mu ~ sigma to ~3e-6"""
+    n = 1_000_003
+    data = O.generate(M.MASS, M.TRUTH[M.MASS], n, seed=91, nstreams=64)
+    say("exp pivot", pivot, "fp32", fp32, "n", n)
+    cx = M.Context(exp_pivot=pivot)
+    mdl = cx.model(M.MASS, fp32=fp32)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    stored = _stored(M.MASS, data, fp32)
+    thetas = [np.array(M.START[M.MASS]), np.array([5.0, 0.005, 0.02, -10.0]), np.array([5.6, 0.13, 0.98, 10.0]),
+              np.array([5.3, 0.02, 0.999, -9.9])]
+    for t, theta in enumerate(thetas):
+        ref = O.nll_sums(M.MASS, stored, theta, mode=2, nthreads=8)
+        absref = O.nll_sums(M.MASS, stored, theta, mode=12, nthreads=8)
+        tol = np.full(len(ref), 2e-6 if fp32 else 1e-12)
+        if t > 0:
+            if fp32:
+                tol[:] = 1e-5
+            else:
+                tol[5:] = 1e-11
+        for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+            out = b.eval(kind, theta)[0]
+            k = len(out)
+            assert np.all(np.abs(out - ref[:k]) <= tol[:k] * absref[:k]), (theta, kind, (out - ref[:k]) / absref[:k])
+    b.close()
+    mdl.close()
+    cx.close()
+
+
 @pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
 @pytest.mark.parametrize("accumulate", [0, 1, 2])
 def test_fp32_accumulation_modes(model, accumulate):
