@@ -48,6 +48,8 @@ typedef struct mfh_options {
   double event_cost;       /* cost above which an event-only term becomes a column (graph cost units) */
   int linear;              /* outputs as linear forms over shared monomials: 0 never, 1 when shorter, 2 always */
   int exp_pivot;           /* linear forms: a sum of exponential terms evaluated around one of them */
+  int host_params;         /* one-segment calls: hoisted terms on the host, passed by value (no prep kernel) */
+  int pad;
 } mfh_options;
 
 typedef struct mfh_fit_out {

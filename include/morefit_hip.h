@@ -83,6 +83,11 @@ typedef struct mf_chunk {
 } mf_chunk;
 
 /* argument block of every generated evaluation kernel (mf_eval_<kind>) */
+/* capacity of the parameters passed by value in mf_eval_args: 128 32-bit words (128 FP32 or 64
+ * FP64 kernel parameters) and 64 FP64 hoisted terms */
+#define MF_INLINE_PARAM_WORDS 128
+#define MF_INLINE_HOIST 64
+
 typedef struct mf_eval_args {
   const void* data;           /* T* SoA event columns */
   const mf_segment* segs;
@@ -109,7 +114,11 @@ typedef struct mf_eval_args {
                                  segment whose totals the final kernel reduces (nseg == 1, neither
                                  MF_EVAL_SKIP_FINAL nor seg_done). */
   int done_value;
-  int pad2;
+  int inline_params;          /* 1: one-segment call whose kernel parameters are pinline (MF_T values,
+                                 [theta, hoisted]) and whose FP64 hoisted terms are hinline, read from
+                                 this argument block (kernarg segment) instead of params / hoist64 */
+  unsigned int pinline[MF_INLINE_PARAM_WORDS];
+  double hinline[MF_INLINE_HOIST];
 } mf_eval_args;
 
 /* mf_eval_args.flags: launch only the per-chunk kernel.  Correct when no active segment has

@@ -46,7 +46,8 @@ class Options(ctypes.Structure):
                 ("grid", ctypes.c_int), ("fast_math", ctypes.c_int), ("accumulate", ctypes.c_int), ("centre", ctypes.c_int),
                 ("unroll", ctypes.c_int), ("balance", ctypes.c_int), ("base2", ctypes.c_int),
                 ("fused_final", ctypes.c_int), ("event_columns", ctypes.c_int), ("event_cost", ctypes.c_double),
-                ("linear", ctypes.c_int), ("exp_pivot", ctypes.c_int)]
+                ("linear", ctypes.c_int), ("exp_pivot", ctypes.c_int),
+                ("host_params", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class FitOut(ctypes.Structure):

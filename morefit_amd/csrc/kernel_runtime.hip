@@ -151,12 +151,22 @@ __device__ __forceinline__ void mf_two_sum(double a, double b, double& s, double
   e = (a - (s - bb)) + (b - bb);
 }
 
+// The argument block as it lies in the kernarg segment (the evaluation kernels' only explicit
+// argument, at offset 0): parameters passed by value are read from there with scalar loads.
+__device__ __forceinline__ const MF_CONST char* mf_args_block() { return (const MF_CONST char*)__builtin_amdgcn_kernarg_segment_ptr(); }
+__device__ __forceinline__ const MF_CONST MF_T* mf_params_of(const mf_eval_args& a) {
+  return a.inline_params ? (const MF_CONST MF_T*)(mf_args_block() + __builtin_offsetof(mf_eval_args, pinline))
+                         : (const MF_CONST MF_T*)a.params;
+}
+
 // Totals of segment s (its F::nout outputs) from the segment sums S of the F::nacc per-event
 // terms, formed by the generated F::combine in FP64 with the segment's hoisted terms
 template <class F>
 __device__ __forceinline__ void mf_write_totals(const mf_eval_args& a, int s, const mf_segment& sg, const double* S) {
   double out[F::nout];
-  const double* h = a.hoist64 ? a.hoist64 + (long long)s * MF_NHOIST : nullptr;
+  const double* h = a.inline_params ? (const double*)(mf_args_block() + __builtin_offsetof(mf_eval_args, hinline))
+                    : a.hoist64     ? a.hoist64 + (long long)s * MF_NHOIST
+                                    : nullptr;
   F::combine(S, h, (double)sg.n, out);
 #pragma unroll
   for (int k = 0; k < F::nout; k++) a.results[sg.out_off + k] = out[k];
@@ -255,14 +265,14 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
   const MF_T* __restrict__ data = (const MF_T*)a.data;
   const MF_CONST mf_chunk* chunks = (const MF_CONST mf_chunk*)a.chunks;
   const MF_CONST mf_segment* segs = (const MF_CONST mf_segment*)a.segs;
-  const MF_CONST MF_T* params = (const MF_CONST MF_T*)a.params;
+  const MF_CONST MF_T* params = mf_params_of(a);
   unsigned int* seg_done = a.seg_done;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
     const mf_chunk ch = chunks[c];
     const mf_segment sg = segs[ch.seg];
     if (sg.mode == 0) continue;
-    const MF_T* __restrict__ P = (const MF_T*)(params + sg.param_off);
+    const MF_T* __restrict__ P = (const MF_T*)(params + (a.inline_params ? 0 : sg.param_off));
     const MF_T* __restrict__ col = data + sg.data_off;
     mf_acc<NO> acc;
     acc.zero();

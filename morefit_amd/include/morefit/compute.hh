@@ -61,6 +61,9 @@ struct compute_options {
   // term whose exponent bounds allow it: exp, log and rcp per event instead of two exp, log and
   // rcp for f Gauss + (1 - f) Exp
   bool hip_exp_pivot = true;
+  // one-segment calls compute the hoisted terms on the host and pass them with theta in the
+  // kernels' argument block (no prep kernel); batches prepare them on the device
+  bool hip_host_params = true;
   double hip_event_cost_threshold = 10.0;
   // segment totals in the work-group that completes a segment (no final kernel).  Off: each
   // work-group's device-scope release fence writes back its XCD's L2, and on MI355X those

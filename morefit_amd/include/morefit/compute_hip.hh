@@ -31,6 +31,7 @@
 
 #include "codegen.hh"
 #include "compute.hh"
+#include "hostprog.hh"
 #include "morefit_hip.h"
 #include "pdf.hh"
 #include "random.hh"
@@ -750,6 +751,10 @@ class HipModel {
   std::vector<int> theta_index;  // position in params of each floating parameter
   std::vector<std::string> obs;
   std::vector<double> centres, lo, hi;
+  // the hoisted terms as a host program over theta: one-segment calls pass [theta, hoisted] by
+  // value instead of running the prep kernel (empty when the block does not fit the arguments)
+  HostProgram<kernelT, evalT> hoist_prog;
+  bool host_params = false;
   int nobs = 0, ntheta = 0, nhoist = 0, nparam = 0;
   int ncol = 0;  // data columns per event: observables, then precomputed event terms
   int nout[4] = {0, 0, 0, 0};
@@ -824,6 +829,9 @@ class HipModel {
     ncol = std::max(ms.ncol, ms.nobs);
     ntheta = ms.ntheta;
     nhoist = ms.nhoist;
+    host_params = be->opts_->hip_host_params && (size_t)(ms.ntheta + ms.nhoist) * sizeof(kernelT) <= MF_INLINE_PARAM_WORDS * 4u &&
+                  ms.nhoist <= MF_INLINE_HOIST;
+    if (host_params) hoist_prog = HostProgram<kernelT, evalT>(LikelihoodGraphs<kernelT, evalT>::raw(g->hoist_exprs), g->theta);
     nparam = ntheta + nhoist;
     nout[EVAL_LH] = 1;
     nout[EVAL_GRAD] = 1 + ntheta;
@@ -1254,9 +1262,22 @@ class HipBatch {
   // just the evaluation kernels of `kind` (segment table and parameters already on device);
   // main_only launches the per-chunk kernel without any reduction of multi-chunk segments (for
   // timing the event loop alone)
-  void launch_kernel(int kind, bool main_only = false, int done_value = 0) {
+  void launch_kernel(int kind, bool main_only = false, int done_value = 0, bool by_value = false) {
     if (nchunks == 0) return;
     mf_eval_args a = eval_args(kind);
+    if (by_value) {
+      // [theta, hoisted] of segment 0 in the argument block, hoisted terms computed here
+      const double* th = theta(0);
+      double h[MF_INLINE_HOIST];
+      m->hoist_prog.run(th, h);
+      kernelT* p = reinterpret_cast<kernelT*>(a.pinline);
+      for (int k = 0; k < m->ntheta; k++) p[k] = (kernelT)th[k];
+      for (int j = 0; j < m->nhoist; j++) {
+        p[m->ntheta + j] = (kernelT)h[j];
+        a.hinline[j] = h[j];
+      }
+      a.inline_params = 1;
+    }
     if (main_only) {
       a.flags |= MF_EVAL_SKIP_FINAL;
       a.seg_done = nullptr;
@@ -1270,9 +1291,21 @@ class HipBatch {
   // the full per-call sequence of one kind: prep (theta and segment modes from mapped host
   // memory) -> per-chunk kernel -> final pass (totals into mapped host memory)
   void enqueue(int kind, bool direct = true, int done_value = 0) {
+    if (by_value(kind, direct)) {
+      if (segs_stale_[kind]) {  // stage the segment's table entry (its mode changed) with a copy
+        hip_check(mf_memcpy_h2d(dev, segs_d[kind].ptr, segs_hk[kind].template as<mf_segment>(), sizeof(mf_segment) * (size_t)S),
+                  "segment table");
+        segs_stale_[kind] = false;
+      }
+      launch_kernel(kind, false, done_value, true);
+      return;
+    }
     prep(kind, direct);
     launch_kernel(kind, false, done_value);
   }
+  // a one-segment direct call: its kernels take theta and the host-computed hoisted terms by
+  // value (no prep kernel)
+  bool by_value(int kind, bool direct) const { return direct && S == 1 && m->host_params && kind > 0; }
   // a one-segment batch whose totals the final kernel reduces: that kernel signals completion
   // through a flag in mapped host memory, which the host polls (a stream synchronisation costs
   // several microseconds more)

@@ -6,13 +6,16 @@
 //     exp2(a_i)..exp2(a_j) / p^m monomials equal the originals at random points, and the pivot
 //     is the term whose exponent bounds qualify;
 //  3. the pivot refuses: an unbounded parameter, a monomial with more powers of 1/p than
-//     exponentials, and a density term that is not exponential leave the monomials unchanged.
+//     exponentials, and a density term that is not exponential leave the monomials unchanged;
+//  4. the host program of a set of expressions (shared subexpressions computed once) returns
+//     bitwise the values of evaluating each expression's graph.
 // Prints "ok" or "FAIL <reason>" and exits non-zero on failure.
 #include <cmath>
 #include <cstdio>
 #include <random>
 
 #include "morefit/graph.hh"
+#include "morefit/hostprog.hh"
 
 using namespace morefit;
 
@@ -192,6 +195,39 @@ int main() {
     auto m3 = monomials(B(NodeKind::Prod, K(0.01), V("x")));
     check(!exp_pivot(m3, th, box(), "pv", pv) && same(m3, monomials(B(NodeKind::Prod, K(0.01), V("x")))),
           "non-exponential density term: no pivot");
+  }
+  // 4. host program == graph evaluation, bitwise
+  {
+    std::vector<G> ex;
+    ex.emplace_back(gauss_exponent());
+    ex.emplace_back(U(NodeKind::Exp2, exp_exponent()));
+    ex.emplace_back(density());
+    ex.emplace_back(U(NodeKind::Erf, P3(V("x"), V("l"), U(NodeKind::Inv, V("s")))));
+    ex.emplace_back(make_node<float, double>(NodeKind::Conditional, [] {
+      std::vector<G> c;
+      c.emplace_back(V("l"));
+      c.emplace_back(K(0.6));
+      c.emplace_back(U(NodeKind::Inv, V("l")));
+      return c;
+    }(), 0.0, "", 0));
+    ex.emplace_back(B(NodeKind::Pow, V("s"), K(-1.5)));
+    std::vector<const ComputeGraphNode<float, double>*> raw;
+    for (auto& e : ex) raw.push_back(e.get());
+    HostProgram<float, double> prog(raw, names);
+    check(prog.outputs() == ex.size(), "host program outputs");
+    std::vector<double> out(ex.size());
+    for (int n = 0; n < 20000; n++) {
+      auto v = sample(rng);
+      if (n == 0) v[3] = 0.0;  // the conditional's other branch
+      prog.run(v.data(), out.data());
+      for (size_t k = 0; k < ex.size(); k++) {
+        const double want = ex[k]->eval(names, v);
+        if (!(out[k] == want || (std::isnan(out[k]) && std::isnan(want)))) {
+          std::printf("FAIL host program %zu: %.17g vs %.17g\n", k, out[k], want);
+          return 1;
+        }
+      }
+    }
   }
   if (fails) return 1;
   std::printf("ok\n");

@@ -220,30 +220,39 @@ def test_fp32_accumulation_modes(model, accumulate):
 
 
 def test_graph_replays_match_eager(ctx):
-    """hipGraph replays (several, with eager work between them) == eager launches, bitwise"""
+    """hipGraph replays (several, with eager work between them) == eager launches: bitwise when
+    both prepare the parameters on the device (host_params=0); with the default one-segment path
+    (hoisted terms computed on the host and passed by value) within 1e-12 per output"""
     n = 300000
     data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=5, nstreams=16)
-    mdl = ctx.model(M.MASS, fp32=True)
-    b = mdl.batch([n])
-    b.upload(0, data)
     thetas = [M.MASS_START, M.MASS_TRUTH, np.array([5.3, 0.025, 0.25, -1.2])]
-    say("graph replay")
-    for t in thetas:
-        eager = b.eval(M.EVAL_GRAD, t, use_graph=False)
-        for _ in range(3):
-            g = b.eval(M.EVAL_GRAD, t, use_graph=True)
-            assert np.array_equal(g, eager)
-            b.eval(M.EVAL_LH, t, use_graph=False)
-    b.close()
-    mdl.close()
+    cdev = M.Context(host_params=0)
+    for cx, exact in ((cdev, True), (ctx, False)):
+        mdl = cx.model(M.MASS, fp32=True)
+        b = mdl.batch([n])
+        b.upload(0, data)
+        say("graph replay, host parameters", not exact)
+        for t in thetas:
+            eager = b.eval(M.EVAL_GRAD, t, use_graph=False)
+            for _ in range(3):
+                g = b.eval(M.EVAL_GRAD, t, use_graph=True)
+                if exact:
+                    assert np.array_equal(g, eager)
+                else:
+                    assert np.all(np.abs(g - eager) <= 1e-12 * np.abs(eager) + 1e-14 * np.max(np.abs(eager))), g - eager
+                b.eval(M.EVAL_LH, t, use_graph=False)
+        b.close()
+        mdl.close()
+    cdev.close()
 
 
 @pytest.mark.parametrize("fp32", [True, False])
 def test_polled_completion_returns_complete_totals(ctx, fp32):
     """A one-segment multi-chunk batch: the final kernel publishes a completion flag that the host
     polls instead of waiting on the stream.  200 calls alternating two parameter points, all
-    kinds: each result bitwise equal to the stream-synchronised result at that point (a flag
-    published before the totals would show the other point's totals)."""
+    kinds: each result bitwise equal to the first polled result at that point and within 1e-12
+    of the stream-synchronised result there (a flag published before the totals would show the
+    other point's totals)."""
     n = 1_000_003
     data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=11, nstreams=16)
     mdl = ctx.model(M.MASS, fp32=fp32)
@@ -254,9 +263,17 @@ def test_polled_completion_returns_complete_totals(ctx, fp32):
     for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
         want = [b.eval(kind, t, use_graph=True) for t in thetas]
         assert not np.array_equal(want[0], want[1])
+        first = [None, None]
         for i in range(200):
             got = b.eval(kind, thetas[i % 2])
-            assert np.array_equal(got, want[i % 2]), (kind, i)
+            w = want[i % 2]
+            # stream-synchronised graph result (parameters prepared on the device): equal to the
+            # polled call's within the host/device hoisted-term rounding; the other point's
+            # totals differ by far more
+            assert np.all(np.abs(got - w) <= 1e-12 * np.abs(w) + 1e-14 * np.max(np.abs(w))), (kind, i, got - w)
+            if first[i % 2] is None:
+                first[i % 2] = got
+            assert np.array_equal(got, first[i % 2]), (kind, i)
     b.close()
     mdl.close()
 
@@ -300,39 +317,53 @@ def test_segment_reduction_in_kernel_and_final_pass(fp32, fused):
 @pytest.mark.parametrize("fp32", [False, True])
 def test_segments_alone_equal_mixed(ctx, fp32):
     """ragged segments (0, 1, 63, 4096, 4097, 100000 events): each segment's sums are bitwise
-    identical evaluated alone (first and later calls) and together with the others, and match
-    the oracle."""
+    identical evaluated alone and together with the others when both prepare the parameters on
+    the device (host_params=0), within 1e-12 of the sums of absolute values with the default
+    one-segment path (hoisted terms computed on the host, passed by value: an ulp of difference
+    in a hoisted normalisation term reaches ~1e-13 in a one-event Hessian), and match the
+    oracle."""
     sizes = [0, 1, 63, 4096, 4097, 100000]
     rs = np.random.default_rng(9)
     datas = [O.generate(M.MASS, M.MASS_TRUTH, s, seed=100 + i, nstreams=2) if s else np.zeros((1, 0)) for i, s in enumerate(sizes)]
     thetas = np.array([M.MASS_START + rs.normal(0, 1e-3, 4) * [1, 0.1, 1, 1] for _ in sizes])
     say("segments", sizes, "fp32", fp32)
     mdl = ctx.model(M.MASS, fp32=fp32)
+    cdev = M.Context(host_params=0)
+    mdev = cdev.model(M.MASS, fp32=fp32)
     mixed = mdl.batch(sizes)
     for i, d in enumerate(datas):
         if sizes[i]:
             mixed.upload(i, d)
     both = mixed.eval(M.EVAL_HESS, thetas)
     for i, d in enumerate(datas):
+        alone_dev = mdev.batch([sizes[i]])
         alone = mdl.batch([sizes[i]])
         if sizes[i]:
+            alone_dev.upload(0, d)
             alone.upload(0, d)
-        r = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
-        assert np.array_equal(r[0], both[i]), (i, r[0] - both[i])
-        # from its second call on, a one-segment batch passes theta to mf_prep by value and
-        # leaves the (unchanged) device segment table alone: still the same bits
         for _ in range(2):
-            r2 = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
-            assert np.array_equal(r2[0], both[i]), (i, r2[0] - both[i])
+            r = alone_dev.eval(M.EVAL_HESS, thetas[i:i + 1])
+            assert np.array_equal(r[0], both[i]), (i, r[0] - both[i])
+        alone_dev.close()
         if sizes[i] == 0:
             assert np.all(r[0] == 0)
+            absref = np.zeros_like(r[0])
         else:
             ref = O.nll_sums(M.MASS, _stored(M.MASS, d, fp32), thetas[i], mode=2, nthreads=0)
             absref = O.nll_sums(M.MASS, _stored(M.MASS, d, fp32), thetas[i], mode=12, nthreads=1)
             tol = 2e-6 if fp32 else 1e-12
             assert np.all(np.abs(r[0] - ref) <= tol * absref + 1e-300)
+        # the default one-segment path: theta and host-computed hoisted terms by value (no prep
+        # kernel), the same sums up to the hoisted terms' rounding, bitwise the same call to call
+        r1 = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
+        for _ in range(2):
+            r2 = alone.eval(M.EVAL_HESS, thetas[i:i + 1])
+            assert np.all(np.abs(r2[0] - both[i]) <= 1e-12 * absref), (i, r2[0] - both[i])
+            assert np.array_equal(r2, r1)
         alone.close()
     mixed.close()
+    mdev.close()
+    cdev.close()
     mdl.close()
 
 
