@@ -953,6 +953,10 @@ class HipBatch {
   DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_;
   int gen_levels_ = 0;
   std::vector<mf_gen_stream> gen_streams_h_;
+  // pinned staging of the tables copied to the device (a copy from pageable memory costs the
+  // runtime ~0.1 ms of staging per call on MI355X): the chunk table, and the generator's tables
+  // and proposal counts
+  HostBuf chunks_stage_, gen_stage_;
   // staging in mapped host memory, read and written by the kernels in place: theta and the
   // per-kind segment tables (modes) in, totals out -- a call's graph holds kernels only
   HostBuf theta_h, segs_hk[4], results_h[4], done_h, segs_pre_h;
@@ -1073,7 +1077,14 @@ class HipBatch {
     }
     if ((int)ch.size() > chunk_cap) throw std::runtime_error("HipBatch: chunk table over capacity");
     nchunks = (int)ch.size();
-    if (nchunks) hip_check(mf_memcpy_h2d(dev, chunks_d.ptr, ch.data(), ch.size() * sizeof(mf_chunk)), "HipBatch chunks");
+    if (nchunks) {
+      // a previous copy out of the staging buffer may still be queued: drain the stream first
+      // (layout changes only)
+      if (chunks_stage_.ptr) hip_check(mf_sync(dev), "HipBatch chunks");
+      chunks_stage_.alloc(dev, ch.size() * sizeof(mf_chunk));
+      std::memcpy(chunks_stage_.ptr, ch.data(), ch.size() * sizeof(mf_chunk));
+      hip_check(mf_memcpy_h2d(dev, chunks_d.ptr, chunks_stage_.ptr, ch.size() * sizeof(mf_chunk)), "HipBatch chunks");
+    }
     for (int k = 1; k <= 3; k++) {
       if (m->nout[k] == 0) continue;
       mf_segment* t = segs_hk[k].template as<mf_segment>();
@@ -1205,10 +1216,20 @@ class HipBatch {
     gen_maxprob_.alloc(dev, S * sizeof(double));
     gen_segs_.alloc(dev, S * sizeof(mf_segment));
     gen_props_.alloc(dev, S * sizeof(unsigned long long));
-    hip_check(mf_memcpy_h2d(dev, gen_streams_.ptr, st.data(), st.size() * sizeof(mf_gen_stream)), "generate streams");
-    hip_check(mf_memcpy_h2d(dev, gen_base_.ptr, base.data(), base.size() * sizeof(unsigned int)), "generate seeds");
-    hip_check(mf_memcpy_h2d(dev, gen_maxprob_.ptr, mp.data(), S * sizeof(double)), "generate maxprob");
-    hip_check(mf_memcpy_h2d(dev, gen_segs_.ptr, segs_h.data(), S * sizeof(mf_segment)), "generate segs");
+    // the four tables in one pinned staging buffer (8-byte aligned parts), then the proposal counts
+    const size_t b_st = st.size() * sizeof(mf_gen_stream), b_base = (base.size() * sizeof(unsigned int) + 7) & ~(size_t)7,
+                 b_mp = S * sizeof(double), b_sg = ((S * sizeof(mf_segment)) + 7) & ~(size_t)7, b_pr = S * sizeof(unsigned long long);
+    gen_stage_.alloc(dev, b_st + b_base + b_mp + b_sg + b_pr);
+    char* stage = (char*)gen_stage_.ptr;
+    std::memcpy(stage, st.data(), b_st);
+    std::memcpy(stage + b_st, base.data(), base.size() * sizeof(unsigned int));
+    std::memcpy(stage + b_st + b_base, mp.data(), b_mp);
+    std::memcpy(stage + b_st + b_base + b_mp, segs_h.data(), S * sizeof(mf_segment));
+    unsigned long long* props_h = (unsigned long long*)(stage + b_st + b_base + b_mp + b_sg);
+    hip_check(mf_memcpy_h2d(dev, gen_streams_.ptr, stage, b_st), "generate streams");
+    hip_check(mf_memcpy_h2d(dev, gen_base_.ptr, stage + b_st, base.size() * sizeof(unsigned int)), "generate seeds");
+    hip_check(mf_memcpy_h2d(dev, gen_maxprob_.ptr, stage + b_st + b_base, b_mp), "generate maxprob");
+    hip_check(mf_memcpy_h2d(dev, gen_segs_.ptr, stage + b_st + b_base + b_mp, S * sizeof(mf_segment)), "generate segs");
     hip_check(mf_fill_zero(dev, gen_props_.ptr, S * sizeof(unsigned long long)), "generate proposals");
     hip_check(mf_launch_seed_streams(dev, gen_powers_.template as<unsigned int>(), levels, gen_streams_.template as<mf_gen_stream>(),
                                      (int)st.size(), events_per_stream, gen_base_.template as<unsigned int>(),
@@ -1225,12 +1246,11 @@ class HipBatch {
     g.proposals = gen_props_.template as<unsigned long long>();
     const int block = 64;
     hip_check(mf_launch_generate(m->mod, &g, (int)((st.size() + block - 1) / block), block), "mf_launch_generate");
-    std::vector<unsigned long long> props(S);
-    hip_check(mf_memcpy_d2h(dev, props.data(), gen_props_.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    hip_check(mf_memcpy_d2h(dev, props_h, gen_props_.ptr, b_pr), "generate proposals");
     hip_check(mf_sync(dev), "generate");
     precompute(slots);
     unsigned long long tot = 0;
-    for (auto p : props) tot += p;
+    for (int k = 0; k < S; k++) tot += props_h[k];
     return tot;
   }
 
