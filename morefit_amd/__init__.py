@@ -82,6 +82,22 @@ CAPI_SYMBOLS = [
 ]
 
 
+_FAST = []
+
+
+def _fast_eval():
+    """morefit_amd._fast.eval (the per-call evaluation through the CPython C API), bound to the
+    loaded library's mfh_batch_eval; None when the extension is not built (the ctypes path)"""
+    if not _FAST:
+        try:
+            from . import _fast
+            _fast.init(ctypes.cast(libs()[1].mfh_batch_eval, ctypes.c_void_p).value)
+            _FAST.append(_fast.eval)
+        except ImportError:
+            _FAST.append(None)
+    return _FAST[0]
+
+
 def libs():
     """Load (once) and return (libmorefit_hip, libmorefit_capi); raise if not built."""
     global _hip, _capi
@@ -324,6 +340,12 @@ class Batch:
         array (S, nout)"""
         if not self._hv:
             raise MorefitError("batch is closed")
+        fast = _fast_eval()
+        if fast is not None:
+            r = fast(self._hv, kind, theta, int(use_graph), self.S, self.model.ntheta, self.model.nout[kind])
+            if type(r) is int:
+                _check(r)
+            return r
         th = np.asarray(theta, dtype=np.float64)
         if th.size != self.S * self.model.ntheta or not th.flags.c_contiguous:
             th = np.ascontiguousarray(np.broadcast_to(th, (self.S, self.model.ntheta)))

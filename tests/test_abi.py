@@ -135,3 +135,19 @@ def test_kernels_fit_register_budget():
                 assert 1 <= r["unroll"] <= 8, (kind, fp32, k, r)
                 assert r["agpr"] == 0 and r["scratch"] == 0, (kind, fp32, k, r)
     assert M.kernel_resources(M.KSTAR, False, fit=True, unroll=8)[M.EVAL_HESS]["unroll"] < 8
+
+
+def test_fast_eval_extension_binds_and_checks_arguments():
+    """morefit_amd._fast (Batch.eval through the CPython C API) is built, bound to the loaded
+    library's mfh_batch_eval, and rejects a malformed call before reaching the library: wrong
+    argument count, a theta of neither nseg x ntheta nor ntheta values, negative sizes"""
+    fast = M._fast_eval()
+    assert fast is not None, "morefit_amd/_fast.so not built"
+    with pytest.raises(TypeError):
+        fast(1, 2, [0.0])
+    with pytest.raises(ValueError):
+        fast(1, 2, [0.0] * 5, 0, 2, 4, 5)  # 5 values for 2 segments x 4 parameters
+    with pytest.raises(ValueError):
+        fast(1, 2, [0.0] * 4, 0, -1, 4, 5)
+    with pytest.raises(ValueError):
+        fast(0, 2, [0.0] * 4, 0, 1, 4, 5)  # null batch handle
