@@ -86,7 +86,8 @@ typedef struct mf_chunk {
 /* capacity of the parameters passed by value in mf_eval_args: 128 32-bit words (128 FP32 or 64
  * FP64 kernel parameters) and 64 FP64 hoisted terms; a model with more takes the prep kernel.
  * The size of the argument block (1.1 KB) was measured: a one-fit call of the N = 10^7 FP32
- * gradient took 23.6 us with it, 27.8 us with a 0.6 KB block and 24.2-24.9 us with 2.1 KB */
+ * gradient (two launches) took 23.6 us with it, 27.8 and 28.0 us with 0.6 and 0.9 KB blocks,
+ * 24.2-24.9 us with 2.1 KB (and 23.3 us with the 0.1 KB block before the parameters moved in) */
 #define MF_INLINE_PARAM_WORDS 128
 #define MF_INLINE_HOIST 64
 
