@@ -61,11 +61,74 @@ __device__ __forceinline__ float mf_atan(float x) { return atanf(x); }
 __device__ __forceinline__ float mf_erf(float x) { return erff(x); }
 __device__ __forceinline__ float mf_pow(float x, float y) { return powf(x, y); }
 #else
-__device__ __forceinline__ double mf_exp(double x) { return exp(x); }
-__device__ __forceinline__ double mf_log(double x) { return log(x); }
+// FP64 exp, log and reciprocal of the per-event code, written out: the library's versions take
+// 42, 98 and 12 VALU instructions on gfx950 and were most of an FP64 event's work.  Each is
+// within about 2 ulp and branch-free (a branch to the library for special arguments, inlined at
+// every unrolled event, doubled the kernels' registers): special arguments are handled by
+// clamping and selects.
+// exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2 / 2 (ln2 split so that k * ln2_hi is exact
+// for |k| < 2^11), e^r by its Taylor polynomial to degree 13 (truncation < 1e-17), times 2^k.
+// The argument is clamped to [-746, 710], where exp underflows to 0 / overflows to inf.
+__device__ __forceinline__ double mf_exp(double x) {
+  const double xc = fmin(fmax(x, -746.0), 710.0);
+  const double k = rint(xc * 0x1.71547652b82fep+0);
+  double r = fma(-k, 0x1.62e42fefa3800p-1, xc);
+  r = fma(-k, 0x1.ef35793c76730p-45, r);
+  double p = 1.0 / 6227020800.0;  // 1/13!
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  const double v = ldexp(p, (int)k);
+  return x == x ? v : x;
+}
+// reciprocal: the hardware approximation refined by two Newton steps; where those give NaN
+// (zero, infinite or NaN argument) the hardware value itself (inf, 0, NaN)
+__device__ __forceinline__ double mf_rcp(double x) {
+  const double r0 = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r0, 1.0);
+  double r = fma(r0, e, r0);
+  e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  return r == r ? r : r0;
+}
+// log: x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1) / (m + 1),
+// |s| <= 0.1716, by its series 2 s + 2 s^3 / 3 + ... + 2 s^21 / 21 (truncation < 3e-17 of it);
+// log x = e ln2_hi + (e ln2_lo + log m).  Zero, negative, infinite, NaN: -inf, NaN, inf, NaN.
+__device__ __forceinline__ double mf_log(double x) {
+  double m = __builtin_amdgcn_frexp_mant(x);
+  int e = __builtin_amdgcn_frexp_exp(x);
+  const bool low = m < 0x1.6a09e667f3bcdp-1;
+  m = low ? 2.0 * m : m;
+  e = low ? e - 1 : e;
+  const double s = (m - 1.0) * mf_rcp(m + 1.0);
+  const double z = s * s;
+  double p = 2.0 / 21.0;
+  p = fma(p, z, 2.0 / 19.0);
+  p = fma(p, z, 2.0 / 17.0);
+  p = fma(p, z, 2.0 / 15.0);
+  p = fma(p, z, 2.0 / 13.0);
+  p = fma(p, z, 2.0 / 11.0);
+  p = fma(p, z, 2.0 / 9.0);
+  p = fma(p, z, 2.0 / 7.0);
+  p = fma(p, z, 2.0 / 5.0);
+  p = fma(p, z, 2.0 / 3.0);
+  const double lm = fma(s * z, p, 2.0 * s);
+  const double de = (double)e;
+  const double v = fma(de, 0x1.62e42fefa3800p-1, fma(de, 0x1.ef35793c76730p-45, lm));
+  return (x > 0.0 && x < __builtin_inf()) ? v : (x == 0.0 ? -__builtin_inf() : (x > 0.0 ? x : __builtin_nan("")));
+}
 __device__ __forceinline__ double mf_exp2(double x) { return exp2(x); }
 __device__ __forceinline__ double mf_log2(double x) { return log2(x); }
-__device__ __forceinline__ double mf_rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ double mf_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ double mf_sin(double x) { return sin(x); }
 __device__ __forceinline__ double mf_cos(double x) { return cos(x); }
