@@ -377,20 +377,48 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
     }
     double v[NO];
     acc.get(v);
+    if constexpr (NO * MF_BLOCK * 8 <= 32768) {
+      // through LDS: every thread stores its NO sums (transposed), then 32 lanes per output
+      // add a column of MF_BLOCK / 32 values each and a 32-lane butterfly joins them: about
+      // MF_BLOCK / 32 + 5 additions per output in 32 threads instead of 6 in every thread
+      __shared__ double s_all[NO * MF_BLOCK];
+      // the thread index through a register move the compiler cannot hoist: computed here, the
+      // LDS addresses do not sit in registers across the event loop
+      const int ti = mf_thread_index();
 #pragma unroll
-    for (int k = 0; k < NO; k++) {
-      double w = mf_wave_sum(v[k]);
-      if (lane == 0) s_red[wave][k] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x < NO) {
-      double t = 0.0;
+      for (int k = 0; k < NO; k++) s_all[k * MF_BLOCK + ti] = v[k];
+      __syncthreads();
+      // 32-lane group ti / 32 takes outputs ti / 32, ti / 32 + MF_BLOCK / 32, ... (both halves of a
+      // wave run the loop body for their own output; the butterfly stays inside a half)
+      const int j = ti & 31;
+      for (int k = ti >> 5; k < NO; k += MF_BLOCK / 32) {
+        double t = 0.0;
+#pragma unroll 4
+        for (int g = 0; g < MF_BLOCK / 32; g++) t += s_all[k * MF_BLOCK + g * 32 + j];
 #pragma unroll
-      for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
-      a.partials[(long long)c * NO + threadIdx.x] = t;
-      s_tot[threadIdx.x] = t;
+        for (int off = 16; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (j == 0) {
+          a.partials[(long long)c * NO + k] = t;
+          s_tot[k] = t;
+        }
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int k = 0; k < NO; k++) {
+        double w = mf_wave_sum(v[k]);
+        if (lane == 0) s_red[wave][k] = w;
+      }
+      __syncthreads();
+      if (threadIdx.x < NO) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < MF_NWAVE; w++) t += s_red[w][threadIdx.x];
+        a.partials[(long long)c * NO + threadIdx.x] = t;
+        s_tot[threadIdx.x] = t;
+      }
+      __syncthreads();
     }
-    __syncthreads();
     // a segment of one chunk is complete here: its totals need no second pass
     if (threadIdx.x == 0 && sg.chunk_end - sg.chunk_begin == 1) mf_write_totals<F>(a, ch.seg, sg, s_tot);
     const int nch = sg.chunk_end - sg.chunk_begin;
