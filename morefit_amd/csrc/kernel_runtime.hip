@@ -597,31 +597,30 @@ __device__ __forceinline__ void mf_generate_impl(const mf_gen_args& g) {
   const MF_T mp = (MF_T)g.maxprob[st.seg];
   MF_T* __restrict__ out = (MF_T*)g.data + sg.data_off;
   unsigned long long props = 0;
-  bool failed = false;
-  for (int i = st.begin; i < st.end; i++) {
+  // One proposal per iteration, accepted events stored in order: a lane goes on to its next
+  // event without waiting for the other lanes of its wave to accept their current one (with a
+  // per-event inner loop the wave ran, per event, the most tries of any of its 64 lanes).  The
+  // stream's draws and accepted events are the same as one event after another.
+  int i = st.begin;
+  unsigned int tries = 0;  // bounded: a density that is NaN or far above maxprob's scale must not hang the stream
+  while (i < st.end) {
     MF_T x[MF_NOBS];
-    // bounded: a density that is NaN or far above maxprob's scale must not hang the stream
-    unsigned int tries = 0;
-    for (;;) {
-      props++;
-      MF_GEN_PROPOSE(x, s);
-      MF_T lp[1];
-      L::eval(x, P, lp);
-      const MF_T p = mf_exp(lp[0]);
-      const MF_T u = mf_uniform(s);
-      if (mp * u < p) break;
-      if (++tries >= MF_GEN_MAX_TRIES) {
-        failed = true;
-        break;
-      }
-    }
-    if (failed) {
+    props++;
+    MF_GEN_PROPOSE(x, s);
+    MF_T lp[1];
+    L::eval(x, P, lp);
+    const MF_T p = mf_exp(lp[0]);
+    const MF_T u = mf_uniform(s);
+    if (mp * u < p) {
+#pragma unroll
+      for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + i] = x[d];
+      i++;
+      tries = 0;
+    } else if (++tries >= MF_GEN_MAX_TRIES) {
       for (int j = i; j < st.end; j++)
         for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + j] = (MF_T)(0.0 / 0.0);
       break;
     }
-#pragma unroll
-    for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + i] = x[d];
   }
 #pragma unroll
   for (int j = 0; j < 4; j++) g.states[4 * k + j] = s[j];
