@@ -269,12 +269,24 @@ __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, 
     const int k = k0 + q;
     double acc = 0.0, err = 0.0;
     if (k < NO) {
-#pragma unroll UNR
-      for (int j = r; j < nch; j += R) {
-        double u, e;
-        mf_two_sum(acc, part[(long long)j * NO + k], u, e);
-        acc = u;
-        err += e;
+      // batches of UNR partials: all loads issued before the additions, so a thread's ~nch / R
+      // loads take one memory latency, not one each (a runtime trip count below the unroll
+      // left them to a one-at-a-time remainder loop); past the end a batch holds zeros, which
+      // the two-sum adds exactly, so the order and the bits are those of the plain loop
+      for (int j0 = r; j0 < nch; j0 += UNR * R) {
+        double b[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+          const int j = j0 + u * R;
+          b[u] = j < nch ? part[(long long)j * NO + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+          double s2, e;
+          mf_two_sum(acc, b[u], s2, e);
+          acc = s2;
+          err += e;
+        }
       }
     }
     double v = acc + err;
