@@ -526,6 +526,7 @@ struct LikelihoodGraphs {
     }
     Ptr lp = pdf->logprob_normalised()->substitute(fixed, fixed_v)->simplify();
     if (centre) lp = centre_observables(*lp, obs, centres);
+    lp = double_angle(*lp)->simplify();
     // event columns come out of log p itself, before differentiation: the derivatives then
     // treat them as opaque variables (products of columns stay products, not new columns)
     Ptr lpe = event_threshold >= 0.0

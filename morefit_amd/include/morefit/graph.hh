@@ -1081,6 +1081,69 @@ GraphPtr<kernelT, evalT> linear_monomial(std::vector<GraphPtr<kernelT, evalT>>& 
   return f.size() == 1 ? std::move(f[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(f));
 }
 
+// ==================================================================== double angles
+// cos(2a) -> 2 cos(a)^2 - 1 and sin(2a) -> 2 sin(a) cos(a) where the graph evaluates cos(a) (and,
+// for the sine, sin(a)) anyway: the K*mumu density's cos(2 phi) and sin(2 phi) then cost products
+// of the cos(phi) and sin(phi) it already has instead of two more library calls per event.
+template <typename kernelT, typename evalT>
+void collect_trig_args(const ComputeGraphNode<kernelT, evalT>& n, std::set<uint64_t>& cos_args, std::set<uint64_t>& sin_args) {
+  if (n.kind() == NodeKind::Cos) cos_args.insert(structural_hash(*n.children_[0]));
+  if (n.kind() == NodeKind::Sin) sin_args.insert(structural_hash(*n.children_[0]));
+  for (const auto& c : n.children_) collect_trig_args(*c, cos_args, sin_args);
+}
+
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> double_angle_rewrite(const ComputeGraphNode<kernelT, evalT>& n, const std::set<uint64_t>& cos_args,
+                                              const std::set<uint64_t>& sin_args) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  std::vector<Ptr> ch;
+  for (const auto& c : n.children_) ch.emplace_back(double_angle_rewrite(*c, cos_args, sin_args));
+  if ((n.kind() == NodeKind::Cos || n.kind() == NodeKind::Sin) && ch[0]->kind() == NodeKind::Prod) {
+    // the argument as 2 * a
+    const auto& f = ch[0]->children_;
+    int two = -1;
+    for (size_t i = 0; i < f.size(); i++)
+      if (f[i]->is_constant() && f[i]->value() == 2.0 && two < 0) two = (int)i;
+    if (two >= 0 && f.size() >= 2) {
+      std::vector<Ptr> rest;
+      for (size_t i = 0; i < f.size(); i++)
+        if ((int)i != two) rest.emplace_back(f[i]->copy());
+      Ptr a = rest.size() == 1 ? std::move(rest[0]) : make_node<kernelT, evalT>(NodeKind::Prod, std::move(rest));
+      const uint64_t h = structural_hash(*a);
+      auto one = [&](NodeKind k) {
+        std::vector<Ptr> v;
+        v.emplace_back(a->copy());
+        return make_node<kernelT, evalT>(k, std::move(v));
+      };
+      if (n.kind() == NodeKind::Cos && cos_args.count(h)) {
+        std::vector<Ptr> p;
+        p.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 2.0));
+        p.emplace_back(one(NodeKind::Cos));
+        p.emplace_back(one(NodeKind::Cos));
+        std::vector<Ptr> sum;
+        sum.emplace_back(make_node<kernelT, evalT>(NodeKind::Prod, std::move(p)));
+        sum.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, -1.0));
+        return make_node<kernelT, evalT>(NodeKind::Sum, std::move(sum));
+      }
+      if (n.kind() == NodeKind::Sin && cos_args.count(h) && sin_args.count(h)) {
+        std::vector<Ptr> p;
+        p.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 2.0));
+        p.emplace_back(one(NodeKind::Sin));
+        p.emplace_back(one(NodeKind::Cos));
+        return make_node<kernelT, evalT>(NodeKind::Prod, std::move(p));
+      }
+    }
+  }
+  return make_node<kernelT, evalT>(n.kind(), std::move(ch), n.value(), n.name(), n.comparator());
+}
+
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> double_angle(const ComputeGraphNode<kernelT, evalT>& g) {
+  std::set<uint64_t> c, s;
+  collect_trig_args(g, c, s);
+  return double_angle_rewrite(g, c, s);
+}
+
 // ==================================================================== interval bounds
 // Bounds of an expression over a box of its variables (observable ranges, parameter limits):
 // plain interval arithmetic, so the bounds hold but may be loose where a variable occurs more
