@@ -8,7 +8,9 @@
 //  3. the pivot refuses: an unbounded parameter, a monomial with more powers of 1/p than
 //     exponentials, and a density term that is not exponential leave the monomials unchanged;
 //  4. the host program of a set of expressions (shared subexpressions computed once) returns
-//     bitwise the values of evaluating each expression's graph.
+//     bitwise the values of evaluating each expression's graph;
+//  5. the double-angle rewrite: cos(2x) and sin(2x) become products of cos(x) and sin(x) where
+//     those occur, the values stay the same, and cos(2x) alone (no cos(x) in the graph) is kept.
 // Prints "ok" or "FAIL <reason>" and exits non-zero on failure.
 #include <cmath>
 #include <cstdio>
@@ -228,6 +230,35 @@ int main() {
         }
       }
     }
+  }
+  // 5. double angles
+  {
+    auto two_x = [] { return B(NodeKind::Prod, K(2.0), V("x")); };
+    // cos(x) sin(x) + cos(2x) + sin(2x) * mu
+    std::vector<G> t;
+    t.emplace_back(B(NodeKind::Prod, U(NodeKind::Cos, V("x")), U(NodeKind::Sin, V("x"))));
+    t.emplace_back(U(NodeKind::Cos, two_x()));
+    t.emplace_back(B(NodeKind::Prod, U(NodeKind::Sin, two_x()), V("mu")));
+    G g = make_node<float, double>(NodeKind::Sum, std::move(t));
+    G r = double_angle(*g);
+    std::set<std::string> trig;
+    std::function<void(const ComputeGraphNode<float, double>&)> walk = [&](const ComputeGraphNode<float, double>& n) {
+      if (n.kind() == NodeKind::Cos || n.kind() == NodeKind::Sin) trig.insert(n.get_kernel());
+      for (const auto& c : n.children_) walk(*c);
+    };
+    walk(*r);
+    check(trig.size() == 2, "double angles: only cos(x) and sin(x) left");
+    double worst = 0.0;
+    for (int n = 0; n < 20000; n++) {
+      auto v = sample(rng);
+      v[0] *= 10.0;  // x over [-3, 3]
+      worst = std::max(worst, std::fabs(g->eval(names, v) - r->eval(names, v)));
+    }
+    if (!(worst < 1e-14)) std::printf("FAIL double angles differ by %g\n", worst), fails++;
+    G lone = U(NodeKind::Cos, two_x());
+    check(double_angle(*lone)->equals(*lone), "cos(2x) without cos(x) kept");
+    G other = B(NodeKind::Sum, U(NodeKind::Cos, V("x")), U(NodeKind::Cos, B(NodeKind::Prod, K(3.0), V("x"))));
+    check(double_angle(*other)->equals(*other), "cos(3x) kept");
   }
   if (fails) return 1;
   std::printf("ok\n");

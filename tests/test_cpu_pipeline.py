@@ -199,7 +199,8 @@ def test_bounds_and_exponential_pivot(tmp_path):
     below by zero); the pivot rewrite of log p and exp(a_i)/p^m monomials is exact to 1e-12, picks
     the term whose exponent bounds qualify, and refuses unbounded parameters, unbalanced
     monomials and non-exponential density terms; the host program of hoisted terms returns the
-    graphs' values bitwise (tests/cpp/pivot_check.cc)"""
+    graphs' values bitwise; the double-angle rewrite keeps the values and removes two
+    trigonometric calls (tests/cpp/pivot_check.cc)"""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = str(tmp_path / "pivot_check")
     subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "morefit_amd", "include"),
