@@ -11,7 +11,7 @@
 //   MF_NHOIST       hoisted parameter-only terms (FP64 on the device prep kernel)
 //   MF_BLOCK        work-group size of the evaluation kernels (multiple of 64)
 //   MF_ACC          0: FP64 accumulation per event, 1: compensated (Kahan) FP32 per lane
-//   MF_FAST_MATH    1: hardware exp2/log2/rcp approximations in FP32
+//   MF_FAST_MATH    1: hardware exp2/log2/rcp/sin/cos approximations in FP32
 //   MF_GEN_LO<d>, MF_GEN_W<d>  generator proposal box per observable (stored coordinates)
 // and, after it, functor structs { static constexpr int nout; static void eval(x, p, o); },
 // mf_hoist_term(k, theta) and the kernel instantiations (MF_EVAL_KERNEL etc. below).  The
@@ -52,8 +52,22 @@ __device__ __forceinline__ float mf_exp2(float x) { return exp2f(x); }
 __device__ __forceinline__ float mf_log2(float x) { return log2f(x); }
 #endif
 __device__ __forceinline__ float mf_sqrt(float x) { return sqrtf(x); }
+#if MF_FAST_MATH
+// The hardware sine / cosine (v_sin_f32 / v_cos_f32 take the argument in turns and are valid
+// for |turns| <= 256): one multiply and one quarter-rate instruction instead of the library's
+// range reduction and polynomials.  Beyond 256 turns (1608 rad) the fraction of the turn is
+// passed, which loses nothing the FP32 argument still holds; below, the turns are passed as
+// they are, keeping the relative accuracy of small negative angles that a fraction would lose.
+__device__ __forceinline__ float mf_turns(float x) {
+  const float t = x * 0.159154943f;  // 1 / (2 pi)
+  return fabsf(t) <= 256.0f ? t : __builtin_amdgcn_fractf(t);
+}
+__device__ __forceinline__ float mf_sin(float x) { return __builtin_amdgcn_sinf(mf_turns(x)); }
+__device__ __forceinline__ float mf_cos(float x) { return __builtin_amdgcn_cosf(mf_turns(x)); }
+#else
 __device__ __forceinline__ float mf_sin(float x) { return sinf(x); }
 __device__ __forceinline__ float mf_cos(float x) { return cosf(x); }
+#endif
 __device__ __forceinline__ float mf_tan(float x) { return tanf(x); }
 __device__ __forceinline__ float mf_asin(float x) { return asinf(x); }
 __device__ __forceinline__ float mf_acos(float x) { return acosf(x); }
