@@ -75,8 +75,8 @@ __device__ __forceinline__ float mf_atan(float x) { return atanf(x); }
 __device__ __forceinline__ float mf_erf(float x) { return erff(x); }
 __device__ __forceinline__ float mf_pow(float x, float y) { return powf(x, y); }
 #else
-// FP64 exp, log and reciprocal of the per-event code, written out: the library's versions take
-// 42, 98 and 12 VALU instructions on gfx950 and were most of an FP64 event's work.  Each is
+// FP64 exp, log, reciprocal, sine and cosine of the per-event code, written out: the library's
+// exp, log and reciprocal take 42, 98 and 12 VALU instructions on gfx950 and were most of an FP64 event's work.  Each is
 // within about 2 ulp and branch-free (a branch to the library for special arguments, inlined at
 // every unrolled event, doubled the kernels' registers): special arguments are handled by
 // clamping and selects.
@@ -144,8 +144,54 @@ __device__ __forceinline__ double mf_log(double x) {
 __device__ __forceinline__ double mf_exp2(double x) { return exp2(x); }
 __device__ __forceinline__ double mf_log2(double x) { return log2(x); }
 __device__ __forceinline__ double mf_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ double mf_sin(double x) { return sin(x); }
-__device__ __forceinline__ double mf_cos(double x) { return cos(x); }
+// sine and cosine: Cody-Waite reduction x = k pi/2 + r, pi/2 split into three doubles (the split
+// is within 6e-50 of pi/2; x - k P1 is exact, being a multiple of 2^-53 below 1 in magnitude), so
+// |r| <= pi/4 + rounding for |x| < 2^50, beyond which the reduction loses accuracy; sin r and cos r
+// by their Taylor polynomials to degree 17 and 16 (truncation < 1e-18 relative at |r| = 0.8);
+// k mod 4 selects the function and the sign.  Infinite and NaN arguments give NaN.  The two share
+// every instruction up to the final selects, so a density with sin(a) and cos(a) pays for one.
+// MF_SINCOS_BEGIN
+__device__ __forceinline__ void mf_sincos_parts(double x, double& s, double& c, int& q) {
+  const double k = rint(x * 0x1.45f306dc9c883p-1);
+  double r = fma(-k, 0x1.921fb54442d18p+0, x);
+  r = fma(-k, 0x1.1a62633145c07p-54, r);
+  r = fma(-k, -0x1.f1976b7ed8fbcp-110, r);
+  const double z = r * r;
+  double ps = 1.0 / 355687428096000.0;  // 1/17!
+  ps = fma(ps, z, -1.0 / 1307674368000.0);
+  ps = fma(ps, z, 1.0 / 6227020800.0);
+  ps = fma(ps, z, -1.0 / 39916800.0);
+  ps = fma(ps, z, 1.0 / 362880.0);
+  ps = fma(ps, z, -1.0 / 5040.0);
+  ps = fma(ps, z, 1.0 / 120.0);
+  ps = fma(ps, z, -1.0 / 6.0);
+  s = fma(r * z, ps, r);
+  double pc = 1.0 / 20922789888000.0;  // 1/16!
+  pc = fma(pc, z, -1.0 / 87178291200.0);
+  pc = fma(pc, z, 1.0 / 479001600.0);
+  pc = fma(pc, z, -1.0 / 3628800.0);
+  pc = fma(pc, z, 1.0 / 40320.0);
+  pc = fma(pc, z, -1.0 / 720.0);
+  pc = fma(pc, z, 1.0 / 24.0);
+  c = fma(z * z, pc, fma(-0.5, z, 1.0));
+  const double kq = k - 4.0 * floor(0.25 * k);  // k mod 4, exact
+  q = kq == kq ? (int)kq : 0;
+}
+__device__ __forceinline__ double mf_sin(double x) {
+  double s, c;
+  int q;
+  mf_sincos_parts(x, s, c, q);
+  const double v = (q & 1) ? c : s;
+  return (q & 2) ? -v : v;
+}
+__device__ __forceinline__ double mf_cos(double x) {
+  double s, c;
+  int q;
+  mf_sincos_parts(x, s, c, q);
+  const double v = (q & 1) ? s : c;
+  return ((q + 1) & 2) ? -v : v;
+}
+// MF_SINCOS_END
 __device__ __forceinline__ double mf_tan(double x) { return tan(x); }
 __device__ __forceinline__ double mf_asin(double x) { return asin(x); }
 __device__ __forceinline__ double mf_acos(double x) { return acos(x); }
