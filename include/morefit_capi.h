@@ -37,7 +37,7 @@ typedef struct mfh_options {
   int block_size;          /* evaluation work-group size (multiple of 64) */
   int events_per_thread;   /* chunk = block_size * events_per_thread */
   int grid;                /* 0 = one work-group per chunk, else persistent grid */
-  int fast_math;           /* FP32 hardware exp/log/rcp/sin/cos */
+  int fast_math;           /* FP32 hardware exp/log/rcp/sqrt/sin/cos */
   int accumulate;          /* FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group then FP64 */
   int centre;              /* observables as offsets from the range centre (FP32 stored so, FP64 computed per event) */
   int unroll;              /* events per thread whose loads are issued together */

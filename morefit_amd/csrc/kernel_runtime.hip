@@ -11,7 +11,7 @@
 //   MF_NHOIST       hoisted parameter-only terms (FP64 on the device prep kernel)
 //   MF_BLOCK        work-group size of the evaluation kernels (multiple of 64)
 //   MF_ACC          0: FP64 accumulation per event, 1: compensated (Kahan) FP32 per lane
-//   MF_FAST_MATH    1: hardware exp2/log2/rcp/sin/cos approximations in FP32
+//   MF_FAST_MATH    1: hardware exp2/log2/rcp/sqrt/sin/cos approximations in FP32
 //   MF_GEN_LO<d>, MF_GEN_W<d>  generator proposal box per observable (stored coordinates)
 // and, after it, functor structs { static constexpr int nout; static void eval(x, p, o); },
 // mf_hoist_term(k, theta) and the kernel instantiations (MF_EVAL_KERNEL etc. below).  The
@@ -51,8 +51,8 @@ __device__ __forceinline__ float mf_rcp(float x) { return __builtin_amdgcn_rcpf(
 __device__ __forceinline__ float mf_exp2(float x) { return exp2f(x); }
 __device__ __forceinline__ float mf_log2(float x) { return log2f(x); }
 #endif
-__device__ __forceinline__ float mf_sqrt(float x) { return sqrtf(x); }
 #if MF_FAST_MATH
+__device__ __forceinline__ float mf_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }  // v_sqrt_f32, 1 ulp
 // The hardware sine / cosine (v_sin_f32 / v_cos_f32 take the argument in turns and are valid
 // for |turns| <= 256): one multiply and one quarter-rate instruction instead of the library's
 // range reduction and polynomials.  Beyond 256 turns (1608 rad) the fraction of the turn is
@@ -65,6 +65,7 @@ __device__ __forceinline__ float mf_turns(float x) {
 __device__ __forceinline__ float mf_sin(float x) { return __builtin_amdgcn_sinf(mf_turns(x)); }
 __device__ __forceinline__ float mf_cos(float x) { return __builtin_amdgcn_cosf(mf_turns(x)); }
 #else
+__device__ __forceinline__ float mf_sqrt(float x) { return sqrtf(x); }  // correctly rounded
 __device__ __forceinline__ float mf_sin(float x) { return sinf(x); }
 __device__ __forceinline__ float mf_cos(float x) { return cosf(x); }
 #endif

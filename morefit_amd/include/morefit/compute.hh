@@ -70,7 +70,7 @@ struct compute_options {
   // write-backs cost more than the final kernel's launch (measured 36.7 vs 19.0 us per
   // gradient call at N = 1e7)
   bool hip_fused_final = false;   // large segments: chunk count a multiple of the resident work-groups (measured: no gain)
-  bool hip_fast_math = true;         // FP32: hardware exp2/log2/rcp/sin/cos (error measured against the oracle)
+  bool hip_fast_math = true;         // FP32: hardware exp2/log2/rcp/sqrt/sin/cos (error measured against the oracle)
   int hip_accumulate = 2;            // FP32 per-lane sums: 0 FP64 per event, 1 Kahan FP32, 2 FP32 per unrolled group + FP64
   // observables as offsets from the range centre: FP32 kernels store them so, FP64 kernels
   // compute them per event from the data as given
