@@ -154,6 +154,30 @@ def test_fp64_base2_matches_oracle(model):
     cx.close()
 
 
+@pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
+def test_fp32_library_math_matches_oracle(model):
+    """FP32 kernels with the device library's exp, log, sqrt, sin and cos (option fast_math=0) in place of
+    the hardware instructions keep the FP32 bar, 2e-6 of the sum of absolute terms."""
+    n = 100000
+    data = O.generate(model, M.TRUTH[model], n, seed=2468, nstreams=16)
+    theta = np.array(M.START[model])
+    say("fp32 library math parity", "model", model, "n", n)
+    cx = M.Context(fast_math=0)
+    mdl = cx.model(model, fp32=True)
+    b = mdl.batch([n])
+    b.upload(0, data)
+    stored = _stored(model, data, True)
+    ref = O.nll_sums(model, stored, theta, mode=2, nthreads=0)
+    absref = O.nll_sums(model, stored, theta, mode=12, nthreads=4)
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        out = b.eval(kind, theta)[0]
+        k = len(out)
+        assert np.all(np.abs(out - ref[:k]) <= 2e-6 * absref[:k]), (kind, out - ref[:k], absref[:k])
+    b.close()
+    mdl.close()
+    cx.close()
+
+
 @pytest.mark.parametrize("pivot", [1, 0])
 @pytest.mark.parametrize("fp32", [False, True])
 def test_exponential_pivot_matches_oracle(fp32, pivot):
