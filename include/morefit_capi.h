@@ -14,6 +14,8 @@
  *   mfh_fit_plain        fitter::fit with the drop-in HIPBlock (FP64 or FP32), one launch per call
  *   mfh_fit_cpu          fitter::fit with the host-compiled CPU backend (reference LLVM semantics)
  *   mfh_fit_weighted     fitter::fit of weighted events with correct_weighted_fit (fitter.hh:894-947, 1327-1432)
+ *   mfh_fit_options      fitter::fit with analytic_gradient / analytic_hessian / kahan_on_accelerator /
+ *                        postrun_hesse switched (fitter.hh:38-106, 796-814, 1110-1124)
  *   mfh_batch_fit        toystudy::toy's fit loop (toystudy.hh:55-111) as one batched study
  *   mfh_serve_run        no counterpart: continuous-batching fit service (serve.hh)
  * Status: 0 on success, else nonzero with mfh_last_error().
@@ -171,6 +173,17 @@ int mfh_fit_cpu(int kind, int fp32, const double* data, long long n, const doubl
 int mfh_fit_weighted(mfh_context* ctx, int kind, int fp32, const double* data, const double* weights, long long n, const double* start,
                      int nthreads, double* values, double* errors, mfh_fit_out* out, double* ms);
 /* -2lnL, gradient (-2 scale) and Hessian through the CPU backend at theta (no device needed) */
+/* fitter::fit with the reference's fitter_options switches (fitter.hh:38-106; its mass benchmark
+ * runs analytic_gradient = analytic_hessian = false, benchmarking/morefit_massfit.cc:84-95):
+ * flags is an OR of MFH_FIT_* bits; ctx NULL fits on the host CPU backend with nthreads. */
+#define MFH_FIT_ANALYTIC_GRADIENT 1
+#define MFH_FIT_ANALYTIC_HESSIAN 2
+#define MFH_FIT_KAHAN_ON_ACCELERATOR 4
+#define MFH_FIT_POSTRUN_HESSE 8
+#define MFH_FIT_ALL 15
+#define MFH_FIT_DEFAULT MFH_FIT_ALL
+int mfh_fit_options(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, int nthreads, int flags,
+                    double* values, double* errors, mfh_fit_out* out, double* ms);
 int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess);
 
 #ifdef __cplusplus

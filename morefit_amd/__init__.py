@@ -78,7 +78,7 @@ CAPI_SYMBOLS = [
     "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states", "mfh_autotune",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
-    "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted",
+    "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted", "mfh_fit_options",
 ]
 
 
@@ -141,6 +141,7 @@ def libs():
         L.mfh_fit_cpu.argtypes = [ip, ip, dp, ctypes.c_longlong, dp, ip, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_fit_weighted.argtypes = [vp, ip, ip, dp, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_eval_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
+        L.mfh_fit_options.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, ip, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_ulonglong), dp, ip,
                                     ctypes.c_double, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_int), dp]
@@ -439,6 +440,24 @@ def fit_weighted(kind, data, weights, start, ctx=None, fp32=False, nthreads=1):
     h = ctx.h if ctx is not None else None
     _check(libs()[1].mfh_fit_weighted(h, kind, int(fp32), _dp(data), _dp(w), data.shape[1], _dp(_f64(start)), nthreads,
                                       _dp(vals), _dp(errs), ctypes.byref(out), ctypes.byref(ms)))
+    return vals, errs, out, ms.value
+
+
+FIT_ANALYTIC_GRADIENT, FIT_ANALYTIC_HESSIAN, FIT_KAHAN_ON_ACCELERATOR, FIT_POSTRUN_HESSE = 1, 2, 4, 8
+FIT_DEFAULT = 15
+
+
+def fit_options(kind, data, start, flags=FIT_DEFAULT, ctx=None, fp32=False, nthreads=1):
+    """fitter::fit with the fitter_options switches of the FIT_* bits (analytic gradient,
+    analytic Hessian, sums on the accelerator, HESSE after MIGRAD): through the device's
+    drop-in path when ctx is given, else the CPU backend."""
+    data = _f64(np.atleast_2d(data))
+    P = NTHETA[kind]
+    vals, errs = np.zeros(P), np.zeros(P)
+    out, ms = FitOut(), ctypes.c_double(0)
+    h = ctx.h if ctx is not None else None
+    _check(libs()[1].mfh_fit_options(h, kind, int(fp32), _dp(data), data.shape[1], _dp(_f64(start)), nthreads, int(flags),
+                                     _dp(vals), _dp(errs), ctypes.byref(out), ctypes.byref(ms)))
     return vals, errs, out, ms.value
 
 

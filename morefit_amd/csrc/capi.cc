@@ -711,7 +711,7 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
 namespace {
 template <typename K, typename Backend, typename Block>
 void run_fit(Backend* be, int kind, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
-             double* ms, const double* weights = nullptr, bool optimize_dimensions = false) {
+             double* ms, const double* weights = nullptr, bool optimize_dimensions = false, int flags = MFH_FIT_DEFAULT) {
   ModelDef<K> def(kind, nullptr);
   // weights ride along as an extra column named "w" that the fitter multiplies log p by
   dimension<double> wdim("w", -1e300, 1e300);
@@ -731,6 +731,10 @@ void run_fit(Backend* be, int kind, const double* data, long long n, const doubl
   fitter_options fo;
   fo.analytic_fisher = weights != nullptr;
   fo.optimize_dimensions = optimize_dimensions;
+  fo.analytic_gradient = (flags & MFH_FIT_ANALYTIC_GRADIENT) != 0;
+  fo.analytic_hessian = (flags & MFH_FIT_ANALYTIC_HESSIAN) != 0;
+  fo.kahan_on_accelerator = (flags & MFH_FIT_KAHAN_ON_ACCELERATOR) != 0;
+  fo.postrun_hesse = (flags & MFH_FIT_POSTRUN_HESSE) != 0;
   fitter<K, double, Backend, Block> f(&fo, be);
   f.fit(def.pdf, def.params, &ev);
   for (size_t k = 0; k < def.params.size(); k++) {
@@ -786,6 +790,23 @@ int mfh_fit_weighted(mfh_context* ctx, int kind, int fp32, const double* data, c
     CPUBackend be(&co);
     if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms, weights);
     else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms, weights);
+  }
+  MFH_CATCH
+}
+
+int mfh_fit_options(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, int nthreads, int flags,
+                    double* values, double* errors, mfh_fit_out* out, double* ms) {
+  MFH_TRY
+  if (flags & ~MFH_FIT_ALL) throw std::invalid_argument("mfh_fit_options: unknown flag bits");
+  if (ctx) {
+    if (fp32) run_fit<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms, nullptr, false, flags);
+    else run_fit<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, data, n, start, values, errors, out, ms, nullptr, false, flags);
+  } else {
+    compute_options co;
+    co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
+    CPUBackend be(&co);
+    if (fp32) run_fit<float, CPUBackend, CPUBlock<float, double>>(&be, kind, data, n, start, values, errors, out, ms, nullptr, false, flags);
+    else run_fit<double, CPUBackend, CPUBlock<double, double>>(&be, kind, data, n, start, values, errors, out, ms, nullptr, false, flags);
   }
   MFH_CATCH
 }

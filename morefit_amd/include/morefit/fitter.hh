@@ -10,9 +10,12 @@
  *  likelihood    -2 sum log p (+ extended term)                         (fitter.hh:1152-1220)
  *  likelihood_and_gradient  -2 sums, zeros for fixed parameters          (fitter.hh:1054-1151)
  *  hessian       -2 sum d2 log p into h[i + j*n] (+2N/nu^2 for yields)  (fitter.hh:1221-1326)
- *  fit           MIGRAD (strategy 2, analytic gradient and Hessian, max 4000 calls,
- *                tolerance 0.01, errordef 1) then HESSE; values and errors written back
- *                into the parameters                                     (fitter.hh:660-1002)
+ *  fit           MIGRAD (strategy 2, max 4000 calls, tolerance 0.01, errordef 1) then
+ *                HESSE; values and errors written back into the parameters (fitter.hh:660-1002).
+ *                analytic_gradient / analytic_hessian = false build no gradient / Hessian
+ *                kernel and MIGRAD and HESSE differentiate numerically (fitter.hh:796-814);
+ *                kahan_on_accelerator = false stores per-event outputs and sums them on the
+ *                host with Kahan's algorithm (fitter.hh:1110-1124)
  *
  * The minimiser is this package's Minuit-algorithm MIGRAD (minimizer.hh); the vendored
  * Minuit2 of the reference is not part of this build.  The Hessian kernel here writes the
@@ -28,6 +31,7 @@
 #include <iomanip>
 #include <iostream>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -97,6 +101,12 @@ class fitter {
   std::vector<double> centres_;
   size_t nevents_ = 0;
   bool weighted_ = false;
+  bool kernels_ready_ = false;
+  // kahan_on_accelerator = false: per-event outputs of the value, gradient, Hessian and Fisher
+  // kernels copied back here and summed on the host
+  int nout_[4] = {0, 0, 0, 0};
+  std::vector<dimension<evalT>> res_dims_;
+  std::unique_ptr<EventVector<kernelT, evalT>> res_buffer_[4];
   // results of the last fit
   int status_ = -1;
   double fval_ = NAN, edm_ = NAN;
@@ -177,11 +187,17 @@ class fitter {
       return r;
     };
     dimension<evalT> res("res", 0.0, 1.0);
+    // per-event outputs summed on the device, or stored and summed on the host
+    const bool kahan = opts_->kahan_on_accelerator;
+    nout_[0] = 1;
+    nout_[1] = opts_->analytic_gradient ? 1 + (int)d1.size() : 0;
+    nout_[2] = opts_->analytic_hessian ? (int)(d1.size() * (d1.size() + 1) / 2) : 0;
+    nout_[3] = opts_->analytic_fisher ? (int)(d1.size() * (d1.size() + 1) / 2) : 0;
     {
       std::vector<Ptr> g;
       g.emplace_back(lp->copy());
       auto k = hoisted(g, buffer_names_, buffer_expressions_, "morefit_parambuffer_");
-      block_.MakeComputeKernel("lh_kernel", nevents_, in_sig, {res}, param_sig(buffer_names_), k, true);
+      block_.MakeComputeKernel("lh_kernel", nevents_, in_sig, {res}, param_sig(buffer_names_), k, kahan);
     }
     if (opts_->analytic_gradient) {
       std::vector<Ptr> g;
@@ -189,7 +205,7 @@ class fitter {
       for (auto& d : d1) g.emplace_back(d->copy());
       auto k = hoisted(g, grad_buffer_names_, grad_buffer_expressions_, "morefit_parambuffer_");
       std::vector<dimension<evalT>> out(k.size(), res);
-      grad_block_.MakeComputeKernel("lh_grad_kernel", nevents_, in_sig, out, param_sig(grad_buffer_names_), k, true);
+      grad_block_.MakeComputeKernel("lh_grad_kernel", nevents_, in_sig, out, param_sig(grad_buffer_names_), k, kahan);
     }
     if (opts_->analytic_hessian) {
       std::vector<Ptr> g;
@@ -197,7 +213,7 @@ class fitter {
         for (size_t j = 0; j <= i; j++) g.emplace_back(factor_common(*d1[i]->diff(floating_names_[j])->simplify()));
       auto k = hoisted(g, hessian_buffer_names_, hessian_buffer_expressions_, "morefit_parambuffer_");
       std::vector<dimension<evalT>> out(k.size(), res);
-      hessian_block_.MakeComputeKernel("lh_hessian_kernel", nevents_, in_sig, out, param_sig(hessian_buffer_names_), k, true);
+      hessian_block_.MakeComputeKernel("lh_hessian_kernel", nevents_, in_sig, out, param_sig(hessian_buffer_names_), k, kahan);
     }
     // Fisher information (fitter.hh:471-497): per event (w d_i log p)(w d_j log p), i >= j, from
     // the derivatives of the weighted log-density
@@ -207,7 +223,7 @@ class fitter {
         for (size_t j = 0; j <= i; j++) g.emplace_back((d1[i]->copy() * d1[j]->copy())->simplify());
       auto k = hoisted(g, fisher_buffer_names_, fisher_buffer_expressions_, "morefit_parambuffer_");
       std::vector<dimension<evalT>> out(k.size(), res);
-      fisher_block_.MakeComputeKernel("lh_fisher_kernel", nevents_, in_sig, out, param_sig(fisher_buffer_names_), k, true);
+      fisher_block_.MakeComputeKernel("lh_fisher_kernel", nevents_, in_sig, out, param_sig(fisher_buffer_names_), k, kahan);
     }
     return true;
   }
@@ -256,6 +272,22 @@ class fitter {
       fisher_block_.SetNevents((int)data->nevents(), (int)data->nevents_padded());
       fisher_block_.SetupParameterBuffer((floating_names_.size() + fisher_buffer_names_.size() + 1) * sizeof(kernelT));
     }
+    if (!opts_->kahan_on_accelerator) {
+      // host result buffers of nout columns, the blocks' output buffers of the same layout
+      // (fitter.hh:583-603)
+      int most = 0;
+      for (int k = 0; k < 4; k++) most = std::max(most, nout_[k]);
+      res_dims_.assign(most, dimension<evalT>("res", 0.0, 1.0));
+      computeT* blocks[4] = {&block_, &grad_block_, &hessian_block_, &fisher_block_};
+      for (int k = 0; k < 4; k++) {
+        res_buffer_[k].reset();
+        if (nout_[k] == 0) continue;
+        std::vector<dimension<evalT>*> dims;
+        for (int j = 0; j < nout_[k]; j++) dims.push_back(&res_dims_[j]);
+        res_buffer_[k].reset(new EventVector<kernelT, evalT>(dims, data->nevents(), data->is_padded(), data->padding()));
+        blocks[k]->SetupOutputBuffer(res_buffer_[k]->buffer_size());
+      }
+    }
   }
 
   template <typename dataT>
@@ -264,7 +296,8 @@ class fitter {
     params_ = params;
     nevents_ = data->nevents();
     weighted_ = data->has_event_weight();
-    if (!refit) make_kernels(pdf, params, data->get_dimensions(), data->event_weight_name());
+    if (!refit || !kernels_ready_) make_kernels(pdf, params, data->get_dimensions(), data->event_weight_name());
+    kernels_ready_ = true;
     bind_data(data);
     return true;
   }
@@ -293,14 +326,27 @@ class fitter {
       if (q->get_name() == p->get_name()) return true;
     return false;
   }
-  template <typename B>
-  std::vector<evalT> run(B& b, const std::vector<Ptr>& exprs) {
+  // one launch of block `which` (0 value, 1 gradient, 2 Hessian, 3 Fisher): its per-output
+  // sums, from the device's compensated reduction or from a host Kahan sum of the stored
+  // per-event outputs (kahan_on_accelerator = false, fitter.hh:1110-1124)
+  std::vector<evalT> run(int which, const std::vector<Ptr>& exprs) {
+    static const char* what[4] = {"likelihood", "gradient (analytic_gradient is off)", "Hessian (analytic_hessian is off)",
+                                  "Fisher matrix (analytic_fisher is off)"};
+    if (!kernels_ready_ || nout_[which] == 0) throw std::runtime_error(std::string("fitter: no ") + what[which] + " kernel");
+    computeT* blocks[4] = {&block_, &grad_block_, &hessian_block_, &fisher_block_};
+    computeT& b = *blocks[which];
     auto t0 = std::chrono::steady_clock::now();
     b.CopyToParameterBuffer(parameter_buffer(exprs));
     b.SubmitKernel();
     b.Finish();
     std::vector<evalT> sums;
-    b.CopyFromKahanBuffer(sums);
+    if (opts_->kahan_on_accelerator) {
+      b.CopyFromKahanBuffer(sums);
+    } else {
+      EventVector<kernelT, evalT>& r = *res_buffer_[which];
+      b.CopyFromOutputBuffer(r);
+      for (int k = 0; k < nout_[which]; k++) sums.push_back(kahan_summation<evalT, kernelT>(r, k));
+    }
     kernel_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return sums;
   }
@@ -308,7 +354,7 @@ class fitter {
   double likelihood() { return likelihood(pdf_, params_, (EventVector<kernelT, evalT>*)nullptr); }
   template <typename dataT = kernelT>
   evalT likelihood(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data) {
-    auto s = run(block_, buffer_expressions_);
+    auto s = run(0, buffer_expressions_);
     double r = s.at(0);
     if (pdf_->is_extended()) {
       double nu = total_yield();
@@ -320,7 +366,7 @@ class fitter {
   template <typename dataT = kernelT>
   evalT likelihood_and_gradient(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, int npar,
                                 double* grad) {
-    auto s = run(grad_block_, grad_buffer_expressions_);
+    auto s = run(1, grad_buffer_expressions_);
     double r = -2.0 * s.at(0);
     int k = 0;
     for (size_t i = 0; i < params_.size(); i++) grad[i] = params_[i]->is_constant() ? 0.0 : -2.0 * s.at(1 + k++);
@@ -335,7 +381,7 @@ class fitter {
   bool hessian(int npar, double* h) { return hessian(pdf_, params_, (EventVector<kernelT, evalT>*)nullptr, npar, h); }
   template <typename dataT = kernelT>
   bool hessian(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, int npar, double* h) {
-    auto s = run(hessian_block_, hessian_buffer_expressions_);
+    auto s = run(2, hessian_buffer_expressions_);
     const size_t N = params_.size();
     for (size_t i = 0; i < N * N; i++) h[i] = 0.0;
     std::vector<int> fl;
@@ -362,7 +408,7 @@ class fitter {
   // yields of an extended density
   bool fisher(int npar, double* f) {
     if (!opts_->analytic_fisher) throw std::runtime_error("fitter::fisher: analytic_fisher is off");
-    auto s = run(fisher_block_, fisher_buffer_expressions_);
+    auto s = run(3, fisher_buffer_expressions_);
     const size_t N = params_.size();
     for (size_t i = 0; i < N * N; i++) f[i] = 0.0;
     std::vector<int> fl;
@@ -429,12 +475,17 @@ class fitter {
       return f->likelihood_and_gradient((int)f->params_.size(), g);
     }
     bool hessian(const double* x, double* h) override {
+      if (!has_hessian()) return false;
       f->update_parameters(x);
       return f->hessian((int)f->params_.size(), h);
     }
     double value_noise() const override {
       return std::is_same<kernelT, float>::value ? fp32_sum_noise((double)f->nevents_) : 0.0;
     }
+    // without the kernels MIGRAD and HESSE differentiate numerically (Minuit2 with an FCN that
+    // has no gradient, or no Hessian function set, fitter.hh:796-814)
+    bool has_gradient() const override { return f->opts_->analytic_gradient; }
+    bool has_hessian() const override { return f->opts_->analytic_hessian; }
   };
 
   std::vector<MinimizerParam> minimizer_params(bool reset) {
@@ -474,7 +525,7 @@ class fitter {
     Fcn fcn(this);
     m.minimize(fcn);
     update_parameters(m.x.data());
-    if (opts_->postrun_hesse && opts_->analytic_hessian) m.hesse(fcn);
+    if (opts_->postrun_hesse) m.hesse(fcn);
     if (weighted_ && opts_->correct_weighted_fit) {
       if (!(opts_->analytic_fisher && opts_->analytic_hessian))
         throw std::runtime_error("correction for weighted fit only supported with analytic hessian and fisher matrix");

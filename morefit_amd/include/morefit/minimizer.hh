@@ -18,7 +18,12 @@
  *  - polish (on by default, not in MINUIT): from the converged point, Newton steps with that
  *    exact Hessian down to EDM 1e-10, so the result is the minimum itself and not any point
  *    inside MIGRAD's EDM tolerance (which allows ~6e-3 sigma);
- *  - HESSE: covariance = 2 up H^-1 in the internal space, mapped back through the Jacobian.
+ *  - HESSE: covariance = 2 up H^-1 in the internal space, mapped back through the Jacobian;
+ *  - objectives without a gradient or a Hessian (the reference fitter's analytic_gradient /
+ *    analytic_hessian = false, fitter.hh:796-814: Minuit2 then differentiates numerically):
+ *    central differences in the internal space, steps a fraction of each parameter's error
+ *    estimate (refreshed from every second difference), the Hessian from differences of the
+ *    analytic gradient where there is one, else of the value.
  *
  * The function interface is synchronous (value, value+gradient, Hessian).  The batched toy
  * engine runs one instance per toy in a fibre and answers the calls from shared launches.
@@ -44,6 +49,12 @@ struct MinimizerFcn {
   // rounding noise of value() (e.g. FP32 per-event sums); value differences below it carry
   // no information and the search then relies on the gradient
   virtual double value_noise() const { return 0.0; }
+  // false: value_grad() is not available and the minimiser differentiates value() numerically
+  // (the reference's analytic_gradient = false, a Minuit2 FCN without a gradient)
+  virtual bool has_gradient() const { return true; }
+  // false: hessian() is not available; the metric seed, the strategy-2 confirmation and HESSE
+  // use second differences (of the analytic gradient where there is one, else of value())
+  virtual bool has_hessian() const { return true; }
 };
 
 // Rounding floor, in -2 lnL units, of a likelihood summed in FP32 over n events.  The kernel
@@ -139,6 +150,12 @@ class Migrad {
   bool polish = true;
   double polish_edm = 1e-10;
   int polish_steps = 4;
+  // numerical derivatives, for objectives without a gradient or a Hessian: central differences
+  // in the internal space with steps of these fractions of each parameter's current error
+  // estimate (sqrt(2 up / d2f/du2), first from the parameter's step size), raised where the
+  // value's rounding noise would otherwise dominate the difference
+  double grad_step_frac = 0.01;
+  double hess_step_frac = 0.05;
 
   // results
   std::vector<double> x;      // external values (all parameters)
@@ -198,40 +215,23 @@ class Migrad {
     const int n = nfree();
     status = -1;
     valid = false;
-    std::vector<double> u(n), g(n), V(n * n), xe(x);
+    std::vector<double> u(n), g(n), V(n * n);
     std::vector<double> d(n), u1(n), g1(n), s(n), y(n), Vy(n);
     for (int k = 0; k < n; k++) u[k] = ext2int(k, p_[free_[k]].value);
-    auto to_ext = [&](const std::vector<double>& uu) {
-      for (int k = 0; k < n; k++) xe[free_[k]] = int2ext(k, uu[k]);
-    };
-    std::vector<double> ge(p_.size()), he(p_.size() * p_.size());
+    init_sigma(u);
+    // value and internal gradient (analytic, or central differences of the value)
     auto eval_fg = [&](const std::vector<double>& uu, std::vector<double>& gg) {
-      to_ext(uu);
-      double f = fcn.value_grad(xe.data(), ge.data());
-      ncalls++;
-      ngrad++;
-      for (int k = 0; k < n; k++) gg[k] = ge[free_[k]] * dext(k, uu[k]);
+      if (fcn.has_gradient()) return grad_at(fcn, uu, gg);
+      double f = value_at(fcn, uu);
+      if (std::isfinite(f)) numeric_gradient(fcn, uu, f, gg);
       return f;
     };
-    auto eval_f = [&](const std::vector<double>& uu) {
-      to_ext(uu);
-      ncalls++;
-      return fcn.value(xe.data());
-    };
-    // internal Hessian -> V = H^-1 (shifted to positive-definite); false on failure
-    auto seed_metric = [&](const std::vector<double>& uu, const std::vector<double>& gg) {
-      to_ext(uu);
-      if (!fcn.hessian(xe.data(), he.data())) return false;
-      nhess++;
-      ncalls++;
-      std::vector<double> H(n * n);
-      const int N = (int)p_.size();
-      for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) {
-          double v = he[free_[i] * N + free_[j]] * dext(i, uu[i]) * dext(j, uu[j]);
-          if (i == j && dext(i, uu[i]) != 0.0) v += (gg[i] / dext(i, uu[i])) * d2ext(i, uu[i]);
-          H[i * n + j] = v;
-        }
+    auto eval_f = [&](const std::vector<double>& uu) { return value_at(fcn, uu); };
+    // internal Hessian at uu (value ff, internal gradient gg there) -> V = H^-1 (shifted to
+    // positive-definite); false on failure
+    auto seed_metric = [&](const std::vector<double>& uu, const std::vector<double>& gg, double ff) {
+      std::vector<double> H;
+      if (!internal_hessian(fcn, uu, gg, ff, H)) return false;
       last_hint_ = H;
       std::vector<double> Hi = H;
       if (!la::spd_inverse(Hi, n)) {
@@ -257,7 +257,7 @@ class Migrad {
 
     double f = eval_fg(u, g);
     if (!std::isfinite(f)) return finish(4, f, NAN, u, n);
-    if (!seed_metric(u, g)) {  // diagonal fallback from the step sizes
+    if (!seed_metric(u, g, f)) {  // diagonal fallback from the step sizes
       std::fill(V.begin(), V.end(), 0.0);
       for (int k = 0; k < n; k++) {
         double s = p_[free_[k]].step / std::max(1e-300, std::fabs(dext(k, u[k])));
@@ -291,7 +291,7 @@ class Migrad {
       if (e < edm_goal) {
         if (strategy >= 2 && !confirmed) {  // confirm with a fresh Hessian
           confirmed = true;
-          if (seed_metric(u, g)) {
+          if (seed_metric(u, g, f)) {
             e = compute_edm(g);
             if (e < edm_goal) {
               if (polish) polish_at_minimum();
@@ -301,7 +301,7 @@ class Migrad {
             break;
           }
         } else {
-          if (polish && seed_metric(u, g)) {
+          if (polish && seed_metric(u, g, f)) {
             e = compute_edm(g);
             polish_at_minimum();
           }
@@ -318,7 +318,7 @@ class Migrad {
         slope += g[i] * t;
       }
       if (!(slope < 0.0)) {  // not a descent direction: restart from the Hessian
-        if (!seed_metric(u, g)) return finish(3, f, e, u, n);
+        if (!seed_metric(u, g, f)) return finish(3, f, e, u, n);
         continue;
       }
       // Full Newton step first, with value and gradient in one call, accepted on sufficient
@@ -372,14 +372,14 @@ class Migrad {
         if (best_a == 0.0) {
           // no acceptable point along the Newton direction.  Inside the value's noise floor
           // the gradient decides: Newton steps that lower EDM; otherwise refresh the metric once.
-          if (noisy && seed_metric(u, g)) {
+          if (noisy && seed_metric(u, g, f)) {
             e = compute_edm(g);
             polish_at_minimum();
             return finish(e < edm_goal ? 0 : 2, f, e, u, n);
           }
           if (!refreshed_) {
             refreshed_ = true;
-            if (seed_metric(u, g)) { e = compute_edm(g); continue; }
+            if (seed_metric(u, g, f)) { e = compute_edm(g); continue; }
           }
           return finish(e < edm_goal ? 0 : 2, f, e, u, n);
         }
@@ -412,23 +412,21 @@ class Migrad {
     return finish(0, f, e, u, n);
   }
 
-  // errors and covariance from the analytic Hessian at the current point
+  // errors and covariance from the Hessian at the current point (analytic, or second
+  // differences when the objective has none)
   bool hesse(MinimizerFcn& fcn) {
-    const int n = nfree(), N = (int)p_.size();
-    std::vector<double> u(n), ge(N), he(N * N);
+    const int n = nfree();
+    std::vector<double> u(n), gi(n), H;
     for (int k = 0; k < n; k++) u[k] = ext2int(k, x[free_[k]]);
-    double f = fcn.value_grad(x.data(), ge.data());
-    ncalls++;
-    if (!fcn.hessian(x.data(), he.data())) return false;
-    nhess++;
-    ncalls++;
-    std::vector<double> H(n * n);
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) {
-        double v = he[free_[i] * N + free_[j]] * dext(i, u[i]) * dext(j, u[j]);
-        if (i == j) v += ge[free_[i]] * d2ext(i, u[i]);
-        H[i * n + j] = v;
-      }
+    if ((int)sig_.size() != n) init_sigma(u);
+    double f;
+    if (fcn.has_gradient()) {
+      f = grad_at(fcn, u, gi);
+    } else {
+      f = value_at(fcn, u);
+      if (std::isfinite(f)) numeric_gradient(fcn, u, f, gi);
+    }
+    if (!std::isfinite(f) || !internal_hessian(fcn, u, gi, f, H)) return false;
     std::vector<double> C = H;
     cov_accurate = la::spd_inverse(C, n);
     if (!cov_accurate) return false;
@@ -438,7 +436,7 @@ class Migrad {
     for (int k = 0; k < n; k++) errors[free_[k]] = std::sqrt(cov[k * n + k]);
     double e = 0.0;
     for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) e += ge[free_[i]] * dext(i, u[i]) * C[i * n + j] * ge[free_[j]] * dext(j, u[j]);
+      for (int j = 0; j < n; j++) e += gi[i] * C[i * n + j] * gi[j];
     edm = 0.5 * e;
     fval = f;
     return true;
@@ -450,6 +448,136 @@ class Migrad {
   std::vector<double> last_hint_;
   bool pd_forced_ = false;
   bool refreshed_ = false;
+  std::vector<double> sig_;  // internal error estimate per floating parameter (numerical steps)
+  std::vector<double> xw_;   // external parameter vector handed to the objective
+
+  bool bounded(int k) const { return p_[free_[k]].has_lo || p_[free_[k]].has_hi; }
+  // the transformed (sin / sqrt) internal coordinates have no meaningful scale above ~1
+  void set_sigma(int k, double v) {
+    if (!(std::isfinite(v) && v > 0.0)) return;
+    sig_[k] = bounded(k) ? std::min(v, 1.0) : v;
+  }
+  void init_sigma(const std::vector<double>& uu) {
+    const int n = nfree();
+    sig_.assign(n, 1.0);
+    for (int k = 0; k < n; k++) set_sigma(k, p_[free_[k]].step / std::max(1e-300, std::fabs(dext(k, uu[k]))));
+  }
+  const double* ext_of(const std::vector<double>& uu) {
+    xw_ = x;
+    for (int k = 0; k < nfree(); k++) xw_[free_[k]] = int2ext(k, uu[k]);
+    return xw_.data();
+  }
+  double value_at(MinimizerFcn& fcn, const std::vector<double>& uu) {
+    ncalls++;
+    return fcn.value(ext_of(uu));
+  }
+  // value and analytic gradient, chained into the internal space
+  double grad_at(MinimizerFcn& fcn, const std::vector<double>& uu, std::vector<double>& gg) {
+    std::vector<double> ge(p_.size());
+    const double f = fcn.value_grad(ext_of(uu), ge.data());
+    ncalls++;
+    ngrad++;
+    for (int k = 0; k < nfree(); k++) gg[k] = ge[free_[k]] * dext(k, uu[k]);
+    return f;
+  }
+  // Rounding floor of value differences: the objective's own noise, at least 8 ulp of |f|.
+  // With f ~ f0 + up (du/sigma)^2 a central difference over h = a sigma has a gradient error of
+  // noise / (2 a up) sigma and a curvature error of 2 noise / (a^2 up) relative: the step
+  // fractions are raised until those stay below 1e-4 sigma and 1e-3 (at most a whole sigma).
+  double noise_floor(const MinimizerFcn& fcn, double f) const {
+    return std::max(fcn.value_noise(), 8.0 * std::numeric_limits<double>::epsilon() * (std::fabs(f) + up));
+  }
+  double grad_frac(double noise) const { return std::min(1.0, std::max(grad_step_frac, 5000.0 * noise / up)); }
+  double hess_frac(double noise) const { return std::min(1.0, std::max(hess_step_frac, std::sqrt(2000.0 * noise / up))); }
+  void note_curvature(int k, double g2) {
+    if (std::isfinite(g2) && g2 > 0.0) set_sigma(k, std::sqrt(2.0 * up / g2));
+  }
+  // internal gradient of value() at uu (value f0 there) by central differences, 2 n calls;
+  // the second differences refresh the step scales
+  void numeric_gradient(MinimizerFcn& fcn, const std::vector<double>& uu, double f0, std::vector<double>& gg) {
+    const int n = nfree();
+    std::vector<double> w(uu);
+    const double a = grad_frac(noise_floor(fcn, f0));
+    for (int k = 0; k < n; k++) {
+      const double h = a * sig_[k];
+      w[k] = uu[k] + h;
+      const double fp = value_at(fcn, w);
+      w[k] = uu[k] - h;
+      const double fm = value_at(fcn, w);
+      w[k] = uu[k];
+      gg[k] = (fp - fm) / (2.0 * h);
+      note_curvature(k, (fp + fm - 2.0 * f0) / (h * h));
+    }
+    ngrad++;
+  }
+  // Internal Hessian at uu (value f0, internal gradient gg there).  Analytic: the external
+  // Hessian through the Jacobian plus the transform's curvature times the gradient.  Without
+  // one: central differences of the analytic gradient (2 n calls), or of the value (diagonal
+  // from f(u +- h_k), off-diagonal from the four corners f(u +- h_j +- h_k)), symmetrised.
+  bool internal_hessian(MinimizerFcn& fcn, const std::vector<double>& uu, const std::vector<double>& gg, double f0,
+                        std::vector<double>& H) {
+    const int n = nfree();
+    const int N = (int)p_.size();
+    H.assign((size_t)n * n, 0.0);
+    if (fcn.has_hessian()) {
+      std::vector<double> he((size_t)N * N);
+      if (!fcn.hessian(ext_of(uu), he.data())) return false;
+      nhess++;
+      ncalls++;
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+          double v = he[free_[i] * N + free_[j]] * dext(i, uu[i]) * dext(j, uu[j]);
+          if (i == j && dext(i, uu[i]) != 0.0) v += (gg[i] / dext(i, uu[i])) * d2ext(i, uu[i]);
+          H[i * n + j] = v;
+        }
+    } else {
+      const double a = hess_frac(noise_floor(fcn, f0));
+      std::vector<double> h(n), w(uu);
+      for (int k = 0; k < n; k++) h[k] = a * sig_[k];
+      if (fcn.has_gradient()) {
+        std::vector<double> gp(n), gm(n);
+        for (int k = 0; k < n; k++) {
+          w[k] = uu[k] + h[k];
+          const double fp = grad_at(fcn, w, gp);
+          w[k] = uu[k] - h[k];
+          const double fm = grad_at(fcn, w, gm);
+          w[k] = uu[k];
+          if (!std::isfinite(fp) || !std::isfinite(fm)) return false;
+          for (int j = 0; j < n; j++) H[j * n + k] = (gp[j] - gm[j]) / (2.0 * h[k]);
+        }
+      } else {
+        std::vector<double> fp(n), fm(n);
+        for (int k = 0; k < n; k++) {
+          w[k] = uu[k] + h[k];
+          fp[k] = value_at(fcn, w);
+          w[k] = uu[k] - h[k];
+          fm[k] = value_at(fcn, w);
+          w[k] = uu[k];
+          H[k * n + k] = (fp[k] + fm[k] - 2.0 * f0) / (h[k] * h[k]);
+        }
+        for (int k = 0; k < n; k++)
+          for (int j = 0; j < k; j++) {
+            double c[4];
+            const int sk[4] = {1, 1, -1, -1}, sj[4] = {1, -1, 1, -1};
+            for (int q = 0; q < 4; q++) {
+              w[k] = uu[k] + sk[q] * h[k];
+              w[j] = uu[j] + sj[q] * h[j];
+              c[q] = value_at(fcn, w);
+            }
+            w[k] = uu[k];
+            w[j] = uu[j];
+            H[k * n + j] = H[j * n + k] = (c[0] - c[1] - c[2] + c[3]) / (4.0 * h[k] * h[j]);
+          }
+      }
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++) H[i * n + j] = H[j * n + i] = 0.5 * (H[i * n + j] + H[j * n + i]);
+      nhess++;
+    }
+    for (int i = 0; i < n * n; i++)
+      if (!std::isfinite(H[i])) return false;
+    for (int k = 0; k < n; k++) note_curvature(k, H[k * n + k]);
+    return true;
+  }
 
   // Zero of the directional derivative phi'(a) = g(u + a d) . d by safeguarded secant steps,
   // starting from phi'(0) = slope0 < 0 and the probe (a1, f_a1, s_a1).  Accepts a point whose
