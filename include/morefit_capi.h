@@ -61,6 +61,8 @@ typedef struct mfh_fit_out {
   int ncalls;
   int iterations;
   int hesse_ok;
+  int thread_moved;  /* batched fits: the fit's fibre resumed on another OS thread (never expected) */
+  int pad;
 } mfh_fit_out;
 
 typedef struct mfh_batch_stats {
@@ -151,11 +153,15 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
  * generated on the device from `seeds` at the class's generation values theta_gen (per class,
  * ntheta values) and are fitted from `start` (per class).  Per request: timings[3] =
  * arrival, admission, completion (s); values/errors (max ntheta per request, NaN-padded);
- * status.  stats[0..6] = ticks, launches, peak live fits, rejected, wall s, peak bytes,
- * generation ms. */
+ * status.  stats[0..7] = ticks, launches, peak live fits, rejected, wall s, peak bytes (event
+ * data of the admitted requests, all columns), generation ms, device bytes of the event arena
+ * that all classes share (the budget: one allocation bounds every class's data together).
+ * request_start (nreq rows of max ntheta values, may be NULL) replaces the class's start
+ * values per request. */
 int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen, const double* start, const int* cls,
                   const long long* n, const unsigned long long* seeds, const double* arrival_s, int nreq, double budget_bytes,
-                  int slots, double* timings, double* values, double* errors, int* status, double* stats);
+                  int slots, double* timings, double* values, double* errors, int* status, double* stats,
+                  const double* request_start);
 
 /* one fit through the drop-in HIPBlock (reference fitter flow); data SoA nobs x n.  The
  * context's event_columns option turns on the fitter's optimize_dimensions (a store kernel

@@ -52,7 +52,7 @@ class Options(ctypes.Structure):
 
 class FitOut(ctypes.Structure):
     _fields_ = [("fval", ctypes.c_double), ("edm", ctypes.c_double), ("status", ctypes.c_int), ("ncalls", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("hesse_ok", ctypes.c_int)]
+                ("iterations", ctypes.c_int), ("hesse_ok", ctypes.c_int), ("thread_moved", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class BatchStats(ctypes.Structure):
@@ -144,7 +144,7 @@ def libs():
         L.mfh_fit_options.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, ip, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_ulonglong), dp, ip,
-                                    ctypes.c_double, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_int), dp]
+                                    ctypes.c_double, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_int), dp, dp]
         _capi = L
     return _hip, _capi
 
@@ -387,7 +387,7 @@ class Batch:
         return vals, errs, list(outs), stats
 
 
-def serve(models, theta_gen, start, cls, n, seeds, arrival_s, budget_bytes, slots=256):
+def serve(models, theta_gen, start, cls, n, seeds, arrival_s, budget_bytes, slots=256, request_start=None):
     """Run the continuous-batching fit service over `models` (request class = index).
     Returns dict with per-request arrival/admit/done times, values, errors, status, stats."""
     nm = len(models)
@@ -407,13 +407,20 @@ def serve(models, theta_gen, start, cls, n, seeds, arrival_s, budget_bytes, slot
     vals = np.zeros((nreq, maxp))
     errs = np.zeros((nreq, maxp))
     status = (ctypes.c_int * nreq)()
-    stats = np.zeros(7)
+    stats = np.zeros(8)
     tg, st = _f64(tg), _f64(st)
+    rst = None
+    if request_start is not None:
+        rst = np.full((nreq, maxp), np.nan)
+        for i, r in enumerate(request_start):
+            rst[i, :len(r)] = r
+        rst = _f64(rst)
     _check(libs()[1].mfh_serve_run(mh, nm, _dp(tg), _dp(st), ci, ni, si, _dp(arr), nreq, float(budget_bytes), int(slots),
-                                   _dp(timings), _dp(vals), _dp(errs), status, _dp(stats)))
+                                   _dp(timings), _dp(vals), _dp(errs), status, _dp(stats), None if rst is None else _dp(rst)))
     return {"arrival": timings[:, 0], "admit": timings[:, 1], "done": timings[:, 2], "values": vals, "errors": errs,
             "status": np.array(list(status)), "ticks": int(stats[0]), "launches": int(stats[1]), "peak_live": int(stats[2]),
-            "rejected": int(stats[3]), "wall_s": stats[4], "peak_bytes": stats[5], "generate_ms": stats[6]}
+            "rejected": int(stats[3]), "wall_s": stats[4], "peak_bytes": stats[5], "generate_ms": stats[6],
+            "arena_bytes": stats[7]}
 
 
 def fit_cpu(kind, data, start, nthreads=1, fp32=False, optimize_dimensions=False):

@@ -606,6 +606,7 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
   MFH_TRY
   b->with([&](auto& bb) {
     std::vector<int> list;
+    if (nfit < 0 || (nfit > 0 && !start)) throw std::invalid_argument("mfh_batch_fit: bad fit count or no start values");
     if (segs) list.assign(segs, segs + nfit);
     else
       for (int s = 0; s < bb.S; s++) list.push_back(s);
@@ -629,6 +630,7 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
         outs[i].ncalls = res[i].ncalls;
         outs[i].iterations = res[i].iterations;
         outs[i].hesse_ok = res[i].hesse_ok;
+        outs[i].thread_moved = res[i].thread_moved;
       }
     }
     if (stats) {
@@ -650,7 +652,7 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
 
 int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen, const double* start, const int* cls,
                   const long long* n, const unsigned long long* seeds, const double* arrival_s, int nreq, double budget_bytes,
-                  int slots, double* timings, double* values, double* errors, int* status, double* stats) {
+                  int slots, double* timings, double* values, double* errors, int* status, double* stats, const double* request_start) {
   MFH_TRY
   if (nmodels <= 0) throw std::invalid_argument("no models");
   const bool fp32 = models[0]->fp32;
@@ -667,6 +669,10 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
     reqs[i].seed = seeds[i];
     reqs[i].arrival_s = arrival_s[i];
     if (cls[i] < 0 || cls[i] >= nmodels) throw std::out_of_range("request class");
+    if (request_start) {
+      const int p = models[cls[i]]->with([](auto& hm) { return hm.ntheta; });
+      reqs[i].start.assign(request_start + (size_t)i * maxp, request_start + (size_t)i * maxp + p);
+    }
   }
   std::stable_sort(reqs.begin(), reqs.end(), [](const ServeRequest& a, const ServeRequest& b) { return a.arrival_s < b.arrival_s; });
   auto serve = [&](auto tag) {
@@ -699,6 +705,7 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
     stats[4] = srv.stats.wall_s;
     stats[5] = srv.stats.peak_bytes;
     stats[6] = srv.stats.generate_ms;
+    stats[7] = srv.stats.arena_bytes;
     return 0;
   };
   if (fp32) serve(float());
@@ -748,6 +755,7 @@ void run_fit(Backend* be, int kind, const double* data, long long n, const doubl
     out->ncalls = f.ncalls_;
     out->iterations = 0;
     out->hesse_ok = !f.cov_.empty();
+    out->thread_moved = 0;
   }
   if (ms) *ms = f.fit_ms_;
 }

@@ -35,6 +35,7 @@ struct BatchFitResult {
   double fval = NAN, edm = NAN;
   int status = -1, ncalls = 0, iterations = 0;
   bool hesse_ok = false;
+  bool thread_moved = false;  // the fit's fibre was resumed on another OS thread (never expected)
 };
 
 struct BatchFitStats {
@@ -63,6 +64,7 @@ class FitScheduler {
     int pending = 0;
     bool done = false;
     BatchFitResult result;
+    std::thread::id first_thread;   // OS thread the fibre first ran on
   };
 
   bool use_graph = false;  // direct launches: on this stack cheaper than a graph replay
@@ -110,14 +112,22 @@ class FitScheduler {
   size_t step(F on_done) {
     const auto ts = std::chrono::steady_clock::now();
     const int T = nthreads();
-    if (T > 1 && tasks_.size() > 1) {
+    if (T > 1) {
+      // every fibre is resumed by the worker it was assigned at add(), however many are live:
+      // code in a fibre may keep thread-local addresses (errno, the C++ exception globals)
+      // across its switches, so it must never continue on another OS thread
       groups_.assign(T, {});
+      int used = 0;
       for (auto& t : tasks_)
-        if (!t.pending && !t.done) groups_[t.worker].push_back(&t);
-      HostThreads::get().run(T, [this](int w) {
-        void* here = nullptr;
-        for (Task* t : groups_[w]) resume(*t, &here);
-      });
+        if (!t.pending && !t.done) {
+          groups_[t.worker].push_back(&t);
+          used = std::max(used, t.worker + 1);
+        }
+      if (used > 0)
+        HostThreads::get().run(used, [this](int w) {
+          void* here = nullptr;
+          for (Task* t : groups_[w]) resume(*t, &here);
+        });
     } else {
       void* here = nullptr;
       for (auto& t : tasks_)
@@ -176,6 +186,7 @@ class FitScheduler {
       for (size_t k = 0; k < ti.size(); k++) th[k] = x[ti[k]];
       t->pending = kind;
       mf_fiber_switch(&t->sp, *t->ret);
+      note_thread(*t);
     }
     double value(const double* x) override {
       post(x, EVAL_LH);
@@ -221,9 +232,16 @@ class FitScheduler {
     t.ret = here;
     mf_fiber_switch(here, t.sp);
   }
+  // inside the fibre, at its start and after each switch back into it: the thread it runs on
+  // must be the one it started on (BatchFitResult::thread_moved reports a violation)
+  static void note_thread(Task& t) {
+    if (t.first_thread == std::thread::id()) t.first_thread = std::this_thread::get_id();
+    else if (t.first_thread != std::this_thread::get_id()) t.result.thread_moved = true;
+  }
   // body of every fibre: the fit, then back to the worker for good
   static void fiber_main(void* arg) {
     Task* t = (Task*)arg;
+    note_thread(*t);
     try {
       t->owner->run_fit(*t);
     } catch (...) {
@@ -287,12 +305,23 @@ class BatchFitter {
   BatchFitStats stats;
   explicit BatchFitter(HipBatch<kernelT, evalT>& batch) : b(batch) {}
 
+  // Each listed segment is fitted once: an index outside the batch, an empty slot or a
+  // repeated index is refused (two fits of one segment would share its parameter slot, and the
+  // later post would overwrite the earlier one before the shared launch).
   std::vector<BatchFitResult> fit(const std::vector<int>& segs, const std::vector<double>& start /* ntheta per listed seg */) {
+    const int P = b.m->ntheta;
+    if (start.size() < segs.size() * (size_t)P) throw std::invalid_argument("BatchFitter::fit: fewer start values than fits");
+    std::vector<char> seen(b.S, 0);
+    for (int s : segs) {
+      if (s < 0 || s >= b.S) throw std::invalid_argument("BatchFitter::fit: segment " + std::to_string(s) + " outside 0.." + std::to_string(b.S - 1));
+      if (!b.active[s]) throw std::invalid_argument("BatchFitter::fit: segment " + std::to_string(s) + " holds no dataset");
+      if (seen[s]) throw std::invalid_argument("BatchFitter::fit: segment " + std::to_string(s) + " listed twice");
+      seen[s] = 1;
+    }
     auto t0 = std::chrono::steady_clock::now();
     FitScheduler<kernelT, evalT> sch;
     sch.use_graph = use_graph;
     sch.run_hesse = run_hesse;
-    const int P = b.m->ntheta;
     for (size_t i = 0; i < segs.size(); i++) sch.add(&b, segs[i], &start[i * P], (long long)i);
     const double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::vector<BatchFitResult> out(segs.size());

@@ -928,6 +928,18 @@ class ArenaAllocator {
   }
 };
 
+// Device storage for event columns with a first-fit allocator over it.  One batch owns its
+// arena, or several batches share one (the fit service's classes: one allocation that bounds
+// the event data of every class together).
+template <typename kernelT>
+struct EventArena {
+  DevBuf buf;
+  ArenaAllocator alloc;
+  EventArena(mf_device* dev, long long elems) : alloc(elems) { buf.alloc(dev, std::max<long long>(1, elems) * sizeof(kernelT)); }
+  long long capacity() const { return alloc.capacity; }
+  size_t device_bytes() const { return buf.bytes; }
+};
+
 // Segments of one model resident in one HBM arena.  Built either with fixed sizes (a toy
 // study, one big dataset) or with a slot capacity and an arena budget, in which case
 // segments are added and removed while other segments keep being evaluated (the fit
@@ -941,7 +953,7 @@ class HipBatch {
   int S = 0;                             // slot capacity
   std::vector<long long> sizes, offsets; // per slot (events, element offset)
   std::vector<char> active;
-  ArenaAllocator arena;
+  std::shared_ptr<EventArena<kernelT>> arena_;  // event columns (owned, or shared with other batches)
   long long total_elems = 0;             // arena capacity in elements
   int chunk = 0, nchunks = 0, chunk_cap = 0;
   int max_seg_chunks = 0;                // most chunks of any active segment (<= 1: no final pass)
@@ -949,7 +961,7 @@ class HipBatch {
   bool segs_stale_[4] = {true, true, true, true};  // per kind: device segment table differs from the mapped one
   bool dirty = true;
   std::vector<mf_segment> segs_h;
-  DevBuf data, params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d;
+  DevBuf params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d;
   // generator scratch, kept between calls (grows as needed)
   DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_;
   int gen_levels_ = 0;
@@ -978,21 +990,28 @@ class HipBatch {
   HipBatch(HipModel<kernelT, evalT>* model, int capacity, size_t arena_bytes) : m(model), dev(model->be->device()) {
     init(capacity, (long long)(arena_bytes / sizeof(kernelT)));
   }
+  // `capacity` slots whose events live in an arena shared with other batches of the same
+  // element type (models of other densities); segments come and go via add/remove
+  HipBatch(HipModel<kernelT, evalT>* model, int capacity, std::shared_ptr<EventArena<kernelT>> shared)
+      : m(model), dev(model->be->device()) {
+    init(capacity, shared->capacity(), shared);
+  }
+  void* data_ptr() const { return arena_->buf.ptr; }
+  kernelT* data_elems() const { return arena_->buf.template as<kernelT>(); }
   ~HipBatch() { drop_graphs(); }
   HipBatch(const HipBatch&) = delete;
   HipBatch& operator=(const HipBatch&) = delete;
 
-  void init(int capacity, long long elems) {
+  void init(int capacity, long long elems, std::shared_ptr<EventArena<kernelT>> shared = nullptr) {
     const compute_options& o = *m->be->opts_;
     S = capacity;
     chunk = m->block * o.hip_events_per_thread;
-    arena.reset(elems);
+    arena_ = shared ? shared : std::make_shared<EventArena<kernelT>>(dev, elems);
     total_elems = elems;
     sizes.assign(S, 0);
     offsets.assign(S, 0);
     active.assign(S, 0);
     segs_h.assign(S, mf_segment{});
-    data.alloc(dev, std::max<long long>(1, elems) * sizeof(kernelT));
     theta_h.alloc_mapped(dev, std::max(1, S * m->ntheta) * sizeof(double));
     params_d.alloc(dev, std::max(1, S * m->nparam) * sizeof(kernelT));
     hoist64_d.alloc(dev, (size_t)std::max(1, S) * std::max(1, m->nhoist) * sizeof(double));
@@ -1027,7 +1046,7 @@ class HipBatch {
     for (int s = 0; s < S && slot < 0; s++)
       if (!active[s]) slot = s;
     if (slot < 0) return -1;
-    long long off = arena.alloc(n * m->ncol);
+    long long off = arena_->alloc.alloc(n * m->ncol);
     if (off < 0) return -1;
     active[slot] = 1;
     sizes[slot] = n;
@@ -1037,7 +1056,7 @@ class HipBatch {
   }
   void remove_segment(int s) {
     if (s < 0 || s >= S || !active[s]) return;
-    arena.release(offsets[s], sizes[s] * m->ncol);
+    arena_->alloc.release(offsets[s], sizes[s] * m->ncol);
     active[s] = 0;
     sizes[s] = 0;
     request_[s] = 0;
@@ -1046,7 +1065,7 @@ class HipBatch {
   bool fits(long long n) const {
     bool slot = false;
     for (int s = 0; s < S && !slot; s++) slot = !active[s];
-    return slot && arena.largest_free() >= ArenaAllocator::round(n * m->ncol);
+    return slot && arena_->alloc.largest_free() >= ArenaAllocator::round(n * m->ncol);
   }
   int active_count() const {
     int c = 0;
@@ -1115,7 +1134,7 @@ class HipBatch {
     std::vector<kernelT> tmp((size_t)n * m->nobs);
     for (int d = 0; d < m->nobs; d++)
       for (long long i = 0; i < n; i++) tmp[d * n + i] = (kernelT)(cols[d * col_stride + i] - m->centres[d]);
-    hip_check(mf_memcpy_h2d(dev, data.template as<kernelT>() + offsets[s], tmp.data(), tmp.size() * sizeof(kernelT)), "HipBatch upload");
+    hip_check(mf_memcpy_h2d(dev, data_elems() + offsets[s], tmp.data(), tmp.size() * sizeof(kernelT)), "HipBatch upload");
     precompute({s});
     hip_check(mf_sync(dev), "HipBatch upload");
   }
@@ -1132,7 +1151,7 @@ class HipBatch {
     }
     for (int s : slots) t[s].mode = active[s] ? 1 : 0;
     mf_eval_args a{};
-    a.data = data.ptr;
+    a.data = data_ptr();
     a.segs = (const mf_segment*)segs_pre_h.dptr;
     a.chunks = chunks_d.template as<mf_chunk>();
     a.nchunks = nchunks;
@@ -1147,7 +1166,7 @@ class HipBatch {
   void download(int s, double* cols) {
     const long long n = sizes[s];
     std::vector<kernelT> tmp((size_t)n * m->nobs);
-    hip_check(mf_memcpy_d2h(dev, tmp.data(), data.template as<kernelT>() + offsets[s], tmp.size() * sizeof(kernelT)), "HipBatch download");
+    hip_check(mf_memcpy_d2h(dev, tmp.data(), data_elems() + offsets[s], tmp.size() * sizeof(kernelT)), "HipBatch download");
     hip_check(mf_sync(dev), "HipBatch download");
     for (int d = 0; d < m->nobs; d++)
       for (long long i = 0; i < n; i++) cols[d * n + i] = (double)tmp[d * n + i] + m->centres[d];
@@ -1237,7 +1256,7 @@ class HipBatch {
                                      gen_states_.template as<unsigned int>()),
               "mf_launch_seed_streams");
     mf_gen_args g{};
-    g.data = data.ptr;
+    g.data = data_ptr();
     g.segs = gen_segs_.template as<mf_segment>();
     g.streams = gen_streams_.template as<mf_gen_stream>();
     g.nstreams = (int)st.size();
@@ -1267,7 +1286,7 @@ class HipBatch {
   }
   mf_eval_args eval_args(int kind) const {
     mf_eval_args a{};
-    a.data = data.ptr;
+    a.data = data_ptr();
     a.segs = segs_d[kind].template as<mf_segment>();
     a.chunks = chunks_d.template as<mf_chunk>();
     a.nchunks = nchunks;

@@ -2,8 +2,9 @@
  * @file serve.hh
  * Fit service with continuous batching.  Requests (a density class, an event count, a
  * seed, an arrival time) arrive while other fits run.  A request is admitted when its
- * class's pool has a free slot and the service's HBM budget covers its events; its dataset
- * is generated on the device into the pool's arena, and its fit joins the shared fibre
+ * class's pool has a free slot and the service's HBM budget covers its events (all its data
+ * columns); its dataset is generated on the device into the one event arena every class
+ * shares (a single allocation of the budget), and its fit joins the shared fibre
  * scheduler.  Every tick issues one launch per (class, kind) over all fits that are waiting
  * for a value, a gradient or a Hessian, so the likelihood evaluations of concurrent fits
  * share launches; a finished fit releases its slot and memory at once.  Results are
@@ -31,6 +32,7 @@ struct ServeRequest {
   long long n = 0;
   uint64_t seed = 0;
   double arrival_s = 0.0;
+  std::vector<double> start;  // the fit's start values (ntheta of its class); empty: the class's
 };
 
 struct ServeResult {
@@ -44,6 +46,7 @@ struct ServeResult {
 struct ServeStats {
   int ticks = 0, launches = 0, peak_live = 0, rejected = 0;
   double wall_s = 0.0, peak_bytes = 0.0, generate_ms = 0.0;
+  double arena_bytes = 0.0;  // device memory holding the event data of all classes together
 };
 
 template <typename kernelT, typename evalT = double>
@@ -70,12 +73,19 @@ class FitServer {
   FitServer(size_t hbm_budget_bytes, int slots_per_class) : budget_bytes(hbm_budget_bytes), slots_(slots_per_class) {}
 
   void add_class(HipModel<kernelT, evalT>* m, const std::vector<double>& theta_gen, const std::vector<double>& start) {
+    if (!arena_) arena_ = std::make_shared<EventArena<kernelT>>(m->be->device(), (long long)(budget_bytes / sizeof(kernelT)));
     Class c;
     c.model = m;
     c.theta_gen = theta_gen;
     c.start = start;
-    c.pool.reset(new HipBatch<kernelT, evalT>(m, slots_, budget_bytes));
+    c.pool.reset(new HipBatch<kernelT, evalT>(m, slots_, arena_));
     classes.push_back(std::move(c));
+    stats.arena_bytes = (double)arena_->device_bytes();
+  }
+  // bytes of the shared arena a request of n events of class c occupies (all its columns:
+  // observables and precomputed event terms)
+  static double request_bytes(const Class& c, long long n) {
+    return (double)ArenaAllocator::round(n * c.model->ncol) * sizeof(kernelT);
   }
 
   // Serves `reqs` (sorted by arrival) in real time; returns one result per request.
@@ -105,7 +115,7 @@ class FitServer {
         size_t i = queue.front();
         const ServeRequest& r = reqs[i];
         Class& c = classes.at(r.cls);
-        const double need = (double)ArenaAllocator::round(r.n * c.model->nobs) * sizeof(kernelT);
+        const double need = request_bytes(c, r.n);
         if (need > (double)budget_bytes) {  // can never fit: reject
           queue.pop_front();
           stats.rejected++;
@@ -142,7 +152,9 @@ class FitServer {
         stats.generate_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tgen).count();
         for (size_t i : admitted[ci]) {
           res[i].admit_s = now();
-          sch.add(c.pool.get(), slot_of[i], c.start.data(), (long long)i);
+          const std::vector<double>& st = reqs[i].start.empty() ? c.start : reqs[i].start;
+          if ((int)st.size() < c.model->ntheta) throw std::invalid_argument("FitServer: request start values too short");
+          sch.add(c.pool.get(), slot_of[i], st.data(), (long long)i);
         }
       }
       if (sch.live() == 0) {
@@ -159,7 +171,7 @@ class FitServer {
         res[i].done_s = now();
         Class& c = classes.at(reqs[i].cls);
         c.pool->remove_segment(slot_of[i]);
-        used -= (double)ArenaAllocator::round(reqs[i].n * c.model->nobs) * sizeof(kernelT);
+        used -= request_bytes(c, reqs[i].n);
         finished++;
       });
       if (sch.live() > 0) sch.launch();
@@ -172,6 +184,7 @@ class FitServer {
 
  private:
   int slots_;
+  std::shared_ptr<EventArena<kernelT>> arena_;
 };
 
 }  // namespace morefit

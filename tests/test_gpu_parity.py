@@ -461,7 +461,23 @@ def test_batched_fits_match_single_fits(ctx, fp32):
         assert oc.status == 0 and np.all(ec > 0), i
         assert np.max(np.abs(vals[i] - vc) / ec) < lim, (i, (vals[i] - vc) / ec)
         np.testing.assert_allclose(errs[i], ec, rtol=1e-3 if fp32 else 1e-6)
+        # each fit's fibre stayed on the host thread it started on, also once the live count
+        # dropped below the worker count
+        assert outs[i].thread_moved == 0, i
     assert stats.ticks > 0
+    b.close()
+    mdl.close()
+
+
+def test_batch_fit_rejects_bad_segment_lists(ctx):
+    """an index outside the batch or listed twice is refused before any fit runs (two fits
+    of one segment would share its parameter slot)"""
+    mdl = ctx.model(M.MASS, fp32=True)
+    b = mdl.batch([1000, 1000])
+    say("bad segment lists")
+    for segs in ([0, 2], [-1], [1, 1]):
+        with pytest.raises(RuntimeError, match="outside|twice"):
+            b.fit(M.MASS_START, segs=segs)
     b.close()
     mdl.close()
 
