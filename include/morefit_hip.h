@@ -24,6 +24,8 @@
  *   mf_module_resources             no counterpart (register budget of each compiled kernel)
  *   mf_module_occupancy             no counterpart (resident work-groups per CU, for balanced chunk tables)
  *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
+ *   mf_launch_eval_noalloc          no counterpart: mf_launch_eval with the library's own static scratch for the
+ *                                   partials, for callers that cannot allocate
  *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
  *   mf_launch_prep_inline           the same with theta passed by value (no counterpart: latency of one call)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
@@ -246,6 +248,20 @@ double mf_module_compile_ms(mf_module* mod);  /* 0 when served from a cache */
 
 /* launches (on the device stream) */
 int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
+
+/* mf_launch_eval that allocates nothing: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  args->partials is ignored: the per-chunk partials go to a static
+ * array of MF_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on until the
+ * process ends; a launch needing more (nchunks x the kernel's partials per chunk x 8 bytes) is
+ * refused with MF_ERR_INVALID before anything is enqueued.  Nothing in the scratch has to start
+ * at zero (each partial is written before it is read), so no clearing is enqueued; in-kernel
+ * segment reduction (args->seg_done) is refused, the final kernel reduces.  All calls on one
+ * GPU share its array: launches that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller.  Inputs, outputs and tables are the caller's, as for
+ * mf_launch_eval. */
+#define MF_NOALLOC_SCRATCH_BYTES (4u << 20)
+int mf_launch_eval_noalloc(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
 /* kernel parameters of segments seg_list[0..nseg) (all 0..nseg-1 when seg_list is NULL) from
  * their FP64 theta: params[s * (ntheta + nhoist) ...] in the module's type and, when hoist64 is
  * set, the hoisted terms in FP64 at hoist64[s * nhoist ...].  With seg_src set, each of those

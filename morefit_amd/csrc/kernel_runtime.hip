@@ -587,9 +587,12 @@ __device__ __forceinline__ void mf_precompute_impl(const mf_eval_args& a) {
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) mf_precompute(mf_eval_args a) { mf_precompute_impl(a); }
 #endif
 
+// NAME##_nacc: the per-chunk partials the kernel writes (F::nacc doubles per chunk), read by the
+// host at module load to bound the launches that bring no partials buffer of their own
 #define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL, FUSED)                                                                    \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL, FUSED>(a); } \
-  extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR>(a); }
+  extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR>(a); } \
+  extern "C" __device__ __attribute__((used)) int NAME##_nacc = FUNCTOR::nacc;
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \
   extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_store_impl<FUNCTOR>(a); }
 

@@ -332,6 +332,12 @@ int code_resources(const std::vector<char>& code, std::vector<mf_kernel_resource
 }
 
 // built-in kernels (compiled into this library for gfx950)
+// Per-chunk partials of mf_launch_eval_noalloc: one array per GPU, for the life of the process.
+// Host code never uses the name itself (on the host it is only a stand-in for the symbol):
+// mf_device_open takes each device's address of it with hipGetSymbolAddress, and launches
+// pass that address (mf_device::scratch).
+static __device__ __align__(256) double scratch[MF_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
 __global__ void k_fill_zero(unsigned int* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -383,6 +389,7 @@ struct mf_module;
 
 struct mf_device {
   int ordinal = 0;
+  double* scratch = nullptr;  // this device's address of `scratch` (mf_launch_eval_noalloc)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   mf_device_props props{};
@@ -394,6 +401,7 @@ struct mf_module {
   mf_device* dev = nullptr;
   hipModule_t mod = nullptr;
   std::map<std::string, hipFunction_t> fns;
+  std::map<std::string, int> nacc;  // evaluation kernel -> FP64 partials per chunk (its <name>_nacc)
   uint64_t hash = 0;
   int refs = 0;
   double compile_ms = 0.0;
@@ -429,6 +437,16 @@ int mf_device_open(int ordinal, mf_device** out) {
     return set_error(MF_ERR_INVALID, "mf_device_open: device %d is %s, this library is built for %s", ordinal, p.gcnArchName, kArch);
   auto* d = new mf_device();
   d->ordinal = ordinal;
+  // the device's own copy of the no-allocation scratch array, with this device current
+  {
+    void* sp = nullptr;
+    hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+      delete d;
+      return set_error(MF_ERR_HIP, "mf_device_open: hipGetSymbolAddress(scratch) on device %d: %s", ordinal, hipGetErrorString(e));
+    }
+    d->scratch = (double*)sp;
+  }
   if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&d->ev0) != hipSuccess ||
       hipEventCreate(&d->ev1) != hipSuccess) {
     delete d;
@@ -629,6 +647,23 @@ int mf_module_load(mf_device* dev, const char* generated_src, const char* option
   m->hash = h;
   m->refs = 1;
   m->compile_ms = ms;
+  // partials per chunk of each evaluation kernel (the module's <kernel>_nacc integers)
+  for (const auto& r : res) {
+    const std::string sym = std::string(r.name) + "_nacc";
+    hipDeviceptr_t dp = nullptr;
+    size_t bytes = 0;
+    if (hipModuleGetGlobal(&dp, &bytes, hm, sym.c_str()) != hipSuccess || bytes != sizeof(int)) {
+      (void)hipGetLastError();
+      continue;
+    }
+    int v = 0;
+    if (hipMemcpyDtoH(&v, dp, sizeof(int)) != hipSuccess) {
+      (void)hipModuleUnload(hm);
+      delete m;
+      return set_error(MF_ERR_HIP, "mf_module_load: reading %s failed", sym.c_str());
+    }
+    m->nacc[r.name] = v;
+  }
   std::lock_guard<std::mutex> lk(dev->mu);
   dev->modules[h] = m;
   *out = m;
@@ -693,12 +728,38 @@ int mf_module_has_kernel(mf_module* mod, const char* kernel) {
   return get_fn(mod, kernel, &fn) == MF_OK ? 1 : 0;
 }
 
+static int launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
+
 int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block) {
   MF_INJECT("mf_launch_eval");
+  return launch_eval(mod, kernel, args, grid, block);
+}
+
+int mf_launch_eval_noalloc(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block) {
+  MF_INJECT("mf_launch_eval_noalloc");
+  if (!mod || !kernel || !args) return set_error(MF_ERR_INVALID, "mf_launch_eval_noalloc: null argument");
+  if (args->seg_done)
+    return set_error(MF_ERR_INVALID, "mf_launch_eval_noalloc: in-kernel segment reduction (seg_done) is not supported; the "
+                                     "final kernel reduces the segments");
+  auto it = mod->nacc.find(kernel);
+  if (it == mod->nacc.end())
+    return set_error(MF_ERR_NOKERNEL, "mf_launch_eval_noalloc: %s is not an evaluation kernel of this module", kernel);
+  const long long need = (long long)(args->nchunks > 0 ? args->nchunks : 0) * it->second * (long long)sizeof(double);
+  if (need > (long long)MF_NOALLOC_SCRATCH_BYTES)
+    return set_error(MF_ERR_INVALID,
+                     "mf_launch_eval_noalloc: %d chunks x %d partials need %lld bytes of scratch; the maximum is %lld "
+                     "(use fewer, larger chunks, or mf_launch_eval with a partials buffer)",
+                     args->nchunks, it->second, need, (long long)MF_NOALLOC_SCRATCH_BYTES);
+  mf_eval_args a = *args;
+  a.partials = mod->dev->scratch;
+  return launch_eval(mod, kernel, &a, grid, block);
+}
+
+static int launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block) {
   if (!mod || !kernel || !args) return set_error(MF_ERR_INVALID, "mf_launch_eval: null argument");
   if (grid <= 0 || block <= 0 || block % 64 != 0 || block > 1024)
     return set_error(MF_ERR_INVALID, "mf_launch_eval: bad launch shape %d x %d", grid, block);
-  if (args->nchunks < 0 || args->nseg < 0 || !args->segs || !args->chunks || !args->params || !args->data)
+  if (args->nchunks < 0 || args->nseg < 0 || !args->segs || !args->chunks || !args->params || !args->data || !args->partials)
     return set_error(MF_ERR_INVALID, "mf_launch_eval: incomplete argument block");
   std::string fin = std::string(kernel) + "_final";
   hipFunction_t ff = nullptr;
