@@ -64,8 +64,12 @@ static void mass_prepare(const double* th, const double* r, mass_hoist* h) {
   double dNG_ds = -(zb * eb2 - za * ea2) / (SQRTPI * s);
   double NE, dNE_dl;
   if (lam == 0.0) {
+    /* the reference's ConditionalEqual branch (pdf.hh:437-439) is the constant b - a, and its
+     * derivative rule (graph.hh:2291-2299) differentiates the branch taken: at lambda = 0
+     * exactly the normalisation contributes nothing to d/dlambda (pinned by the lambda = 0
+     * points of tests/golden/ref_graph.json; the limit lambda -> 0 would give (b^2 - a^2)/2) */
     NE = b - a;
-    dNE_dl = 0.5 * (b * b - a * a);
+    dNE_dl = 0.0;
   } else {
     double eb = exp(lam * b), ea = exp(lam * a);
     NE = (eb - ea) / lam;

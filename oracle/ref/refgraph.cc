@@ -133,6 +133,12 @@ int main() {
       for (int i = 0; i < 8; i++) v.push_back(kth[t][i]);
       kstar_pts.push_back(v);
     }
+  // lambda = 0: the ConditionalEqual branch of the exponential's normalisation (pdf.hh:437-439);
+  // drawn last so that the points above keep their values
+  for (int k = 0; k < 8; k++) {
+    double m = a + (b - a) * u01();
+    mass_pts.push_back({m, 5.3, 0.03, 0.5, 0.0});
+  }
   printf("{\n  \"generator\": \"oracle/ref/refgraph.cc over the reference graph.hh\",\n  \"mass_range\": [%.17g, %.17g],\n", a, b);
   emit("mass", mass_logprob(a, b), {"m"}, {"mu", "sigma", "f", "lambda"}, mass_pts, false);
   emit("kstar", kstar_logprob(), {"ctl", "ctk", "phi"}, {"FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"}, kstar_pts, true);

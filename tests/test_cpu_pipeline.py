@@ -249,3 +249,33 @@ def test_weighted_fit_covariance_correction_matches_oracle_sandwich():
     np.testing.assert_allclose(e, np.sqrt(np.diag(C)), rtol=1e-6)
     # and it differs from the uncorrected HESSE errors (the correction is live)
     assert np.max(np.abs(e / np.sqrt(np.diag(2 * Hi)) - 1)) > 0.01
+
+
+def test_derivative_rules_match_calculus(tmp_path):
+    """every derivative rule of graph.hh (power with base and/or exponent depending on the
+    variable, asin, acos, tan, atan, sin, cos, erf, exp, log, sqrt, inverse, conditional)
+    against closed forms and central differences, first and second order
+    (tests/cpp/diff_rules_check.cc)"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "diff_rules_check")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "morefit_amd", "include"),
+                           os.path.join(root, "tests", "cpp", "diff_rules_check.cc"), "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
+def test_cpu_eval_at_lambda_zero_matches_oracle():
+    """lambda = 0: the exponential's normalisation takes its ConditionalEqual branch
+    (reference pdf.hh:437-439); the CPU backend's sums equal the oracle's, which follows the
+    reference's own lambda = 0 derivatives (tests/golden/ref_graph.json)"""
+    theta = np.array([5.3, 0.03, 0.5, 0.0])
+    data = O.generate(O.MASS, theta, 20000, seed=5, nstreams=4)
+    v, g, H = M.eval_cpu(O.MASS, data, theta, nthreads=2)
+    ref = O.nll_sums(O.MASS, data, theta, mode=2, nthreads=0)
+    absref = O.nll_sums(O.MASS, data, theta, mode=12, nthreads=1)
+    assert abs(v - (-2 * ref[0])) <= 1e-12 * abs(ref[0]) * 2
+    assert np.all(np.abs(g - (-2 * ref[1:5])) <= 2e-12 * absref[1:5])
+    assert np.all(np.abs(H - (-2 * _hess_full(O.MASS, ref[5:]))) <= 1e-11 * 2 * _hess_full(O.MASS, absref[5:]) + 1e-300)
+    # mode 1 (closed form) and mode 2 (jets) of the oracle agree there as well
+    ref1 = O.nll_sums(O.MASS, data, theta, mode=1, nthreads=0)
+    assert np.all(np.abs(ref1 - ref[:5]) <= 1e-12 * absref[:5])

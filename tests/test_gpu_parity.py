@@ -75,6 +75,38 @@ def test_eval_matches_oracle(ctx, model, fp32):
 
 
 @pytest.mark.parametrize("fp32", [False, True])
+def test_mass_at_lambda_zero_matches_oracle(ctx, fp32):
+    """lambda = 0 takes the ConditionalEqual branch of the exponential's normalisation
+    (reference pdf.hh:437-439): b - a, whose derivative rule contributes nothing at lambda = 0
+    exactly (pinned by the lambda = 0 points of tests/golden/ref_graph.json).  Through both
+    places the hoisted terms are computed: the host program of a one-segment call (passed by
+    value) and the device prep kernel of a two-segment batch."""
+    n = 60000
+    theta = np.array([5.3, 0.03, 0.5, 0.0])
+    data = O.generate(M.MASS, theta, n, seed=99, nstreams=8)
+    stored = _stored(M.MASS, data, fp32)
+    ref = O.nll_sums(M.MASS, stored, theta, mode=2, nthreads=0)
+    absref = O.nll_sums(M.MASS, stored, theta, mode=12, nthreads=4)
+    tol = 2e-6 if fp32 else 1e-12
+    mdl = ctx.model(M.MASS, fp32=fp32)
+    one = mdl.batch([n])
+    one.upload(0, data)
+    two = mdl.batch([n, 1000])
+    two.upload(0, data)
+    two.upload(1, data[:, :1000])
+    for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+        say("lambda = 0", "fp32", fp32, "kind", kind)
+        for b in (one, two):
+            out = b.eval(kind, theta)[0]
+            k = len(out)
+            assert np.all(np.isfinite(out))
+            assert np.all(np.abs(out - ref[:k]) <= tol * absref[:k]), (kind, out - ref[:k], absref[:k])
+    one.close()
+    two.close()
+    mdl.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
 def test_event_columns_match_oracle(fp32):
     """Angular density with event-only terms precomputed into data columns (event_columns=1):
     after an upload, every sum within the bars of test_eval_matches_oracle; after generation
