@@ -20,5 +20,7 @@
 #include "morefit/compute_hip.hh"
 #include "morefit/minimizer.hh"
 #include "morefit/fitter.hh"
+#include "morefit/generator.hh"
+#include "morefit/toystudy.hh"
 
 #endif

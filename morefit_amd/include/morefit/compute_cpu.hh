@@ -5,7 +5,10 @@
  * into a shared object and loaded with dlopen (the reference JIT-compiles C with Clang/ORC);
  * contiguous per-thread event slices with the last thread taking the remainder, an in-loop
  * Kahan sum per output in evalT, and a Kahan sum of the thread partials (compute_llvm.hh:
- * 556-709, 785-850).  It runs the whole graph -> code -> fit pipeline where no GPU exists.
+ * 556-709, 785-850).  Generation (MakeGenerateKernel) runs one xoshiro128++ stream per thread
+ * over the same contiguous slices, accept-reject as the reference's generate kernel
+ * (compute_llvm.hh:362-525, 751-784).  It runs the whole graph -> code -> fit pipeline where
+ * no GPU exists.
  */
 #ifndef MOREFIT_COMPUTE_CPU_HH
 #define MOREFIT_COMPUTE_CPU_HH
@@ -84,6 +87,42 @@ inline std::string cpu_source(const ModuleSource& ms, const std::string& kernel_
   return s.str();
 }
 
+// host translation unit of a generate kernel: the density p (one functor output) and
+// accept-reject over events [from, to) of one xoshiro128++ stream (the reference's generate
+// kernel, compute_llvm.hh:380-500): every observable proposed uniformly in its range, accepted
+// when maxprob * u < p; after 2^20 rejections in a row (a density that is NaN, or far above
+// the envelope) the remaining events of the slice are NaN
+inline std::string cpu_generate_source(const ModuleSource& ms) {
+  std::ostringstream s;
+  s << cpu_source(ms, "mf_cpu_unused")
+    << "#include <cstdint>\n"
+    << "static inline MF_T mf_xoshiro_u(uint32_t* s) {\n"
+    << "  const uint32_t r = ((s[0] + s[3]) << 7 | (s[0] + s[3]) >> 25) + s[0];\n"
+    << "  const uint32_t t = s[1] << 9;\n"
+    << "  s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = (s[3] << 11) | (s[3] >> 21);\n"
+    << "  return (MF_T)(r * 2.3283064365386963e-10);\n}\n"
+    << "extern \"C\" void mf_cpu_gen(long long stride, long long from, long long to, uint32_t* st, MF_T* out, double maxprob,\n"
+    << "                            const double* lo, const double* width) {\n"
+    << "  const MF_T mp = (MF_T)maxprob;\n"
+    << "  uint32_t s[4] = {st[0], st[1], st[2], st[3]};\n"
+    << "  const MF_T* p = nullptr;\n"
+    << "  for (long long i = from; i < to; i++) {\n"
+    << "    unsigned long tries = 0;\n"
+    << "    for (;;) {\n"
+    << "      MF_T x[" << ms.nobs << "];\n"
+    << "      for (int d = 0; d < " << ms.nobs << "; d++) x[d] = (MF_T)(lo[d] + width[d] * (double)mf_xoshiro_u(s));\n"
+    << "      MF_T o[1];\n      mf_f_block::eval(x, p, o);\n"
+    << "      if (mp * mf_xoshiro_u(s) < o[0]) {\n"
+    << "        for (int d = 0; d < " << ms.nobs << "; d++) out[d * stride + i] = x[d];\n"
+    << "        break;\n      }\n"
+    << "      if (++tries >= (1ul << 20)) {\n"
+    << "        for (long long j = i; j < to; j++) for (int d = 0; d < " << ms.nobs << "; d++) out[d * stride + j] = (MF_T)NAN;\n"
+    << "        i = to;\n        break;\n      }\n"
+    << "    }\n  }\n"
+    << "  for (int j = 0; j < 4; j++) st[j] = s[j];\n}\n";
+  return s.str();
+}
+
 // compiles (or reuses) a shared object for `src`; returns the dlopen handle
 inline void* cpu_compile(const std::string& src, const compute_options& o) {
   std::string key = src + "\n" + o.cpu_compiler + " " + o.cpu_flags;
@@ -118,6 +157,7 @@ template <typename kernelT, typename evalT, typename seedT = uint32_t>
 class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
   typedef void (*sum_fn)(const kernelT*, long long, long long, long long, const kernelT*, double*);
   typedef void (*store_fn)(const kernelT*, long long, long long, long long, const kernelT*, kernelT*, long long);
+  typedef void (*gen_fn)(long long, long long, long long, uint32_t*, kernelT*, double, const double*, const double*);
   struct State {
     CPUBackend* be = nullptr;
     std::shared_ptr<std::vector<kernelT>> input, output;
@@ -127,6 +167,11 @@ class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
     void* handle = nullptr;
     sum_fn fsum = nullptr;
     store_fn fstore = nullptr;
+    gen_fn fgen = nullptr;
+    bool generate = false;
+    double maxprob = 1.0;
+    std::vector<double> gen_lo, gen_width;
+    std::vector<uint32_t> seeds;  // 4 words per stream
     bool kahan = true;
     int nout = 0;
     long long nevents = 0, stride = 0;
@@ -173,13 +218,20 @@ class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
   bool CopyToInputBuffer(const EventVector<kernelT, evalT>& data) override { return CopyToInputBufferAny(data); }
   template <typename srcT, typename std::enable_if<!std::is_same<srcT, kernelT>::value, int>::type = 0>
   bool CopyToInputBuffer(const EventVector<srcT, evalT>& data) { return CopyToInputBufferAny(data); }
-  bool CopyToSeedBuffer(const EventVector<seedT, evalT>&) override { return true; }
+  // one stream per thread: row t holds the 4 state words of thread t's xoshiro128++
+  bool CopyToSeedBuffer(const EventVector<seedT, evalT>& data) override {
+    s_->seeds.assign(4 * data.nevents(), 0u);
+    for (size_t k = 0; k < data.nevents(); k++)
+      for (unsigned j = 0; j < 4; j++) s_->seeds[4 * k + j] = (uint32_t)data(k, j);
+    return true;
+  }
   bool CopyToParameterBuffer(const std::vector<kernelT>& params) override {
     s_->params = params;
     if (s_->params.empty()) s_->params.push_back(kernelT(0));
     return true;
   }
   bool CopyFromOutputBuffer(EventVector<kernelT, evalT>& data) override {
+    if (!s_->output) throw std::runtime_error("CPUBlock::CopyFromOutputBuffer: no output buffer");
     std::copy(s_->output->begin(), s_->output->begin() + std::min(s_->output->size(), data.buffer_size() / sizeof(kernelT)), data.get_data());
     return true;
   }
@@ -207,12 +259,64 @@ class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
     s_->fstore = (store_fn)::dlsym(s_->handle, "mf_cpu_store");
     if (!s_->fsum || !s_->fstore) throw std::runtime_error("CPU backend: symbols missing");
     s_->kahan = kahan_summation;
+    s_->generate = false;
     s_->nout = (int)graphs.size();
+    if (s_->nevents == 0) s_->nevents = s_->stride = nevents;
+    return true;
+  }
+  bool MakeGenerateKernel(std::string name, unsigned int nevents, std::vector<dimension<evalT>> input_signature,
+                          std::vector<dimension<evalT>> output_signature, const std::vector<std::string>& params,
+                          const std::vector<std::unique_ptr<ComputeGraphNode<kernelT, evalT>>>& graphs, evalT maxprob = 1.0) override {
+    if (graphs.size() != 1) throw std::runtime_error("CPUBlock::MakeGenerateKernel: expects the density as one graph");
+    ModuleSource ms;
+    ms.fp32 = std::is_same<kernelT, float>::value;
+    ms.nobs = (int)output_signature.size();
+    std::vector<std::string> obs;
+    s_->gen_lo.clear();
+    s_->gen_width.clear();
+    for (auto& d : output_signature) {
+      obs.push_back(d.get_name());
+      s_->gen_lo.push_back(d.get_min());
+      s_->gen_width.push_back(d.get_max() - d.get_min());
+    }
+    std::vector<const ComputeGraphNode<kernelT, evalT>*> outs{graphs[0].get()};
+    ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
+    std::string src = cpu_generate_source(ms);
+    if (s_->be->opts_->print_kernel) std::cout << src << std::endl;
+    if (s_->handle) ::dlclose(s_->handle);
+    s_->handle = cpu_compile(src, *s_->be->opts_);
+    s_->fgen = (gen_fn)::dlsym(s_->handle, "mf_cpu_gen");
+    if (!s_->fgen) throw std::runtime_error("CPU backend: generate symbol missing");
+    s_->fsum = nullptr;
+    s_->fstore = nullptr;
+    s_->generate = true;
+    s_->maxprob = maxprob;
+    s_->nout = ms.nobs;
     if (s_->nevents == 0) s_->nevents = s_->stride = nevents;
     return true;
   }
   bool SubmitKernel() override {
     State& s = *s_;
+    if (s.generate) {
+      // thread t owns events [t * n / nt, ...) (the last thread the remainder) and stream t
+      const unsigned nt = (unsigned)(s.seeds.size() / 4);
+      if (nt == 0) throw std::runtime_error("CPUBlock::SubmitKernel: generator without seeds");
+      if (!s.output || (long long)s.output->size() < s.stride * s.nout) s.output = std::make_shared<std::vector<kernelT>>(s.stride * s.nout);
+      const long long size = s.nevents / nt;
+      auto slice = [&](unsigned t) {
+        long long from = t * size, to = (t == nt - 1) ? s.nevents : from + size;
+        s.fgen(s.stride, from, to, &s.seeds[4 * t], s.output->data(), s.maxprob, s.gen_lo.data(), s.gen_width.data());
+      };
+      if (nt == 1) {
+        slice(0);
+      } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++) th.emplace_back(slice, t);
+        for (auto& t : th) t.join();
+      }
+      return true;
+    }
+    if (!s.input) throw std::runtime_error("CPUBlock::SubmitKernel: no input buffer");
     const kernelT* data = s.input->data();
     const kernelT* p = s.params.empty() ? nullptr : s.params.data();
     unsigned int nt = std::max(1u, s.be->opts_->llvm_nthreads);

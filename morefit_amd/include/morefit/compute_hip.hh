@@ -301,14 +301,19 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
   bool CopyToInputBuffer(const EventVector<srcT, evalT>& data) { return CopyToInputBufferAny(data); }
 
   bool CopyToSeedBuffer(const EventVector<seedT, evalT>& data) override {
-    // reference layout: 4 seed columns of one row per work item; here one row per stream
+    // reference layout: 4 seed columns of one row per work item; here one row per stream.
+    // Stream k owns events [k * per, (k + 1) * per): per = 1, or n / max_generator_workitems + 1
+    // when there are more events than work items (compute_opencl.hh:436-440, 596-602)
     State& s = *s_;
     const size_t ns = data.nevents();
     std::vector<unsigned int> st(4 * ns);
     for (size_t k = 0; k < ns; k++)
       for (int j = 0; j < 4; j++) st[4 * k + j] = (unsigned int)data(k, (unsigned)j);
     const long long n = s.nevents;
-    const long long per = ns > 0 ? (n + (long long)ns - 1) / (long long)ns : n;
+    const int maxw = opts().opencl_max_generator_workitems;
+    const long long per = (maxw > 0 && n > maxw) ? n / maxw + 1 : 1;
+    if (ns == 0 || (long long)ns * per < n)
+      throw std::runtime_error("HIPBlock::CopyToSeedBuffer: " + std::to_string(ns) + " seeds cannot cover " + std::to_string(n) + " events");
     std::vector<mf_gen_stream> streams;
     for (size_t k = 0; k < ns; k++) {
       long long b = (long long)k * per, e = std::min(n, b + per);
