@@ -66,6 +66,12 @@ extern "C" {
  * the per-chunk kernel <kernel> and then, when the module has it, <kernel>_final (one
  * 256-thread work-group per segment, fixed-order reduction of the partials of segments with
  * more than one chunk; a one-chunk segment's totals come from the per-chunk kernel). */
+/* Event indices inside a segment are 32-bit (mf_segment.n, mf_chunk.begin/end, the kernels'
+ * event index, which runs up to end + unroll x block): a segment holds at most
+ * MF_MAX_SEGMENT_EVENTS events (8.6 GB of FP32 mass events).  Hosts refuse larger segments;
+ * a larger dataset is split into several segments whose totals add up. */
+#define MF_MAX_SEGMENT_EVENTS 2146435072LL /* 2^31 - 2^20 */
+
 typedef struct mf_segment {
   long long data_off;
   long long stride;

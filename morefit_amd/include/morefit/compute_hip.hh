@@ -158,6 +158,14 @@ struct HostBuf {
   template <typename T> T* as() const { return (T*)ptr; }
 };
 
+// refuses an event count a segment cannot hold (32-bit event indices in the kernels)
+inline long long check_segment_events(long long n, const char* who) {
+  if (n < 0 || n > MF_MAX_SEGMENT_EVENTS)
+    throw std::invalid_argument(std::string(who) + ": " + std::to_string(n) + " events in one segment; at most " +
+                                std::to_string(MF_MAX_SEGMENT_EVENTS) + " (split the dataset into segments)");
+  return n;
+}
+
 // one segment's chunk table [chunk_begin, chunk_end) for n events at `chunk` events per chunk
 // Splits segment `seg` of n events into chunks of about `chunk` events.  With `resident` (the
 // work-groups the whole device holds at once) a segment of at least half that many chunks is
@@ -165,6 +173,8 @@ struct HostBuf {
 // is full (e.g. 10^7 events at 8192 per chunk: 1221 chunks on a 1024-group device would run
 // 1.19 rounds; 1024 chunks of 9766 run one).
 inline void append_chunks(std::vector<mf_chunk>& chunks, mf_segment& sg, int seg, long long n, int chunk, int resident = 0) {
+  check_segment_events(n, "append_chunks");
+  if (chunk <= 0) throw std::invalid_argument("append_chunks: chunk size " + std::to_string(chunk));
   sg.chunk_begin = (int)chunks.size();
   if (resident > 0 && n > chunk) {
     const long long nch0 = (n + chunk - 1) / chunk;
@@ -276,6 +286,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
   }
   bool SetupKahanBuffer(unsigned long int nbytes) override { return true; }
   bool SetNevents(int nevents, int nevents_padded = -1) override {
+    check_segment_events(nevents, "HIPBlock::SetNevents");
     if (nevents != s_->nevents || (nevents_padded > 0 ? nevents_padded : nevents) != s_->nevents_padded) s_->tables_ready = false;
     s_->nevents = nevents;
     s_->nevents_padded = nevents_padded > 0 ? nevents_padded : nevents;
@@ -1047,6 +1058,7 @@ class HipBatch {
   // ---------------------------------------------------------------- membership
   // returns the slot, or -1 when no slot or no contiguous arena space is left
   int add_segment(long long n) {
+    check_segment_events(n, "HipBatch::add_segment");
     int slot = -1;
     for (int s = 0; s < S && slot < 0; s++)
       if (!active[s]) slot = s;

@@ -30,6 +30,7 @@
 #include <functional>
 #include <iomanip>
 #include <iostream>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -236,6 +237,9 @@ class fitter {
   // upload a dataset (FP64 host data may feed an FP32 kernel) and bind the blocks to it
   template <typename dataT>
   void bind_data(EventVector<dataT, evalT>* data) {
+    // the blocks take int event counts (reference compute.hh:120)
+    if (data->nevents_padded() > (size_t)std::numeric_limits<int>::max())
+      throw std::invalid_argument("fitter: " + std::to_string(data->nevents_padded()) + " events do not fit the blocks' int event counts");
     nevents_ = data->nevents();
     apply_centres(block_, centres_);
     if (opts_->optimize_dimensions) {

@@ -29,6 +29,7 @@
 #include <chrono>
 #include <iomanip>
 #include <iostream>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -89,6 +90,8 @@ class generator {
   bool generate(unsigned int nevents, PDF<kernelT, evalT>* prob, std::vector<parameter<evalT>*> params, EventVector<kernelT, evalT>& result,
                 bool regen = false) {
     const auto t0 = std::chrono::steady_clock::now();
+    if (nevents > (unsigned)std::numeric_limits<int>::max())
+      throw std::invalid_argument("generator: " + std::to_string(nevents) + " events do not fit the blocks' int event counts");
     const evalT maxprob = prob->get_max();
     const int pad = backend_->required_padding();
     if (opts_->rndtype == generator_options::on_accelerator) {

@@ -279,3 +279,16 @@ def test_cpu_eval_at_lambda_zero_matches_oracle():
     # mode 1 (closed form) and mode 2 (jets) of the oracle agree there as well
     ref1 = O.nll_sums(O.MASS, data, theta, mode=1, nthreads=0)
     assert np.all(np.abs(ref1 - ref[:5]) <= 1e-12 * absref[:5])
+
+
+def test_segment_event_limit_is_guarded(tmp_path):
+    """segments beyond the kernels' 32-bit event indices are refused on the host, and the
+    largest allowed one stays in range (tests/cpp/segment_limits_check.cc)"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "morefit_amd", "lib")
+    exe = str(tmp_path / "segment_limits_check")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "include"), "-I", os.path.join(root, "morefit_amd", "include"),
+                           os.path.join(root, "tests", "cpp", "segment_limits_check.cc"), "-o", exe, "-L", lib, "-lmorefit_hip",
+                           "-Wl,-rpath," + lib])
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
