@@ -3,12 +3,16 @@
 Workload (the configuration the headline metric is quoted on): the mass-fit density
 f*Gauss(mu, sigma) + (1-f)*Exp(lambda) on m in [5.0, 5.6], N = 10,000,000 events generated
 on the device by seeded accept-reject, one dataset.  A step is one -2lnL + gradient
-evaluation over all N events at the fit's start values: parameter upload, FP64 hoisted
-terms, the fused per-event kernel with its deterministic reduction, totals back to the host
-(a replayed hipGraph).  `value` is evaluations per second of the FP32-compensated path;
-the FP64 path, accuracy against the FP64 oracle, the kernel's HBM roofline fraction, the
-batched toy study (4096 toys, Poisson(5000) events) against the plain per-fit FP64 path and
-the CPU baseline (the oracle, single thread) are reported beside it.
+evaluation over all N events at the fit's start values: FP64 hoisted terms computed on the
+host and passed with theta in the kernel arguments, the per-chunk kernel and the
+deterministic final reduction launched directly, completion polled, totals read from mapped
+host memory.  `value` is evaluations per second of the FP32-compensated path; the FP64 path,
+accuracy against the FP64 oracle, the kernel's HBM roofline fraction, the angular
+configuration, the batched toy study (4096 toys, Poisson(5000) events) and the reference's
+toy protocol (100 toys x N = 1e3..1e6) against the plain per-fit FP64 path (kernels reused
+from fit to fit, and built anew per fit), the fit service, and the CPU baselines (the host
+CPU backend with the reference LLVM backend's semantics, on 1 thread and all threads; a
+config-1 fit; CPU toys) are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-toys] [--no-cpu]
        python bench.py --profile   (counter workload: the headline FP32 gradient and value
@@ -77,12 +81,23 @@ def measured_traffic(current_shape):
 
 def plain_context(M, O):
     """a default-shape context for the plain per-fit FP64 path, its modules compiled by one
-    untimed fit per density (the plain path builds its kernels at the first fit)"""
+    untimed fit per density (the plain path builds its kernels at the first fit), and one
+    plain fitter session per density that reuses its kernels from fit to fit (refit, as the
+    reference's toy study does, toystudy.hh:76-77), warmed by one fit"""
     cp = M.Context(device=0)
+    cp.sessions = {}
     for kind in (M.MASS, M.KSTAR):
         d = O.generate(kind, M.TRUTH[kind], 2000, seed=1, nstreams=4)
         cp.fit_plain(kind, d, M.TRUTH[kind], fp32=False)
+        cp.sessions[kind] = M.PlainFitter(kind, fp32=False, ctx=cp)
+        cp.sessions[kind].fit(d, M.TRUTH[kind])
     return cp
+
+
+def close_plain(cp):
+    for sess in cp.sessions.values():
+        sess.close()
+    cp.close()
 
 
 def serve_workload(ctx, cplain, M, O, nreq, rate, seed=11, plain_subset=12):
@@ -117,16 +132,20 @@ def serve_workload(ctx, cplain, M, O, nreq, rate, seed=11, plain_subset=12):
         row = run(np.cumsum(gaps / lam))
         row.update({"arrival_rate_per_s": lam, "load": label})
         rows.append(row)
-    t_plain = 0.0
+    t_plain = t_scratch = 0.0
     for i in range(plain_subset):
         d = O.generate(int(cls[i]), truth[cls[i]], int(n[i]), seed=int(seeds[i]) % (2**63), nstreams=64)
         t0 = time.perf_counter()
+        cplain.sessions[int(cls[i])].fit(d, truth[cls[i]])
+        t1 = time.perf_counter()
         cplain.fit_plain(int(cls[i]), d, truth[cls[i]], fp32=False)
-        t_plain += time.perf_counter() - t0
+        t_plain += t1 - t0
+        t_scratch += time.perf_counter() - t1
     # the subset's mix of sizes and classes differs from the whole stream's: scale by events
     ev_sub = float(np.sum(n[:plain_subset] * np.where(cls[:plain_subset] == 1, 3, 1)))
     ev_all = float(np.sum(n * np.where(cls == 1, 3, 1)))
     plain_rate = nreq / (t_plain * ev_all / ev_sub)
+    scratch_rate = nreq / (t_scratch * ev_all / ev_sub)
     mass.close()
     kst.close()
     fixed = rows[-1]
@@ -134,10 +153,132 @@ def serve_workload(ctx, cplain, M, O, nreq, rate, seed=11, plain_subset=12):
             "hbm_budget_bytes": 64 << 30, "saturation": sat, "sustained_fits_per_s": sustained, "rates": rows,
             "p50_latency_ms": fixed["p50_latency_ms"], "p99_latency_ms": fixed["p99_latency_ms"],
             "fits_per_s": fixed["fits_per_s"], "arrival_rate_per_s": rate,
-            "plain_fp64_fits_per_s": plain_rate,
-            "plain_fp64_sample": f"{plain_subset} requests fitted one at a time (FP64 plain path), "
-                                 "time scaled to the whole stream by its event count (angular events weighted x3)",
-            "speedup_vs_plain": sustained / plain_rate}
+            "plain_fp64_fits_per_s": plain_rate, "plain_fp64_from_scratch_fits_per_s": scratch_rate,
+            "plain_fp64_sample": f"{plain_subset} requests fitted one at a time through the plain FP64 path with its kernels "
+                                 "reused from fit to fit (and, beside it, built anew per fit), time scaled to the whole "
+                                 "stream by its event count (angular events weighted x3)",
+            "speedup_vs_plain": sustained / plain_rate, "speedup_vs_plain_from_scratch": sustained / scratch_rate}
+
+
+def reference_protocol(M, cplain, repeats):
+    """The reference's toy-study benchmark (benchmarking/morefit_massfit.cc:64-107): 100 toys of
+    the mass density at N = 1e3, 1e4, 1e5 and 1e6 events, repeated.  Batched FP32 path: the
+    toys generated on the device (one seed per toy) and fitted in one batched study, from the
+    generating values as the reference does; plain FP64 path: the same datasets fitted one at a
+    time with kernels reused (a subset at 1e6)."""
+    rows = []
+    ctx = M.Context(device=0)
+    m = ctx.model(M.MASS, fp32=True)
+    for n in (1000, 10000, 100000, 1000000):
+        ntoys = 100
+        b = m.batch([n] * ntoys)
+        walls = []
+        for r in range(repeats):
+            t0 = time.perf_counter()
+            b.generate(M.MASS_TRUTH, list(range(1000 * r + 1, 1000 * r + ntoys + 1)))
+            v, e, o, st = b.fit(M.MASS_TRUTH)
+            walls.append(time.perf_counter() - t0)
+        conv = sum(1 for x in o if x.status == 0)
+        nplain = ntoys if n < 1000000 else 10
+        tp = 0.0
+        for i in range(nplain):
+            d = b.download(i)
+            t1 = time.perf_counter()
+            cplain.sessions[M.MASS].fit(d, M.MASS_TRUTH)
+            tp += time.perf_counter() - t1
+        rows.append({"events": n, "toys": ntoys, "repeats": repeats, "batched_fp32_study_ms_mean": 1e3 * float(np.mean(walls)),
+                     "batched_fp32_study_ms_rms": 1e3 * float(np.std(walls, ddof=1)) if repeats > 1 else 0.0,
+                     "batched_fp32_toy_fits_per_s": ntoys / float(np.mean(walls)), "converged": conv,
+                     "plain_fp64_toy_fits_per_s": nplain / tp, "plain_fp64_sample_toys": nplain})
+        log(f"reference protocol N={n}: batched {rows[-1]['batched_fp32_toy_fits_per_s']:.0f} toys/s "
+            f"(study {rows[-1]['batched_fp32_study_ms_mean']:.1f} ms incl. generation), plain FP64 {rows[-1]['plain_fp64_toy_fits_per_s']:.1f} toys/s")
+        b.close()
+    m.close()
+    ctx.close()
+    return rows
+
+
+def cpu_baselines(M, O, data, start, seconds):
+    """The host CPU backend (generated C++ per density with the semantics of the reference's
+    LLVM backend: FP64, 4 events per AVX2 vector with libmvec, 4 interleaved Kahan lanes,
+    contiguous slices per thread), on this machine's cores: -2lnL+gradient evaluations/s at
+    N = 1e7 (a 2e6-event sample, scaled) and of the angular density at N = 1e6, built for the
+    host (-march=native) and for the reference's default target (x86-64-v3), on 1 thread and
+    on all threads; a config-1 fit (N = 1e5) with the analytic and with the reference
+    benchmark's numerical derivatives; toys of Poisson(5000) events fitted one after another
+    (kernels reused), on 1 thread and on all threads (one toy per thread)."""
+    import threading
+    nproc = os.cpu_count() or 1
+    # the one-GPU share of the machine's host cores that work may use (16), or fewer if the
+    # process is limited to fewer
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    native = "-O3 -march=native -fno-math-errno"
+    v3 = "-O3 -march=x86-64-v3 -fno-math-errno"
+    sample = 2_000_000
+    d = data[:, :sample]
+    out = {"threads_all": cores, "nproc": nproc}
+
+    def eval_rate(kind, dd, theta, threads, flags, scale_to):
+        p = M.PlainFitter(kind, fp32=False, nthreads=threads, cpu_flags=flags)
+        p.time_eval(dd, theta, 1)
+        iters, el = 2, 0.0
+        while True:
+            _, _, el = p.time_eval(dd, theta, iters)
+            if el > seconds / 8 or iters >= 4096:
+                break
+            iters *= 2
+        p.close()
+        return iters / el * dd.shape[1] / scale_to, iters
+
+    for label, threads, flags in (("native_1thread", 1, native), ("native_all", cores, native), ("v3_1thread", 1, v3), ("v3_all", cores, v3)):
+        r, it = eval_rate(M.MASS, d, start, threads, flags, N_EVENTS)
+        out["mass_1e7_" + label] = {"evals_per_s": r, "threads": threads, "flags": flags, "sample": f"{it} evaluations over {sample} events, scaled to 1e7"}
+    dk = O.generate(M.KSTAR, M.KSTAR_TRUTH, 200000, seed=3, nstreams=16)
+    for label, threads in (("native_1thread", 1), ("native_all", cores)):
+        r, it = eval_rate(M.KSTAR, dk, M.KSTAR_START, threads, native, 1_000_000)
+        out["angular_1e6_" + label] = {"evals_per_s": r, "threads": threads, "flags": native, "sample": f"{it} evaluations over 200000 events, scaled to 1e6"}
+    # config 1: a full MIGRAD + HESSE fit at N = 1e5
+    d1 = O.generate(M.MASS, M.MASS_TRUTH, 100000, seed=229387429, nstreams=16)
+    for label, flags in (("analytic", M.FIT_DEFAULT), ("numerical", M.FIT_KAHAN_ON_ACCELERATOR | M.FIT_POSTRUN_HESSE)):
+        for threads in (1, cores):
+            M.fit_options(M.MASS, d1, M.MASS_START, flags=flags, nthreads=threads)  # kernels compiled and cached
+            t0 = time.perf_counter()
+            v, e, o, ms = M.fit_options(M.MASS, d1, M.MASS_START, flags=flags, nthreads=threads)
+            out[f"fit_1e5_{label}_{threads}threads"] = {"wall_ms": 1e3 * (time.perf_counter() - t0), "fit_ms": ms,
+                                                         "status": o.status, "calls": o.ncalls}
+    # config 4 on the CPU: toys of Poisson(5000) events, one fit after another, kernels reused
+    rs = np.random.default_rng(4)
+    toys = [O.generate(M.MASS, M.MASS_TRUTH, int(k), seed=100 + i, nstreams=4) for i, k in enumerate(rs.poisson(5000, 64))]
+    starts = M.toy_starts(M.MASS, [t.shape[1] for t in toys], seed=4)
+
+    def toy_worker(idx, res, barrier=None):
+        # a session per thread, its kernels built by one untimed fit; then the timed fits
+        p = M.PlainFitter(M.MASS, fp32=False, nthreads=1)
+        p.fit(toys[0], starts[0])
+        if barrier is not None:
+            barrier.wait()
+        t0 = time.perf_counter()
+        for i in idx:
+            p.fit(toys[i], starts[i])
+        res.append((t0, time.perf_counter()))
+        p.close()
+
+    res1 = []
+    toy_worker(range(1, 33), res1)
+    out["toys_1thread"] = {"toy_fits_per_s": 32 / (res1[0][1] - res1[0][0]), "toys": 32, "events_mean": 5000}
+    resn, ths = [], []
+    per = 16
+    bar = threading.Barrier(cores)
+    for w in range(cores):
+        ths.append(threading.Thread(target=toy_worker, args=([1 + (w * per + j) % 63 for j in range(per)], resn, bar)))
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    span = max(e for _, e in resn) - min(b for b, _ in resn)
+    out["toys_all_threads"] = {"toy_fits_per_s": cores * per / span, "threads": cores, "toys": cores * per, "events_mean": 5000,
+                               "note": "one fitter session per thread, toys fitted concurrently (the ctypes calls release the GIL)"}
+    return out
 
 
 def angular_workload(M, O):
@@ -245,6 +386,7 @@ def main():
     ap.add_argument("--toys", type=int, default=4096)
     ap.add_argument("--plain-toys", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--protocol-repeats", type=int, default=3)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-angular", action="store_true")
@@ -403,25 +545,38 @@ def main():
         ok = np.array([o.status == 0 for o in to])
         pulls = (tv[ok] - M.MASS_TRUTH) / te[ok]
         batched_rate = args.toys / tfit
-        # plain per-fit FP64 GPU path on a subset (same toys, one fit at a time, one launch per call)
-        tplain = 0.0
+        # plain per-fit FP64 GPU path on a subset (same toys, one fit at a time, one launch per
+        # call): with its kernels reused from toy to toy (refit, the reference toy study's way)
+        # and, beside it, built anew for every fit
+        tplain = tscratch = 0.0
+        plain_dev = []
         for i in range(args.plain_toys):
             d = bt.download(i)
             t1 = time.perf_counter()
+            pv, pe, po, _ = cplain.sessions[M.MASS].fit(d, starts[i])
+            t2 = time.perf_counter()
             cplain.fit_plain(M.MASS, d, starts[i], fp32=False)
-            tplain += time.perf_counter() - t1
+            tplain += t2 - t1
+            tscratch += time.perf_counter() - t2
+            if po.status == 0 and to[i].status == 0:
+                plain_dev.append(float(np.max(np.abs(tv[i] - pv) / pe)))
         plain_rate = args.plain_toys / tplain
+        scratch_rate = args.plain_toys / tscratch
         toys = {"toys": args.toys, "mean_events": float(sizes.mean()), "generate_s": tgen, "fit_s": tfit,
                 "toy_fits_per_s": batched_rate, "study_fits_per_s_incl_generation": args.toys / (tgen + tfit),
-                "plain_fp64_toy_fits_per_s": plain_rate,
-                "speedup_vs_plain": batched_rate / plain_rate, "converged": int(ok.sum()),
+                "plain_fp64_toy_fits_per_s": plain_rate, "plain_fp64_from_scratch_toy_fits_per_s": scratch_rate,
+                "speedup_vs_plain": batched_rate / plain_rate, "speedup_vs_plain_from_scratch": batched_rate / scratch_rate,
+                "batched_fp32_vs_plain_fp64_max_dtheta_over_sigma": max(plain_dev) if plain_dev else None,
+                "converged": int(ok.sum()),
                 "pull_mean": pulls.mean(axis=0).round(4).tolist(), "pull_width": pulls.std(axis=0).round(4).tolist(),
                 "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
                 "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms, "host_threads": tst.host_threads,
                 "setup_ms": tst.setup_ms, "step_ms": tst.step_ms,
                 "launch_shape": shape(mt, tunet["best"]),
                 "autotune_table_us": [[list(c), round(u, 2)] for c, u in tunet["table"]]}
-        log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain ({batched_rate / plain_rate:.1f}x)")
+        log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain with kernel reuse ({batched_rate / plain_rate:.1f}x), "
+            f"{scratch_rate:.1f} plain from scratch; FP32 batched vs FP64 plain max {max(plain_dev) if plain_dev else float('nan'):.2e} sigma")
+        toys["reference_protocol"] = reference_protocol(M, cplain, args.protocol_repeats)
         bt.close()
         mt.close()
         ctxt.close()
@@ -434,29 +589,33 @@ def main():
             f"one at a time; " + "; ".join(f"{r['load']} ({r['arrival_rate_per_s']:.0f}/s): p50 {r['p50_latency_ms']:.1f} ms "
                                            f"p99 {r['p99_latency_ms']:.1f} ms" for r in serving["rates"]))
 
-    # ---------------------------------------------------------------- CPU baseline (oracle)
-    cpu = cpu1 = None
+    # ---------------------------------------------------------------- CPU baselines
+    cpu = cpu1 = cpu_port = cpu_more = None
     if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu_more = cpu_baselines(M, O, data, start, args.cpu_seconds)
+        nat, nat1 = cpu_more["mass_1e7_native_all"], cpu_more["mass_1e7_native_1thread"]
+        what = ("-2lnL+grad of the mass density through the host CPU backend (generated C++, FP64, AVX2 4-event vectors with "
+                "libmvec, 4 interleaved Kahan lanes, contiguous slices per thread: the reference LLVM backend's semantics), "
+                "-march=native; ")
+        cpu = {"value": nat["evals_per_s"], "unit": "evals/s", "cores": nat["threads"], "kind": "port",
+               "sample": what + nat["sample"] + f"; {cpu_more['nproc']} host CPUs seen, {nat['threads']} threads used"}
+        cpu1 = {"value": nat1["evals_per_s"], "unit": "evals/s", "cores": 1, "kind": "port", "sample": what + nat1["sample"]}
+        # the scalar oracle loop, as before, for comparison
         sample = 2_000_000
         d = data[:, :sample]
-
-        def cpu_rate(threads, seconds):
-            t0 = time.perf_counter()
-            reps = 0
-            while time.perf_counter() - t0 < seconds:
-                O.nll_sums(O.MASS, d, start, mode=1, nthreads=threads)
-                reps += 1
-            el = time.perf_counter() - t0
-            return reps, 1.0 / (el / reps * (n / sample))
-
-        cores = max(1, min(16, len(os.sched_getaffinity(0))))
-        reps1, rate1 = cpu_rate(1, args.cpu_seconds / 2)
-        repsn, raten = cpu_rate(cores, args.cpu_seconds / 2)
-        what = (f"evaluations of -2lnL+grad over the first {sample} of the {n} events (FP64, per-output Kahan, "
-                f"compute_llvm.hh semantics: contiguous slices per thread), scaled to N={n}")
-        cpu = {"value": raten, "unit": "evals/s", "cores": cores, "kind": "port", "sample": f"{repsn} {what}"}
-        cpu1 = {"value": rate1, "unit": "evals/s", "cores": 1, "kind": "port", "sample": f"{reps1} {what}"}
-        log(f"cpu baseline: {raten:.3f} evals/s ({cores} threads), {rate1:.3f} evals/s (1 thread)")
+        t1 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t1 < 2.0:
+            O.nll_sums(O.MASS, d, start, mode=1, nthreads=1)
+            reps += 1
+        cpu_port = {"value": 1.0 / ((time.perf_counter() - t1) / reps * (n / sample)), "unit": "evals/s", "cores": 1,
+                    "kind": "port", "sample": f"{reps} evaluations of the scalar oracle loop over {sample} events, scaled to N={n}"}
+        log(f"cpu baseline ({time.perf_counter() - t0:.0f} s): {cpu['value']:.2f} evals/s ({cpu['cores']} threads), {cpu1['value']:.2f} "
+            f"(1 thread), x86-64-v3 1 thread {cpu_more['mass_1e7_v3_1thread']['evals_per_s']:.2f}; scalar oracle {cpu_port['value']:.2f}; "
+            f"config-1 fit {cpu_more['fit_1e5_analytic_1threads']['wall_ms']:.0f} ms (analytic, 1 thread), "
+            f"{cpu_more['fit_1e5_numerical_1threads']['wall_ms']:.0f} ms (numerical); toys {cpu_more['toys_1thread']['toy_fits_per_s']:.1f}/s "
+            f"(1 thread), {cpu_more['toys_all_threads']['toy_fits_per_s']:.1f}/s ({cpu_more['toys_all_threads']['threads']} threads)")
 
     out = {
         "metric": "-2lnL+grad evals/s; toy fits/s vs plain GPU path; p50/p99 fit latency",
@@ -481,6 +640,8 @@ def main():
                      "kernel_ms": k32_ms, "algorithmic_bytes": bytes32, "kernels_of_a_call_ms": k32_pair_ms},
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
+        "cpu_baseline_scalar_oracle": cpu_port,
+        "cpu_baselines": cpu_more,
         "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9,
                  "launch_shape": shape(m64, tune64["best"])},
         "lh_kernel_ms_fp32": k32_lh_ms,
@@ -499,7 +660,7 @@ def main():
     m64.close()
     ctx.close()
     ctx64.close()
-    cplain.close()
+    close_plain(cplain)
     print(json.dumps(out), flush=True)
 
 

@@ -16,6 +16,7 @@
  *   mfh_fit_weighted     fitter::fit of weighted events with correct_weighted_fit (fitter.hh:894-947, 1327-1432)
  *   mfh_fit_options      fitter::fit with analytic_gradient / analytic_hessian / kahan_on_accelerator /
  *                        postrun_hesse switched (fitter.hh:38-106, 796-814, 1110-1124)
+ *   mfh_plain_*          fitter::fit per dataset with kernels reused (refit, toystudy.hh:76-77), device or CPU
  *   mfh_batch_fit        toystudy::toy's fit loop (toystudy.hh:55-111) as one batched study
  *   mfh_serve_run        no counterpart: continuous-batching fit service (serve.hh)
  * Status: 0 on success, else nonzero with mfh_last_error().
@@ -190,6 +191,20 @@ int mfh_fit_weighted(mfh_context* ctx, int kind, int fp32, const double* data, c
 #define MFH_FIT_DEFAULT MFH_FIT_ALL
 int mfh_fit_options(mfh_context* ctx, int kind, int fp32, const double* data, long long n, const double* start, int nthreads, int flags,
                     double* values, double* errors, mfh_fit_out* out, double* ms);
+/* A plain fitter session (fitter::fit one dataset at a time, kernels built at the first fit and
+ * reused by the next ones with refit, as the reference's toy study does, toystudy.hh:76-77):
+ * ctx given: the device's drop-in HIPBlock path; ctx NULL: the host CPU backend with nthreads
+ * threads (cpu_flags: host compiler flags of its kernels, NULL for the default).  flags as for
+ * mfh_fit_options.  mfh_plain_eval binds a dataset and times `iters` -2lnL + gradient
+ * evaluations at theta (seconds), returning the last value and gradient.  A session is not
+ * thread-safe; sessions are independent of each other. */
+typedef struct mfh_plain mfh_plain;
+int mfh_plain_create(mfh_context* ctx, int kind, int fp32, int nthreads, int flags, const char* cpu_flags, mfh_plain** out);
+int mfh_plain_fit(mfh_plain* p, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
+                  double* ms);
+int mfh_plain_eval(mfh_plain* p, const double* data, long long n, const double* theta, int iters, double* value, double* grad,
+                   double* seconds);
+int mfh_plain_destroy(mfh_plain* p);
 int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess);
 
 #ifdef __cplusplus

@@ -80,6 +80,7 @@ CAPI_SYMBOLS = [
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted", "mfh_fit_options",
+    "mfh_plain_create", "mfh_plain_fit", "mfh_plain_eval", "mfh_plain_destroy",
 ]
 
 
@@ -143,6 +144,10 @@ def libs():
         L.mfh_fit_weighted.argtypes = [vp, ip, ip, dp, dp, ctypes.c_longlong, dp, ip, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_eval_cpu.argtypes = [ip, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
         L.mfh_fit_options.argtypes = [vp, ip, ip, dp, ctypes.c_longlong, dp, ip, ip, dp, dp, ctypes.POINTER(FitOut), dp]
+        L.mfh_plain_create.argtypes = [vp, ip, ip, ip, ip, ctypes.c_char_p, pp]
+        L.mfh_plain_fit.argtypes = [vp, dp, ctypes.c_longlong, dp, dp, dp, ctypes.POINTER(FitOut), dp]
+        L.mfh_plain_eval.argtypes = [vp, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
+        L.mfh_plain_destroy.argtypes = [vp]
         L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_ulonglong), dp, ip,
                                     ctypes.c_double, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_int), dp, dp]
@@ -467,6 +472,40 @@ def fit_options(kind, data, start, flags=FIT_DEFAULT, ctx=None, fp32=False, nthr
     _check(libs()[1].mfh_fit_options(h, kind, int(fp32), _dp(data), data.shape[1], _dp(_f64(start)), nthreads, int(flags),
                                      _dp(vals), _dp(errs), ctypes.byref(out), ctypes.byref(ms)))
     return vals, errs, out, ms.value
+
+
+class PlainFitter:
+    """fitter::fit one dataset at a time with the kernels built once (refit): on the device's
+    drop-in path when ctx is given, else on the CPU backend (nthreads, cpu_flags)."""
+
+    def __init__(self, kind, fp32=False, ctx=None, nthreads=1, flags=FIT_DEFAULT, cpu_flags=None):
+        self.kind = kind
+        self.h = ctypes.c_void_p()
+        _check(libs()[1].mfh_plain_create(ctx.h if ctx is not None else None, kind, int(fp32), nthreads, int(flags),
+                                          cpu_flags.encode() if cpu_flags else None, ctypes.byref(self.h)))
+
+    def fit(self, data, start):
+        data = _f64(np.atleast_2d(data))
+        P = NTHETA[self.kind]
+        vals, errs = np.zeros(P), np.zeros(P)
+        out, ms = FitOut(), ctypes.c_double(0)
+        _check(libs()[1].mfh_plain_fit(self.h, _dp(data), data.shape[1], _dp(_f64(start)), _dp(vals), _dp(errs),
+                                       ctypes.byref(out), ctypes.byref(ms)))
+        return vals, errs, out, ms.value
+
+    def time_eval(self, data, theta, iters):
+        """-2lnL and gradient at theta, and the seconds `iters` evaluations took"""
+        data = _f64(np.atleast_2d(data))
+        v, sec = ctypes.c_double(0), ctypes.c_double(0)
+        g = np.zeros(NTHETA[self.kind])
+        _check(libs()[1].mfh_plain_eval(self.h, _dp(data), data.shape[1], _dp(_f64(theta)), int(iters), ctypes.byref(v), _dp(g),
+                                        ctypes.byref(sec)))
+        return v.value, g, sec.value
+
+    def close(self):
+        if self.h:
+            libs()[1].mfh_plain_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
 
 def eval_cpu(kind, data, theta, nthreads=1):

@@ -819,6 +819,135 @@ int mfh_fit_options(mfh_context* ctx, int kind, int fp32, const double* data, lo
   MFH_CATCH
 }
 
+}  // extern "C"
+
+// A plain fitter session: one density, one backend, kernels built at the first fit and reused
+// by later ones (fitter::fit with refit, as the reference's toy study reuses them,
+// toystudy.hh:76-77).
+struct mfh_plain {
+  virtual ~mfh_plain() = default;
+  virtual void fit(const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out, double* ms) = 0;
+  virtual void eval(const double* data, long long n, const double* theta, int iters, double* value, double* grad, double* seconds) = 0;
+};
+
+namespace {
+template <typename K, typename Backend, typename Block>
+struct PlainSession : mfh_plain {
+  compute_options co;
+  std::unique_ptr<Backend> own;
+  Backend* be;
+  ModelDef<K> def;
+  fitter_options fo;
+  fitter<K, double, Backend, Block> f;
+  bool built = false;
+  // shared: the context's backend; null: a backend of this session's own (the CPU backend with
+  // nthreads threads and, when given, the host compiler flags cpu_flags)
+  PlainSession(Backend* shared, int kind, int flags, int nthreads, const char* cpu_flags)
+      : co(cpu_options(nthreads, cpu_flags)),
+        own(shared ? nullptr : new Backend(&co)),
+        be(shared ? shared : own.get()),
+        def(kind, nullptr),
+        fo(options(flags)),
+        f(&fo, be) {}
+  static compute_options cpu_options(int nthreads, const char* cpu_flags) {
+    compute_options c;
+    c.llvm_nthreads = nthreads > 0 ? nthreads : 1;
+    if (cpu_flags && *cpu_flags) c.cpu_flags = cpu_flags;
+    return c;
+  }
+  static fitter_options options(int flags) {
+    fitter_options o;
+    o.analytic_gradient = (flags & MFH_FIT_ANALYTIC_GRADIENT) != 0;
+    o.analytic_hessian = (flags & MFH_FIT_ANALYTIC_HESSIAN) != 0;
+    o.kahan_on_accelerator = (flags & MFH_FIT_KAHAN_ON_ACCELERATOR) != 0;
+    o.postrun_hesse = (flags & MFH_FIT_POSTRUN_HESSE) != 0;
+    return o;
+  }
+  EventVector<double, double> events(const double* data, long long n) {
+    if (!data || n <= 0) throw std::invalid_argument("plain session: no events");
+    EventVector<double, double> ev(def.dimv, (unsigned long)n);
+    for (size_t d = 0; d < def.dimv.size(); d++)
+      for (long long i = 0; i < n; i++) ev(i, (unsigned)d) = data[d * n + i];
+    return ev;
+  }
+  void fit(const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out, double* ms) override {
+    auto ev = events(data, n);
+    for (size_t k = 0; k < def.params.size(); k++) {
+      def.params[k]->set_value(start[k]);
+      def.params[k]->set_start_value(start[k]);
+    }
+    f.fit(def.pdf, def.params, &ev, built);
+    built = true;
+    for (size_t k = 0; k < def.params.size(); k++) {
+      values[k] = def.params[k]->get_value();
+      errors[k] = def.params[k]->get_error();
+    }
+    if (out) {
+      out->fval = f.fval_;
+      out->edm = f.edm_;
+      out->status = f.status_;
+      out->ncalls = f.ncalls_;
+      out->iterations = 0;
+      out->hesse_ok = !f.cov_.empty();
+      out->thread_moved = 0;
+    }
+    if (ms) *ms = f.fit_ms_;
+  }
+  void eval(const double* data, long long n, const double* theta, int iters, double* value, double* grad, double* seconds) override {
+    if (!fo.analytic_gradient) throw std::invalid_argument("plain session: evaluation timing needs the analytic gradient");
+    auto ev = events(data, n);
+    f.prepare(def.pdf, def.params, &ev, built);
+    built = true;
+    for (size_t k = 0; k < def.params.size(); k++) def.params[k]->set_value(theta[k]);
+    const size_t N = def.params.size();
+    std::vector<double> g(N);
+    double v = f.likelihood_and_gradient((int)N, g.data());
+    auto t0 = std::chrono::steady_clock::now();
+    for (int it = 0; it < iters; it++) v = f.likelihood_and_gradient((int)N, g.data());
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (value) *value = v;
+    if (grad) std::memcpy(grad, g.data(), N * sizeof(double));
+  }
+};
+}  // namespace
+
+extern "C" {
+
+int mfh_plain_create(mfh_context* ctx, int kind, int fp32, int nthreads, int flags, const char* cpu_flags, mfh_plain** out) {
+  MFH_TRY
+  if (!out) throw std::invalid_argument("mfh_plain_create: null output");
+  if (flags & ~MFH_FIT_ALL) throw std::invalid_argument("mfh_plain_create: unknown flag bits");
+  if (ctx) {
+    if (fp32) *out = new PlainSession<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, flags, 0, nullptr);
+    else *out = new PlainSession<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, flags, 0, nullptr);
+  } else {
+    if (fp32) *out = new PlainSession<float, CPUBackend, CPUBlock<float, double>>(nullptr, kind, flags, nthreads, cpu_flags);
+    else *out = new PlainSession<double, CPUBackend, CPUBlock<double, double>>(nullptr, kind, flags, nthreads, cpu_flags);
+  }
+  MFH_CATCH
+}
+
+int mfh_plain_fit(mfh_plain* p, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
+                  double* ms) {
+  MFH_TRY
+  if (!p || !start || !values || !errors) throw std::invalid_argument("mfh_plain_fit: null argument");
+  p->fit(data, n, start, values, errors, out, ms);
+  MFH_CATCH
+}
+
+int mfh_plain_eval(mfh_plain* p, const double* data, long long n, const double* theta, int iters, double* value, double* grad,
+                   double* seconds) {
+  MFH_TRY
+  if (!p || !theta || iters < 0) throw std::invalid_argument("mfh_plain_eval: bad argument");
+  p->eval(data, n, theta, iters, value, grad, seconds);
+  MFH_CATCH
+}
+
+int mfh_plain_destroy(mfh_plain* p) {
+  delete p;
+  return 0;
+}
+
 int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess) {
   MFH_TRY
   compute_options co;

@@ -129,13 +129,21 @@ inline std::string emit_shift_prologue(KernelEmitter& em, const std::vector<std:
   return pro;
 }
 
+// vector_type (host CPU backend): the per-event values are of that SIMD type, one lane per
+// event -- x and o point to vectors, parameters stay scalar, and every temporary goes through
+// mf_bc (a broadcast of scalar subexpressions, the identity on vectors)
 template <typename kernelT, typename evalT>
 std::string emit_functor(const std::string& name, const std::vector<const ComputeGraphNode<kernelT, evalT>*>& acc,
                          const std::vector<std::string>& observables, const std::vector<std::string>& param_names,
                          bool fp32, size_t* statements = nullptr, const CombineForm* form = nullptr,
-                         const std::vector<double>* shift = nullptr) {
+                         const std::vector<double>* shift = nullptr, const std::string& vector_type = "") {
   KernelEmitter em(fp32);
-  const std::string pro = emit_shift_prologue(em, observables, shift, fp32 ? "float" : "double");
+  const std::string vt = vector_type.empty() ? "MF_T" : vector_type;
+  if (!vector_type.empty()) {
+    em.T = vector_type;
+    em.wrap = "mf_bc";
+  }
+  const std::string pro = emit_shift_prologue(em, observables, shift, vector_type.empty() ? (fp32 ? "float" : "double") : vector_type);
   for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
   for (auto* g : acc) em.count_products(*g);
   std::vector<std::string> res;
@@ -143,10 +151,11 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
   const size_t nout = form ? form->size() : acc.size();
   std::ostringstream s;
   s << "struct " << name << " {\n  static constexpr int nacc = " << acc.size() << ";\n  static constexpr int nout = " << nout << ";\n"
-    << "  __device__ __forceinline__ static void eval(const MF_T* __restrict__ x, const MF_T* __restrict__ p, MF_T* __restrict__ o) {\n"
+    << "  __device__ __forceinline__ static void eval(const " << vt << "* __restrict__ x, const MF_T* __restrict__ p, " << vt
+    << "* __restrict__ o) {\n"
     << "    (void)x; (void)p;\n"
     << pro << em.body.str();
-  for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << res[k] << ";\n";
+  for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << (vector_type.empty() ? res[k] : "mf_bc(" + res[k] + ")") << ";\n";
   s << "  }\n  __device__ __forceinline__ static void combine(const double* S, const double* h, double n, double* out) {\n"
     << "    (void)S; (void)h; (void)n;\n";
   for (size_t k = 0; k < nout; k++) {

@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -55,8 +56,32 @@ inline uint64_t cpu_fnv1a(const std::string& s) {
   return h;
 }
 
-// host translation unit around generated functors
-inline std::string cpu_source(const ModuleSource& ms, const std::string& kernel_fn) {
+// FP64 SIMD code of the host backend: 4 events per AVX2 vector, the transcendentals from
+// glibc's vector math library (libmvec, its AVX2 variants _ZGVdN4v_*), the interleaved
+// 4-lane Kahan sums of the reference's vectorised LLVM kernels (compute_llvm.hh:556-568,
+// 656-709)
+inline std::string cpu_vector_prelude() {
+  std::ostringstream s;
+  s << "#include <immintrin.h>\n#include <cstring>\n"
+    << "typedef double mf_v4d __attribute__((vector_size(32)));\n"
+    << "static inline mf_v4d mf_bc(mf_v4d v) { return v; }\n"
+    << "static inline mf_v4d mf_bc(double v) { return mf_v4d{v, v, v, v}; }\n";
+  const char* fns[] = {"exp", "log", "exp2", "log2", "sin", "cos", "tan", "asin", "acos", "atan", "erf"};
+  for (auto* f : fns)
+    s << "extern \"C\" __m256d _ZGVdN4v_" << f << "(__m256d);\n"
+      << "static inline mf_v4d mf_" << f << "(mf_v4d x) { return (mf_v4d)_ZGVdN4v_" << f << "((__m256d)x); }\n";
+  s << "extern \"C\" __m256d _ZGVdN4vv_pow(__m256d, __m256d);\n"
+    << "static inline mf_v4d mf_pow(mf_v4d x, mf_v4d y) { return (mf_v4d)_ZGVdN4vv_pow((__m256d)x, (__m256d)y); }\n"
+    << "static inline mf_v4d mf_pow(mf_v4d x, double y) { return mf_pow(x, mf_bc(y)); }\n"
+    << "static inline mf_v4d mf_pow(double x, mf_v4d y) { return mf_pow(mf_bc(x), y); }\n"
+    << "static inline mf_v4d mf_sqrt(mf_v4d x) { return (mf_v4d)_mm256_sqrt_pd((__m256d)x); }\n"
+    << "static inline mf_v4d mf_rcp(mf_v4d x) { return 1.0 / x; }\n";
+  return s.str();
+}
+
+// host translation unit around generated functors; `vector_functor` (FP64 only): the same
+// outputs as a SIMD functor mf_v_block, which the sum then runs 4 events at a time
+inline std::string cpu_source(const ModuleSource& ms, const std::string& kernel_fn, const std::string& vector_functor = "") {
   std::ostringstream s;
   s << "#include <cmath>\n#include <cstddef>\n"
     << "#define __device__\n#define __forceinline__ inline\n"
@@ -71,6 +96,37 @@ inline std::string cpu_source(const ModuleSource& ms, const std::string& kernel_
     << "static inline MF_T mf_pow(MF_T x, MF_T y) { return std::pow(x, y); }\n"
     << "static inline double mfd_pow(double x, double y) { return std::pow(x, y); }\n"
     << ms.functors << "\n";
+  if (!vector_functor.empty()) {
+    s << cpu_vector_prelude() << vector_functor << "\n";
+    // 4 interleaved lane sums with in-loop Kahan per output; events of the tail go to the lane
+    // of their position in the slice; the lanes' sums then added with Kahan (the lanes'
+    // carries dropped, as in the reference's final pass)
+    s << "extern \"C\" void " << kernel_fn << "_sum(const MF_T* data, long long stride, long long from, long long to, const MF_T* p, double* out) {\n"
+      << "  const int NO = mf_f_block::nout;\n  mf_v4d sum[NO], c[NO];\n"
+      << "  for (int k = 0; k < NO; k++) { sum[k] = mf_bc(0.0); c[k] = mf_bc(0.0); }\n"
+      << "  long long i = from;\n"
+      << "  for (; i + 4 <= to; i += 4) {\n    mf_v4d x[" << ms.nobs << "];\n"
+      << "    for (int d = 0; d < " << ms.nobs << "; d++) std::memcpy(&x[d], data + d * stride + i, sizeof(mf_v4d));\n"
+      << "    mf_v4d o[NO];\n    mf_v_block::eval(x, p, o);\n"
+      << "    for (int k = 0; k < NO; k++) { const mf_v4d y = o[k] - c[k]; const mf_v4d t = sum[k] + y; c[k] = (t - sum[k]) - y; sum[k] = t; }\n"
+      << "  }\n"
+      << "  double ls[NO][4], lc[NO][4];\n"
+      << "  for (int k = 0; k < NO; k++) for (int j = 0; j < 4; j++) { ls[k][j] = sum[k][j]; lc[k][j] = c[k][j]; }\n"
+      << "  for (; i < to; i++) {\n    const int j = (int)((i - from) & 3);\n    MF_T x[" << ms.nobs << "];\n"
+      << "    for (int d = 0; d < " << ms.nobs << "; d++) x[d] = data[d * stride + i];\n"
+      << "    MF_T o[NO];\n    mf_f_block::eval(x, p, o);\n"
+      << "    for (int k = 0; k < NO; k++) { double y = (double)o[k] - lc[k][j]; volatile double t = ls[k][j] + y; lc[k][j] = (t - ls[k][j]) - y; ls[k][j] = t; }\n"
+      << "  }\n"
+      << "  for (int k = 0; k < NO; k++) {\n    double s = 0.0, cc = 0.0;\n"
+      << "    for (int j = 0; j < 4; j++) { double y = ls[k][j] - cc; volatile double t = s + y; cc = (t - s) - y; s = t; }\n"
+      << "    out[k] = s;\n  }\n}\n";
+    s << "extern \"C\" void " << kernel_fn << "_store(const MF_T* data, long long stride, long long from, long long to, const MF_T* p, MF_T* out, long long ostride) {\n"
+      << "  const int NO = mf_f_block::nout;\n"
+      << "  for (long long i = from; i < to; i++) {\n    MF_T x[" << ms.nobs << "];\n"
+      << "    for (int d = 0; d < " << ms.nobs << "; d++) x[d] = data[d * stride + i];\n"
+      << "    MF_T o[NO];\n    mf_f_block::eval(x, p, o);\n    for (int k = 0; k < NO; k++) out[k * ostride + i] = o[k];\n  }\n}\n";
+    return s.str();
+  }
   // sums: in-loop Kahan per output in double (evalT), events [from, to)
   s << "extern \"C\" void " << kernel_fn << "_sum(const MF_T* data, long long stride, long long from, long long to, const MF_T* p, double* out) {\n"
     << "  const int NO = mf_f_block::nout;\n  double sum[NO], c[NO];\n  for (int k = 0; k < NO; k++) { sum[k] = 0.0; c[k] = 0.0; }\n"
@@ -125,6 +181,9 @@ inline std::string cpu_generate_source(const ModuleSource& ms) {
 
 // compiles (or reuses) a shared object for `src`; returns the dlopen handle
 inline void* cpu_compile(const std::string& src, const compute_options& o) {
+  // one compilation at a time per process (the temporary files are named by process)
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
   std::string key = src + "\n" + o.cpu_compiler + " " + o.cpu_flags;
   char name[64];
   std::snprintf(name, sizeof(name), "/cpu_%016llx", (unsigned long long)cpu_fnv1a(key));
@@ -138,7 +197,7 @@ inline void* cpu_compile(const std::string& src, const compute_options& o) {
       f << src;
     }
     std::string tmpso = base + "." + std::to_string(::getpid()) + ".so";
-    std::string cmd = o.cpu_compiler + " " + o.cpu_flags + " -std=c++17 -fPIC -shared " + cc + " -o " + tmpso + " 2>&1";
+    std::string cmd = o.cpu_compiler + " " + o.cpu_flags + " -std=c++17 -fPIC -shared " + cc + " -o " + tmpso + " -lmvec -lm 2>&1";
     FILE* pp = ::popen(cmd.c_str(), "r");
     std::string log;
     char buf[512];
@@ -251,7 +310,13 @@ class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
     std::vector<const ComputeGraphNode<kernelT, evalT>*> outs;
     for (auto& g : graphs) outs.push_back(g.get());
     ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
-    std::string src = cpu_source(ms, "mf_cpu");
+    // FP64 sums run 4 events per AVX2 vector when vectorisation is on (the reference's
+    // llvm_vectorization, width 4)
+    const compute_options& o = *s_->be->opts_;
+    std::string vf;
+    if (!ms.fp32 && o.llvm_vectorization && kahan_summation)
+      vf = emit_functor("mf_v_block", outs, obs, params, false, nullptr, nullptr, nullptr, "mf_v4d");
+    std::string src = cpu_source(ms, "mf_cpu", vf);
     if (s_->be->opts_->print_kernel) std::cout << src << std::endl;
     if (s_->handle) ::dlclose(s_->handle);
     s_->handle = cpu_compile(src, *s_->be->opts_);

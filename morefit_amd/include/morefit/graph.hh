@@ -1406,7 +1406,9 @@ bool exp_pivot(std::vector<GraphPtr<kernelT, evalT>>& monos, const std::vector<s
 // `bindings` maps variable names to C expressions (observables / parameter slots).
 // Math goes through mf_* wrappers that the kernel prologue defines per precision.
 struct KernelEmitter {
-  std::string T;          // "float" or "double"
+  std::string T;          // type of the emitted temporaries: "float", "double", or a vector type
+  std::string S;          // the scalar type (literal casts, conditions)
+  std::string wrap;       // applied to every temporary's expression (vector code: broadcast of scalars)
   bool fp32 = false;
   std::ostringstream body;
   std::unordered_map<std::string, std::string> memo;
@@ -1425,11 +1427,11 @@ struct KernelEmitter {
   std::unordered_map<uint64_t, int> prod_freq;
   std::unordered_map<const void*, uint64_t> hash_memo;
 
-  explicit KernelEmitter(bool is_fp32) : T(is_fp32 ? "float" : "double"), fp32(is_fp32) {}
+  explicit KernelEmitter(bool is_fp32) : T(is_fp32 ? "float" : "double"), S(T), fp32(is_fp32) {}
 
   std::string literal(double v) const {
-    if (std::isinf(v)) return v > 0 ? "(" + T + ")(1.0/0.0)" : "(" + T + ")(-1.0/0.0)";
-    if (std::isnan(v)) return "(" + T + ")(0.0/0.0)";
+    if (std::isinf(v)) return v > 0 ? "(" + S + ")(1.0/0.0)" : "(" + S + ")(-1.0/0.0)";
+    if (std::isnan(v)) return "(" + S + ")(0.0/0.0)";
     char buf[64];
     if (fp32) std::snprintf(buf, sizeof(buf), "%af", (double)(float)v);
     else std::snprintf(buf, sizeof(buf), "%a", v);
@@ -1444,7 +1446,7 @@ struct KernelEmitter {
   }
   std::string fresh(const std::string& key, const std::string& expr) {
     std::string n = tmp + std::to_string(next++);
-    body << "  const " << T << " " << n << " = " << expr << ";\n";
+    body << "  const " << T << " " << n << " = " << (wrap.empty() ? expr : wrap + "(" + expr + ")") << ";\n";
     statements++;
     cost += op_cost(expr);
     memo[key] = n;
@@ -1556,7 +1558,8 @@ struct KernelEmitter {
       }
       case NodeKind::Conditional: {
         static const char* cs[] = {"==", "!=", ">", "<", ">=", "<="};
-        e = "(" + ops[0] + " " + cs[n.comparator()] + " (" + T + ")0) ? " + ops[1] + " : " + ops[2];
+        if (wrap.empty()) e = "(" + ops[0] + " " + cs[n.comparator()] + " (" + S + ")0) ? " + ops[1] + " : " + ops[2];
+        else e = "(" + ops[0] + " " + cs[n.comparator()] + " (" + S + ")0) ? " + wrap + "(" + ops[1] + ") : " + wrap + "(" + ops[2] + ")";
         break;
       }
       default: e = fn(n.kind()) + "(" + ops[0] + ")"; break;
