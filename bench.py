@@ -18,6 +18,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-toys] [--no-cpu
        python bench.py --profile   (counter workload: the headline FP32 gradient and value
                                     kernels at N = 10^7 and the copy kernel over 40 MB, 20
                                     launches each, no JSON; for rocprofv3 --pmc passes)
+       python bench.py --profile others   (the same for the other gradient kernels: FP64 mass at
+                                    N = 10^7, angular FP32 and FP64 at N = 10^6, the 4096-toy
+                                    FP32 launch; each in its autotuned shape, 20 launches)
 Progress goes to stderr; stdout carries exactly one JSON line.
 """
 import argparse
@@ -51,11 +54,12 @@ def shape(model, tuned):
             "segment_reduction": "in-kernel" if model.fused[M.EVAL_GRAD] else "final kernel"}
 
 
-def tuned_context(M, kind, fp32, sizes, label, **options):
-    """a context whose launch shape the autotuner picked for this workload"""
+def tuned_context(M, kind, fp32, sizes, label, candidates=None, **options):
+    """a context whose launch shape the autotuner picked for this workload (among `candidates`,
+    default: the built-in set for its size)"""
     ctx = M.Context(device=0, **options)
     t0 = time.perf_counter()
-    r = ctx.autotune(kind, fp32, sizes, eval_kind=M.EVAL_GRAD)
+    r = ctx.autotune(kind, fp32, sizes, eval_kind=M.EVAL_GRAD, candidates=candidates)
     log(f"autotune {label}: block {r['best'][0]}, {r['best'][1]} events/thread, unroll {r['best'][2]} "
         f"({min(u for _, u in r['table']):.2f} us; {len(r['table'])} shapes in {time.perf_counter() - t0:.1f} s)")
     return ctx, r
@@ -77,6 +81,30 @@ def measured_traffic(current_shape):
     if "traffic_bytes_per_launch" not in kv:
         return None, None
     return float(kv["traffic_bytes_per_launch"]), kv.get("launch_shape", "").strip()
+
+
+OTHERS_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp64_mass_angular_toys_counters.txt")
+
+
+def others_traffic():
+    """HBM bytes per launch of the secondary gradient kernels from their committed counter
+    summary, by workload label ({} without it)"""
+    try:
+        lines = open(OTHERS_FILE).read().splitlines()
+    except OSError:
+        return {}
+    out = {}
+    for l in lines:
+        if l.startswith("traffic_bytes_per_launch["):
+            k, v = l.split("]:", 1)
+            out[k[len("traffic_bytes_per_launch["):]] = float(v)
+    return out
+
+
+def roofline_row(label, kernel_ms, algorithmic_bytes, traffic):
+    gbs = algorithmic_bytes / (kernel_ms * 1e-3) / 1e9
+    return {"workload": label, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": traffic.get(label), "kernel_ms": kernel_ms, "algorithmic_bytes": algorithmic_bytes}
 
 
 def plain_context(M, O):
@@ -375,6 +403,33 @@ def profile_workload():
     log("profile: copy 40 MB", ctx.copy_ceiling(40_000_000, iters=20), "GB/s")
 
 
+def profile_others():
+    """Gradient kernels below the headline, for hardware-counter passes: FP64 mass at N = 10^7,
+    angular FP32 and FP64 at N = 10^6, and the 4096-toy FP32 batch (Poisson(5000) events per
+    toy), each in a fixed launch shape (the one the autotuner picked for it without a profiler
+    attached: tuning under counter collection, which serialises the dispatches, picks
+    others), 20 back-to-back launches of the per-chunk kernel, labelled on stderr by grid size"""
+    import morefit_amd as M
+    jobs = [("mass FP64 N=1e7", M.MASS, False, [N_EVENTS], (256, 32, 8)),
+            ("angular FP32 N=1e6", M.KSTAR, True, [1_000_000], (128, 8, 8)),
+            ("angular FP64 N=1e6", M.KSTAR, False, [1_000_000], (256, 8, 8)),
+            ("4096 toys FP32", M.MASS, True, list(np.random.default_rng(229387429).poisson(5000, 4096)), (256, 32, 8))]
+    for label, kind, fp32, sizes, fixed in jobs:
+        ctx, tune = tuned_context(M, kind, fp32, sizes, label, candidates=[fixed])
+        m = ctx.model(kind, fp32=fp32)
+        b = m.batch(sizes)
+        b.generate(M.TRUTH[kind], list(range(1, len(sizes) + 1)))
+        s = shape(m, tune["best"])
+        ms = b.time_kernel(M.EVAL_GRAD, iters=20, main_only=True)
+        chunk = s["block"] * s["events_per_thread"]
+        nch = int(sum((int(n) + chunk - 1) // chunk for n in sizes))
+        log(f"profile {label}: block={s['block']} ept={s['events_per_thread']} unroll={s['unroll']['grad']} vgprs={s['vgprs']['grad']} "
+            f"grid={nch * s['block']} ({nch} chunks), {ms * 1e3:.2f} us per launch, {sum(sizes)} events")
+        b.close()
+        m.close()
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -387,7 +442,7 @@ def main():
     ap.add_argument("--plain-toys", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--protocol-repeats", type=int, default=3)
-    ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--profile", nargs="?", const="headline", default=None, choices=["headline", "others"])
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-angular", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=300)
@@ -398,8 +453,11 @@ def main():
               "(the likelihood path is measured on one MI355X's share; multi-GPU sharding is a design note only).",
               file=sys.stderr)
         sys.exit(2)
-    if args.profile:
+    if args.profile == "headline":
         profile_workload()
+        return
+    if args.profile == "others":
+        profile_others()
         return
 
     import morefit_amd as M
@@ -545,6 +603,7 @@ def main():
         ok = np.array([o.status == 0 for o in to])
         pulls = (tv[ok] - M.MASS_TRUTH) / te[ok]
         batched_rate = args.toys / tfit
+        toy_kernel_ms = bt.time_kernel(M.EVAL_GRAD, iters=50, main_only=not mt.fused[M.EVAL_GRAD])
         # plain per-fit FP64 GPU path on a subset (same toys, one fit at a time, one launch per
         # call): with its kernels reused from toy to toy (refit, the reference toy study's way)
         # and, beside it, built anew for every fit
@@ -572,7 +631,7 @@ def main():
                 "ticks": tst.ticks, "launches": [tst.launches_lh, tst.launches_grad, tst.launches_hess],
                 "host_ms": tst.host_ms, "gpu_wait_ms": tst.gpu_wait_ms, "host_threads": tst.host_threads,
                 "setup_ms": tst.setup_ms, "step_ms": tst.step_ms,
-                "launch_shape": shape(mt, tunet["best"]),
+                "launch_shape": shape(mt, tunet["best"]), "grad_kernel_ms": toy_kernel_ms,
                 "autotune_table_us": [[list(c), round(u, 2)] for c, u in tunet["table"]]}
         log(f"toys: {batched_rate:.0f} fits/s batched vs {plain_rate:.1f} plain with kernel reuse ({batched_rate / plain_rate:.1f}x), "
             f"{scratch_rate:.1f} plain from scratch; FP32 batched vs FP64 plain max {max(plain_dev) if plain_dev else float('nan'):.2e} sigma")
@@ -617,6 +676,16 @@ def main():
             f"{cpu_more['fit_1e5_numerical_1threads']['wall_ms']:.0f} ms (numerical); toys {cpu_more['toys_1thread']['toy_fits_per_s']:.1f}/s "
             f"(1 thread), {cpu_more['toys_all_threads']['toy_fits_per_s']:.1f}/s ({cpu_more['toys_all_threads']['threads']} threads)")
 
+    # the other gradient kernels against the same roofline (traffic from their counter summary)
+    ot = others_traffic()
+    other_rows = [roofline_row("mass FP64 N=1e7", k64_ms, n * 8, ot)]
+    if angular:
+        other_rows.append(roofline_row("angular FP32 N=1e6", angular["fp32"]["kernel_us"] * 1e-3, 12_000_000, ot))
+        other_rows.append(roofline_row("angular FP64 N=1e6", angular["fp64"]["kernel_us"] * 1e-3, 24_000_000, ot))
+    if toys:
+        other_rows.append(roofline_row("4096 toys FP32", toys["grad_kernel_ms"], int(round(toys["mean_events"] * toys["toys"])) * 4, ot))
+        other_rows[-1]["launch_shape_bench"] = toys["launch_shape"]
+
     out = {
         "metric": "-2lnL+grad evals/s; toy fits/s vs plain GPU path; p50/p99 fit latency",
         "value": value,
@@ -645,6 +714,7 @@ def main():
         "fp64": {"evals_per_s": v64, "kernel_ms": k64_ms, "GBps": n * 8 / (k64_ms * 1e-3) / 1e9,
                  "launch_shape": shape(m64, tune64["best"])},
         "lh_kernel_ms_fp32": k32_lh_ms,
+        "roofline_other_kernels": other_rows,
         "call_breakdown": step,
         "accuracy": acc,
         "copy_ceiling_GBps": copy_gbs,

@@ -113,14 +113,17 @@ int mfh_model_kernels(int kind, int fp32, int with_hessian, const mfh_options* o
 int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, const double* data, long long n, double event_cost,
                            int linear, double* out);
 /* launch-shape autotuner: for each candidate (block, events_per_thread, unroll) triple in
- * cand[3 * ncand] (NULL: 16 built-in shapes, block 256/512 x 8..64 events per thread x unroll
- * 4/8) times `eval_kind` evaluations (per-chunk kernel + final pass, mean of `iters`
+ * cand[3 * ncand] (NULL: the built-in shapes of mfh_autotune_candidates for the sizes' total) times `eval_kind` evaluations (per-chunk kernel + final pass, mean of `iters`
  * back-to-back launches) of the configuration density on a batch of nseg segments of the
  * given sizes generated on the device at its generating values; sets the context's launch
  * options to the fastest (used by models created afterwards), returns it in best[3] and the
  * per-candidate us in us_per_launch[ncand] (may be NULL).  No counterpart in the reference. */
 int mfh_autotune(mfh_context* ctx, int kind, int fp32, const long long* sizes, int nseg, int eval_kind, const int* cand, int ncand,
                  int iters, int* best, double* us_per_launch);
+/* the built-in candidate shapes for a workload of total_events events: block 256/512 x 8..64
+ * events per thread x unroll 4/8 from 4e6 events, block 128..512 x 2..16 events per thread
+ * below; up to `cap` triples into cand, their number in *count */
+int mfh_autotune_candidates(long long total_events, int* cand, int cap, int* count);
 /* without a device: xoshiro128++ states of streams first .. first+count-1 of `seed` by the
  * jump-ahead matrices the device seeding uses (out: 4 words per stream) */
 int mfh_stream_states(unsigned long long seed, long long first, int count, unsigned int* out);

@@ -76,7 +76,7 @@ HIP_SYMBOLS = [
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
-    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states", "mfh_autotune",
+    "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states", "mfh_autotune", "mfh_autotune_candidates",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted", "mfh_fit_options",
@@ -124,6 +124,7 @@ def libs():
         L.mfh_model_precompile.argtypes = [ip, ip, ip, ctypes.POINTER(Options), dp]
         L.mfh_model_kernels.argtypes = [ip, ip, ip, ctypes.POINTER(Options), ip, ctypes.POINTER(ctypes.c_int)]
         L.mfh_model_host_outputs.argtypes = [ip, ip, ip, dp, dp, ctypes.c_longlong, ctypes.c_double, ip, dp]
+        L.mfh_autotune_candidates.argtypes = [ctypes.c_longlong, ctypes.POINTER(ctypes.c_int), ip, ctypes.POINTER(ctypes.c_int)]
         L.mfh_stream_states.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ip, ctypes.POINTER(ctypes.c_uint32)]
         L.mfh_autotune.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip, ip, ctypes.POINTER(ctypes.c_int), ip, ip,
                                    ctypes.POINTER(ctypes.c_int), dp]
@@ -206,6 +207,16 @@ def stream_states(seed, first, count):
     return np.frombuffer(out, dtype=np.uint32).reshape(count, 4).copy()
 
 
+def autotune_candidates(total_events):
+    """the autotuner's built-in launch shapes (block, events per thread, unroll) for a workload
+    of total_events events"""
+    cand = (ctypes.c_int * 3 * 64)()
+    n = ctypes.c_int(0)
+    _check(libs()[1].mfh_autotune_candidates(int(total_events), ctypes.cast(cand, ctypes.POINTER(ctypes.c_int)), 64, ctypes.byref(n)))
+    flat = list(ctypes.cast(cand, ctypes.POINTER(ctypes.c_int))[:3 * n.value])
+    return [tuple(flat[3 * i:3 * i + 3]) for i in range(n.value)]
+
+
 def precompile(kind, fp32, with_hessian=True, **kw):
     """Compile a configuration module for gfx950 into the on-disk cache (no device needed)."""
     ms = ctypes.c_double(0)
@@ -241,7 +252,7 @@ class Context:
         kernels by timing `eval_kind` evaluations of the configuration density on a batch of
         the given segment sizes (generated on the device).  Models created afterwards use it.
         Returns {"best": (block, ept, unroll), "table": [((block, ept, unroll), us), ...]}."""
-        cands = candidates or [(b, e, u) for b in (256, 512) for e in (8, 16, 32, 64) for u in (4, 8)]
+        cands = candidates or autotune_candidates(int(np.sum(np.asarray(sizes, dtype=np.int64))))
         flat = (ctypes.c_int * (3 * len(cands)))(*[int(v) for c in cands for v in c])
         sz = (ctypes.c_longlong * len(sizes))(*[int(s) for s in sizes])
         best = (ctypes.c_int * 3)()

@@ -424,8 +424,11 @@ int mfh_autotune(mfh_context* ctx, int kind, int fp32, const long long* sizes, i
   std::vector<LaunchShape> cs;
   if (cand && ncand > 0)
     for (int i = 0; i < ncand; i++) cs.push_back({cand[3 * i], cand[3 * i + 1], cand[3 * i + 2]});
-  else
-    cs = default_launch_candidates();
+  else {
+    long long tot = 0;
+    for (int i = 0; i < nseg; i++) tot += sizes[i];
+    cs = launch_candidates(tot);
+  }
   for (auto& s : cs)
     if (s.block < 64 || s.block > 1024 || s.block % 64 || s.events_per_thread < 1 || s.unroll < 1)
       throw std::invalid_argument("mfh_autotune: bad candidate launch shape");
@@ -477,6 +480,19 @@ int mfh_autotune(mfh_context* ctx, int kind, int fp32, const long long* sizes, i
   best[2] = win.unroll;
   if (us_per_launch)
     for (size_t i = 0; i < us.size(); i++) us_per_launch[i] = us[i];
+  MFH_CATCH
+}
+
+int mfh_autotune_candidates(long long total_events, int* cand, int cap, int* count) {
+  MFH_TRY
+  if (!count || (cap > 0 && !cand)) throw std::invalid_argument("mfh_autotune_candidates: null argument");
+  auto cs = launch_candidates(total_events);
+  *count = (int)cs.size();
+  for (int i = 0; i < cap && i < (int)cs.size(); i++) {
+    cand[3 * i] = cs[i].block;
+    cand[3 * i + 1] = cs[i].events_per_thread;
+    cand[3 * i + 2] = cs[i].unroll;
+  }
   MFH_CATCH
 }
 

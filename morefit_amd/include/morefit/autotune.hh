@@ -13,7 +13,8 @@
  *
  * No counterpart in the reference (its OpenCL work-group size is a fixed option,
  * compute.hh:33); the candidates span the shapes that matter on gfx950: 4-8 waves per
- * work-group, 8-64 events per thread.
+ * work-group and 8-64 events per thread for large workloads, 2-8 waves and 2-16 events per
+ * thread below a few million events (launch_candidates).
  */
 #ifndef MOREFIT_AUTOTUNE_HH
 #define MOREFIT_AUTOTUNE_HH
@@ -36,6 +37,25 @@ inline std::vector<LaunchShape> default_launch_candidates() {
   for (int block : {256, 512})
     for (int ept : {8, 16, 32, 64})
       for (int unroll : {4, 8}) c.push_back({block, ept, unroll});
+  return c;
+}
+
+// Candidates for a workload of `total_events` events.  From a few million events up the
+// default set (4-8 waves per work-group, 8-64 events per thread) already gives every CU
+// several work-groups; below, smaller chunks are needed for that (10^6 events at 256 x 16 are
+// 245 work-groups on 256 CUs), so the set moves to 2-8 waves and 2-16 events per thread (an
+// unroll above the events per thread is lowered to them).
+inline std::vector<LaunchShape> launch_candidates(long long total_events) {
+  if (total_events >= 4000000) return default_launch_candidates();
+  std::vector<LaunchShape> c;
+  for (int block : {128, 256, 512})
+    for (int ept : {2, 4, 8, 16})
+      for (int unroll : {4, 8}) {
+        const LaunchShape s{block, ept, std::min(unroll, ept)};
+        bool seen = false;
+        for (auto& q : c) seen = seen || (q.block == s.block && q.events_per_thread == s.events_per_thread && q.unroll == s.unroll);
+        if (!seen) c.push_back(s);
+      }
   return c;
 }
 
