@@ -30,6 +30,8 @@ extern "C" {
 
 #define MFH_MASS 0   /* f Gauss(mu, sigma) + (1-f) Exp(lambda) on [lo, hi]; theta = mu, sigma, f, lambda */
 #define MFH_KSTAR 1  /* P-wave B->K*ll angular density; theta = FL, S3, S4, S5, AFB, S7, S8, S9 */
+#define MFH_CB 2     /* Crystal Ball (pdf.hh:195-419) on [lo, hi] (5.0, 5.6); theta = mu, sigma, alpha, n */
+#define MFH_POLY 3   /* 1 + c1 x + c2 x^2 + c3 x^3 (pdf.hh:716-892) on [lo, hi] (-1, 1); theta = c1, c2, c3 */
 
 typedef struct mfh_context mfh_context;
 typedef struct mfh_model mfh_model;

@@ -12,21 +12,26 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
-MASS, KSTAR = 0, 1
+MASS, KSTAR, CB, POLY = 0, 1, 2, 3
 EVAL_LH, EVAL_GRAD, EVAL_HESS = 1, 2, 3
-NTHETA = {MASS: 4, KSTAR: 8}
-NOBS = {MASS: 1, KSTAR: 3}
-THETA_NAMES = {MASS: ["mu", "sigma", "f", "lambda"], KSTAR: ["FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"]}
+NTHETA = {MASS: 4, KSTAR: 8, CB: 4, POLY: 3}
+NOBS = {MASS: 1, KSTAR: 3, CB: 1, POLY: 1}
+THETA_NAMES = {MASS: ["mu", "sigma", "f", "lambda"], KSTAR: ["FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"],
+               CB: ["mu", "sigma", "alpha", "n"], POLY: ["c1", "c2", "c3"]}
 MASS_TRUTH = np.array([5.28, 0.02, 0.3, -1.5])
 KSTAR_TRUTH = np.array([0.7, 0.0, -0.1, -0.2, 0.1, 0.0, 0.0, 0.0])
-TRUTH = {MASS: MASS_TRUTH, KSTAR: KSTAR_TRUTH}
+CB_TRUTH = np.array([5.28, 0.02, 1.5, 3.0])
+POLY_TRUTH = np.array([0.3, -0.2, 0.1])
+TRUTH = {MASS: MASS_TRUTH, KSTAR: KSTAR_TRUTH, CB: CB_TRUTH, POLY: POLY_TRUTH}
 # start values: +10 % of the window width for mu, +10 % of the value otherwise, zeros stay zero
 MASS_START = np.array([5.28 + 0.06, 0.022, 0.33, -1.65])
 KSTAR_START = np.array([0.77, 0.0, -0.11, -0.22, 0.11, 0.0, 0.0, 0.0])
-START = {MASS: MASS_START, KSTAR: KSTAR_START}
+START = {MASS: MASS_START, KSTAR: KSTAR_START, CB: np.array([5.285, 0.021, 1.6, 3.3]), POLY: np.array([0.33, -0.22, 0.11])}
 # statistical errors of one fit at the truth, per parameter, scaled as 1/sqrt(N) from N_REF events
 SIGMA_REF = {MASS: (np.array([1.57e-4, 1.41e-4, 1.93e-3, 2.24e-2]), 100000),
-             KSTAR: (np.array([2.9e-3, 3.8e-3, 4.7e-3, 4.7e-3, 2.6e-3, 4.8e-3, 4.8e-3, 3.8e-3]), 50000)}
+             KSTAR: (np.array([2.9e-3, 3.8e-3, 4.7e-3, 4.7e-3, 2.6e-3, 4.8e-3, 4.8e-3, 3.8e-3]), 50000),
+             CB: (np.array([7.9e-5, 6.3e-5, 1.7e-2, 8.2e-2]), 100000),
+             POLY: (np.array([1.25e-2, 9.2e-3, 1.88e-2]), 100000)}
 
 
 def toy_starts(kind, sizes, seed):

@@ -76,6 +76,23 @@ struct ModelDef {
       pdfs.emplace_back(new KstarmumuAngularPDF<K, double>(dims[0].get(), dims[1].get(), dims[2].get(), pars[0].get(), pars[1].get(),
                                                            pars[2].get(), pars[3].get(), pars[4].get(), pars[5].get(), pars[6].get(),
                                                            pars[7].get()));
+    } else if (k == MFH_CB) {
+      // Crystal Ball signal on the mass range: its tail joins the core at mu - alpha sigma
+      double lo = range ? range[0] : 5.0, hi = range ? range[1] : 5.6;
+      dims.emplace_back(new dimension<double>("m", lo, hi));
+      pars.emplace_back(new parameter<double>("mu", 5.28, lo, hi, 0.01));
+      pars.emplace_back(new parameter<double>("sigma", 0.02, 0.005, 0.13, 0.001));
+      pars.emplace_back(new parameter<double>("alpha", 1.5, 0.2, 5.0, 0.01));
+      pars.emplace_back(new parameter<double>("n", 3.0, 1.05, 20.0, 0.1));
+      pdfs.emplace_back(new CrystalballPDF<K, double>(dims[0].get(), pars[0].get(), pars[1].get(), pars[2].get(), pars[3].get()));
+    } else if (k == MFH_POLY) {
+      // third-order polynomial 1 + c1 x + c2 x^2 + c3 x^3 on [-1, 1]
+      double lo = range ? range[0] : -1.0, hi = range ? range[1] : 1.0;
+      dims.emplace_back(new dimension<double>("x", lo, hi));
+      const char* names[3] = {"c1", "c2", "c3"};
+      const double truth[3] = {0.3, -0.2, 0.1};
+      for (int i = 0; i < 3; i++) pars.emplace_back(new parameter<double>(names[i], truth[i], -1.0, 1.0, 0.01));
+      pdfs.emplace_back(new PolynomialPDF<K, double>(dims[0].get(), {pars[0].get(), pars[1].get(), pars[2].get()}));
     } else {
       throw std::invalid_argument("unknown model kind " + std::to_string(k));
     }

@@ -17,8 +17,16 @@
 #define SQRTPI 1.7724538509055160
 #define KSTAR_C (9.0 / 32.0 / M_PI)
 
-int orc_npar(int model) { return model == ORC_MASS ? 4 : (model == ORC_KSTAR ? 8 : -1); }
-int orc_nobs(int model) { return model == ORC_MASS ? 1 : (model == ORC_KSTAR ? 3 : -1); }
+int orc_npar(int model) {
+  switch (model) {
+    case ORC_MASS: return 4;
+    case ORC_KSTAR: return 8;
+    case ORC_CB: return 4;
+    case ORC_POLY: return 3;
+  }
+  return -1;
+}
+int orc_nobs(int model) { return model == ORC_KSTAR ? 3 : (model >= 0 && model <= ORC_POLY ? 1 : -1); }
 
 /* ------------------------------------------------------------------------------------
  * Mass model (pdf.hh:147-154, 431-440, 526-558):
@@ -161,10 +169,72 @@ static jet kstar_jet(const double* x, const double* th) {
   return jet_scale(q, KSTAR_C);
 }
 
+/* ------------------------------------------------------------------------------------
+ * Crystal Ball (pdf.hh:195-419) on [a, b], theta = [mu, sigma, alpha, n]:
+ *   t = (x - mu)/sigma; core exp(-t^2/2) for x - mu + alpha sigma > 0, else tail
+ *   A (B - t)^-n, A = (n/alpha)^n exp(-alpha^2/2), B = n/alpha - alpha;
+ *   norm = tail on [a, m] + core on [m, b], m = mu - alpha sigma, C = mu + sigma B:
+ *   tail = A sigma^n [(C-m)^(1-n) - (C-a)^(1-n)]/(n-1)   (n != 1; A sigma log((C-a)/(C-m)) at n = 1)
+ *   core = sigma sqrt(pi/2) [erf((mu-m)/(sqrt2 sigma)) - erf((mu-b)/(sqrt2 sigma))]
+ * The branches are chosen by value and differentiated as taken (the reference's Conditional
+ * derivative rule).
+ * ---------------------------------------------------------------------------------- */
+static jet jet_pow(jet u, jet v) { return jet_exp(jet_mul(v, jet_log(u))); }
+
+static jet cb_jet(double x, const double* th, const double* r) {
+  const int n = 4;
+  jet mu = jet_var(n, th[0], 0), sg = jet_var(n, th[1], 1), al = jet_var(n, th[2], 2), nn = jet_var(n, th[3], 3);
+  double a = r[0], b = r[1];
+  jet A = jet_mul(jet_pow(jet_div(nn, al), nn), jet_exp(jet_scale(jet_mul(al, al), -0.5)));
+  jet B = jet_sub(jet_div(nn, al), al);
+  jet C = jet_add(mu, jet_mul(sg, B));
+  jet m = jet_sub(mu, jet_mul(al, sg));
+  jet t = jet_div(jet_addc(jet_scale(mu, -1.0), x), sg);
+  jet p;
+  if (x - th[0] + th[2] * th[1] > 0.0) p = jet_exp(jet_scale(jet_mul(t, t), -0.5));
+  else p = jet_div(A, jet_pow(jet_sub(B, t), nn));
+  jet tail;
+  if (th[3] - 1.0 != 0.0)
+    tail = jet_div(jet_mul(jet_mul(A, jet_pow(sg, nn)),
+                           jet_sub(jet_pow(jet_sub(C, m), jet_addc(jet_scale(nn, -1.0), 1.0)),
+                                   jet_pow(jet_addc(C, -a), jet_addc(jet_scale(nn, -1.0), 1.0)))),
+                   jet_addc(nn, -1.0));
+  else
+    tail = jet_mul(jet_mul(A, sg), jet_sub(jet_log(jet_addc(C, -a)), jet_log(jet_sub(C, m))));
+  jet s2 = jet_scale(sg, SQRT2);
+  jet core = jet_mul(jet_scale(sg, sqrt(M_PI / 2.0)),
+                     jet_sub(jet_erf(jet_div(jet_sub(mu, m), s2)), jet_erf(jet_div(jet_addc(mu, -b), s2))));
+  return jet_div(p, jet_add(tail, core));
+}
+
+/* ------------------------------------------------------------------------------------
+ * Polynomial (pdf.hh:716-892) on [a, b], theta = [c1, c2, c3]: p = 1 + c1 x + c2 x^2 + c3 x^3
+ * over its integral P(b) - P(a), P(x) = x + c1 x^2/2 + c2 x^3/3 + c3 x^4/4.
+ * ---------------------------------------------------------------------------------- */
+static jet poly_jet(double x, const double* th, const double* r) {
+  const int n = 3;
+  jet c1 = jet_var(n, th[0], 0), c2 = jet_var(n, th[1], 1), c3 = jet_var(n, th[2], 2);
+  double a = r[0], b = r[1];
+  jet p = jet_addc(jet_add(jet_add(jet_scale(c1, x), jet_scale(c2, x * x)), jet_scale(c3, x * x * x)), 1.0);
+  jet P = jet_addc(jet_add(jet_add(jet_scale(c1, (b * b - a * a) / 2.0), jet_scale(c2, (b * b * b - a * a * a) / 3.0)),
+                           jet_scale(c3, (b * b * b * b - a * a * a * a) / 4.0)),
+                   b - a);
+  return jet_div(p, P);
+}
+
 /* ------------------------------------------------------------------------------------ */
+static jet model_jet(int model, const double* x, const double* theta, const double* range) {
+  switch (model) {
+    case ORC_MASS: return mass_jet(x[0], theta, range);
+    case ORC_CB: return cb_jet(x[0], theta, range);
+    case ORC_POLY: return poly_jet(x[0], theta, range);
+  }
+  return kstar_jet(x, theta);
+}
+
 void orc_event_jet(int model, const double* x, const double* theta, const double* range,
                    double* logp, double* grad, double* hess_full) {
-  jet p = model == ORC_MASS ? mass_jet(x[0], theta, range) : kstar_jet(x, theta);
+  jet p = model_jet(model, x, theta, range);
   jet lp = jet_log(p);
   int n = lp.n;
   if (logp) *logp = lp.v;
@@ -175,6 +245,11 @@ void orc_event_jet(int model, const double* x, const double* theta, const double
 }
 
 double orc_event_grad(int model, const double* x, const double* theta, const double* range, double* grad) {
+  if (model == ORC_CB || model == ORC_POLY) {
+    double lp;
+    orc_event_jet(model, x, theta, range, &lp, grad, NULL);
+    return lp;
+  }
   if (model == ORC_MASS) {
     mass_hoist h;
     mass_prepare(theta, range, &h);
@@ -184,6 +259,7 @@ double orc_event_grad(int model, const double* x, const double* theta, const dou
 }
 
 double orc_prob(int model, const double* x, const double* theta, const double* range) {
+  if (model == ORC_CB || model == ORC_POLY) return model_jet(model, x, theta, range).v;
   if (model == ORC_MASS) {
     mass_hoist h;
     mass_prepare(theta, range, &h);
@@ -193,6 +269,18 @@ double orc_prob(int model, const double* x, const double* theta, const double* r
 }
 
 double orc_get_max(int model, const double* th, const double* r) {
+  if (model == ORC_CB) { /* the unnormalised density peaks at 1 (x = mu) */
+    double x = th[0];
+    return model_jet(model, &x, th, r).v;
+  }
+  if (model == ORC_POLY) { /* largest value on a fine grid, with a 0.1 % margin */
+    double best = 0.0;
+    for (int i = 0; i <= 100000; i++) {
+      double x = r[0] + (r[1] - r[0]) * i / 100000.0, v = model_jet(model, &x, th, r).v;
+      if (v > best) best = v;
+    }
+    return 1.001 * best;
+  }
   if (model == ORC_MASS) {
     double mu = th[0], s = th[1], f = th[2], lam = th[3], a = r[0], b = r[1];
     double NG = 0.5 * (erf((b - mu) / (SQRT2 * s)) - erf((a - mu) / (SQRT2 * s)));
@@ -231,7 +319,7 @@ static int n_outputs(int model, int mode) {
 static void event_outputs(int model, int mode, const double* x, const double* theta,
                           const double* range, const mass_hoist* mh, double* o) {
   int P = orc_npar(model);
-  if (mode <= 1) {
+  if (mode <= 1 && (model == ORC_MASS || model == ORC_KSTAR)) {
     if (model == ORC_MASS) o[0] = mass_event(x[0], mh, mode ? o + 1 : NULL);
     else o[0] = kstar_event(x, theta, mode ? o + 1 : NULL);
     return;
@@ -295,7 +383,7 @@ int orc_nll(int model, const double* data, long n, long stride, const double* th
     mode -= 10;
     if (nthreads == 0) nthreads = 1;
   }
-  if (model != ORC_MASS && model != ORC_KSTAR) return -1;
+  if (model < ORC_MASS || model > ORC_POLY) return -1;
   if (mode < 0 || mode > 2 || n < 0) return -1;
   if (nthreads == 0) return nll_precise(model, data, n, stride, theta, range, mode, out);
   if (nthreads > 256) nthreads = 256;
@@ -414,7 +502,7 @@ long orc_generate(int model, const double* theta, const double* range, double ma
           double lo = range[2 * d], hi = range[2 * d + 1];
           x[d] = lo + (hi - lo) * (orc_xoshiro128pp_next(st) * 2.3283064365386963e-10);
         }
-        double p = model == ORC_MASS ? mass_p(x[0], &mh) : kstar_p(x, theta);
+        double p = model == ORC_MASS ? mass_p(x[0], &mh) : model == ORC_KSTAR ? kstar_p(x, theta) : orc_prob(model, x, theta, range);
         double u = orc_xoshiro128pp_next(st) * 2.3283064365386963e-10;
         if (maxprob * u < p) break;
       }

@@ -15,6 +15,7 @@
  *                   fitter.hh:1290-1301 (Hessian packed i>=j, row by row)
  *   PRNG            random.hh:160-185 (Splitmix64), random.hh:333-465 (Xoshiro128++ with
  *                   jump), random.hh:42-83 (Poisson), compute_opencl.hh:540-575 (accept-reject)
+ *   Crystal Ball    pdf.hh:195-419; polynomial pdf.hh:716-892 (value, jets, maxima)
  *   maxima          pdf.hh:185-189, 467-476, 621-631; physicspdfs.hh:209-213
  *
  * Parity pins (tests/test_oracle.py): golden vectors written by the reference's own
@@ -30,7 +31,7 @@
 extern "C" {
 #endif
 
-enum { ORC_MASS = 0, ORC_KSTAR = 1 };
+enum { ORC_MASS = 0, ORC_KSTAR = 1, ORC_CB = 2, ORC_POLY = 3 };
 
 /* number of parameters / observables of a model */
 int orc_npar(int model);
@@ -38,7 +39,10 @@ int orc_nobs(int model);
 
 /* range: [lo_0, hi_0, lo_1, hi_1, ...] of the observables
  * mass theta  = [mu, sigma, f, lambda]           (order of benchmarking/morefit_massfit.cc:62)
- * kstar theta = [FL, S3, S4, S5, AFB, S7, S8, S9] (order of physicspdfs.hh:33-34)           */
+ * kstar theta = [FL, S3, S4, S5, AFB, S7, S8, S9] (order of physicspdfs.hh:33-34)
+ * cb theta    = [mu, sigma, alpha, n]             (CrystalballPDF, pdf.hh:195-419)
+ * poly theta  = [c1, c2, c3]                      (PolynomialPDF of degree 3, pdf.hh:716-892)
+ * The Crystal Ball and polynomial densities are evaluated by jets on every path.         */
 
 /* value, gradient and full Hessian of log p(x; theta) for one event, by jets */
 void orc_event_jet(int model, const double* x, const double* theta, const double* range,

@@ -9,12 +9,13 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MASS, KSTAR = 0, 1
-NPAR = {MASS: 4, KSTAR: 8}
-NOBS = {MASS: 1, KSTAR: 3}
+MASS, KSTAR, CB, POLY = 0, 1, 2, 3
+NPAR = {MASS: 4, KSTAR: 8, CB: 4, POLY: 3}
+NOBS = {MASS: 1, KSTAR: 3, CB: 1, POLY: 1}
 MASS_RANGE = np.array([5.0, 5.6])
 KSTAR_RANGE = np.array([-1.0, 1.0, -1.0, 1.0, -np.pi, np.pi])
-RANGES = {MASS: MASS_RANGE, KSTAR: KSTAR_RANGE}
+POLY_RANGE = np.array([-1.0, 1.0])
+RANGES = {MASS: MASS_RANGE, KSTAR: KSTAR_RANGE, CB: MASS_RANGE, POLY: POLY_RANGE}
 
 _lib = None
 
