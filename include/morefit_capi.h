@@ -212,6 +212,23 @@ int mfh_plain_eval(mfh_plain* p, const double* data, long long n, const double* 
 int mfh_plain_destroy(mfh_plain* p);
 int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta, int nthreads, double* value, double* grad, double* hess);
 
+/* Projection of a density onto observable `var` (plotter<...>::plot, reference
+ * plotter.hh:218-640, integral kernel plotter.hh:138-216): the integral of the density at theta
+ * over nbins * multiplier fine bins of [range_min, range_max] (equal values: the observable's
+ * range), run by the integral kernel on the device (ctx) or on the CPU backend (ctx NULL,
+ * nthreads threads); the histogram of data (SoA, n events, weights NULL or per event),
+ * residuals and pulls.  Outputs, each may be NULL: pdf_integrals [nbins * multiplier],
+ * counts, errors, pulls [nbins].  filename (NULL: no file): a matplotlib script (".py",
+ * plot_type MFH_PLOT_MATPLOTLIB) or a ROOT macro (".C", MFH_PLOT_ROOT); flags MFH_PLOT_PULLS
+ * adds the pull panel.  replots > 0 repeats the projection that many times with the kernel of
+ * the first (plot with replot); ms: duration of the last call. */
+#define MFH_PLOT_ROOT 0
+#define MFH_PLOT_MATPLOTLIB 1
+#define MFH_PLOT_PULLS 1
+int mfh_plot(mfh_context* ctx, int kind, int fp32, int nthreads, const double* theta, const double* data, const double* weights,
+             long long n, int var, int nbins, int multiplier, double range_min, double range_max, int plot_type, int flags,
+             const char* filename, int replots, double* pdf_integrals, double* counts, double* errors, double* pulls, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,7 +85,7 @@ CAPI_SYMBOLS = [
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
     "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted", "mfh_fit_options",
-    "mfh_plain_create", "mfh_plain_fit", "mfh_plain_eval", "mfh_plain_destroy",
+    "mfh_plain_create", "mfh_plain_fit", "mfh_plain_eval", "mfh_plain_destroy", "mfh_plot",
 ]
 
 
@@ -154,6 +154,8 @@ def libs():
         L.mfh_plain_fit.argtypes = [vp, dp, ctypes.c_longlong, dp, dp, dp, ctypes.POINTER(FitOut), dp]
         L.mfh_plain_eval.argtypes = [vp, dp, ctypes.c_longlong, dp, ip, dp, dp, dp]
         L.mfh_plain_destroy.argtypes = [vp]
+        L.mfh_plot.argtypes = [vp, ip, ip, ip, dp, dp, dp, ctypes.c_longlong, ip, ip, ip, ctypes.c_double, ctypes.c_double, ip, ip,
+                               ctypes.c_char_p, ip, dp, dp, dp, dp, dp]
         L.mfh_serve_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ip, dp, dp, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_ulonglong), dp, ip,
                                     ctypes.c_double, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_int), dp, dp]
@@ -532,3 +534,25 @@ def eval_cpu(kind, data, theta, nthreads=1):
     g, h = np.zeros(P), np.zeros(P * P)
     _check(libs()[1].mfh_eval_cpu(kind, _dp(data), data.shape[1], _dp(_f64(theta)), nthreads, ctypes.byref(v), _dp(g), _dp(h)))
     return v.value, g, h.reshape(P, P)
+
+
+PLOT_ROOT, PLOT_MATPLOTLIB = 0, 1
+
+
+def plot(kind, theta, data, var=0, nbins=100, multiplier=10, range_min=0.0, range_max=0.0, weights=None, ctx=None, fp32=False,
+         nthreads=1, filename=None, plot_type=PLOT_MATPLOTLIB, pulls=False, replots=0):
+    """plotter::plot: the density at theta integrated over nbins*multiplier fine bins of
+    observable var (integral kernel on the device when ctx is given, else the CPU backend),
+    the histogram of data, residuals and pulls; filename writes a matplotlib script (.py) or a
+    ROOT macro (.C).  Returns a dict of numpy arrays and the last call's milliseconds."""
+    data = _f64(np.atleast_2d(data))
+    n = data.shape[1]
+    fine = np.zeros(nbins * multiplier)
+    counts, errors, pl = np.zeros(nbins), np.zeros(nbins), np.zeros(nbins)
+    ms = ctypes.c_double(0)
+    w = _f64(weights) if weights is not None else None
+    _check(libs()[1].mfh_plot(ctx.h if ctx is not None else None, kind, int(fp32), nthreads, _dp(_f64(theta)), _dp(data),
+                              _dp(w) if w is not None else None, n, var, nbins, multiplier, range_min, range_max, plot_type,
+                              1 if pulls else 0, filename.encode() if filename else None, replots, _dp(fine), _dp(counts),
+                              _dp(errors), _dp(pl), ctypes.byref(ms)))
+    return {"pdf_integrals": fine, "counts": counts, "errors": errors, "pulls": pl, "ms": ms.value}

@@ -13,6 +13,7 @@
 
 #include "morefit.hh"
 #include "morefit/autotune.hh"
+#include "morefit/plotter.hh"
 #include "morefit/batch.hh"
 #include "morefit/serve.hh"
 #include "morefit_capi.h"
@@ -39,7 +40,8 @@ int fail(const std::string& what) {
   return 0;
 
 // the two configuration densities with the parameter limits and steps of the reference
-// benchmarks (benchmarking/morefit_massfit.cc:48-57, benchmarking/morefit_kstarmumu.cc:41-52)
+// benchmarks (benchmarking/morefit_massfit.cc:48-57, benchmarking/morefit_kstarmumu.cc:41-52),
+// and the Crystal Ball and third-order polynomial densities
 template <typename K>
 struct ModelDef {
   typedef K kernel_type;
@@ -1001,6 +1003,77 @@ int mfh_eval_cpu(int kind, const double* data, long long n, const double* theta,
   if (value) *value = v;
   if (grad) std::memcpy(grad, g.data(), N * sizeof(double));
   if (hess) std::memcpy(hess, h.data(), N * N * sizeof(double));
+  MFH_CATCH
+}
+
+}  // extern "C"
+
+namespace {
+template <typename K, typename Backend, typename Block>
+void run_plot(Backend* be, int kind, const double* theta, const double* data, const double* weights, long long n, int var, int nbins,
+              int multiplier, double range_min, double range_max, int plot_type, int flags, const char* filename, int replots,
+              double* pdf_integrals, double* counts, double* errors, double* pulls, double* ms) {
+  ModelDef<K> def(kind, nullptr);
+  if (var < 0 || var >= (int)def.dimv.size()) throw std::invalid_argument("mfh_plot: observable index out of range");
+  if (nbins <= 0 || multiplier <= 0) throw std::invalid_argument("mfh_plot: nbins and multiplier must be positive");
+  if (n < 0 || n > (long long)std::numeric_limits<int>::max()) throw std::invalid_argument("mfh_plot: event count out of range");
+  dimension<double> wdim("w", -1e300, 1e300);
+  std::vector<dimension<double>*> dims = def.dimv;
+  if (weights) dims.push_back(&wdim);
+  EventVector<K, double> ev(dims, (unsigned long)n);
+  for (size_t d = 0; d < def.dimv.size(); d++)
+    for (long long i = 0; i < n; i++) ev(i, (unsigned)d) = (K)data[d * n + i];
+  if (weights) {
+    for (long long i = 0; i < n; i++) ev(i, (unsigned)def.dimv.size()) = (K)weights[i];
+    ev.set_event_weight("w");
+  }
+  for (size_t k = 0; k < def.params.size(); k++) def.params[k]->set_value(theta[k]);
+  plotter_options po;
+  po.plotter = plot_type == MFH_PLOT_MATPLOTLIB ? plotter_options::MatPlotLib : plotter_options::Root;
+  po.pdf_bin_multiplier = multiplier;
+  po.plot_pulls = (flags & MFH_PLOT_PULLS) != 0;
+  po.print_level = 0;
+  plotter<K, double, Backend, Block> pl(&po, be);
+  const std::string var_name = def.dimv[(size_t)var]->get_name();
+  pl.plot(def.pdf, def.params, &ev, var_name, filename ? filename : "", "", (unsigned)nbins, range_min, range_max, false);
+  for (int r = 0; r < replots; r++) pl.plot(def.pdf, def.params, &ev, var_name, "", "", (unsigned)nbins, range_min, range_max, true);
+  const auto& P = pl.last;
+  if (pdf_integrals) std::copy(P.pdf_integrals.begin(), P.pdf_integrals.end(), pdf_integrals);
+  if (counts) std::copy(P.counts.begin(), P.counts.end(), counts);
+  if (errors) std::copy(P.errors.begin(), P.errors.end(), errors);
+  if (pulls) std::copy(P.pulls.begin(), P.pulls.end(), pulls);
+  if (ms) *ms = pl.last_ms;
+}
+}  // namespace
+
+extern "C" {
+
+int mfh_plot(mfh_context* ctx, int kind, int fp32, int nthreads, const double* theta, const double* data, const double* weights,
+             long long n, int var, int nbins, int multiplier, double range_min, double range_max, int plot_type, int flags,
+             const char* filename, int replots, double* pdf_integrals, double* counts, double* errors, double* pulls, double* ms) {
+  MFH_TRY
+  if (!theta || (n > 0 && !data)) throw std::invalid_argument("mfh_plot: null theta or data");
+  if (ctx) {
+    if (fp32)
+      run_plot<float, HIPBackend, HIPBlock<float, double>>(ctx->be.get(), kind, theta, data, weights, n, var, nbins, multiplier, range_min,
+                                                            range_max, plot_type, flags, filename, replots, pdf_integrals, counts, errors,
+                                                            pulls, ms);
+    else
+      run_plot<double, HIPBackend, HIPBlock<double, double>>(ctx->be.get(), kind, theta, data, weights, n, var, nbins, multiplier,
+                                                              range_min, range_max, plot_type, flags, filename, replots, pdf_integrals,
+                                                              counts, errors, pulls, ms);
+  } else {
+    compute_options co;
+    co.llvm_nthreads = nthreads > 0 ? nthreads : 1;
+    CPUBackend be(&co);
+    if (fp32)
+      run_plot<float, CPUBackend, CPUBlock<float, double>>(&be, kind, theta, data, weights, n, var, nbins, multiplier, range_min, range_max,
+                                                            plot_type, flags, filename, replots, pdf_integrals, counts, errors, pulls, ms);
+    else
+      run_plot<double, CPUBackend, CPUBlock<double, double>>(&be, kind, theta, data, weights, n, var, nbins, multiplier, range_min,
+                                                              range_max, plot_type, flags, filename, replots, pdf_integrals, counts, errors,
+                                                              pulls, ms);
+  }
   MFH_CATCH
 }
 

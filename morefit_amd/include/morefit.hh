@@ -22,5 +22,6 @@
 #include "morefit/fitter.hh"
 #include "morefit/generator.hh"
 #include "morefit/toystudy.hh"
+#include "morefit/plotter.hh"
 
 #endif

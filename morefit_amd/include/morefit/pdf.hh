@@ -146,10 +146,11 @@ class ExponentialPDF : public PDF<kernelT, evalT> {
                                             (Exp<kernelT, evalT>(A() * to()) - Exp<kernelT, evalT>(A() * from())) / A());
   }
   Ptr definite_integral() const override {
-    Ptr hi = Variable<kernelT, evalT>(x()->get_to_name()), lo = Variable<kernelT, evalT>(x()->get_from_name());
-    return ConditionalEqual<kernelT, evalT>(
-        A(), (hi->copy() - lo->copy()) / norm(),
-        (Exp<kernelT, evalT>(A() * std::move(hi)) - Exp<kernelT, evalT>(A() * std::move(lo))) / (A() * norm()));
+    // the branches are built first: argument evaluation order is unspecified
+    Ptr width = Variable<kernelT, evalT>(x()->get_to_name()) - Variable<kernelT, evalT>(x()->get_from_name());
+    Ptr expdiff = Exp<kernelT, evalT>(A() * Variable<kernelT, evalT>(x()->get_to_name())) -
+                  Exp<kernelT, evalT>(A() * Variable<kernelT, evalT>(x()->get_from_name()));
+    return ConditionalEqual<kernelT, evalT>(A(), std::move(width) / norm(), std::move(expdiff) / (A() * norm()));
   }
   evalT get_max() const override {
     evalT a = alpha()->get_value();

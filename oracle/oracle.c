@@ -419,6 +419,79 @@ int orc_nll(int model, const double* data, long n, long stride, const double* th
 }
 
 /* ------------------------------------------------------------------------------------
+ * Bin integrals (the plotter's integral_kernel, plotter.hh:138-216, over
+ * PDF::definite_integral, pdf.hh:75-82 and physicspdfs.hh:82-164): the normalised density
+ * integrated over [lo, hi] in observable `var`, the other observables over their full ranges.
+ * Restated by quadrature of orc_prob, not by the antiderivatives the product uses:
+ * composite Gauss-Legendre (4 panels of 20 nodes, split at the Crystal Ball join), and
+ * 16 x 16 nodes over the angular model's other observables.
+ * ---------------------------------------------------------------------------------- */
+#define GL_N 20
+#define GL_M 16
+
+static void gauss_legendre(int n, double* x, double* w) { /* Newton on P_n, textbook */
+  for (int i = 0; i < n; i++) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; it++) {
+      double p1 = 1.0, p0 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        double pm = p0;
+        p0 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p0 - (j - 1.0) * pm) / j;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+static double marginal(int model, const double* th, const double* r, int var, double v,
+                       const double* mx, const double* mw) {
+  if (orc_nobs(model) == 1) return orc_prob(model, &v, th, r);
+  int o1 = (var + 1) % 3, o2 = (var + 2) % 3;
+  double h1 = 0.5 * (r[2 * o1 + 1] - r[2 * o1]), c1 = 0.5 * (r[2 * o1 + 1] + r[2 * o1]);
+  double h2 = 0.5 * (r[2 * o2 + 1] - r[2 * o2]), c2 = 0.5 * (r[2 * o2 + 1] + r[2 * o2]);
+  double pt[3], s = 0.0;
+  pt[var] = v;
+  for (int i = 0; i < GL_M; i++)
+    for (int j = 0; j < GL_M; j++) {
+      pt[o1] = c1 + h1 * mx[i];
+      pt[o2] = c2 + h2 * mx[j];
+      s += mw[i] * mw[j] * orc_prob(model, pt, th, r);
+    }
+  return h1 * h2 * s;
+}
+
+double orc_bin_integral(int model, const double* th, const double* r, int var, double lo, double hi) {
+  double gx[GL_N], gw[GL_N], mx[GL_M], mw[GL_M];
+  gauss_legendre(GL_N, gx, gw);
+  gauss_legendre(GL_M, mx, mw);
+  double cut[3];
+  int nc = 0;
+  cut[nc++] = lo;
+  if (model == ORC_CB) {
+    double join = th[0] - th[2] * th[1];
+    if (join > lo && join < hi) cut[nc++] = join;
+  }
+  cut[nc++] = hi;
+  double total = 0.0;
+  for (int c = 0; c + 1 < nc; c++) {
+    const int panels = 4;
+    double w = (cut[c + 1] - cut[c]) / panels;
+    for (int p = 0; p < panels; p++) {
+      double a = cut[c] + p * w, h = 0.5 * w, m = a + h, s = 0.0;
+      for (int i = 0; i < GL_N; i++) s += gw[i] * marginal(model, th, r, var, m + h * gx[i], mx, mw);
+      total += h * s;
+    }
+  }
+  return total;
+}
+
+/* ------------------------------------------------------------------------------------
  * PRNG (random.hh:160-185, 333-465, 42-83)
  * ---------------------------------------------------------------------------------- */
 uint64_t orc_splitmix64_next(uint64_t* state) {

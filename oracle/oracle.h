@@ -56,6 +56,10 @@ double orc_prob(int model, const double* x, const double* theta, const double* r
 /* PDF::get_max() of the reference (upper bound used by accept-reject) */
 double orc_get_max(int model, const double* theta, const double* range);
 
+/* integral of the normalised density over [lo, hi] in observable var, the others over their
+ * full ranges (plotter bins), by Gauss-Legendre quadrature of orc_prob */
+double orc_bin_integral(int model, const double* theta, const double* range, int var, double lo, double hi);
+
 /* Per-output sums over events.  data is SoA: observable d of event i at data[d*stride + i].
  * mode 0: out[0] = sum log p
  * mode 1: + out[1..P] = sum dlog p/dtheta

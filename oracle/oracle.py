@@ -36,6 +36,8 @@ def lib():
         L.orc_prob.restype = ctypes.c_double
         L.orc_get_max.argtypes = [ctypes.c_int, dp, dp]
         L.orc_get_max.restype = ctypes.c_double
+        L.orc_bin_integral.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_bin_integral.restype = ctypes.c_double
         L.orc_nll.argtypes = [ctypes.c_int, dp, ctypes.c_long, ctypes.c_long, dp, dp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
         L.orc_splitmix64_next.restype = ctypes.c_uint64
@@ -94,6 +96,15 @@ def get_max(model, theta, rng=None):
     theta = _f64(theta)
     rng = _f64(RANGES[model] if rng is None else rng)
     return lib().orc_get_max(model, _dp(theta), _dp(rng))
+
+
+def bin_integrals(model, theta, var, lo, hi, rng=None):
+    """integral of the normalised density over each [lo[i], hi[i]] in observable var (the
+    others over their full ranges), by quadrature"""
+    theta = _f64(theta)
+    rng = _f64(RANGES[model] if rng is None else rng)
+    return np.array([lib().orc_bin_integral(model, _dp(theta), _dp(rng), int(var), float(a), float(b))
+                     for a, b in zip(np.atleast_1d(lo), np.atleast_1d(hi))])
 
 
 def nll_sums(model, data, theta, mode=1, nthreads=1, rng=None):
