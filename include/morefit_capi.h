@@ -65,7 +65,12 @@ typedef struct mfh_fit_out {
   int iterations;
   int hesse_ok;
   int thread_moved;  /* batched fits: the fit's fibre resumed on another OS thread (never expected) */
-  int pad;
+  int nonfinite_evals;         /* evaluations whose sums came back non-finite (failure detection) */
+  long long nonfinite_events;  /* events with non-finite per-event terms at such a point (-1: unknown) */
+  long long first_nonfinite;   /* the smallest such event index (-1: none or unknown) */
+  long long out_of_range;      /* events with an observable outside its range or not a number (plain
+                                  fits: of the bound data; batched fits: found by the diagnostic pass,
+                                  which runs only after a non-finite evaluation) */
 } mfh_fit_out;
 
 typedef struct mfh_batch_stats {
@@ -152,6 +157,11 @@ long long mfh_batch_bytes(mfh_batch* b);
 /* fits of the listed segments (NULL = all) from start values (nfit x ntheta) */
 int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, int use_graph, double* values, double* errors,
                   mfh_fit_out* outs, mfh_batch_stats* stats);
+/* failure detection (the reference's NaN check of the host sum, fitter.hh:1197-1204, as a
+ * device pass): for each listed segment at theta (nseg x ntheta), out[3 i ..] = events whose
+ * per-event log-density terms are not all finite, the smallest such event index (-1: none),
+ * events with an observable outside its range or not a number */
+int mfh_batch_check(mfh_batch* b, const int* segs, int nseg, const double* theta, long long* out);
 
 /* Fit service with continuous batching over the given models (request class = index into
  * `models`, all of one precision): requests arrive at arrival_s (seconds after the start),

@@ -26,6 +26,7 @@
  *   mf_launch_eval                  SubmitKernel: per-event kernel + kahan_<name> passes (compute_opencl.hh:880-995)
  *   mf_launch_eval_noalloc          no counterpart: mf_launch_eval with the library's own static scratch for the
  *                                   partials, for callers that cannot allocate
+ *   mf_launch_check                 the NaN check of the host sum (fitter.hh:1197-1204), as a per-event device pass
  *   mf_launch_prep                  host eval() of the hoisted parameter terms (fitter.hh:1063-1072)
  *   mf_launch_prep_inline           the same with theta passed by value (no counterpart: latency of one call)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
@@ -268,6 +269,13 @@ int mf_launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args,
  * mf_launch_eval. */
 #define MF_NOALLOC_SCRATCH_BYTES (4u << 20)
 int mf_launch_eval_noalloc(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
+/* a diagnostic kernel of the module (MF_CHECK_KERNEL: its functor's per-event terms over the
+ * active segments of args): args->store points at unsigned int [args->nseg][3] that the caller
+ * sets to {0, 0xffffffff, 0} per segment; the kernel adds the events whose terms are not all
+ * finite, lowers the second word to the smallest such event index, and adds the events with an
+ * observable outside the density's range (or not a number; modules generated without ranges
+ * leave that word alone).  args->partials and args->results are not used. */
+int mf_launch_check(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block);
 /* kernel parameters of segments seg_list[0..nseg) (all 0..nseg-1 when seg_list is NULL) from
  * their FP64 theta: params[s * (ntheta + nhoist) ...] in the module's type and, when hoist64 is
  * set, the hoisted terms in FP64 at hoist64[s * nhoist ...].  With seg_src set, each of those

@@ -57,7 +57,9 @@ class Options(ctypes.Structure):
 
 class FitOut(ctypes.Structure):
     _fields_ = [("fval", ctypes.c_double), ("edm", ctypes.c_double), ("status", ctypes.c_int), ("ncalls", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("hesse_ok", ctypes.c_int), ("thread_moved", ctypes.c_int), ("pad", ctypes.c_int)]
+                ("iterations", ctypes.c_int), ("hesse_ok", ctypes.c_int), ("thread_moved", ctypes.c_int),
+                ("nonfinite_evals", ctypes.c_int), ("nonfinite_events", ctypes.c_longlong), ("first_nonfinite", ctypes.c_longlong),
+                ("out_of_range", ctypes.c_longlong)]
 
 
 class BatchStats(ctypes.Structure):
@@ -77,13 +79,13 @@ HIP_SYMBOLS = [
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_prep_inline", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
     "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped", "mf_launch_seed_streams", "mf_module_occupancy",
-    "mf_launch_eval_noalloc",
+    "mf_launch_eval_noalloc", "mf_launch_check",
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",
     "mfh_model_create", "mfh_model_destroy", "mfh_model_info", "mfh_model_source", "mfh_model_precompile", "mfh_model_kernels", "mfh_model_host_outputs", "mfh_stream_states", "mfh_autotune", "mfh_autotune_candidates",
     "mfh_batch_create", "mfh_batch_destroy", "mfh_batch_upload", "mfh_batch_download", "mfh_batch_generate",
-    "mfh_batch_eval", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
+    "mfh_batch_eval", "mfh_batch_check", "mfh_batch_time_kernel", "mfh_batch_time_eval", "mfh_batch_time_graph", "mfh_batch_bytes", "mfh_batch_fit",
     "mfh_fit_plain", "mfh_fit_cpu", "mfh_eval_cpu", "mfh_serve_run", "mfh_fit_weighted", "mfh_fit_options",
     "mfh_plain_create", "mfh_plain_fit", "mfh_plain_eval", "mfh_plain_destroy", "mfh_plot",
 ]
@@ -139,6 +141,7 @@ def libs():
         L.mfh_batch_download.argtypes = [vp, ip, dp]
         L.mfh_batch_generate.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_ulonglong), ip, ctypes.POINTER(ctypes.c_ulonglong)]
         L.mfh_batch_eval.argtypes = [vp, ip, dp, ip, dp]
+        L.mfh_batch_check.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ip, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.mfh_batch_time_kernel.argtypes = [vp, ip, ip, ip, dp]
         L.mfh_batch_time_graph.argtypes = [vp, ip, ip, dp]
         L.mfh_batch_time_eval.argtypes = [vp, ip, ip, ip, dp]
@@ -397,6 +400,18 @@ class Batch:
         us = ctypes.c_double(0)
         _check(libs()[1].mfh_batch_time_eval(self.h, kind, iters, int(use_graph), ctypes.byref(us)))
         return us.value
+
+    def check(self, theta, segs=None):
+        """failure detection at theta (one row per listed segment): per segment the events with
+        non-finite log-density terms, the first of them (-1: none), the events out of range"""
+        P = self.model.ntheta
+        lst = list(range(self.S)) if segs is None else list(segs)
+        n = len(lst)
+        th = _f64(np.broadcast_to(np.asarray(theta, dtype=np.float64), (n, P)))
+        out = np.zeros(3 * n, dtype=np.int64)
+        _check(libs()[1].mfh_batch_check(self.h, (ctypes.c_int * n)(*lst), n, _dp(th),
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        return out.reshape(n, 3)
 
     def fit(self, start, segs=None, use_graph=False):
         P = self.model.ntheta

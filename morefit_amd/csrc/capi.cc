@@ -636,6 +636,22 @@ long long mfh_batch_bytes(mfh_batch* b) {
   return b->with([&](auto& bb) { return (long long)bb.data_bytes(); });
 }
 
+int mfh_batch_check(mfh_batch* b, const int* segs, int nseg, const double* theta, long long* out) {
+  MFH_TRY
+  if (!b || nseg < 0 || (nseg > 0 && (!segs || !theta || !out))) throw std::invalid_argument("mfh_batch_check: bad arguments");
+  b->with([&](auto& bb) {
+    std::vector<int> list(segs, segs + nseg);
+    std::vector<double> th(theta, theta + (size_t)nseg * bb.m->ntheta);
+    auto d = bb.diagnose(list, th);
+    for (int i = 0; i < nseg; i++) {
+      out[3 * i] = d[i].nonfinite;
+      out[3 * i + 1] = d[i].first;
+      out[3 * i + 2] = d[i].outside;
+    }
+  });
+  MFH_CATCH
+}
+
 int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, int use_graph, double* values, double* errors,
                   mfh_fit_out* outs, mfh_batch_stats* stats) {
   MFH_TRY
@@ -666,6 +682,10 @@ int mfh_batch_fit(mfh_batch* b, const int* segs, int nfit, const double* start, 
         outs[i].iterations = res[i].iterations;
         outs[i].hesse_ok = res[i].hesse_ok;
         outs[i].thread_moved = res[i].thread_moved;
+        outs[i].nonfinite_evals = res[i].nonfinite_evals;
+        outs[i].nonfinite_events = res[i].nonfinite_evals > 0 ? res[i].nonfinite_events : 0;
+        outs[i].first_nonfinite = res[i].first_nonfinite_event;
+        outs[i].out_of_range = res[i].out_of_range_events;
       }
     }
     if (stats) {
@@ -751,6 +771,15 @@ int mfh_serve_run(mfh_model* const* models, int nmodels, const double* theta_gen
 }  // extern "C"
 
 namespace {
+template <typename F>
+void fill_failure(mfh_fit_out& out, const F& f) {
+  out.thread_moved = 0;
+  out.nonfinite_evals = f.nonfinite_evals_;
+  out.nonfinite_events = f.nonfinite_evals_ > 0 ? f.nonfinite_events_ : 0;
+  out.first_nonfinite = f.first_nonfinite_event_;
+  out.out_of_range = f.out_of_range_events_;
+}
+
 template <typename K, typename Backend, typename Block>
 void run_fit(Backend* be, int kind, const double* data, long long n, const double* start, double* values, double* errors, mfh_fit_out* out,
              double* ms, const double* weights = nullptr, bool optimize_dimensions = false, int flags = MFH_FIT_DEFAULT) {
@@ -790,7 +819,7 @@ void run_fit(Backend* be, int kind, const double* data, long long n, const doubl
     out->ncalls = f.ncalls_;
     out->iterations = 0;
     out->hesse_ok = !f.cov_.empty();
-    out->thread_moved = 0;
+    fill_failure(*out, f);
   }
   if (ms) *ms = f.fit_ms_;
 }
@@ -924,7 +953,7 @@ struct PlainSession : mfh_plain {
       out->ncalls = f.ncalls_;
       out->iterations = 0;
       out->hesse_ok = !f.cov_.empty();
-      out->thread_moved = 0;
+      fill_failure(*out, f);
     }
     if (ms) *ms = f.fit_ms_;
   }

@@ -558,6 +558,64 @@ __device__ __forceinline__ void mf_store_impl(const mf_eval_args& a) {
   }
 }
 
+// ------------------------------------------------------------------ diagnostics
+// For sums that came back non-finite: per active segment, the events whose per-event terms are
+// not all finite, the smallest such event index, and the events with an observable outside the
+// density's range or not a number.  a.store points at unsigned int [nseg][3] = {count, first,
+// outside}, set by the host to {0, 0xffffffff, 0}.  One ballot per wave and event step, one
+// atomic per wave and counter.  Finiteness is a test of the exponent bits, which no
+// floating-point mode folds away.
+__device__ __forceinline__ bool mf_finite(float v) { return (__float_as_uint(v) & 0x7f800000u) != 0x7f800000u; }
+__device__ __forceinline__ bool mf_finite(double v) {
+  return ((unsigned long long)__double_as_longlong(v) & 0x7ff0000000000000ull) != 0x7ff0000000000000ull;
+}
+template <class F>
+__device__ __forceinline__ void mf_check_impl(const mf_eval_args& a) {
+  constexpr int NO = F::nacc;
+  const MF_T* __restrict__ data = (const MF_T*)a.data;
+  unsigned int* __restrict__ out = (unsigned int*)a.store;
+  const MF_CONST MF_T* params = mf_params_of(a);
+#ifdef MF_OBS_LO
+  const MF_T lo[MF_NOBS] = MF_OBS_LO, hi[MF_NOBS] = MF_OBS_HI;  // stored coordinates
+#endif
+  for (int c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+    const mf_chunk ch = a.chunks[c];
+    const mf_segment sg = a.segs[ch.seg];
+    if (sg.mode == 0) continue;
+    const MF_T* __restrict__ P = (const MF_T*)(params + (a.inline_params ? 0 : sg.param_off));
+    const MF_T* __restrict__ col = data + sg.data_off;
+    // the trip count is the same for every lane, so every lane reaches the ballots
+    for (int i0 = ch.begin; i0 < ch.end; i0 += MF_BLOCK) {
+      const int i = i0 + (int)threadIdx.x;
+      bool bad = false, outside = false;
+      if (i < ch.end) {
+        MF_T x[MF_NCOL];
+#pragma unroll
+        for (int d = 0; d < MF_NCOL; d++) x[d] = col[d * sg.stride + i];
+#ifdef MF_OBS_LO
+#pragma unroll
+        for (int d = 0; d < MF_NOBS; d++) outside = outside || !(x[d] >= lo[d] && x[d] <= hi[d]);
+#endif
+        MF_T o[NO];
+        F::eval(x, P, o);
+#pragma unroll
+        for (int k = 0; k < NO; k++) bad = bad || !mf_finite(o[k]);
+      }
+      const unsigned long long mb = __ballot(bad), mo = __ballot(outside);
+      if ((threadIdx.x & 63) == 0) {
+        unsigned int* o3 = out + 3 * ch.seg;
+        if (mb) {
+          atomicAdd(o3, (unsigned int)__popcll(mb));
+          atomicMin(o3 + 1, (unsigned int)(i + __ffsll((unsigned long long)mb) - 1));
+        }
+        if (mo) atomicAdd(o3 + 2, (unsigned int)__popcll(mo));
+      }
+    }
+  }
+}
+#define MF_CHECK_KERNEL(NAME, FUNCTOR) \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_check_impl<FUNCTOR>(a); }
+
 // ------------------------------------------------------------------ event columns
 // Event-only terms of the density (generated mf_event_terms: FP64 from the stored observables)
 // into columns MF_NOBS..MF_NCOL-1 of every segment whose mode is set, rounded once to MF_T.

@@ -755,6 +755,23 @@ int mf_launch_eval_noalloc(mf_module* mod, const char* kernel, const mf_eval_arg
   return launch_eval(mod, kernel, &a, grid, block);
 }
 
+int mf_launch_check(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block) {
+  MF_INJECT("mf_launch_check");
+  if (!mod || !kernel || !args) return set_error(MF_ERR_INVALID, "mf_launch_check: null argument");
+  if (grid <= 0 || block <= 0 || block % 64 != 0 || block > 1024)
+    return set_error(MF_ERR_INVALID, "mf_launch_check: bad launch shape %d x %d", grid, block);
+  if (args->nchunks < 0 || args->nseg < 0 || !args->segs || !args->chunks || !args->params || !args->data || !args->store)
+    return set_error(MF_ERR_INVALID, "mf_launch_check: incomplete argument block");
+  if (args->nchunks == 0) return MF_OK;
+  hipFunction_t fn;
+  int rc = get_fn(mod, kernel, &fn);
+  if (rc != MF_OK) return rc;
+  mf_eval_args a = *args;
+  void* kp[] = {(void*)&a};
+  MF_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, mod->dev->stream, kp, nullptr), "mf_launch_check");
+  return MF_OK;
+}
+
 static int launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* args, int grid, int block) {
   if (!mod || !kernel || !args) return set_error(MF_ERR_INVALID, "mf_launch_eval: null argument");
   if (grid <= 0 || block <= 0 || block % 64 != 0 || block > 1024)

@@ -36,6 +36,11 @@ struct BatchFitResult {
   int status = -1, ncalls = 0, iterations = 0;
   bool hesse_ok = false;
   bool thread_moved = false;  // the fit's fibre was resumed on another OS thread (never expected)
+  // evaluations whose outputs came back non-finite; at the last such point, from the diagnostic
+  // kernel: events with non-finite per-event terms, the first of them (-1: none), and events
+  // with an observable outside its range or not a number
+  int nonfinite_evals = 0;
+  long long nonfinite_events = 0, first_nonfinite_event = -1, out_of_range_events = 0;
 };
 
 struct BatchFitStats {
@@ -65,6 +70,7 @@ class FitScheduler {
     bool done = false;
     BatchFitResult result;
     std::thread::id first_thread;   // OS thread the fibre first ran on
+    std::vector<double> bad_theta;  // theta of the last evaluation that came back non-finite
   };
 
   bool use_graph = false;  // direct launches: on this stack cheaper than a graph replay
@@ -135,6 +141,7 @@ class FitScheduler {
     }
     for (auto it = tasks_.begin(); it != tasks_.end();) {
       if (it->done) {
+        if (it->result.nonfinite_evals > 0) diagnose(*it);
         on_done(*it);
         FiberStacks::get().release(it->stack);
         it = tasks_.erase(it);
@@ -187,6 +194,14 @@ class FitScheduler {
       t->pending = kind;
       mf_fiber_switch(&t->sp, *t->ret);
       note_thread(*t);
+      const double* r = t->b->result(t->seg, kind);
+      bool finite = true;
+      for (int k = 0; k < t->b->m->nout[kind]; k++) finite = finite && std::isfinite(r[k]);
+      if (!finite) {
+        t->result.nonfinite_evals++;
+        t->bad_theta.resize(ti.size());
+        for (size_t k = 0; k < ti.size(); k++) t->bad_theta[k] = x[ti[k]];
+      }
     }
     double value(const double* x) override {
       post(x, EVAL_LH);
@@ -228,6 +243,18 @@ class FitScheduler {
   // per step: the tasks worker w resumes (its own, in order)
   std::vector<std::vector<Task*>> groups_;
 
+  // where the sums of a finished fit came back non-finite: count the offending events at the
+  // last such theta on the device (the calling thread issues all device work)
+  static void diagnose(Task& t) {
+    try {
+      auto d = t.b->diagnose({t.seg}, t.bad_theta);
+      t.result.nonfinite_events = d[0].nonfinite;
+      t.result.first_nonfinite_event = d[0].first;
+      t.result.out_of_range_events = d[0].outside;
+    } catch (const std::exception&) {
+      t.result.nonfinite_events = -1;  // the diagnostic itself failed; the fit's status stands
+    }
+  }
   static void resume(Task& t, void** here) {
     t.ret = here;
     mf_fiber_switch(here, t.sp);

@@ -40,6 +40,8 @@ struct ModuleSource {
   int acc = 1;
   bool fast_math = false;
   std::vector<double> gen_lo, gen_width;  // proposal box per observable, in stored coordinates
+  std::vector<double> obs_lo, obs_hi;      // ranges of the observables in stored coordinates, for the
+                                           // diagnostic kernel (empty: no range test)
   std::string functors;                    // functor structs
   std::string hoist;                       // mf_hoist_term definition
   std::string event_terms;                 // mf_event_terms definition (ncol > nobs)
@@ -68,6 +70,21 @@ struct ModuleSource {
       s << buf;
     }
     s << " } while (0)\n";
+    if ((int)obs_lo.size() == nobs && (int)obs_hi.size() == nobs && nobs > 0) {
+      // rounded to the kernel type as the uploaded observables are, so an event at a boundary
+      // compares equal to it
+      auto list = [&](const std::vector<double>& v) {
+        std::string r = "{";
+        for (int d = 0; d < nobs; d++) {
+          char buf[64];
+          if (fp32) std::snprintf(buf, sizeof(buf), "%s%af", d ? ", " : "", (double)(float)v[d]);
+          else std::snprintf(buf, sizeof(buf), "%s%a", d ? ", " : "", v[d]);
+          r += buf;
+        }
+        return r + "}";
+      };
+      s << "#define MF_OBS_LO " << list(obs_lo) << "\n#define MF_OBS_HI " << list(obs_hi) << "\n";
+    }
     return s.str();
   }
   // generated module text: prelude, the runtime marker, then functors/hoist/kernels

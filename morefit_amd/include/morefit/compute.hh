@@ -136,6 +136,10 @@ class ComputeBlock {
   virtual bool SubmitKernel() = 0;
   virtual bool Finish() = 0;
   virtual bool ProvidesKahan() { return false; }
+  // events whose per-event outputs of the current summing kernel are not all finite at the
+  // current parameter buffer, and the smallest such index (-1: none); false when the block
+  // cannot tell (no counterpart in the reference, which prints its per-event buffer instead)
+  virtual bool CountNonFinite(long long* count, long long* first) { return false; }
 };
 
 }  // namespace morefit

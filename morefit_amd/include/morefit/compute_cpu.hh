@@ -17,6 +17,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -413,6 +414,24 @@ class CPUBlock : public ComputeBlock<kernelT, evalT, seedT> {
     return true;
   }
   bool Finish() override { return true; }
+  // the per-event outputs of the block's functor over its events, counted on the host
+  bool CountNonFinite(long long* count, long long* first) override {
+    State& s = *s_;
+    if (!s.fstore || s.generate || !s.input || s.nout == 0) return false;
+    std::vector<kernelT> o((size_t)s.nout * (size_t)std::max<long long>(1, s.nevents));
+    s.fstore(s.input->data(), s.stride, 0, s.nevents, s.params.empty() ? nullptr : s.params.data(), o.data(), s.nevents);
+    *count = 0;
+    *first = -1;
+    for (long long i = 0; i < s.nevents; i++) {
+      bool ok = true;
+      for (int k = 0; k < s.nout; k++) ok = ok && std::isfinite((double)o[(size_t)k * s.nevents + i]);
+      if (!ok) {
+        if (*first < 0) *first = i;
+        (*count)++;
+      }
+    }
+    return true;
+  }
 };
 
 }  // namespace morefit

@@ -381,8 +381,12 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     ms.functors = emit_functor("mf_f_block", outs, obs, params, ms.fp32, &ms.statements);
     ms.hoist = empty_hoist();
     s.kernel = name;
-    if (kahan_summation) ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final});
-    else ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
+    if (kahan_summation) {
+      ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final});
+      ms.kernels.push_back("MF_CHECK_KERNEL(" + name + "_check, mf_f_block)");
+    } else {
+      ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
+    }
     compile(ms);
     s.nout = (int)graphs.size();
     s.kahan = kahan_summation;
@@ -474,6 +478,29 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
   }
   bool Finish() override {
     hip_check(mf_sync(dev()), "HIPBlock::Finish");
+    return true;
+  }
+  // the summing kernel's diagnostic twin (NAME_check) over the block's events
+  bool CountNonFinite(long long* count, long long* first) override {
+    State& s = *s_;
+    if (!s.mod || !s.kahan || s.generate || !s.input || s.params.bytes == 0) return false;
+    build_tables();
+    unsigned int c[3] = {0u, 0xffffffffu, 0u};
+    DevBuf out(dev(), sizeof(c));
+    hip_check(mf_memcpy_h2d(dev(), out.ptr, c, sizeof(c)), "CountNonFinite");
+    mf_eval_args a{};
+    a.data = s.input->ptr;
+    a.segs = s.segs.template as<mf_segment>();
+    a.chunks = s.chunks.template as<mf_chunk>();
+    a.nchunks = nchunks_;
+    a.nseg = 1;
+    a.params = s.params.ptr;
+    a.store = out.ptr;
+    hip_check(mf_launch_check(s.mod, (s.kernel + "_check").c_str(), &a, std::max(1, nchunks_), s.block), "mf_launch_check");
+    hip_check(mf_memcpy_d2h(dev(), c, out.ptr, sizeof(c)), "CountNonFinite");
+    hip_check(mf_sync(dev()), "CountNonFinite");
+    *count = c[0];
+    *first = c[1] == 0xffffffffu ? -1 : (long long)c[1];
     return true;
   }
   mf_module* module() const { return s_->mod; }
@@ -735,6 +762,8 @@ struct LikelihoodGraphs {
     for (size_t d = 0; d < obs.size(); d++) {
       ms.gen_lo.push_back(lo[d] - sc[d]);
       ms.gen_width.push_back(hi[d] - lo[d]);
+      ms.obs_lo.push_back(lo[d] - sc[d]);
+      ms.obs_hi.push_back(hi[d] - sc[d]);
     }
     auto pn = param_names();
     const auto cols = columns();
@@ -753,7 +782,7 @@ struct LikelihoodGraphs {
     const bool f = o.hip_fused_final;
     ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f}, {"mf_eval_grad", "mf_f_grad", u, f}};
     if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u, f});
-    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)"};
+    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)", "MF_CHECK_KERNEL(mf_check_lh, mf_f_lh)"};
     if (!ev_exprs.empty()) ms.kernels.push_back("MF_PRECOMPUTE_KERNEL");
     return ms;
   }
@@ -992,6 +1021,7 @@ class HipBatch {
   int done_seq_ = 0;
   mf_graph* graphs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<int> request_;
+  DevBuf check_segs_, check_out_;  // diagnostic pass: its segment table and counters
 
   // fixed segments, packed back to back
   HipBatch(HipModel<kernelT, evalT>* model, const std::vector<long long>& seg_sizes) : m(model), dev(model->be->device()) {
@@ -1412,6 +1442,56 @@ class HipBatch {
     }
     hip_check(mf_sync(dev), "HipBatch run");
   }
+  // Per listed slot, at the slot's theta in `thetas` (ntheta each): the events whose per-event
+  // log-density terms are not all finite, the smallest such event index (-1: none), and the
+  // events with an observable outside its range or not a number -- the diagnostic kernel
+  // mf_check_lh, run for fits whose sums came back non-finite (the reference's NaN check of
+  // its host sum, fitter.hh:1197-1204).  Kernel parameters of every slot are recomputed from
+  // the mapped theta first, as before any launch.
+  struct Diagnosis {
+    long long nonfinite = 0, first = -1, outside = 0;
+  };
+  std::vector<Diagnosis> diagnose(const std::vector<int>& slots, const std::vector<double>& thetas) {
+    if (thetas.size() < slots.size() * (size_t)m->ntheta) throw std::invalid_argument("HipBatch::diagnose: fewer theta values than slots");
+    rebuild();
+    std::vector<Diagnosis> r(slots.size());
+    std::vector<mf_segment> t(segs_h);
+    for (auto& e : t) e.mode = 0;
+    for (size_t i = 0; i < slots.size(); i++) {
+      const int s = slots[i];
+      if (s < 0 || s >= S || !active[s]) throw std::invalid_argument("HipBatch::diagnose: slot " + std::to_string(s) + " holds no dataset");
+      t[s].mode = 1;
+      std::memcpy(theta(s), &thetas[i * (size_t)m->ntheta], m->ntheta * sizeof(double));
+    }
+    if (slots.empty() || nchunks == 0) return r;
+    std::vector<unsigned int> c(3 * (size_t)S, 0u);
+    for (int s = 0; s < S; s++) c[3 * (size_t)s + 1] = 0xffffffffu;
+    check_segs_.alloc(dev, t.size() * sizeof(mf_segment));
+    check_out_.alloc(dev, c.size() * sizeof(unsigned int));
+    hip_check(mf_memcpy_h2d(dev, check_segs_.ptr, t.data(), t.size() * sizeof(mf_segment)), "diagnose segments");
+    hip_check(mf_memcpy_h2d(dev, check_out_.ptr, c.data(), c.size() * sizeof(unsigned int)), "diagnose counters");
+    hip_check(mf_launch_prep(m->mod, (const double*)theta_h.dptr, params_d.ptr, hoist64_d.template as<double>(), nullptr, S, nullptr, nullptr),
+              "mf_launch_prep");
+    mf_eval_args a{};
+    a.data = data_ptr();
+    a.segs = check_segs_.template as<mf_segment>();
+    a.chunks = chunks_d.template as<mf_chunk>();
+    a.nchunks = nchunks;
+    a.nseg = S;
+    a.params = params_d.ptr;
+    a.store = check_out_.ptr;
+    hip_check(mf_launch_check(m->mod, "mf_check_lh", &a, grid(), m->block), "mf_launch_check");
+    hip_check(mf_memcpy_d2h(dev, c.data(), check_out_.ptr, c.size() * sizeof(unsigned int)), "diagnose counters");
+    hip_check(mf_sync(dev), "diagnose");
+    for (size_t i = 0; i < slots.size(); i++) {
+      const size_t s = (size_t)slots[i];
+      r[i].nonfinite = c[3 * s];
+      r[i].first = c[3 * s + 1] == 0xffffffffu ? -1 : (long long)c[3 * s + 1];
+      r[i].outside = c[3 * s + 2];
+    }
+    return r;
+  }
+
   // one launch per kind that any segment requested; requests are cleared
   int run_requested(bool use_graph) {
     int launches = 0;

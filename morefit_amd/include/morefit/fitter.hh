@@ -114,6 +114,13 @@ class fitter {
   int ncalls_ = 0, ngrad_ = 0, nhess_ = 0;
   std::vector<double> cov_;
   double fit_ms_ = 0.0, kernel_ms_ = 0.0;
+  // failure detection of the last fit: evaluations whose sums came back non-finite; at the
+  // first of them, the events with non-finite per-event terms and the first such index (from
+  // the block's diagnostic pass, -1: not known) and the parameter values; events of the bound
+  // data with an observable outside its range or not a number
+  int nonfinite_evals_ = 0;
+  long long nonfinite_events_ = -1, first_nonfinite_event_ = -1, out_of_range_events_ = 0;
+  std::vector<double> nonfinite_params_;
 
   fitter(fitter_options* options, backendT* backend)
       : opts_(options),
@@ -241,6 +248,22 @@ class fitter {
     if (data->nevents_padded() > (size_t)std::numeric_limits<int>::max())
       throw std::invalid_argument("fitter: " + std::to_string(data->nevents_padded()) + " events do not fit the blocks' int event counts");
     nevents_ = data->nevents();
+    // events outside the observables' ranges (or not a number) are fitted as they are, as in
+    // the reference; their number is reported with the fit
+    out_of_range_events_ = 0;
+    std::vector<std::pair<unsigned, dimension<evalT>*>> cols;
+    for (auto* d : pdf_->dimensions()) {
+      const int j = data->dim_index(d->get_name());
+      if (j >= 0) cols.emplace_back((unsigned)j, d);
+    }
+    for (size_t i = 0; i < data->nevents(); i++) {
+      bool in = true;
+      for (auto& c : cols) {
+        const double x = (double)(*data)(i, c.first);
+        in = in && x >= c.second->get_min() && x <= c.second->get_max();
+      }
+      if (!in) out_of_range_events_++;
+    }
     apply_centres(block_, centres_);
     if (opts_->optimize_dimensions) {
       // observables in, observables and event columns out; the kernels read that buffer
@@ -352,7 +375,39 @@ class fitter {
       for (int k = 0; k < nout_[which]; k++) sums.push_back(kahan_summation<evalT, kernelT>(r, k));
     }
     kernel_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    bool finite = true;
+    for (auto v : sums) finite = finite && std::isfinite((double)v);
+    if (!finite) note_nonfinite(which, b);
     return sums;
+  }
+
+  // A non-finite sum (the reference checks the host sum of the likelihood for NaN and stops
+  // the program, fitter.hh:1197-1204): counted, and at the first one in a fit the offending
+  // events are counted by the block (on the device for the HIP block) or, for per-event
+  // outputs on the host, here; MIGRAD then treats the point as invalid.
+  void note_nonfinite(int which, computeT& b) {
+    if (nonfinite_evals_++ > 0) return;
+    nonfinite_params_.clear();
+    for (auto* p : params_) nonfinite_params_.push_back(p->get_value());
+    long long count = -1, first = -1;
+    if (opts_->kahan_on_accelerator) {
+      if (!b.CountNonFinite(&count, &first)) count = first = -1;
+    } else {
+      const EventVector<kernelT, evalT>& r = *res_buffer_[which];
+      count = 0;
+      for (size_t i = 0; i < r.nevents(); i++) {
+        bool ok = true;
+        for (int k = 0; k < nout_[which]; k++) ok = ok && std::isfinite((double)r(i, (unsigned)k));
+        if (!ok && count++ == 0) first = (long long)i;
+      }
+    }
+    nonfinite_events_ = count;
+    first_nonfinite_event_ = first;
+    if (opts_->print_level > 0) {
+      std::cout << "sum of the " << (which == 0 ? "likelihood" : which == 1 ? "gradient" : which == 2 ? "Hessian" : "Fisher")
+                << " terms is not finite: " << count << " events with non-finite terms, the first at " << first << std::endl;
+      for (auto* p : params_) std::cout << "parameter values: " << p->get_name() << " " << p->get_value() << std::endl;
+    }
   }
 
   double likelihood() { return likelihood(pdf_, params_, (EventVector<kernelT, evalT>*)nullptr); }
@@ -519,6 +574,9 @@ class fitter {
   bool fit(PDF<kernelT, evalT>* pdf, std::vector<parameter<evalT>*> params, EventVector<dataT, evalT>* data, bool refit = false) {
     auto t0 = std::chrono::steady_clock::now();
     kernel_ms_ = 0.0;
+    nonfinite_evals_ = 0;
+    nonfinite_events_ = first_nonfinite_event_ = -1;
+    nonfinite_params_.clear();
     prepare(pdf, params, data, refit);
     Migrad m(minimizer_params(true));
     m.strategy = (int)opts_->minuit_strategy;
