@@ -124,7 +124,8 @@ typedef struct mf_eval_args {
                                  kernel stores done_value there (after a system-scope fence), so the
                                  host can poll it instead of waiting on the stream.  Only with one
                                  segment whose totals the final kernel reduces (nseg == 1, neither
-                                 MF_EVAL_SKIP_FINAL nor seg_done). */
+                                 MF_EVAL_SKIP_FINAL nor seg_done), or with one segment of a single
+                                 chunk launched as one work-group (the per-chunk kernel signals). */
   int done_value;
   int inline_params;          /* 1: one-segment call whose kernel parameters are pinline (MF_T values,
                                  [theta, hoisted]) and whose FP64 hoisted terms are hinline, read from

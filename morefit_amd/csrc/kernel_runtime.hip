@@ -492,8 +492,15 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       }
       __syncthreads();
     }
-    // a segment of one chunk is complete here: its totals need no second pass
-    if (threadIdx.x == 0 && sg.chunk_end - sg.chunk_begin == 1) mf_write_totals<F>(a, ch.seg, sg, s_tot);
+    // a segment of one chunk is complete here: its totals need no second pass; a launch of one
+    // chunk publishes them to the host, then the flag it polls (as the final kernel does)
+    if (threadIdx.x == 0 && sg.chunk_end - sg.chunk_begin == 1) {
+      mf_write_totals<F>(a, ch.seg, sg, s_tot);
+      if (a.done_flag && a.nchunks == 1) {
+        __threadfence_system();
+        __hip_atomic_store(a.done_flag, a.done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
     const int nch = sg.chunk_end - sg.chunk_begin;
     if (FUSED && nch > 1 && seg_done) {
       __shared__ int s_last;

@@ -781,8 +781,9 @@ static int launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* a
   std::string fin = std::string(kernel) + "_final";
   hipFunction_t ff = nullptr;
   const bool final = args->nseg > 0 && !args->seg_done && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff);
-  if (args->done_flag && (args->nseg != 1 || !final || args->nchunks == 0))
-    return set_error(MF_ERR_INVALID, "mf_launch_eval: a done flag needs the final kernel over exactly one segment");
+  if (args->done_flag && (args->nseg != 1 || args->nchunks == 0 || !(final || (args->nchunks == 1 && grid == 1))))
+    return set_error(MF_ERR_INVALID,
+                     "mf_launch_eval: a done flag needs one segment, reduced by the final kernel or held in a single chunk");
   if (args->nchunks == 0) return MF_OK;
   hipFunction_t fn;
   int rc = get_fn(mod, kernel, &fn);

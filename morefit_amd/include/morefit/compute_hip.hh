@@ -1393,10 +1393,10 @@ class HipBatch {
   // a one-segment direct call: its kernels take theta and the host-computed hoisted terms by
   // value (no prep kernel)
   bool by_value(int kind, bool direct) const { return direct && S == 1 && m->host_params && kind > 0; }
-  // a one-segment batch whose totals the final kernel reduces: that kernel signals completion
-  // through a flag in mapped host memory, which the host polls (a stream synchronisation costs
-  // several microseconds more)
-  bool can_poll(int kind) const { return S == 1 && nchunks > 0 && max_seg_chunks > 1 && !fused_[kind]; }
+  // a one-segment batch whose totals the final kernel reduces, or which is one chunk: the last
+  // kernel signals completion through a flag in mapped host memory, which the host polls (a
+  // stream synchronisation costs several microseconds more)
+  bool can_poll(int kind) const { return S == 1 && (nchunks == 1 || (nchunks > 0 && max_seg_chunks > 1 && !fused_[kind])); }
   // wait for done_value in the flag: a bounded spin, then the stream (which reports any error)
   void wait_done(int done_value) {
     const int* f = done_h.template as<int>();
