@@ -309,6 +309,51 @@ def cpu_baselines(M, O, data, start, seconds):
     return out
 
 
+def call_latency_and_extras(M, O, ctx, data, start):
+    """Per-call latency of the FP32 mass evaluation at smaller N (the same launch shape; the
+    fixed cost of a call shows below 10^6 events), the plotter (integral kernel over 100 x 10
+    fine bins of the mass density on the device, histogram and pulls of 10^6 events; bin
+    integrals checked against the oracle's quadrature) and the failure-detection pass at
+    N = 10^7."""
+    out = {"call_latency": []}
+    m = ctx.model(M.MASS, fp32=True)
+    for n in (1000, 10_000, 100_000, 1_000_000):
+        b = m.batch([n])
+        b.upload(0, data[:, :n])
+        for _ in range(20):
+            b.eval(M.EVAL_GRAD, start)
+        iters = 300
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            b.eval(M.EVAL_GRAD, start)
+        py = (time.perf_counter() - t0) / iters * 1e6
+        out["call_latency"].append({"events": n, "python_us": py, "cpp_loop_us": b.time_eval(M.EVAL_GRAD, iters=iters),
+                                    "kernels_of_a_call_us": b.time_kernel(M.EVAL_GRAD, iters=iters) * 1e3})
+        b.close()
+    b = m.batch([data.shape[1]])
+    b.upload(0, data)
+    b.check(start)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        r = b.check(start)
+    out["diagnostic_check_us"] = (time.perf_counter() - t0) / 20 * 1e6
+    out["diagnostic_check_result"] = r[0].tolist()
+    b.close()
+    m.close()
+    sub = data[:, :1_000_000]
+    t0 = time.perf_counter()
+    p = M.plot(M.MASS, M.MASS_TRUTH, sub, nbins=100, multiplier=10, ctx=ctx, fp32=False)
+    plot_ms = (time.perf_counter() - t0) * 1e3
+    rp = M.plot(M.MASS, M.MASS_TRUTH, sub, nbins=100, multiplier=10, ctx=ctx, fp32=False, replots=10)
+    e = np.linspace(5.0, 5.6, 1001)
+    ref = O.bin_integrals(O.MASS, M.MASS_TRUTH, 0, e[:-1], e[1:])
+    ok = p["counts"] > 0
+    out["plot"] = {"events": sub.shape[1], "bins": 100, "fine_bins": 1000, "plot_ms": plot_ms, "replot_ms": rp["ms"],
+                   "fp64_max_abs_bin_err_vs_oracle": float(np.max(np.abs(p["pdf_integrals"] - ref))),
+                   "pull_mean": float(np.mean(p["pulls"][ok])), "pull_rms": float(np.sqrt(np.mean(p["pulls"][ok] ** 2)))}
+    return out
+
+
 def angular_workload(M, O):
     """The angular configuration: K*mumu P-wave density in (cos theta_l, cos theta_K, phi), 8 free
     parameters, N = 10^6 events generated on the device (FP64 generator), FP32 kernels in the
@@ -560,6 +605,12 @@ def main():
                         "fp32_vs_fp64_max_dtheta_over_sigma": float(np.max(np.abs(f32[0] - f64[0]) / e64f[0])),
                         "fp64_vs_near_start_max_dtheta_over_sigma": float(np.max(np.abs(f64[0] - v64f[0]) / e64f[0]))}
 
+    # ---------------------------------------------------------------- smaller fits, plotter, diagnostics
+    extras = call_latency_and_extras(M, O, ctx, data, start)
+    log("call latency (us): " + ", ".join(f"N={r['events']:.0e} {r['python_us']:.1f}" for r in extras["call_latency"]) +
+        f"; plot of 1e6 events {extras['plot']['plot_ms']:.2f} ms (re-plot {extras['plot']['replot_ms']:.2f} ms); "
+        f"diagnostic pass {extras['diagnostic_check_us']:.1f} us")
+
     # ---------------------------------------------------------------- HBM ceilings
     copy_gbs = ctx.copy_ceiling(1 << 31, iters=5)
     stream = None
@@ -721,6 +772,9 @@ def main():
         "streaming": stream,
         "toys": toys,
         "serving": serving,
+        "call_latency": extras["call_latency"],
+        "plotter": extras["plot"],
+        "diagnostic_check_us": extras["diagnostic_check_us"],
         "angular": angular,
         "device": name,
     }
