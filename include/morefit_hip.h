@@ -31,6 +31,7 @@
  *   mf_launch_prep_inline           the same with theta passed by value (no counterpart: latency of one call)
  *   mf_launch_generate              SubmitKernel of gen_kernel (compute_opencl.hh:540-575)
  *   mf_launch_seed_streams          host seeding of per-work-item generator states (compute_opencl.hh:522-539), on the device
+ *   mf_launch_expand_streams        no counterpart (the per-stream event ranges built on the device from per-slot counts)
  *   mf_sync                         Finish (clFinish)
  *   mf_graph_*                      no counterpart (per-iteration launch sequence as a hipGraph)
  *   mf_timer_*, mf_bench_copy       no counterpart (measurement)
@@ -297,6 +298,12 @@ int mf_launch_generate(mf_module* mod, const mf_gen_args* args, int grid, int bl
  * words each (the layout of random.hh xoshiro_jump_powers).  One thread per stream. */
 int mf_launch_seed_streams(mf_device* dev, const unsigned int* powers, int levels, const mf_gen_stream* streams, int nstreams,
                            int events_per_stream, const unsigned int* seg_base, unsigned int* states);
+/* the stream table of a generation on the device, from per-slot counts (all device pointers):
+ * slot l (of nslots) owns streams [first[l], first[l + 1]) (first[nslots] == nstreams), its
+ * j-th stream covering events [j * events_per_stream, min(n[l], (j + 1) * events_per_stream))
+ * of segment slot[l].  One thread per stream. */
+int mf_launch_expand_streams(mf_device* dev, const long long* first, const int* slot, const long long* n, int nslots,
+                             int events_per_stream, mf_gen_stream* streams, int nstreams);
 
 /* hipGraph capture of everything enqueued on the device stream between begin and end */
 int mf_graph_begin(mf_device* dev);

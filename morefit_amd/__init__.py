@@ -79,7 +79,7 @@ HIP_SYMBOLS = [
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_prep_inline", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
     "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped", "mf_launch_seed_streams", "mf_module_occupancy",
-    "mf_launch_eval_noalloc", "mf_launch_check",
+    "mf_launch_eval_noalloc", "mf_launch_check", "mf_launch_expand_streams",
 ]
 CAPI_SYMBOLS = [
     "mfh_last_error", "mfh_default_options", "mfh_open", "mfh_close", "mfh_device_name", "mfh_copy_ceiling",

@@ -728,44 +728,64 @@ __device__ __forceinline__ MF_T mf_uniform(unsigned int* s) { return (MF_T)(mf_x
 template <class L>
 __device__ __forceinline__ void mf_generate_impl(const mf_gen_args& g) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.nstreams) return;
-  const mf_gen_stream st = g.streams[k];
-  const mf_segment sg = g.segs[st.seg];
-  unsigned int s[4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) s[j] = g.states[4 * k + j];
-  const MF_T* __restrict__ P = (const MF_T*)g.params + sg.param_off;
-  const MF_T mp = (MF_T)g.maxprob[st.seg];
-  MF_T* __restrict__ out = (MF_T*)g.data + sg.data_off;
+  const bool active = k < g.nstreams;  // every lane stays to the end: the wave reduces the proposal counts
+  int seg = -1;
   unsigned long long props = 0;
-  // One proposal per iteration, accepted events stored in order: a lane goes on to its next
-  // event without waiting for the other lanes of its wave to accept their current one (with a
-  // per-event inner loop the wave ran, per event, the most tries of any of its 64 lanes).  The
-  // stream's draws and accepted events are the same as one event after another.
-  int i = st.begin;
-  unsigned int tries = 0;  // bounded: a density that is NaN or far above maxprob's scale must not hang the stream
-  while (i < st.end) {
-    MF_T x[MF_NOBS];
-    props++;
-    MF_GEN_PROPOSE(x, s);
-    MF_T lp[1];
-    L::eval(x, P, lp);
-    const MF_T p = mf_exp(lp[0]);
-    const MF_T u = mf_uniform(s);
-    if (mp * u < p) {
+  if (active) {
+    const mf_gen_stream st = g.streams[k];
+    seg = st.seg;
+    const mf_segment sg = g.segs[st.seg];
+    unsigned int s[4];
 #pragma unroll
-      for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + i] = x[d];
-      i++;
-      tries = 0;
-    } else if (++tries >= MF_GEN_MAX_TRIES) {
-      for (int j = i; j < st.end; j++)
-        for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + j] = (MF_T)(0.0 / 0.0);
-      break;
+    for (int j = 0; j < 4; j++) s[j] = g.states[4 * k + j];
+    const MF_T* __restrict__ P = (const MF_T*)g.params + sg.param_off;
+    const MF_T mp = (MF_T)g.maxprob[st.seg];
+    MF_T* __restrict__ out = (MF_T*)g.data + sg.data_off;
+    // One proposal per iteration, accepted events stored in order: a lane goes on to its next
+    // event without waiting for the other lanes of its wave to accept their current one (with a
+    // per-event inner loop the wave ran, per event, the most tries of any of its 64 lanes).  The
+    // stream's draws and accepted events are the same as one event after another.
+    int i = st.begin;
+    unsigned int tries = 0;  // bounded: a density that is NaN or far above maxprob's scale must not hang the stream
+    while (i < st.end) {
+      MF_T x[MF_NOBS];
+      props++;
+      MF_GEN_PROPOSE(x, s);
+      MF_T lp[1];
+      L::eval(x, P, lp);
+      const MF_T p = mf_exp(lp[0]);
+      const MF_T u = mf_uniform(s);
+      if (mp * u < p) {
+#pragma unroll
+        for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + i] = x[d];
+        i++;
+        tries = 0;
+      } else if (++tries >= MF_GEN_MAX_TRIES) {
+        for (int j = i; j < st.end; j++)
+          for (int d = 0; d < MF_NOBS; d++) out[d * sg.stride + j] = (MF_T)(0.0 / 0.0);
+        break;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) g.states[4 * k + j] = s[j];
+  }
+  // proposal counts: one atomic per wave when its streams belong to one segment (the common case),
+  // else one per stream; device-scope atomics on a few counters serialise (~10 ns each), and one per
+  // stream had cost milliseconds at 10^5-10^6 streams
+  if (g.proposals) {
+    const int lane = threadIdx.x & 63;
+    if (__ballot(active)) {  // lane 0 holds the wave's lowest stream, so it is active
+      const int s0 = __shfl(seg, 0);
+      if (__ballot(active && seg != s0) == 0) {
+        unsigned long long v = props;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) atomicAdd(g.proposals + s0, v);
+      } else if (active) {
+        atomicAdd(g.proposals + seg, props);
+      }
     }
   }
-#pragma unroll
-  for (int j = 0; j < 4; j++) g.states[4 * k + j] = s[j];
-  if (g.proposals) atomicAdd(g.proposals + st.seg, props);
 }
 
 #define MF_GENERATE_KERNEL(FUNCTOR) \

@@ -376,6 +376,25 @@ __global__ void __launch_bounds__(256) k_seed_streams(const unsigned int* __rest
   for (int w = 0; w < 4; w++) states[4 * (size_t)g + w] = s[w];
 }
 
+// generator stream table from per-slot counts: stream g belongs to the slot l whose range
+// [first[l], first[l + 1]) holds g (binary search over the nslots + 1 prefix sums); it is the
+// (g - first[l])-th stream of that slot, events [j * eps, min(n, (j + 1) * eps))
+__global__ void __launch_bounds__(256) k_expand_streams(const long long* __restrict__ first, const int* __restrict__ slot,
+                                                        const long long* __restrict__ n, int nslots, int eps,
+                                                        mf_gen_stream* __restrict__ out, int nstreams) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nstreams) return;
+  int lo = 0, hi = nslots;  // invariant: first[lo] <= g < first[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const long long j = g - first[lo];
+  const long long b = j * eps, e = b + eps < n[lo] ? b + eps : n[lo];
+  out[g] = mf_gen_stream{slot[lo], (int)b, (int)e, 0};
+}
+
 typedef float mf_v4f __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_copy(const mf_v4f* __restrict__ src, mf_v4f* __restrict__ dst, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -845,6 +864,20 @@ int mf_launch_seed_streams(mf_device* dev, const unsigned int* powers, int level
   hipLaunchKernelGGL(k_seed_streams, dim3((nstreams + 255) / 256), dim3(256), 0, dev->stream, powers, levels, streams, nstreams,
                      events_per_stream, seg_base, states);
   MF_HIP(hipGetLastError(), "mf_launch_seed_streams");
+  return MF_OK;
+}
+
+int mf_launch_expand_streams(mf_device* dev, const long long* first, const int* slot, const long long* n, int nslots,
+                             int events_per_stream, mf_gen_stream* streams, int nstreams) {
+  MF_INJECT("mf_launch_expand_streams");
+  if (!dev || nslots < 0 || nstreams < 0 || events_per_stream <= 0 ||
+      (nstreams > 0 && (nslots == 0 || !first || !slot || !n || !streams)))
+    return set_error(MF_ERR_INVALID, "mf_launch_expand_streams: bad argument");
+  if (nstreams == 0) return MF_OK;
+  MF_HIP(hipSetDevice(dev->ordinal), "mf_launch_expand_streams");
+  hipLaunchKernelGGL(k_expand_streams, dim3((nstreams + 255) / 256), dim3(256), 0, dev->stream, first, slot, n, nslots,
+                     events_per_stream, streams, nstreams);
+  MF_HIP(hipGetLastError(), "mf_launch_expand_streams");
   return MF_OK;
 }
 

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -1008,9 +1009,8 @@ class HipBatch {
   std::vector<mf_segment> segs_h;
   DevBuf params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d;
   // generator scratch, kept between calls (grows as needed)
-  DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_;
+  DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_, gen_slots_;
   int gen_levels_ = 0;
-  std::vector<mf_gen_stream> gen_streams_h_;
   // pinned staging of the tables copied to the device (a copy from pageable memory costs the
   // runtime ~0.1 ms of staging per call on MI355X): the chunk table, and the generator's tables
   // and proposal counts
@@ -1251,8 +1251,9 @@ class HipBatch {
     const int L = (int)slots.size();
     for (int i = 0; i < L; i++) std::memcpy(theta(slots[i]), &theta_gen[(size_t)i * m->ntheta], m->ntheta * sizeof(double));
     prep();
-    std::vector<mf_gen_stream>& st = gen_streams_h_;
-    st.clear();
+    // per listed slot: its streams' prefix sum (the stream table itself is built on the device)
+    std::vector<long long> first(1, 0), counts;
+    std::vector<int> slot_of;
     std::vector<unsigned int> base(4 * (size_t)S, 0u);
     std::vector<double> mp(S, 1.0);
     long long max_streams = 1;
@@ -1264,10 +1265,13 @@ class HipBatch {
       for (int w = 0; w < 4; w++) base[4 * (size_t)s + w] = r.getSeed(w);
       const long long ns = (sizes[s] + events_per_stream - 1) / events_per_stream;
       max_streams = std::max(max_streams, ns);
-      for (long long k = 0; k < ns; k++)
-        st.push_back({s, (int)(k * events_per_stream), (int)std::min<long long>(sizes[s], (k + 1) * events_per_stream), 0});
+      first.push_back(first.back() + ns);
+      counts.push_back(sizes[s]);
+      slot_of.push_back(s);
     }
-    if (st.empty()) return 0;
+    const long long nst = first.back();
+    if (nst == 0) return 0;
+    if (nst > (long long)std::numeric_limits<int>::max()) throw std::invalid_argument("HipBatch::generate: too many generator streams");
     int levels = 1;
     while ((1ll << levels) < max_streams) levels++;
     if (gen_levels_ < levels) {
@@ -1277,42 +1281,53 @@ class HipBatch {
       hip_check(mf_memcpy_h2d(dev, gen_powers_.ptr, pw.data(), pw.size() * sizeof(uint32_t)), "generate jump powers");
       gen_levels_ = lv;
     }
-    gen_streams_.alloc(dev, st.size() * sizeof(mf_gen_stream));
-    gen_states_.alloc(dev, st.size() * 4 * sizeof(unsigned int));
+    gen_streams_.alloc(dev, (size_t)nst * sizeof(mf_gen_stream));
+    gen_states_.alloc(dev, (size_t)nst * 4 * sizeof(unsigned int));
     gen_base_.alloc(dev, base.size() * sizeof(unsigned int));
     gen_maxprob_.alloc(dev, S * sizeof(double));
     gen_segs_.alloc(dev, S * sizeof(mf_segment));
     gen_props_.alloc(dev, S * sizeof(unsigned long long));
-    // the four tables in one pinned staging buffer (8-byte aligned parts), then the proposal counts
-    const size_t b_st = st.size() * sizeof(mf_gen_stream), b_base = (base.size() * sizeof(unsigned int) + 7) & ~(size_t)7,
-                 b_mp = S * sizeof(double), b_sg = ((S * sizeof(mf_segment)) + 7) & ~(size_t)7, b_pr = S * sizeof(unsigned long long);
-    gen_stage_.alloc(dev, b_st + b_base + b_mp + b_sg + b_pr);
+    // the tables in one pinned staging buffer (8-byte aligned parts), then the proposal counts:
+    // per-slot stream prefix sums, event counts and slot numbers, seeds, envelopes, segments
+    const size_t b_first = (L + 1) * sizeof(long long), b_cnt = L * sizeof(long long), b_slot = (L * sizeof(int) + 7) & ~(size_t)7,
+                 b_base = (base.size() * sizeof(unsigned int) + 7) & ~(size_t)7, b_mp = S * sizeof(double),
+                 b_sg = ((S * sizeof(mf_segment)) + 7) & ~(size_t)7, b_pr = S * sizeof(unsigned long long);
+    gen_slots_.alloc(dev, b_first + b_cnt + b_slot);
+    gen_stage_.alloc(dev, b_first + b_cnt + b_slot + b_base + b_mp + b_sg + b_pr);
     char* stage = (char*)gen_stage_.ptr;
-    std::memcpy(stage, st.data(), b_st);
-    std::memcpy(stage + b_st, base.data(), base.size() * sizeof(unsigned int));
-    std::memcpy(stage + b_st + b_base, mp.data(), b_mp);
-    std::memcpy(stage + b_st + b_base + b_mp, segs_h.data(), S * sizeof(mf_segment));
-    unsigned long long* props_h = (unsigned long long*)(stage + b_st + b_base + b_mp + b_sg);
-    hip_check(mf_memcpy_h2d(dev, gen_streams_.ptr, stage, b_st), "generate streams");
-    hip_check(mf_memcpy_h2d(dev, gen_base_.ptr, stage + b_st, base.size() * sizeof(unsigned int)), "generate seeds");
-    hip_check(mf_memcpy_h2d(dev, gen_maxprob_.ptr, stage + b_st + b_base, b_mp), "generate maxprob");
-    hip_check(mf_memcpy_h2d(dev, gen_segs_.ptr, stage + b_st + b_base + b_mp, S * sizeof(mf_segment)), "generate segs");
+    std::memcpy(stage, first.data(), b_first);
+    std::memcpy(stage + b_first, counts.data(), b_cnt);
+    std::memcpy(stage + b_first + b_cnt, slot_of.data(), L * sizeof(int));
+    char* rest = stage + b_first + b_cnt + b_slot;
+    std::memcpy(rest, base.data(), base.size() * sizeof(unsigned int));
+    std::memcpy(rest + b_base, mp.data(), b_mp);
+    std::memcpy(rest + b_base + b_mp, segs_h.data(), S * sizeof(mf_segment));
+    unsigned long long* props_h = (unsigned long long*)(rest + b_base + b_mp + b_sg);
+    char* slots_d = (char*)gen_slots_.ptr;
+    hip_check(mf_memcpy_h2d(dev, slots_d, stage, b_first + b_cnt + b_slot), "generate slot table");
+    hip_check(mf_memcpy_h2d(dev, gen_base_.ptr, rest, base.size() * sizeof(unsigned int)), "generate seeds");
+    hip_check(mf_memcpy_h2d(dev, gen_maxprob_.ptr, rest + b_base, b_mp), "generate maxprob");
+    hip_check(mf_memcpy_h2d(dev, gen_segs_.ptr, rest + b_base + b_mp, S * sizeof(mf_segment)), "generate segs");
     hip_check(mf_fill_zero(dev, gen_props_.ptr, S * sizeof(unsigned long long)), "generate proposals");
+    hip_check(mf_launch_expand_streams(dev, (const long long*)slots_d, (const int*)(slots_d + b_first + b_cnt),
+                                       (const long long*)(slots_d + b_first), L, events_per_stream,
+                                       gen_streams_.template as<mf_gen_stream>(), (int)nst),
+              "mf_launch_expand_streams");
     hip_check(mf_launch_seed_streams(dev, gen_powers_.template as<unsigned int>(), levels, gen_streams_.template as<mf_gen_stream>(),
-                                     (int)st.size(), events_per_stream, gen_base_.template as<unsigned int>(),
+                                     (int)nst, events_per_stream, gen_base_.template as<unsigned int>(),
                                      gen_states_.template as<unsigned int>()),
               "mf_launch_seed_streams");
     mf_gen_args g{};
     g.data = data_ptr();
     g.segs = gen_segs_.template as<mf_segment>();
     g.streams = gen_streams_.template as<mf_gen_stream>();
-    g.nstreams = (int)st.size();
+    g.nstreams = (int)nst;
     g.states = gen_states_.template as<unsigned int>();
     g.params = params_d.ptr;
     g.maxprob = gen_maxprob_.template as<double>();
     g.proposals = gen_props_.template as<unsigned long long>();
     const int block = 64;
-    hip_check(mf_launch_generate(m->mod, &g, (int)((st.size() + block - 1) / block), block), "mf_launch_generate");
+    hip_check(mf_launch_generate(m->mod, &g, (int)((nst + block - 1) / block), block), "mf_launch_generate");
     hip_check(mf_memcpy_d2h(dev, props_h, gen_props_.ptr, b_pr), "generate proposals");
     hip_check(mf_sync(dev), "generate");
     precompute(slots);
