@@ -492,6 +492,10 @@ def main():
     ap.add_argument("--no-angular", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
+    # a fixed launch shape "block,events_per_thread,unroll" for the N = 1e7 FP32 / FP64 workloads
+    # instead of the autotuner's pick (profiled runs: a profiler's per-dispatch cost skews tuning)
+    ap.add_argument("--shape32", default=None)
+    ap.add_argument("--shape64", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -511,8 +515,9 @@ def main():
     n = N_EVENTS
     start = M.MASS_START.copy()
     # launch shapes picked by the autotuner for each workload
-    ctx, tune32 = tuned_context(M, M.MASS, True, [n], "mass N=1e7 FP32")
-    ctx64, tune64 = tuned_context(M, M.MASS, False, [n], "mass N=1e7 FP64")
+    fixed = lambda v: [tuple(int(x) for x in v.split(","))] if v else None
+    ctx, tune32 = tuned_context(M, M.MASS, True, [n], "mass N=1e7 FP32", candidates=fixed(args.shape32))
+    ctx64, tune64 = tuned_context(M, M.MASS, False, [n], "mass N=1e7 FP64", candidates=fixed(args.shape64))
     name, cus = ctx.device_name()
     log("device", name, cus, "CUs")
 
@@ -542,9 +547,11 @@ def main():
     # dominant kernel, mf_eval_grad -- the per-chunk kernel alone when a final kernel reduces the
     # segments, the whole kernel when it reduces them itself -- and of a call's kernels together
     fused32 = m32.fused[M.EVAL_GRAD]
-    k32_pair_ms = b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps))
-    k32_ms = k32_pair_ms if fused32 else b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True)
-    k32_lh_ms = b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps), main_only=not fused32)
+    # each figure the median of 5 timing loops (a single loop now and then catches interference)
+    med = lambda f: float(np.median([f() for _ in range(5)]))
+    k32_pair_ms = med(lambda: b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps)))
+    k32_ms = k32_pair_ms if fused32 else med(lambda: b32.time_kernel(M.EVAL_GRAD, iters=max(50, args.steps), main_only=True))
+    k32_lh_ms = med(lambda: b32.time_kernel(M.EVAL_LH, iters=max(50, args.steps), main_only=not fused32))
     bytes32 = n * 4
     achieved = bytes32 / (k32_ms * 1e-3) / 1e9
     log(f"fp32 grad kernel {k32_ms * 1e3:.2f} us -> {achieved:.0f} GB/s (with the segment reduction {k32_pair_ms * 1e3:.2f} us); "
