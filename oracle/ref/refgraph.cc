@@ -2,9 +2,9 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into oracle/_ref/ from the
 // reference headers where they lie (nothing is copied).  pdf.hh cannot be compiled here
-// (it includes Eigen, which is absent), so this driver assembles the two configuration
-// densities with the reference's graph builders exactly as pdf.hh:147-154, 431-440,
-// 498-554 and physicspdfs.hh:39-77 assemble them, then uses the reference's own
+// (it includes Eigen, which is absent), so this driver assembles the densities with the
+// reference's graph builders exactly as pdf.hh:147-154, 431-440, 498-554, 207-260 (Crystal
+// Ball), 725-755 (polynomial) and physicspdfs.hh:39-77 assemble them, then uses the reference's own
 // substitute/simplify/diff/eval (graph.hh) the way fitter.hh:336, 405-407, 434-440 and
 // 1071-1072 use them.  Output: JSON golden vectors on stdout.
 #include <cassert>
@@ -83,6 +83,47 @@ static Ptr kstar_logprob() {
   return std::make_unique<SumNode<K, E>>(std::move(logprob), std::make_unique<NegNode<K, E>>(std::move(lognorm)));
 }
 
+// CrystalballPDF prob/norm (pdf.hh:207-260) on [a,b], then PDF::logprob_normalised (pdf.hh:103-106)
+static Ptr cb_logprob(double a, double b) {
+  Ptr prob = ConditionalLarger(V("m") - V("mu") + V("alpha") * V("sigma"),
+                               Exp<K, E>(-0.5 * (V("m") - V("mu")) * (V("m") - V("mu")) / (V("sigma") * V("sigma"))),
+                               Pow<K, E>(V("n") / V("alpha"), V("n")) * Exp<K, E>(-0.5 * V("alpha") * V("alpha")) /
+                                   Pow<K, E>(V("n") / V("alpha") - V("alpha") - (V("m") - V("mu")) / V("sigma"), V("n")));
+  Ptr A = Pow<K, E>(V("n") / V("alpha"), V("n")) * Exp<K, E>(-0.5 * V("alpha") * V("alpha"));
+  Ptr B = V("n") / V("alpha") - V("alpha");
+  Ptr Cc = V("sigma") * B->copy() + V("mu");
+  Ptr mid = V("mu") - V("alpha") * V("sigma");
+  Ptr gaussint = Sqrt<K, E>(M_PI / 2.0) * V("sigma") *
+                 (Erf<K, E>((V("mu") - mid->copy()) / (V("sigma") * sqrt(2.0))) - Erf<K, E>((V("mu") - C(b)) / (V("sigma") * sqrt(2.0))));
+  Ptr powint = ConditionalUnequal(V("n") - 1.0,
+                                  A->copy() * Pow<K, E>(V("sigma"), V("n")) *
+                                      (Pow<K, E>(Cc->copy() - mid->copy(), 1.0 - V("n")) - Pow<K, E>(Cc->copy() - C(a), 1.0 - V("n"))) /
+                                      (V("n") - 1.0),
+                                  A->copy() * V("sigma") * (Log<K, E>(Cc->copy() - C(a)) - Log<K, E>(Cc->copy() - mid->copy())));
+  Ptr norm = powint->copy() + gaussint->copy();
+  Ptr logprob = std::make_unique<LogNode<K, E>>(std::move(prob));
+  Ptr lognorm = std::make_unique<LogNode<K, E>>(std::move(norm));
+  return std::make_unique<SumNode<K, E>>(std::move(logprob), std::make_unique<NegNode<K, E>>(std::move(lognorm)));
+}
+// PolynomialPDF prob/norm (pdf.hh:725-755) with coefficients c1..c3 on [a,b] (Horner forms)
+static Ptr poly_logprob(double a, double b) {
+  const char* c[3] = {"c1", "c2", "c3"};
+  const int k = 3;
+  Ptr prob = V(c[k - 1]);
+  for (int i = 1; i < k; i++) prob = std::move(prob) * V("x") + V(c[k - i - 1]);
+  prob = std::move(prob) * V("x") + 1.0;
+  auto anti = [&](double x) {  // x (1 + c1 x / 2 + c2 x^2 / 3 + c3 x^3 / 4)
+    Ptr q = C(1.0 / (k + 1)) * V(c[k - 1]);
+    for (int i = 1; i < k; i++) q = std::move(q) * C(x) + C(1.0 / (k + 1 - i)) * V(c[k - i - 1]);
+    q = std::move(q) * C(x) + 1.0;
+    return std::move(q) * C(x);
+  };
+  Ptr norm = anti(b) - anti(a);
+  Ptr logprob = std::make_unique<LogNode<K, E>>(std::move(prob));
+  Ptr lognorm = std::make_unique<LogNode<K, E>>(std::move(norm));
+  return std::make_unique<SumNode<K, E>>(std::move(logprob), std::make_unique<NegNode<K, E>>(std::move(lognorm)));
+}
+
 static void emit(const char* model, Ptr lp, const std::vector<std::string>& obs, const std::vector<std::string>& pars,
                  const std::vector<std::vector<double>>& points, bool last) {
   // fitter.hh:336 (simplify), 405-407 (first derivatives), 434-440 (second derivatives, i>=j)
@@ -139,9 +180,26 @@ int main() {
     double m = a + (b - a) * u01();
     mass_pts.push_back({m, 5.3, 0.03, 0.5, 0.0});
   }
+  // Crystal Ball on the mass window (both sides of the join; n = 1 takes the tail integral's
+  // logarithmic branch) and the third-order polynomial on [-1, 1]; drawn after the points above
+  std::vector<std::vector<double>> cb_pts, poly_pts;
+  const double cbth[4][4] = {{5.28, 0.02, 1.5, 3.0}, {5.30, 0.05, 0.5, 1.2}, {5.25, 0.015, 3.0, 10.0}, {5.40, 0.08, 1.0, 1.0}};
+  for (int t = 0; t < 4; t++)
+    for (int k = 0; k < 8; k++) {
+      double m = a + (b - a) * u01();
+      cb_pts.push_back({m, cbth[t][0], cbth[t][1], cbth[t][2], cbth[t][3]});
+    }
+  const double pth[4][3] = {{0.3, -0.2, 0.1}, {0.0, 0.0, 0.0}, {-0.9, 0.5, 0.2}, {0.2, 0.9, -0.3}};
+  for (int t = 0; t < 4; t++)
+    for (int k = 0; k < 8; k++) {
+      double x = -1.0 + 2.0 * u01();
+      poly_pts.push_back({x, pth[t][0], pth[t][1], pth[t][2]});
+    }
   printf("{\n  \"generator\": \"oracle/ref/refgraph.cc over the reference graph.hh\",\n  \"mass_range\": [%.17g, %.17g],\n", a, b);
   emit("mass", mass_logprob(a, b), {"m"}, {"mu", "sigma", "f", "lambda"}, mass_pts, false);
-  emit("kstar", kstar_logprob(), {"ctl", "ctk", "phi"}, {"FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"}, kstar_pts, true);
+  emit("kstar", kstar_logprob(), {"ctl", "ctk", "phi"}, {"FL", "S3", "S4", "S5", "AFB", "S7", "S8", "S9"}, kstar_pts, false);
+  emit("cb", cb_logprob(a, b), {"m"}, {"mu", "sigma", "alpha", "n"}, cb_pts, false);
+  emit("poly", poly_logprob(-1.0, 1.0), {"x"}, {"c1", "c2", "c3"}, poly_pts, true);
   printf("}\n");
   return 0;
 }

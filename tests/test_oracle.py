@@ -33,7 +33,7 @@ def _relmax(a, b):
     return float(np.max(np.abs(a - b)) / scale)
 
 
-@pytest.mark.parametrize("name,model", [("mass", O.MASS), ("kstar", O.KSTAR)])
+@pytest.mark.parametrize("name,model", [("mass", O.MASS), ("kstar", O.KSTAR), ("cb", O.CB), ("poly", O.POLY)])
 def test_oracle_matches_reference_graph(name, model):
     """log p, gradient and Hessian of the oracle == the reference graph.hh's own symbolic
     derivatives evaluated by its interpreter (tests/golden/ref_graph.json)."""

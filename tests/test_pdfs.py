@@ -1,7 +1,8 @@
 """Crystal Ball and polynomial densities (reference pdf.hh:195-419 and 716-892) and the box
 integrals of all densities (definite_integral, used by the plotter's bin integrals).
 
-Oracle pins (independent of the oracle's own formulas):
+Oracle pins (independent of the oracle's own formulas), beside the golden vectors of the
+reference's own graph engine for both densities (tests/test_oracle.py, tests/golden/ref_graph.json):
   * Crystal Ball: scipy.stats.crystalball (beta = alpha, m = n, loc = mu, scale = sigma),
     truncated to the window, for the density, and quad for the normalisation; continuity of
     the density and of its slope at the join mu - alpha sigma (what fixes A and B);
