@@ -83,6 +83,31 @@ def measured_traffic(current_shape):
     return float(kv["traffic_bytes_per_launch"]), kv.get("launch_shape", "").strip()
 
 
+# VALU issue rate of MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz;
+# a quarter-rate transcendental occupies its SIMD for 4 issue slots
+VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def issue_floor(kernel_ms):
+    """the FP32 mf_eval_grad's instruction-issue floor at N = 10^7 from the committed counter
+    summary (VALU wave instructions per launch, transcendentals counted 4x) and the measured
+    kernel's fraction of it; None without the summary"""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    kv = dict(l.split(":", 1) for l in lines if l.startswith(("valu_wave_instructions_per_launch:",
+                                                             "valu_trans_wave_instructions_per_launch:")))
+    if len(kv) < 2:
+        return None
+    valu, trans = float(kv["valu_wave_instructions_per_launch"]), float(kv["valu_trans_wave_instructions_per_launch"])
+    floor_ms = (valu + 3.0 * trans) / VALU_WAVE_INSTR_PER_S * 1e3
+    return {"floor_ms": floor_ms, "kernel_ms": kernel_ms, "frac": floor_ms / kernel_ms,
+            "valu_wave_instructions": valu, "transcendental_wave_instructions": trans,
+            "basis": "counter summary profiles/mf_eval_grad_fp32_mass_1e7_traffic.txt (SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32); "
+                     "6.144e11 VALU wave instructions/s, transcendentals at quarter rate"}
+
+
 OTHERS_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp64_mass_angular_toys_counters.txt")
 
 
@@ -433,10 +458,11 @@ def angular_workload(M, O):
 
 def profile_workload():
     """Headline kernels only, for hardware-counter passes: FP32 mf_eval_grad and mf_eval_lh at
-    N = 10^7 (generated on the device) in the autotuned launch shape, then the copy kernel
-    over 40 MB (the counters' scale)."""
+    N = 10^7 (generated on the device) in the benched launch shape (block 512, 16 events per
+    thread, unroll 8), then the copy kernel over 40 MB (the counters' scale)."""
     import morefit_amd as M
-    ctx, tune = tuned_context(M, M.MASS, True, [N_EVENTS], "mass N=1e7 FP32")
+    # the launch shape the unprofiled bench picks (a profiler's per-dispatch cost skews tuning)
+    ctx, tune = tuned_context(M, M.MASS, True, [N_EVENTS], "mass N=1e7 FP32", candidates=[(512, 16, 8)])
     m32 = ctx.model(M.MASS, fp32=True)
     s = shape(m32, tune["best"])
     log("profile launch_shape: block=%d ept=%d acc=%d unroll=%d reduction=%s" % (
@@ -769,7 +795,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_launch_shape": traffic_shape,
                      "kernel": "mf_eval_grad (fp32, " + ("event loop and segment reduction)" if fused32 else "per-chunk kernel; mf_eval_grad_final reduces the segment)"),
-                     "kernel_ms": k32_ms, "algorithmic_bytes": bytes32, "kernels_of_a_call_ms": k32_pair_ms},
+                     "kernel_ms": k32_ms, "algorithmic_bytes": bytes32, "kernels_of_a_call_ms": k32_pair_ms,
+                     "issue_floor": issue_floor(k32_ms)},
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
         "cpu_baseline_scalar_oracle": cpu_port,
