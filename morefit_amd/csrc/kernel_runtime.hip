@@ -260,6 +260,24 @@ struct mf_acc {
 #endif
 };
 
+// ------------------------------------------------------------------ event loads
+// Raw buffer resource over `n` elements from `base` (gfx9 descriptor word 3: 32-bit data
+// format, no swizzle, no stride).  Loads through it take a per-lane byte offset (VGPR) and a
+// uniform byte offset (SGPR), so walking a chunk changes scalar registers only.  The range
+// check is a safety net: the evaluation kernels load in-range events only.
+typedef __amdgpu_buffer_rsrc_t mf_rsrc_t;
+__device__ __forceinline__ mf_rsrc_t mf_make_rsrc(const MF_T* base, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, n * (int)sizeof(MF_T), 0x00020000);
+}
+// non-temporal load (aux bit 1: the nt cache policy; every event is read once per launch)
+__device__ __forceinline__ MF_T mf_load_nt(mf_rsrc_t r, int voff, int soff) {
+#if MF_FP32
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+#else
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+#endif
+}
+
 // butterfly over the 64 lanes of a wavefront: every lane ends with the same total and the
 // order of additions is fixed, so the result is deterministic
 __device__ __forceinline__ double mf_wave_sum(double v) {
@@ -412,14 +430,23 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
     const MF_T* __restrict__ col = data + sg.data_off;
     mf_acc<NO> acc;
     acc.zero();
-    int i = ch.begin + (int)threadIdx.x;
+    // one buffer resource per column based at the chunk's first event: a load's address is the
+    // lane's byte offset (one VGPR for the whole loop) plus a uniform offset in an SGPR, so the
+    // event loop spends no vector instructions on addresses; the trip count of the unrolled
+    // part is uniform (whole groups of U * MF_BLOCK events), the rest goes event by event
+    const int len = ch.end - ch.begin;
+    mf_rsrc_t rs[MF_NCOL];
+#pragma unroll
+    for (int d = 0; d < MF_NCOL; d++) rs[d] = mf_make_rsrc(col + d * sg.stride + ch.begin, len);
+    const int lane_off = (int)threadIdx.x * (int)sizeof(MF_T);
+    int j = 0;
     // U events per thread per step: their loads are issued together, then computed
-    for (; i + (U - 1) * MF_BLOCK < ch.end; i += U * MF_BLOCK) {
+    for (; j + U * MF_BLOCK <= len; j += U * MF_BLOCK) {
       MF_T x[U][MF_NCOL];
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int d = 0; d < MF_NCOL; d++) x[u][d] = __builtin_nontemporal_load(col + d * sg.stride + i + u * MF_BLOCK);
+        for (int d = 0; d < MF_NCOL; d++) x[u][d] = mf_load_nt(rs[d], lane_off, (j + u * MF_BLOCK) * (int)sizeof(MF_T));
 #if MF_FP32 && MF_ACC == 2
       MF_T g[NO];
       F::eval(x[0], P, g);
@@ -440,10 +467,10 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
       }
 #endif
     }
-    for (; i < ch.end; i += MF_BLOCK) {
+    for (int i = j + (int)threadIdx.x; i < len; i += MF_BLOCK) {
       MF_T x[MF_NCOL];
 #pragma unroll
-      for (int d = 0; d < MF_NCOL; d++) x[d] = __builtin_nontemporal_load(col + d * sg.stride + i);
+      for (int d = 0; d < MF_NCOL; d++) x[d] = mf_load_nt(rs[d], i * (int)sizeof(MF_T), 0);
       MF_T o[NO];
       F::eval(x, P, o);
       acc.add(o);
