@@ -54,7 +54,7 @@ typedef struct mfh_options {
   int linear;              /* outputs as linear forms over shared monomials: 0 never, 1 when shorter, 2 always */
   int exp_pivot;           /* linear forms: a sum of exponential terms evaluated around one of them */
   int host_params;         /* one-segment calls: hoisted terms on the host, passed by value (no prep kernel) */
-  int pad;
+  int prefetch;            /* evaluation kernels load the next event batch while computing the current one */
 } mfh_options;
 
 typedef struct mfh_fit_out {

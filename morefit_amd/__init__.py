@@ -52,7 +52,7 @@ class Options(ctypes.Structure):
                 ("unroll", ctypes.c_int), ("balance", ctypes.c_int), ("base2", ctypes.c_int),
                 ("fused_final", ctypes.c_int), ("event_columns", ctypes.c_int), ("event_cost", ctypes.c_double),
                 ("linear", ctypes.c_int), ("exp_pivot", ctypes.c_int),
-                ("host_params", ctypes.c_int), ("pad", ctypes.c_int)]
+                ("host_params", ctypes.c_int), ("prefetch", ctypes.c_int)]
 
 
 class FitOut(ctypes.Structure):

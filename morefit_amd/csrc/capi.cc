@@ -124,6 +124,7 @@ void apply_options(compute_options& co, const mfh_options* o) {
   co.hip_linear_outputs = o->linear;
   co.hip_exp_pivot = o->exp_pivot != 0;
   co.hip_host_params = o->host_params != 0;
+  co.hip_prefetch = o->prefetch != 0;
 }
 }  // namespace
 
@@ -177,6 +178,7 @@ void mfh_default_options(mfh_options* o) {
   o->linear = 1;
   o->exp_pivot = 1;
   o->host_params = 1;
+  o->prefetch = 1;
 }
 
 int mfh_open(const mfh_options* opts, mfh_context** out) {

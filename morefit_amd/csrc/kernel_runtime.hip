@@ -35,6 +35,12 @@
 #define MF_GEN_MAX_TRIES (1u << 20)
 #define MF_NWAVE (MF_BLOCK / 64)
 #define MF_CONST __attribute__((address_space(4)))  // constant memory: read-only for the kernel's lifetime
+#ifndef MF_PREFETCH
+#define MF_PREFETCH 1       // evaluation kernels load the next event group before computing this one
+#endif
+#ifndef MF_PREFETCH_MAX
+#define MF_PREFETCH_MAX 32  // ... when a group holds at most this many values per thread
+#endif
 
 // ------------------------------------------------------------------ math
 #if MF_FP32
@@ -269,12 +275,16 @@ typedef __amdgpu_buffer_rsrc_t mf_rsrc_t;
 __device__ __forceinline__ mf_rsrc_t mf_make_rsrc(const MF_T* base, int n) {
   return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, n * (int)sizeof(MF_T), 0x00020000);
 }
-// non-temporal load (aux bit 1: the nt cache policy; every event is read once per launch)
+// event loads with the cache policy MF_LOAD_AUX (buffer instruction aux bits; default 2, the nt
+// policy: every event is read once per launch)
+#ifndef MF_LOAD_AUX
+#define MF_LOAD_AUX 2
+#endif
 __device__ __forceinline__ MF_T mf_load_nt(mf_rsrc_t r, int voff, int soff) {
 #if MF_FP32
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, MF_LOAD_AUX));
 #else
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, MF_LOAD_AUX));
 #endif
 }
 
@@ -411,7 +421,7 @@ __device__ __forceinline__ void mf_segment_totals(const mf_eval_args& a, int s, 
 // space pointers (this kernel never writes them): uniform loads then go through the scalar
 // cache into SGPRs, which the compiler cannot otherwise prove safe once the loop holds the
 // counter atomics and fences.
-template <class F, int U, bool FUSED>
+template <class F, int U, bool FUSED, bool PF>
 __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
   constexpr int NO = F::nacc;
   __shared__ double s_red[MF_NWAVE][NO];
@@ -439,14 +449,17 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
 #pragma unroll
     for (int d = 0; d < MF_NCOL; d++) rs[d] = mf_make_rsrc(col + d * sg.stride + ch.begin, len);
     const int lane_off = (int)threadIdx.x * (int)sizeof(MF_T);
-    int j = 0;
-    // U events per thread per step: their loads are issued together, then computed
-    for (; j + U * MF_BLOCK <= len; j += U * MF_BLOCK) {
-      MF_T x[U][MF_NCOL];
+    // U events per thread per step: their loads are issued together, then computed.  With
+    // MF_PREFETCH the loads of the next group are issued before the current group is computed
+    // (U * MF_NCOL more registers), so a wave waits on memory once per chunk, not per group
+    const int ngroups = len / (U * MF_BLOCK);
+    auto load_group = [&](MF_T (&dst)[U][MF_NCOL], int g) {
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int d = 0; d < MF_NCOL; d++) x[u][d] = mf_load_nt(rs[d], lane_off, (j + u * MF_BLOCK) * (int)sizeof(MF_T));
+        for (int d = 0; d < MF_NCOL; d++) dst[u][d] = mf_load_nt(rs[d], lane_off, (g * U + u) * MF_BLOCK * (int)sizeof(MF_T));
+    };
+    auto compute_group = [&](const MF_T (&x)[U][MF_NCOL]) {
 #if MF_FP32 && MF_ACC == 2
       MF_T g[NO];
       F::eval(x[0], P, g);
@@ -466,8 +479,38 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
         acc.add(o);
       }
 #endif
+    };
+    if constexpr (PF && MF_PREFETCH && U * MF_NCOL <= MF_PREFETCH_MAX) {
+      // two register sets in turn: group g + 1 loads into one while group g is computed from
+      // the other (no copies between them).  Every load sits on an unconditional path to the
+      // group computed next, so the compiler's wait counts stay exact (a load issued under a
+      // condition made it wait for all loads in flight, the prefetched ones included)
+      MF_T xa[U][MF_NCOL], xb[U][MF_NCOL];
+      if (ngroups > 0) {
+        load_group(xa, 0);
+        int gi = 0;
+        for (; gi + 2 < ngroups; gi += 2) {
+          load_group(xb, gi + 1);
+          compute_group(xa);
+          load_group(xa, gi + 2);
+          compute_group(xb);
+        }
+        if (gi + 1 < ngroups) {
+          load_group(xb, gi + 1);
+          compute_group(xa);
+          compute_group(xb);
+        } else {
+          compute_group(xa);
+        }
+      }
+    } else {
+      for (int gi = 0; gi < ngroups; gi++) {
+        MF_T x[U][MF_NCOL];
+        load_group(x, gi);
+        compute_group(x);
+      }
     }
-    for (int i = j + (int)threadIdx.x; i < len; i += MF_BLOCK) {
+    for (int i = ngroups * U * MF_BLOCK + (int)threadIdx.x; i < len; i += MF_BLOCK) {
       MF_T x[MF_NCOL];
 #pragma unroll
       for (int d = 0; d < MF_NCOL; d++) x[d] = mf_load_nt(rs[d], i * (int)sizeof(MF_T), 0);
@@ -681,8 +724,10 @@ __device__ __forceinline__ void mf_precompute_impl(const mf_eval_args& a) {
 
 // NAME##_nacc: the per-chunk partials the kernel writes (F::nacc doubles per chunk), read by the
 // host at module load to bound the launches that bring no partials buffer of their own
-#define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL, FUSED)                                                                    \
-  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_eval_impl<FUNCTOR, UNROLL, FUSED>(a); } \
+#define MF_EVAL_KERNEL(NAME, FUNCTOR, UNROLL, FUSED, PREFETCH)                                                \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) {                                  \
+    mf_eval_impl<FUNCTOR, UNROLL, FUSED, PREFETCH>(a);                                                           \
+  }                                                                                                               \
   extern "C" __global__ void __launch_bounds__(MF_FINAL_BLOCK) NAME##_final(mf_eval_args a) { mf_final_impl<FUNCTOR>(a); } \
   extern "C" __device__ __attribute__((used)) int NAME##_nacc = FUNCTOR::nacc;
 #define MF_STORE_KERNEL(NAME, FUNCTOR) \

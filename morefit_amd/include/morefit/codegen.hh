@@ -22,11 +22,13 @@
 namespace morefit {
 
 // one MF_EVAL_KERNEL instantiation: kernel name, functor, events per thread per load batch,
-// and whether the kernel reduces multi-chunk segments itself (else its _final kernel does)
+// whether the kernel reduces multi-chunk segments itself (else its _final kernel does), and
+// whether it loads the next batch while computing the current one (two register sets)
 struct EvalKernelSpec {
   std::string name, functor;
   int unroll = 1;
   bool fused = true;
+  bool prefetch = true;
 };
 
 struct ModuleSource {
@@ -92,7 +94,8 @@ struct ModuleSource {
     std::ostringstream s;
     s << "// generated by morefit codegen.hh\n" << prelude() << "//@MF_RUNTIME@\n" << functors << "\n" << hoist << "\n" << event_terms << "\n";
     for (const auto& e : evals)
-      s << "MF_EVAL_KERNEL(" << e.name << ", " << e.functor << ", " << e.unroll << ", " << (e.fused ? 1 : 0) << ")\n";
+      s << "MF_EVAL_KERNEL(" << e.name << ", " << e.functor << ", " << e.unroll << ", " << (e.fused ? 1 : 0) << ", "
+        << (e.prefetch ? 1 : 0) << ")\n";
     for (const auto& k : kernels) s << k << "\n";
     return s.str();
   }
@@ -154,17 +157,30 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
                          const std::vector<std::string>& observables, const std::vector<std::string>& param_names,
                          bool fp32, size_t* statements = nullptr, const CombineForm* form = nullptr,
                          const std::vector<double>* shift = nullptr, const std::string& vector_type = "") {
-  KernelEmitter em(fp32);
   const std::string vt = vector_type.empty() ? "MF_T" : vector_type;
-  if (!vector_type.empty()) {
-    em.T = vector_type;
-    em.wrap = "mf_bc";
+  // the product chains in the searched operand order and in plain frequency order: the one with
+  // the lower emitted cost is kept
+  auto run = [&](KernelEmitter& em, std::string& pro, std::vector<std::string>& res) {
+    if (!vector_type.empty()) {
+      em.T = vector_type;
+      em.wrap = "mf_bc";
+    }
+    pro = emit_shift_prologue(em, observables, shift, vector_type.empty() ? (fp32 ? "float" : "double") : vector_type);
+    for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
+    for (auto* g : acc) em.count_products(*g);
+    for (auto* g : acc) res.push_back(em.emit(*g));
+  };
+  KernelEmitter em(fp32), em2(fp32);
+  em2.search_order = false;
+  std::string pro, pro2;
+  std::vector<std::string> res, res2;
+  run(em, pro, res);
+  run(em2, pro2, res2);
+  if (em2.cost < em.cost) {
+    std::swap(em, em2);
+    std::swap(pro, pro2);
+    std::swap(res, res2);
   }
-  const std::string pro = emit_shift_prologue(em, observables, shift, vector_type.empty() ? (fp32 ? "float" : "double") : vector_type);
-  for (size_t k = 0; k < param_names.size(); k++) em.bindings[param_names[k]] = "p[" + std::to_string(k) + "]";
-  for (auto* g : acc) em.count_products(*g);
-  std::vector<std::string> res;
-  for (auto* g : acc) res.push_back(em.emit(*g));
   const size_t nout = form ? form->size() : acc.size();
   std::ostringstream s;
   s << "struct " << name << " {\n  static constexpr int nacc = " << acc.size() << ";\n  static constexpr int nout = " << nout << ";\n"

@@ -75,6 +75,9 @@ struct compute_options {
   // observables as offsets from the range centre: FP32 kernels store them so, FP64 kernels
   // compute them per event from the data as given
   bool hip_centre_observables = true;
+  // evaluation kernels issue the loads of their next event batch before computing the current
+  // one (two register sets; dropped per kernel where the registers do not fit)
+  bool hip_prefetch = true;
   int hip_generator_events_per_stream = 64;
   std::string hip_compile_options;   // extra hipRTC options
   // host-compiled CPU backend

@@ -43,8 +43,9 @@ inline void hip_check(int rc, const char* what) {
   if (rc != MF_OK) throw std::runtime_error(std::string(what) + ": " + mf_last_error());
 }
 
-// Halve the unroll of every evaluation kernel whose registers do not fit the 256
-// architectural VGPRs (it allocates AGPRs or scratch) until every kernel fits.  A kernel that
+// Fit every evaluation kernel into the 256 architectural VGPRs: a kernel that allocates AGPRs or
+// scratch first drops the prefetch of its next event batch (one register set instead of two),
+// then has its unroll halved until it fits.  A kernel that
 // needs AGPRs even at unroll 1 is an error: mf_module_load refuses such modules, because the
 // compiler was found to leave loop-carried AGPR copies unset in lanes outside a divergent
 // branch.  Each pass compiles through the code-object cache, so the load that follows finds
@@ -61,7 +62,10 @@ inline void fit_register_budget(ModuleSource& ms, const std::string& options, st
     for (auto& e : ms.evals)
       for (int k = 0; k < n; k++) {
         if (e.name != r[k].name || (r[k].agpr_count == 0 && r[k].private_segment_bytes == 0)) continue;
-        if (e.unroll > 1) {
+        if (e.prefetch) {
+          e.prefetch = false;  // one register set for the event batch
+          changed = true;
+        } else if (e.unroll > 1) {
           e.unroll /= 2;
           changed = true;
         } else if (e.fused) {
@@ -383,7 +387,7 @@ class HIPBlock : public ComputeBlock<kernelT, evalT, seedT> {
     ms.hoist = empty_hoist();
     s.kernel = name;
     if (kahan_summation) {
-      ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final});
+      ms.evals.push_back({name, "mf_f_block", std::max(1, opts().hip_unroll), opts().hip_fused_final, opts().hip_prefetch});
       ms.kernels.push_back("MF_CHECK_KERNEL(" + name + "_check, mf_f_block)");
     } else {
       ms.kernels.push_back("MF_STORE_KERNEL(" + name + ", mf_f_block)");
@@ -621,12 +625,16 @@ struct LikelihoodGraphs {
       return kf;
     };
     // outputs as linear forms over the kind's distinct monomials (false: expansion too large)
-    auto build_linear = [&](const std::vector<Ptr>& outs, KindForm& kf) {
+    // (canon: one shift per event variable, linear_canonical_shifts)
+    auto build_linear = [&](const std::vector<Ptr>& outs, KindForm& kf, bool canon) {
       std::map<std::string, int> index;
       std::vector<Ptr> monos;
+      std::vector<std::vector<LinearTerm<kernelT, evalT>>> all(outs.size());
+      for (size_t k = 0; k < outs.size(); k++)
+        if (!linear_expand(*outs[k], theta, all[k], 64)) return false;
+      if (canon && !linear_canonical_shifts(all, theta, 64)) return false;
       for (size_t k = 0; k < outs.size(); k++) {
-        std::vector<LinearTerm<kernelT, evalT>> terms;
-        if (!linear_expand(*outs[k], theta, terms, 64)) return false;
+        std::vector<LinearTerm<kernelT, evalT>>& terms = all[k];
         std::vector<std::string> keys;
         std::map<std::string, std::vector<Ptr>> coefs;
         for (auto& t : terms) {
@@ -667,20 +675,29 @@ struct LikelihoodGraphs {
       return true;
     };
     // issue cost of the statements the code generator emits for a kind's per-event terms
-    // (after CSE), in full-rate operations
+    // (after CSE, the cheaper of its two product orders, as emit_functor keeps), in full-rate
+    // operations
     auto statements = [&](const std::vector<Ptr>& acc) {
-      KernelEmitter em(std::is_same<kernelT, float>::value);
-      std::vector<std::string> cols(obs);
-      cols.insert(cols.end(), ev_names.begin(), ev_names.end());
-      for (size_t d = 0; d < cols.size(); d++) em.bindings[cols[d]] = "x[" + std::to_string(d) + "]";
-      std::vector<std::string> pn(theta);
-      pn.insert(pn.end(), hoist_names.begin(), hoist_names.end());
-      for (size_t k = 0; k < pn.size(); k++) em.bindings[pn[k]] = "p[" + std::to_string(k) + "]";
-      for (auto& g : acc) em.count_products(*g);
-      for (auto& g : acc) em.emit(*g);
-      return em.cost;
+      double best = INFINITY;
+      for (bool search : {true, false}) {
+        KernelEmitter em(std::is_same<kernelT, float>::value);
+        em.search_order = search;
+        std::vector<std::string> cols(obs);
+        cols.insert(cols.end(), ev_names.begin(), ev_names.end());
+        for (size_t d = 0; d < cols.size(); d++) em.bindings[cols[d]] = "x[" + std::to_string(d) + "]";
+        std::vector<std::string> pn(theta);
+        pn.insert(pn.end(), hoist_names.begin(), hoist_names.end());
+        for (size_t k = 0; k < pn.size(); k++) em.bindings[pn[k]] = "p[" + std::to_string(k) + "]";
+        for (auto& g : acc) em.count_products(*g);
+        for (auto& g : acc) em.emit(*g);
+        best = std::min(best, em.cost);
+      }
+      return best;
     };
-    auto make_kind = [&](std::vector<Ptr> outs, bool allow_linear) {
+    // canon: linear forms may take one shift per event variable (value and gradient kinds only:
+    // in second derivatives the shift differences enter squared, and the FP32 Hessian sums
+    // lost accuracy to the cancellation between the expanded terms)
+    auto make_kind = [&](std::vector<Ptr> outs, bool allow_linear, bool canon) {
       if (base2)
         for (auto& g : outs) g = to_base2(*g)->simplify();
       if (linear == 0 || !allow_linear) return build_scaled(outs);
@@ -692,35 +709,41 @@ struct LikelihoodGraphs {
         hoist_exprs.clear();
         for (auto& e : exprs0) hoist_exprs.emplace_back(e->copy());
       };
-      KindForm b;
-      if (!build_linear(outs, b)) {
+      // candidates: linear forms with one shift per event variable (c = 1) or as expanded
+      // (c = 0), and every output summed as it is; the one with the lowest emitted cost wins
+      // (ties: the earlier).  Accumulating a term costs about 1.25 operations per event (an FP32
+      // add, and an FP64 convert + add per unrolled group).
+      double cost[2] = {INFINITY, INFINITY};
+      for (int c = canon ? 1 : 0; c >= 0; c--) {
+        KindForm b;
+        if (build_linear(outs, b, c == 1)) cost[c] = statements(b.acc) + 1.25 * b.acc.size();
         restore();
-        return build_scaled(outs);
       }
-      // accumulating a term costs about 1.25 operations per event (an FP32 add, and an FP64
-      // convert + add per unrolled group)
-      const double cost_b = statements(b.acc) + 1.25 * b.acc.size();
-      if (linear == 2) return b;
-      restore();
+      const int cb = cost[1] <= cost[0] ? 1 : 0;
+      if (std::isfinite(cost[cb]) && linear == 2) {
+        KindForm b;
+        build_linear(outs, b, cb == 1);
+        return b;
+      }
       KindForm a = build_scaled(outs);
       const double cost_a = statements(a.acc) + 1.25 * a.acc.size();
-      if (cost_a <= cost_b) return a;
+      if (!std::isfinite(cost[cb]) || cost_a <= cost[cb]) return a;
       restore();
-      KindForm b2;
-      build_linear(outs, b2);
-      return b2;
+      KindForm b;
+      build_linear(outs, b, cb == 1);
+      return b;
     };
     gen = lp->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
     {
       std::vector<Ptr> o;
       o.emplace_back(lpe->copy());
-      lh = make_kind(std::move(o), true);
+      lh = make_kind(std::move(o), true, true);
     }
     {
       std::vector<Ptr> o;
       o.emplace_back(lpe->copy());
       for (auto& g : d1) o.emplace_back(g->copy());
-      grad = make_kind(std::move(o), true);
+      grad = make_kind(std::move(o), true, true);
     }
     if (with_hessian) {
       std::vector<Ptr> o;
@@ -731,7 +754,7 @@ struct LikelihoodGraphs {
       // (E''/p - (E'/p)^2 for a dominant component), and a linear form moves that cancellation
       // to segment sums of 10^6 terms, costing FP64 digits (1e-12 of the reference) where f or
       // lambda sit near their limits; the Hessian is evaluated once per fit
-      hess = make_kind(std::move(o), std::is_same<kernelT, float>::value);
+      hess = make_kind(std::move(o), std::is_same<kernelT, float>::value, false);
     }
     for (auto& g : hess.acc) nodes += g->node_count();
   }
@@ -781,8 +804,9 @@ struct LikelihoodGraphs {
     if (!ev_exprs.empty()) ms.event_terms = emit_fp64_function("mf_event_terms", "x", raw(ev_exprs), obs, sh);
     const int u = std::max(1, o.hip_unroll);
     const bool f = o.hip_fused_final;
-    ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f}, {"mf_eval_grad", "mf_f_grad", u, f}};
-    if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u, f});
+    const bool pf = o.hip_prefetch;
+    ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f, pf}, {"mf_eval_grad", "mf_f_grad", u, f, pf}};
+    if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u, f, pf});
     ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)", "MF_CHECK_KERNEL(mf_check_lh, mf_f_lh)"};
     if (!ev_exprs.empty()) ms.kernels.push_back("MF_PRECOMPUTE_KERNEL");
     return ms;

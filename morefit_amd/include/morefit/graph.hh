@@ -1065,6 +1065,105 @@ bool linear_expand(const ComputeGraphNode<kernelT, evalT>& n, const std::vector<
   }
 }
 
+// One shift per event variable.  Terms of a kind's outputs can hold the same event variable
+// under different shifts, x + c1 and x + c2 (the mass gradient: m - mu from the Gaussian and
+// m - <m> from the exponential's normalisation), each a factor of its own monomials.  With
+// the shift that occurs in the most terms as x + c*, every other one is written
+//   x + c = (x + c*) + (c - c*)
+// (the difference is parameter-only), so monomials such as (m - <m>) r become (m - mu) r and
+// r, which a kernel accumulates with one multiply-add instead of an add and a multiply-add,
+// and the centred difference that keeps FP32 sums accurate is kept.  false: the expansion
+// would exceed max_terms terms in one output.
+template <typename kernelT, typename evalT>
+bool linear_canonical_shifts(std::vector<std::vector<LinearTerm<kernelT, evalT>>>& outputs, const std::vector<std::string>& params,
+                             size_t max_terms) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  typedef ComputeGraphNode<kernelT, evalT> Node;
+  // the event variable of a shifted factor (x + c(params)), "" for any other factor
+  auto shifted_var = [&](const Node& f) -> std::string {
+    if (!linear_shift(f, params)) return "";
+    for (const auto& c : f.children_)
+      if (c->contains_other_variables(params)) return c->name();
+    return "";
+  };
+  // the parameter-only part c of x + c
+  auto shift_of = [&](const Node& f) {
+    std::vector<Ptr> c;
+    for (const auto& ch : f.children_)
+      if (!ch->contains_other_variables(params)) c.emplace_back(ch->copy());
+    return c.size() == 1 ? std::move(c[0]) : make_node<kernelT, evalT>(NodeKind::Sum, std::move(c));
+  };
+  std::map<std::string, std::map<std::string, int>> count;  // variable -> shift text -> terms holding it
+  std::map<std::string, const Node*> repr;
+  for (const auto& terms : outputs)
+    for (const auto& t : terms) {
+      std::set<std::string> seen;
+      for (const auto& f : t.factors) {
+        const std::string v = shifted_var(*f);
+        if (v.empty()) continue;
+        const std::string key = f->get_kernel();
+        if (seen.insert(key).second) count[v][key]++;
+        repr.emplace(key, f.get());
+      }
+    }
+  std::map<std::string, std::string> canon;  // variable -> text of its canonical shift
+  for (const auto& [v, shifts] : count) {
+    if (shifts.size() < 2) continue;
+    std::string best;
+    int most = -1;
+    for (const auto& [key, n] : shifts)
+      if (n > most) {
+        most = n;
+        best = key;
+      }
+    canon[v] = best;
+  }
+  if (canon.empty()) return true;
+  std::map<std::string, Ptr> canon_node;
+  for (const auto& [v, key] : canon) canon_node[v] = repr.at(key)->copy();
+  size_t nterms = 0;
+  std::function<bool(LinearTerm<kernelT, evalT>, std::vector<LinearTerm<kernelT, evalT>>&)> expand =
+      [&](LinearTerm<kernelT, evalT> t, std::vector<LinearTerm<kernelT, evalT>>& done) {
+        int at = -1;
+        std::string v;
+        for (size_t i = 0; i < t.factors.size() && at < 0; i++) {
+          v = shifted_var(*t.factors[i]);
+          if (!v.empty() && canon.count(v) && t.factors[i]->get_kernel() != canon[v]) at = (int)i;
+        }
+        if (at < 0) {
+          done.push_back(std::move(t));
+          return ++nterms <= max_terms;
+        }
+        // (x + c) F = (x + c*) F + (c - c*) F
+        std::vector<Ptr> d;
+        d.emplace_back(shift_of(*t.factors[at]));
+        std::vector<Ptr> neg;
+        neg.emplace_back(shift_of(*canon_node[v]));
+        d.emplace_back(make_node<kernelT, evalT>(NodeKind::Neg, std::move(neg)));
+        Ptr delta = make_node<kernelT, evalT>(NodeKind::Sum, std::move(d))->simplify();
+        LinearTerm<kernelT, evalT> a, b;
+        a.coef = t.coef ? t.coef->copy() : Ptr();
+        b.coef = linear_mul<kernelT, evalT>(t.coef ? t.coef->copy() : Ptr(), std::move(delta));
+        for (size_t i = 0; i < t.factors.size(); i++) {
+          if ((int)i == at) {
+            a.factors.push_back(canon_node[v]->copy());
+            continue;
+          }
+          a.factors.push_back(t.factors[i]->copy());
+          b.factors.push_back(t.factors[i]->copy());
+        }
+        return expand(std::move(a), done) && expand(std::move(b), done);
+      };
+  for (auto& terms : outputs) {
+    std::vector<LinearTerm<kernelT, evalT>> done;
+    nterms = 0;
+    for (auto& t : terms)
+      if (!expand(std::move(t), done)) return false;
+    terms = std::move(done);
+  }
+  return true;
+}
+
 // Canonical monomial of a term: factors sorted by their emitted text (key "" for the constant 1)
 template <typename kernelT, typename evalT>
 GraphPtr<kernelT, evalT> linear_monomial(std::vector<GraphPtr<kernelT, evalT>>& factors, std::string& key) {
@@ -1416,15 +1515,21 @@ struct KernelEmitter {
   int next = 0;
   int statements = 0;
   // issue cost of the emitted statements in full-rate VALU operations: a transcendental that the
-  // hardware evaluates (FP32 exp2, log2, rcp, sqrt, sin, cos) is quarter rate; FP64 ones and the
-  // rest are library sequences
+  // hardware evaluates (FP32 exp2, log2, rcp, sqrt, sin, cos) issues at half rate on gfx950 (8
+  // cycles per wave64 against 4, measured); FP64 ones and the rest are library sequences
   double cost = 0.0;
   std::string prefix = "mf_";  // math wrapper prefix ("mfd_" for the FP64 hoisted terms)
   std::string tmp = "t";       // SSA name prefix
   // how often each operand (by structural hash) occurs in the products of the graphs about
-  // to be emitted; products are then emitted as chains in falling frequency, so operands
-  // shared by several products form common prefixes that are computed once
+  // to be emitted; products are then emitted as chains with their operands in one global order
+  // (prod_rank), so operands shared by several products form common prefixes that are computed
+  // once.  The order is the one with the fewest distinct chain prefixes over all products: all
+  // orders are tried for up to 7 distinct operands, larger sets start from falling frequency and
+  // move single operands while that removes prefixes.
   std::unordered_map<uint64_t, int> prod_freq;
+  bool search_order = true;  // false: falling frequency, ties by emitted text
+  std::unordered_map<uint64_t, int> prod_rank;
+  std::vector<std::vector<uint64_t>> prod_lists;  // operand hashes of each distinct product
   std::unordered_map<const void*, uint64_t> hash_memo;
 
   explicit KernelEmitter(bool is_fp32) : T(is_fp32 ? "float" : "double"), S(T), fp32(is_fp32) {}
@@ -1438,10 +1543,11 @@ struct KernelEmitter {
     return buf;
   }
   double op_cost(const std::string& expr) const {
+    if (!expr.empty() && expr[0] == '-') return 0.0;  // a negation folds into its user (a source modifier)
     if (expr.compare(0, prefix.size(), prefix) != 0) return 1.0;
     const std::string f = expr.substr(prefix.size(), expr.find('(') - prefix.size());
     const bool hw = f == "exp2" || f == "log2" || f == "rcp" || f == "sqrt" || f == "sin" || f == "cos";
-    if (fp32) return hw ? 4.0 : (f == "exp" || f == "log") ? 5.0 : 20.0;
+    if (fp32) return hw ? 2.0 : (f == "exp" || f == "log") ? 5.0 : 20.0;
     return (f == "rcp" || f == "sqrt") ? 8.0 : 20.0;
   }
   std::string fresh(const std::string& key, const std::string& expr) {
@@ -1486,10 +1592,88 @@ struct KernelEmitter {
     if (n.kind() == NodeKind::Prod) {
       // products that contain a factor (a repeated factor counts once per product)
       std::unordered_set<uint64_t> seen;
-      for (const auto& c : n.children_)
+      std::vector<uint64_t> ops;
+      for (const auto& c : n.children_) {
         if (seen.insert(node_hash(*c)).second) prod_freq[node_hash(*c)]++;
+        if (!c->is_constant()) ops.push_back(node_hash(*c));
+      }
+      if (ops.size() > 1) prod_lists.push_back(std::move(ops));
     }
     for (const auto& c : n.children_) count_products(*c);
+    prod_rank.clear();
+  }
+  // multiplications the chains of all counted products take with operands in the order `rank`
+  size_t chain_cost(const std::unordered_map<uint64_t, int>& rank) const {
+    std::set<std::vector<uint64_t>> prefixes;
+    for (auto ops : prod_lists) {
+      std::stable_sort(ops.begin(), ops.end(), [&](uint64_t a, uint64_t b) { return rank.at(a) < rank.at(b); });
+      for (size_t k = 2; k <= ops.size(); k++) prefixes.insert(std::vector<uint64_t>(ops.begin(), ops.begin() + k));
+    }
+    return prefixes.size();
+  }
+  void order_products() {
+    std::vector<uint64_t> f;
+    for (const auto& [h, n] : prod_freq) f.push_back(h);
+    // falling frequency, then hash: the starting order and the tie-break
+    std::sort(f.begin(), f.end(), [&](uint64_t a, uint64_t b) { return prod_freq[a] != prod_freq[b] ? prod_freq[a] > prod_freq[b] : a < b; });
+    auto ranks = [&](const std::vector<uint64_t>& order) {
+      std::unordered_map<uint64_t, int> r;
+      for (size_t i = 0; i < order.size(); i++) r[order[i]] = (int)i;
+      return r;
+    };
+    prod_rank = ranks(f);
+    // only operands of products with more than two operands take part in chains
+    std::set<uint64_t> in_chains;
+    for (const auto& ops : prod_lists)
+      if (ops.size() > 2) in_chains.insert(ops.begin(), ops.end());
+    std::vector<uint64_t> sub;
+    for (uint64_t h : f)
+      if (in_chains.count(h)) sub.push_back(h);
+    if (sub.size() < 2) return;
+    auto with_rest = [&](const std::vector<uint64_t>& head) {
+      std::vector<uint64_t> o(head);
+      for (uint64_t h : f)
+        if (!in_chains.count(h)) o.push_back(h);
+      return o;
+    };
+    if (sub.size() > 7) {
+      // more operands: moves of one operand to another place while they remove prefixes
+      size_t best = chain_cost(prod_rank);
+      for (bool improved = true; improved;) {
+        improved = false;
+        for (size_t i = 0; i < sub.size(); i++)
+          for (size_t j = 0; j < sub.size(); j++) {
+            if (i == j) continue;
+            std::vector<uint64_t> o(sub);
+            const uint64_t h = o[i];
+            o.erase(o.begin() + i);
+            o.insert(o.begin() + j, h);
+            const size_t c = chain_cost(ranks(with_rest(o)));
+            if (c < best) {
+              best = c;
+              sub = o;
+              improved = true;
+            }
+          }
+      }
+      prod_rank = ranks(with_rest(sub));
+      return;
+    }
+    std::vector<int> perm(sub.size());
+    for (size_t i = 0; i < perm.size(); i++) perm[i] = (int)i;
+    size_t best = chain_cost(prod_rank);
+    std::vector<uint64_t> best_order = f;
+    while (std::next_permutation(perm.begin(), perm.end())) {
+      std::vector<uint64_t> h;
+      for (int i : perm) h.push_back(sub[i]);
+      const std::vector<uint64_t> o = with_rest(h);
+      const size_t c = chain_cost(ranks(o));
+      if (c < best) {
+        best = c;
+        best_order = o;
+      }
+    }
+    prod_rank = ranks(best_order);
   }
 
 
@@ -1516,14 +1700,20 @@ struct KernelEmitter {
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
     if (n.kind() == NodeKind::Prod && !prod_freq.empty() && ops.size() > 2) {
+      if (search_order && prod_rank.empty()) order_products();
       std::vector<size_t> idx(ops.size());
-      std::vector<int> freq(ops.size());
+      std::vector<int> rank(ops.size());
       for (size_t i = 0; i < ops.size(); i++) {
         idx[i] = i;
-        auto f = prod_freq.find(node_hash(*n.children_[i]));
-        freq[i] = f == prod_freq.end() ? 0 : f->second;
+        if (search_order) {
+          auto r = prod_rank.find(node_hash(*n.children_[i]));
+          rank[i] = r == prod_rank.end() ? (int)prod_rank.size() : r->second;
+        } else {
+          auto f = prod_freq.find(node_hash(*n.children_[i]));
+          rank[i] = f == prod_freq.end() ? 0 : -f->second;  // falling frequency
+        }
       }
-      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return freq[a] != freq[b] ? freq[a] > freq[b] : ops[a] < ops[b]; });
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return rank[a] != rank[b] ? rank[a] < rank[b] : ops[a] < ops[b]; });
       std::string acc = ops[idx[0]];
       for (size_t k = 1; k < idx.size(); k++) {
         // each step keyed like a two-factor product node (operands sorted), so chains and
