@@ -84,13 +84,16 @@ def measured_traffic(current_shape):
 
 
 # VALU issue rate of MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles at 2.4 GHz;
-# a quarter-rate transcendental occupies its SIMD for 4 issue slots
+# an FP32 transcendental (v_exp/v_log/v_rcp/v_sqrt/v_sin/v_cos_f32) occupies its SIMD for 8 cycles,
+# two issue slots (half rate on gfx950: morefit_amd/csrc/calib/issue_rate.hip,
+# profiles/valu_issue_rate_calibration.txt)
 VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+TRANS_EXTRA_SLOTS = 1.0
 
 
 def issue_floor(kernel_ms):
     """the FP32 mf_eval_grad's instruction-issue floor at N = 10^7 from the committed counter
-    summary (VALU wave instructions per launch, transcendentals counted 4x) and the measured
+    summary (VALU wave instructions per launch, transcendentals counted 2x) and the measured
     kernel's fraction of it; None without the summary"""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
@@ -101,11 +104,11 @@ def issue_floor(kernel_ms):
     if len(kv) < 2:
         return None
     valu, trans = float(kv["valu_wave_instructions_per_launch"]), float(kv["valu_trans_wave_instructions_per_launch"])
-    floor_ms = (valu + 3.0 * trans) / VALU_WAVE_INSTR_PER_S * 1e3
+    floor_ms = (valu + TRANS_EXTRA_SLOTS * trans) / VALU_WAVE_INSTR_PER_S * 1e3
     return {"floor_ms": floor_ms, "kernel_ms": kernel_ms, "frac": floor_ms / kernel_ms,
             "valu_wave_instructions": valu, "transcendental_wave_instructions": trans,
             "basis": "counter summary profiles/mf_eval_grad_fp32_mass_1e7_traffic.txt (SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32); "
-                     "6.144e11 VALU wave instructions/s, transcendentals at quarter rate"}
+                     "6.144e11 VALU wave instructions/s, transcendentals at half rate (calib/issue_rate.hip)"}
 
 
 OTHERS_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp64_mass_angular_toys_counters.txt")
@@ -458,11 +461,11 @@ def angular_workload(M, O):
 
 def profile_workload():
     """Headline kernels only, for hardware-counter passes: FP32 mf_eval_grad and mf_eval_lh at
-    N = 10^7 (generated on the device) in the benched launch shape (block 512, 16 events per
-    thread, unroll 8), then the copy kernel over 40 MB (the counters' scale)."""
+    N = 10^7 (generated on the device) in the benched launch shape (block 256, 32 events per
+    thread, unroll 4), then the copy kernel over 40 MB (the counters' scale)."""
     import morefit_amd as M
     # the launch shape the unprofiled bench picks (a profiler's per-dispatch cost skews tuning)
-    ctx, tune = tuned_context(M, M.MASS, True, [N_EVENTS], "mass N=1e7 FP32", candidates=[(512, 16, 8)])
+    ctx, tune = tuned_context(M, M.MASS, True, [N_EVENTS], "mass N=1e7 FP32", candidates=[(256, 32, 4)])
     m32 = ctx.model(M.MASS, fp32=True)
     s = shape(m32, tune["best"])
     log("profile launch_shape: block=%d ept=%d acc=%d unroll=%d reduction=%s" % (
@@ -481,10 +484,10 @@ def profile_others():
     attached: tuning under counter collection, which serialises the dispatches, picks
     others), 20 back-to-back launches of the per-chunk kernel, labelled on stderr by grid size"""
     import morefit_amd as M
-    jobs = [("mass FP64 N=1e7", M.MASS, False, [N_EVENTS], (256, 32, 8)),
-            ("angular FP32 N=1e6", M.KSTAR, True, [1_000_000], (128, 8, 8)),
-            ("angular FP64 N=1e6", M.KSTAR, False, [1_000_000], (256, 8, 8)),
-            ("4096 toys FP32", M.MASS, True, list(np.random.default_rng(229387429).poisson(5000, 4096)), (256, 32, 8))]
+    jobs = [("mass FP64 N=1e7", M.MASS, False, [N_EVENTS], (256, 32, 4)),
+            ("angular FP32 N=1e6", M.KSTAR, True, [1_000_000], (256, 16, 8)),
+            ("angular FP64 N=1e6", M.KSTAR, False, [1_000_000], (256, 16, 8)),
+            ("4096 toys FP32", M.MASS, True, list(np.random.default_rng(229387429).poisson(5000, 4096)), (256, 64, 8))]
     for label, kind, fp32, sizes, fixed in jobs:
         ctx, tune = tuned_context(M, kind, fp32, sizes, label, candidates=[fixed])
         m = ctx.model(kind, fp32=fp32)
@@ -648,18 +651,27 @@ def main():
     copy_gbs = ctx.copy_ceiling(1 << 31, iters=5)
     stream = None
     if not args.no_stream:
+        # the same kernels in steady state: 2 GB of events (beyond the 256 MB last-level cache, so
+        # every event comes from HBM), launch shape tuned for that size; the per-chunk kernel alone
+        # (the per-event kernel's rate) and with the final pass
         ns = 500_000_000
-        ms = ctx.model(M.MASS, fp32=True)
+        cts, tunes = tuned_context(M, M.MASS, True, [ns], "mass N=5e8 FP32 (streaming)")
+        ms = cts.model(M.MASS, fp32=True)
         bs = ms.batch([ns])
         bs.generate(M.MASS_TRUTH, [7])
-        ks = bs.time_kernel(M.EVAL_GRAD, iters=20)
-        kls = bs.time_kernel(M.EVAL_LH, iters=20)
-        stream = {"events": ns, "bytes": ns * 4, "grad_kernel_ms": ks, "grad_GBps": ns * 4 / (ks * 1e-3) / 1e9,
-                  "lh_kernel_ms": kls, "lh_GBps": ns * 4 / (kls * 1e-3) / 1e9}
+        ks = bs.time_kernel(M.EVAL_GRAD, iters=20, main_only=True)
+        ksf = bs.time_kernel(M.EVAL_GRAD, iters=20)
+        kls = bs.time_kernel(M.EVAL_LH, iters=20, main_only=True)
+        stream = {"events": ns, "bytes": ns * 4, "launch_shape": shape(ms, tunes["best"]), "grad_kernel_ms": ks,
+                  "grad_GBps": ns * 4 / (ks * 1e-3) / 1e9, "grad_with_final_ms": ksf, "lh_kernel_ms": kls,
+                  "lh_GBps": ns * 4 / (kls * 1e-3) / 1e9}
         stream["grad_frac"] = stream["grad_GBps"] / HBM_PEAK_GBS
-        log(f"streaming 2 GB: grad {stream['grad_GBps']:.0f} GB/s, lh {stream['lh_GBps']:.0f} GB/s, copy ceiling {copy_gbs:.0f} GB/s")
+        stream["lh_frac"] = stream["lh_GBps"] / HBM_PEAK_GBS
+        log(f"streaming 2 GB: grad {stream['grad_GBps']:.0f} GB/s ({stream['grad_frac']:.2f} of peak; with the final pass "
+            f"{ns * 4 / (ksf * 1e-3) / 1e9:.0f} GB/s), lh {stream['lh_GBps']:.0f} GB/s, copy ceiling {copy_gbs:.0f} GB/s")
         bs.close()
         ms.close()
+        cts.close()
 
     # ---------------------------------------------------------------- angular configuration
     angular = None if args.no_angular else angular_workload(M, O)
