@@ -1,5 +1,5 @@
 // chunk_flags.hip -- calibration of the whole call (launch + polled completion, host clock) for
-// three ways to reduce a segment of 1221 chunks (the headline workload's) after a per-chunk pass
+// four ways to reduce a segment of 1221 chunks (the headline workload's) after a per-chunk pass
 // of realistic weight (512 threads x 16 float events, exp2 + log2 + reciprocal + 6 sums per event):
 //  mode 0: per-chunk kernel + one-work-group final kernel writing host totals and the flag (the
 //          engine's scheme);
@@ -7,7 +7,12 @@
 //          (release fence + agent-scope store); work-group G-1, after its own chunk, polls all
 //          flags (bounded spin; a timeout is reported in the totals), then reduces and signals;
 //  mode 2: one kernel; every work-group counts on one device-scope counter between two fences;
-//          the one that counts last reduces and signals.
+//          the one that counts last reduces and signals;
+//  mode 3: one kernel; every work-group writes its partials through to memory (agent-scope stores),
+//          waits for their acknowledgement and counts with relaxed atomics on one of 16 counters on
+//          separate 256-byte lines, the last of each counting on a top counter (no release fence,
+//          whose L2 write-back serialises); the last overall takes an acquire fence, reduces and
+//          signals.
 // Also the per-chunk kernel alone, back to back (HIP events).  All three totals are printed.
 #include <hip/hip_runtime.h>
 
@@ -108,10 +113,39 @@ __global__ void __launch_bounds__(NT) k_chunk(const float* data, double* part, i
       for (int gg = 0; gg < NT / 32; gg++) s += s_all[k * NT + gg * 32 + j];
 #pragma unroll
       for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (j == 0) part[g * NO + k] = s;
+      if (j == 0) {
+        if (MODE == 3) __hip_atomic_store(part + g * NO + k, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else part[g * NO + k] = s;
+      }
     }
   }
   if (MODE == 0) return;
+  if (MODE == 3) {
+    __builtin_amdgcn_s_waitcnt(0);  // the partial stores acknowledged
+    __syncthreads();
+    if (t == 0) {
+      constexpr int C = 16, LINE = 64;
+      const int q = g % C;
+      const unsigned n1 = (unsigned)((G - q + C - 1) / C);
+      int last = 0;
+      unsigned prev = __hip_atomic_fetch_add(cnt + (1 + q) * LINE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == n1 - 1) {
+        __hip_atomic_store(cnt + (1 + q) * LINE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == C - 1) {
+          __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          last = 1;
+        }
+      }
+      s_last = last;
+    }
+    __syncthreads();
+    if (s_last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      reduce_all<NT>(part, out, flag, v, 0);
+    }
+    return;
+  }
   __syncthreads();
   if (MODE == 1) {
     if (t == 0) {
@@ -161,8 +195,8 @@ int main() {
   CK(hipMalloc(&part, G * NO * sizeof(double)));
   CK(hipMalloc(&cflags, G * sizeof(int)));
   CK(hipMemset(cflags, 0, G * sizeof(int)));
-  CK(hipMalloc(&cnt, 64));
-  CK(hipMemset(cnt, 0, 64));
+  CK(hipMalloc(&cnt, 17 * 64 * 4));
+  CK(hipMemset(cnt, 0, 17 * 64 * 4));
   CK(hipHostMalloc(&hout, 64 * sizeof(double), hipHostMallocMapped));
   CK(hipHostGetDevicePointer((void**)&hout_d, hout, 0));
   CK(hipHostMalloc(&flag, 64, hipHostMallocMapped));
@@ -180,9 +214,9 @@ int main() {
   const float a = -1.5f, b = 7.0f, c = 0.25f;
   int v = 0;
   const int it = 3000;
-  double tot[3][NO + 1];
+  double tot[4][NO + 1];
   for (int pass = 0; pass < 2; pass++) {
-    for (int mode = 0; mode < 3; mode++) {
+    for (int mode = 0; mode < 4; mode++) {
       clk::time_point t0;
       for (int i = -200; i < it; i++) {
         if (i == 0) t0 = clk::now();
@@ -192,8 +226,10 @@ int main() {
           k_final<<<1, 1024, 0, s>>>(part, hout_d, flag_d, v);
         } else if (mode == 1) {
           k_chunk<1><<<G, NT, 0, s>>>(data, part, cflags, cnt, hout_d, flag_d, v, a, b, c);
-        } else {
+        } else if (mode == 2) {
           k_chunk<2><<<G, NT, 0, s>>>(data, part, cflags, cnt, hout_d, flag_d, v, a, b, c);
+        } else {
+          k_chunk<3><<<G, NT, 0, s>>>(data, part, cflags, cnt, hout_d, flag_d, v, a, b, c);
         }
         if (!poll(v)) return std::printf("flag not set (mode %d)\n", mode), 1;
       }
@@ -213,7 +249,7 @@ int main() {
     CK(hipEventElapsedTime(&ms, e0, e1));
     std::printf("chunk kernel alone (back to back) %.2f us\n", ms * 1e3 / 500);
   }
-  for (int k = 0; k < NO; k++) std::printf("total %d: %.17g %.17g %.17g\n", k, tot[0][k], tot[1][k], tot[2][k]);
+  for (int k = 0; k < NO; k++) std::printf("total %d: %.17g %.17g %.17g %.17g\n", k, tot[0][k], tot[1][k], tot[2][k], tot[3][k]);
   CK(hipStreamDestroy(s));
   return 0;
 }

@@ -60,7 +60,7 @@ __device__ __forceinline__ float mf_log2(float x) { return log2f(x); }
 #if MF_FAST_MATH
 __device__ __forceinline__ float mf_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }  // v_sqrt_f32, 1 ulp
 // The hardware sine / cosine (v_sin_f32 / v_cos_f32 take the argument in turns and are valid
-// for |turns| <= 256): one multiply and one quarter-rate instruction instead of the library's
+// for |turns| <= 256): one multiply and one half-rate instruction instead of the library's
 // range reduction and polynomials.  Beyond 256 turns (1608 rad) the fraction of the turn is
 // passed, which loses nothing the FP32 argument still holds; below, the turns are passed as
 // they are, keeping the relative accuracy of small negative angles that a fraction would lose.
