@@ -74,6 +74,39 @@ def test_eval_matches_oracle(ctx, model, fp32):
     mdl.close()
 
 
+@pytest.mark.parametrize("model", [M.MASS, M.KSTAR])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_prefetch_off_matches_prefetch_on(model, fp32):
+    """the evaluation kernels without the prefetch of the next event batch (one register set,
+    prefetch=0) at a size with many chunks and a ragged tail: within the oracle bars, and the
+    value kernel bitwise the same as with the prefetch (the same events in the same order per
+    lane; only when the loads are issued changes)"""
+    n = 1_000_003
+    data = O.generate(model, M.TRUTH[model], n, seed=31, nstreams=64)
+    theta = np.array(M.START[model])
+    say("prefetch off", "model", model, "fp32", fp32, "n", n)
+    stored = _stored(model, data, fp32)
+    ref = O.nll_sums(model, stored, theta, mode=2, nthreads=8)
+    absref = O.nll_sums(model, stored, theta, mode=12, nthreads=8)
+    tol = 2e-6 if fp32 else 1e-12
+    outs = []
+    for pf in (0, 1):
+        cx = M.Context(prefetch=pf)
+        mdl = cx.model(model, fp32=fp32, with_hessian=False)
+        b = mdl.batch([n])
+        b.upload(0, data)
+        r = [b.eval(kind, theta)[0] for kind in (M.EVAL_LH, M.EVAL_GRAD)]
+        for out in r:
+            k = len(out)
+            assert np.all(np.abs(out - ref[:k]) <= tol * absref[:k]), (pf, out - ref[:k], absref[:k])
+        outs.append((r, mdl.unroll[M.EVAL_LH]))
+        b.close()
+        mdl.close()
+        cx.close()
+    if outs[0][1] == outs[1][1]:
+        assert np.array_equal(outs[0][0][0], outs[1][0][0])
+
+
 @pytest.mark.parametrize("fp32", [False, True])
 def test_mass_at_lambda_zero_matches_oracle(ctx, fp32):
     """lambda = 0 takes the ConditionalEqual branch of the exponential's normalisation
