@@ -133,6 +133,9 @@ typedef struct mf_eval_args {
                                  this argument block (kernarg segment) instead of params / hoist64 */
   unsigned int pinline[MF_INLINE_PARAM_WORDS];
   double hinline[MF_INLINE_HOIST];
+  const double* moments;      /* [nseg][MF_NMOM] data moments of each segment (sums of observable-only
+                                 terms, computed once per dataset by the module's mf_eval_mom) that the
+                                 outputs' combination reads, or NULL when the module has none */
 } mf_eval_args;
 
 /* mf_eval_args.flags: launch only the per-chunk kernel.  Correct when no active segment has

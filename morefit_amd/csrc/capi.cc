@@ -361,16 +361,19 @@ int mfh_model_host_outputs(int kind, int fp32, int which, const double* theta, c
     const std::vector<std::string> obs_names(g.obs);
     std::vector<double> xv(g.obs.size());
     const auto& kf = which == 1 ? g.lh : (which == 2 ? g.grad : g.hess);
-    const size_t na = kf.acc.size(), no = kf.form.size();
-    std::vector<double> av(na);
+    const size_t na = kf.acc.size(), no = kf.form.size(), nm = g.mom_exprs.size();
+    std::vector<double> av(na), mv(nm);
     for (long long i = 0; i < n; i++) {
       for (size_t dd = 0; dd < g.obs.size(); dd++) xv[dd] = vals[off + dd] = data[dd * n + i] - g.centres[dd];
       for (size_t e = 0; e < g.ev_exprs.size(); e++) vals[eoff + e] = g.ev_exprs[e]->eval(obs_names, xv);
       for (size_t j = 0; j < na; j++) av[j] = (double)kf.acc[j]->eval(names, vals);
-      // the kernels' FP64 combination of the accumulated terms, for one event (n = 1)
+      for (size_t j = 0; j < nm; j++) mv[j] = (double)g.mom_exprs[j]->eval(names, vals);
+      // the kernels' FP64 combination of the accumulated terms and data moments, for one event
+      // (n = 1)
       for (size_t k = 0; k < no; k++) {
         double v = 0.0;
-        for (const auto& t : kf.form[k]) v += t.mult * (t.hoist >= 0 ? hv[t.hoist] : 1.0) * (t.acc >= 0 ? av[t.acc] : 1.0);
+        for (const auto& t : kf.form[k])
+          v += t.mult * (t.hoist >= 0 ? hv[t.hoist] : 1.0) * (t.mom >= 0 ? mv[t.mom] : t.acc >= 0 ? av[t.acc] : 1.0);
         out[i * no + k] = v;
       }
     }

@@ -31,6 +31,11 @@
 #ifndef MF_NCOL
 #define MF_NCOL MF_NOBS
 #endif
+#ifndef MF_NMOM
+#define MF_NMOM 0  // data moments per segment (mf_eval_args.moments)
+#endif
+// read by the host at module load: launches of the evaluation kernels need the moments
+extern "C" __device__ __attribute__((used)) int mf_nmom = MF_NMOM;
 #define MF_NPARAM (MF_NTHETA + MF_NHOIST)
 #define MF_GEN_MAX_TRIES (1u << 20)
 #define MF_NWAVE (MF_BLOCK / 64)
@@ -319,7 +324,8 @@ __device__ __forceinline__ void mf_write_totals(const mf_eval_args& a, int s, co
   const double* h = a.inline_params ? (const double*)(mf_args_block() + __builtin_offsetof(mf_eval_args, hinline))
                     : a.hoist64     ? a.hoist64 + (long long)s * MF_NHOIST
                                     : nullptr;
-  F::combine(S, h, (double)sg.n, out);
+  const double* M = MF_NMOM > 0 && a.moments ? a.moments + (long long)s * (MF_NMOM > 0 ? MF_NMOM : 1) : nullptr;
+  F::combine(S, h, (double)sg.n, M, out);
 #pragma unroll
   for (int k = 0; k < F::nout; k++) a.results[sg.out_off + k] = out[k];
 }
@@ -646,9 +652,9 @@ __device__ __forceinline__ bool mf_finite(float v) { return (__float_as_uint(v) 
 __device__ __forceinline__ bool mf_finite(double v) {
   return ((unsigned long long)__double_as_longlong(v) & 0x7ff0000000000000ull) != 0x7ff0000000000000ull;
 }
-template <class F>
+template <class F, class FM>
 __device__ __forceinline__ void mf_check_impl(const mf_eval_args& a) {
-  constexpr int NO = F::nacc;
+  constexpr int NO = F::nacc, NM = FM::nacc;
   const MF_T* __restrict__ data = (const MF_T*)a.data;
   unsigned int* __restrict__ out = (unsigned int*)a.store;
   const MF_CONST MF_T* params = mf_params_of(a);
@@ -677,6 +683,13 @@ __device__ __forceinline__ void mf_check_impl(const mf_eval_args& a) {
         F::eval(x, P, o);
 #pragma unroll
         for (int k = 0; k < NO; k++) bad = bad || !mf_finite(o[k]);
+        // the observable-only terms summed through the data moments belong to the event too
+        if constexpr (NM > 0) {
+          MF_T om[NM > 0 ? NM : 1];
+          FM::eval(x, P, om);
+#pragma unroll
+          for (int k = 0; k < NM; k++) bad = bad || !mf_finite(om[k]);
+        }
       }
       const unsigned long long mb = __ballot(bad), mo = __ballot(outside);
       if ((threadIdx.x & 63) == 0) {
@@ -690,8 +703,15 @@ __device__ __forceinline__ void mf_check_impl(const mf_eval_args& a) {
     }
   }
 }
+// a module without data moments
+struct mf_no_moments {
+  static constexpr int nacc = 0;
+  __device__ __forceinline__ static void eval(const MF_T*, const MF_T*, MF_T*) {}
+};
 #define MF_CHECK_KERNEL(NAME, FUNCTOR) \
-  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_check_impl<FUNCTOR>(a); }
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_check_impl<FUNCTOR, mf_no_moments>(a); }
+#define MF_CHECK_KERNEL_MOM(NAME, FUNCTOR, MOMENTS) \
+  extern "C" __global__ void __launch_bounds__(MF_BLOCK) NAME(mf_eval_args a) { mf_check_impl<FUNCTOR, MOMENTS>(a); }
 
 // ------------------------------------------------------------------ event columns
 // Event-only terms of the density (generated mf_event_terms: FP64 from the stored observables)

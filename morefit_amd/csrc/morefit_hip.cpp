@@ -421,6 +421,7 @@ struct mf_module {
   hipModule_t mod = nullptr;
   std::map<std::string, hipFunction_t> fns;
   std::map<std::string, int> nacc;  // evaluation kernel -> FP64 partials per chunk (its <name>_nacc)
+  int nmom = 0;                      // data moments per segment its outputs read (mf_nmom)
   uint64_t hash = 0;
   int refs = 0;
   double compile_ms = 0.0;
@@ -683,6 +684,19 @@ int mf_module_load(mf_device* dev, const char* generated_src, const char* option
     }
     m->nacc[r.name] = v;
   }
+  {
+    hipDeviceptr_t dp = nullptr;
+    size_t bytes = 0;
+    if (hipModuleGetGlobal(&dp, &bytes, hm, "mf_nmom") == hipSuccess && bytes == sizeof(int)) {
+      if (hipMemcpyDtoH(&m->nmom, dp, sizeof(int)) != hipSuccess) {
+        (void)hipModuleUnload(hm);
+        delete m;
+        return set_error(MF_ERR_HIP, "mf_module_load: reading mf_nmom failed");
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+  }
   std::lock_guard<std::mutex> lk(dev->mu);
   dev->modules[h] = m;
   *out = m;
@@ -797,6 +811,11 @@ static int launch_eval(mf_module* mod, const char* kernel, const mf_eval_args* a
     return set_error(MF_ERR_INVALID, "mf_launch_eval: bad launch shape %d x %d", grid, block);
   if (args->nchunks < 0 || args->nseg < 0 || !args->segs || !args->chunks || !args->params || !args->data || !args->partials)
     return set_error(MF_ERR_INVALID, "mf_launch_eval: incomplete argument block");
+  if (mod->nmom > 0 && !args->moments && mod->nacc.count(kernel) && std::strcmp(kernel, "mf_eval_mom") != 0)
+    return set_error(MF_ERR_INVALID,
+                     "mf_launch_eval: the outputs of %s read the segments' data moments (mf_eval_args.moments, %d per segment); "
+                     "compute them with mf_eval_mom first",
+                     kernel, mod->nmom);
   std::string fin = std::string(kernel) + "_final";
   hipFunction_t ff = nullptr;
   const bool final = args->nseg > 0 && !args->seg_done && !(args->flags & MF_EVAL_SKIP_FINAL) && mod_has(mod, fin.c_str(), &ff);

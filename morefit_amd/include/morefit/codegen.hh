@@ -51,6 +51,7 @@ struct ModuleSource {
   std::vector<std::string> kernels;        // other instantiation lines
   size_t statements = 0;                   // SSA statements emitted (all functors)
   int nacc[4] = {0, 0, 0, 0};              // per evaluation kind: per-event terms accumulated
+  int nmom = 0;                            // data moments per segment (mf_eval_mom's outputs)
 
   std::string prelude() const {
     std::ostringstream s;
@@ -62,7 +63,8 @@ struct ModuleSource {
       << "#define MF_NHOIST " << nhoist << "\n"
       << "#define MF_BLOCK " << block << "\n"
       << "#define MF_ACC " << acc << "\n"
-      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n";
+      << "#define MF_FAST_MATH " << (fast_math ? 1 : 0) << "\n"
+      << "#define MF_NMOM " << nmom << "\n";
     s << "#define MF_GEN_PROPOSE(x, s) do {";
     for (int d = 0; d < nobs; d++) {
       double lo = d < (int)gen_lo.size() ? gen_lo[d] : 0.0, w = d < (int)gen_width.size() ? gen_width[d] : 1.0;
@@ -102,12 +104,14 @@ struct ModuleSource {
 };
 
 // One term of an output's combination from the segment sums of a kind's accumulated per-event
-// terms: mult * (hoist >= 0 ? h[hoist] : 1) * (acc >= 0 ? S[acc] : n), h the segment's FP64
-// hoisted terms, n its event count.
+// terms: mult * (hoist >= 0 ? h[hoist] : 1) * (mom >= 0 ? M[mom] : acc >= 0 ? S[acc] : n), h the
+// segment's FP64 hoisted terms, M its data moments (sums of observable-only terms, computed once
+// per dataset), n its event count.
 struct CombineTerm {
   int acc = -1;
   int hoist = -1;
   double mult = 1.0;
+  int mom = -1;
 };
 typedef std::vector<std::vector<CombineTerm>> CombineForm;  // per output
 
@@ -120,15 +124,19 @@ inline std::string emit_combine_terms(const std::vector<CombineTerm>& terms) {
     std::snprintf(buf, sizeof(buf), "%a", terms[i].mult);
     s << (i ? " + " : "") << buf;
     if (terms[i].hoist >= 0) s << " * h[" << terms[i].hoist << "]";
-    s << " * " << (terms[i].acc >= 0 ? "S[" + std::to_string(terms[i].acc) + "]" : std::string("n"));
+    s << " * "
+      << (terms[i].mom >= 0   ? "M[" + std::to_string(terms[i].mom) + "]"
+          : terms[i].acc >= 0 ? "S[" + std::to_string(terms[i].acc) + "]"
+                              : std::string("n"));
   }
   return s.str();
 }
 
-// struct NAME { nacc; nout; eval(x, p, o); combine(S, h, n, out) }: eval computes the nacc
+// struct NAME { nacc; nout; eval(x, p, o); combine(S, h, n, M, out) }: eval computes the nacc
 // per-event terms `acc` (their sums over a segment's events are accumulated) from the columns
 // x[d] and kernel parameters p[k]; combine forms the nout outputs from those sums S, the
-// segment's FP64 hoisted terms h and its event count n, in FP64 (form; null: out[k] = S[k]).
+// segment's FP64 hoisted terms h, its event count n and its data moments M, in FP64 (form;
+// null: out[k] = S[k]).
 // shift (optional, one entry per observable): the graphs' variable d is x[d] + shift[d],
 // computed once per event (offsets from the range centre for data stored as given).
 inline std::string emit_shift_prologue(KernelEmitter& em, const std::vector<std::string>& observables, const std::vector<double>* shift,
@@ -189,8 +197,9 @@ std::string emit_functor(const std::string& name, const std::vector<const Comput
     << "    (void)x; (void)p;\n"
     << pro << em.body.str();
   for (size_t k = 0; k < res.size(); k++) s << "    o[" << k << "] = " << (vector_type.empty() ? res[k] : "mf_bc(" + res[k] + ")") << ";\n";
-  s << "  }\n  __device__ __forceinline__ static void combine(const double* S, const double* h, double n, double* out) {\n"
-    << "    (void)S; (void)h; (void)n;\n";
+  s << "  }\n  __device__ __forceinline__ static void combine(const double* S, const double* h, double n, const double* M, "
+       "double* out) {\n"
+    << "    (void)S; (void)h; (void)n; (void)M;\n";
   for (size_t k = 0; k < nout; k++) {
     if (form) s << "    out[" << k << "] = " << emit_combine_terms((*form)[k]) << ";\n";
     else s << "    out[" << k << "] = S[" << k << "];\n";

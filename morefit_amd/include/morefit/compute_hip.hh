@@ -538,6 +538,10 @@ struct LikelihoodGraphs {
   // event-only terms kept as extra data columns (in terms of the stored observables)
   std::vector<std::string> ev_names;
   std::vector<Ptr> ev_exprs;
+  // data moments: observable-only terms whose sums over a segment are computed once per dataset
+  // (mf_eval_mom) and enter the outputs' combination (CombineTerm::mom); var^k of one observable
+  std::vector<std::string> mom_keys;
+  std::vector<Ptr> mom_exprs;
   size_t nodes = 0;
 
   // base2: rewrite exp/log into the hardware's exp2/log2 (FP32 kernels), see to_base2;
@@ -549,9 +553,11 @@ struct LikelihoodGraphs {
   // operations (linear_expand), 2 always linear forms where they exist; pivot: a density that
   // is a sum of exponential terms is evaluated around one of them in the linear forms (one
   // exponential fewer per event, exp_pivot)
+  // moments: with the pivot, the pivot exponent's share of log p is summed from data moments
+  // and the other exponentials' arguments are taken in Horner form (pivot_moments)
   LikelihoodGraphs(PDF<kernelT, evalT>* pdf, const std::vector<parameter<evalT>*>& params, bool centre, bool with_hessian,
                    bool base2 = false, double hoist_threshold = 0.0, double event_threshold = -1.0, int linear = 1,
-                   bool pivot = true, bool centre_in_kernel = false) {
+                   bool pivot = true, bool centre_in_kernel = false, bool moments = true) {
     std::vector<std::string> fixed;
     std::vector<double> fixed_v;
     std::map<std::string, Interval> box;  // observable ranges (as stored) and parameter limits
@@ -624,12 +630,134 @@ struct LikelihoodGraphs {
       }
       return kf;
     };
+    // index of the data moment var^k (registered on first use)
+    auto moment_index = [&](const std::string& var, int k) {
+      const std::string key = var + "^" + std::to_string(k);
+      auto it = std::find(mom_keys.begin(), mom_keys.end(), key);
+      if (it != mom_keys.end()) return (int)(it - mom_keys.begin());
+      Ptr v = make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, var);
+      if (k > 1) {
+        std::vector<Ptr> ch;
+        ch.emplace_back(std::move(v));
+        ch.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, (double)k));
+        v = make_node<kernelT, evalT>(NodeKind::Pow, std::move(ch));
+      }
+      mom_keys.push_back(key);
+      mom_exprs.emplace_back(std::move(v));
+      return (int)mom_keys.size() - 1;
+    };
+    // With the exponential pivot, log p = a_k + log q and the other terms enter as
+    // E(a_i - a_k).  When the pivot exponent a_k is a polynomial in one observable x (the
+    // exponential density: lambda x), (1) its share of every output, coefficient * sum_events a_k,
+    // is a combination of the segment's event count and moments sum x^j, so the kernels sum log q
+    // alone; (2) each a_i - a_k, then a polynomial in x, is written in Horner form in the shift
+    // t = x + s the monomials use most (the Gaussian's m - mu): for f Gauss + (1 - f) Exp the
+    // kernels no longer compute lambda x per event, and (m - mu)^2 / 2 sigma^2 - lambda m costs two
+    // multiply-adds.  Everything is rewritten or nothing (false).
+    std::function<bool(std::vector<Ptr>&, CombineForm&, std::vector<std::map<int, Ptr>>&, const std::string&, const ComputeGraphNode<kernelT, evalT>&)>
+        pivot_moments = [&](std::vector<Ptr>& monos, CombineForm& form, std::vector<std::map<int, Ptr>>& cexpr, const std::string& pv,
+                            const ComputeGraphNode<kernelT, evalT>& pe) -> bool {
+      std::string var;
+      int nvar = 0;
+      for (const auto& o : obs)
+        if (pe.variable_in_tree(o)) {
+          var = o;
+          nvar++;
+        }
+      for (const auto& e : ev_names)
+        if (pe.variable_in_tree(e)) nvar++;
+      std::vector<Ptr> pa;
+      if (nvar != 1 || !poly_in(pe, var, theta, pa, 4)) return false;
+      // the log p monomial: a_k (as pv) + log q
+      int i0 = -1;
+      for (size_t i = 0; i < monos.size(); i++) {
+        const auto& m = *monos[i];
+        if (m.kind() != NodeKind::Sum || m.children_.size() != 2) continue;
+        for (const auto& c : m.children_)
+          if (c->kind() == NodeKind::Variable && c->name() == pv) i0 = (int)i;
+      }
+      if (i0 < 0) return false;
+      // the shift of x used most in the monomials (Horner variable t = x + s), else x itself
+      std::map<std::string, int> count;
+      std::map<std::string, const ComputeGraphNode<kernelT, evalT>*> repr;
+      std::function<void(const ComputeGraphNode<kernelT, evalT>&)> collect = [&](const ComputeGraphNode<kernelT, evalT>& n) {
+        if (linear_shift(n, theta) && n.variable_in_tree(var)) {
+          count[n.get_kernel()]++;
+          repr.emplace(n.get_kernel(), &n);
+          return;
+        }
+        for (const auto& c : n.children_) collect(*c);
+      };
+      for (const auto& m : monos) collect(*m);
+      Ptr tnode = make_node<kernelT, evalT>(NodeKind::Variable, {}, 0.0, var), snode;
+      int most = 0;
+      for (const auto& [key, c] : count)
+        if (c > most) {
+          most = c;
+          tnode = repr[key]->copy();
+        }
+      if (most > 0) {
+        std::vector<Ptr> sp;
+        for (const auto& c : tnode->children_)
+          if (!c->contains_other_variables(theta)) sp.emplace_back(c->copy());
+        snode = sp.size() == 1 ? std::move(sp[0]) : make_node<kernelT, evalT>(NodeKind::Sum, std::move(sp));
+      }
+      // E(a_i - pv) -> E(Horner(a_i - a_k)) in every monomial
+      bool ok = true;
+      std::function<Ptr(const ComputeGraphNode<kernelT, evalT>&)> rewrite = [&](const ComputeGraphNode<kernelT, evalT>& n) -> Ptr {
+        if ((n.kind() == NodeKind::Exp || n.kind() == NodeKind::Exp2) && n.children_[0]->variable_in_tree(pv)) {
+          Ptr e = n.children_[0]->replace_variable(pv, pe)->simplify();
+          std::vector<Ptr> c;
+          if (!poly_in(*e, var, theta, c, 4)) {
+            ok = false;
+            return n.copy();
+          }
+          std::vector<Ptr> arg;
+          arg.emplace_back(horner_in_shift(c, *tnode, snode.get()));
+          return make_node<kernelT, evalT>(n.kind(), std::move(arg));
+        }
+        std::vector<Ptr> ch;
+        for (const auto& c : n.children_) ch.emplace_back(rewrite(*c));
+        return make_node<kernelT, evalT>(n.kind(), std::move(ch), n.value(), n.name(), n.comparator());
+      };
+      std::vector<Ptr> out;
+      for (size_t i = 0; i < monos.size(); i++) {
+        if ((int)i == i0) {
+          for (const auto& c : monos[i]->children_)
+            if (!(c->kind() == NodeKind::Variable && c->name() == pv)) out.emplace_back(rewrite(*c));
+        } else {
+          out.emplace_back(rewrite(*monos[i]));
+        }
+        if (!ok || out.back()->variable_in_tree(pv)) return false;
+      }
+      // each output's coefficient c of log p: c * a_k = c a_0 n + sum_j c a_j sum x^j
+      CombineForm f2(form);
+      for (size_t k = 0; k < f2.size(); k++) {
+        auto it = cexpr[k].find(i0);
+        if (it == cexpr[k].end()) continue;
+        for (size_t j = 0; j < pa.size(); j++) {
+          if (!pa[j]) continue;
+          std::vector<Ptr> pr;
+          pr.emplace_back(it->second->copy());
+          pr.emplace_back(pa[j]->copy());
+          CombineTerm t;
+          if (j > 0) t.mom = moment_index(var, (int)j);
+          coefficient(make_node<kernelT, evalT>(NodeKind::Prod, std::move(pr)), t);
+          if (t.hoist < 0 && t.mult == 0.0) continue;
+          f2[k].push_back(t);
+        }
+      }
+      monos = std::move(out);
+      form = std::move(f2);
+      return true;
+    };
     // outputs as linear forms over the kind's distinct monomials (false: expansion too large)
     // (canon: one shift per event variable, linear_canonical_shifts)
     auto build_linear = [&](const std::vector<Ptr>& outs, KindForm& kf, bool canon) {
       std::map<std::string, int> index;
       std::vector<Ptr> monos;
       std::vector<std::vector<LinearTerm<kernelT, evalT>>> all(outs.size());
+      std::vector<std::map<int, Ptr>> cexpr(outs.size());  // each output's coefficient of each monomial, unhoisted
       for (size_t k = 0; k < outs.size(); k++)
         if (!linear_expand(*outs[k], theta, all[k], 64)) return false;
       if (canon && !linear_canonical_shifts(all, theta, 64)) return false;
@@ -653,6 +781,7 @@ struct LikelihoodGraphs {
           Ptr c = cl.size() == 1 ? std::move(cl[0]) : make_node<kernelT, evalT>(NodeKind::Sum, std::move(cl));
           CombineTerm t;
           t.acc = key.empty() ? -1 : index[key];
+          if (t.acc >= 0 && moments) cexpr[k][t.acc] = c->copy();
           coefficient(std::move(c), t);
           if (t.hoist < 0 && t.mult == 0.0) continue;
           ft.push_back(t);
@@ -664,12 +793,13 @@ struct LikelihoodGraphs {
       // E(a_i - a_k) up to 2^40 in FP32 (a Hessian monomial's product of two stays below
       // 2^127), 2^400 in FP64
       if (pivot) kf.pivot = exp_pivot(monos, theta, box, pv, pe, std::is_same<kernelT, float>::value ? 40.0 : 400.0);
-      if (kf.pivot) pe = pe->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
+      const bool pv_gone = kf.pivot && moments && pivot_moments(monos, kf.form, cexpr, pv, *pe);
+      if (kf.pivot && !pv_gone) pe = pe->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
       for (auto& m : monos) {
         Ptr h = m->optimize_buffering_constant_terms(hoist_names, hoist_exprs, theta, prefix, thr)->simplify();
         // the pivot exponent substituted after simplification: a_i - a_k stays a difference
         // of the two exponents as computed, sharing a_k with log p = a_k + log q
-        kf.acc.emplace_back(kf.pivot ? h->replace_variable(pv, *pe) : std::move(h));
+        kf.acc.emplace_back(kf.pivot && !pv_gone ? h->replace_variable(pv, *pe) : std::move(h));
       }
       kf.linear = true;
       return true;
@@ -704,10 +834,13 @@ struct LikelihoodGraphs {
       std::vector<std::string> names0(hoist_names);
       std::vector<Ptr> exprs0;
       for (auto& e : hoist_exprs) exprs0.emplace_back(e->copy());
+      const size_t nmom0 = mom_exprs.size();
       auto restore = [&] {
         hoist_names = names0;
         hoist_exprs.clear();
         for (auto& e : exprs0) hoist_exprs.emplace_back(e->copy());
+        mom_keys.resize(nmom0);
+        mom_exprs.resize(nmom0);
       };
       // candidates: linear forms with one shift per event variable (c = 1) or as expanded
       // (c = 0), and every output summed as it is; the one with the lowest emitted cost wins
@@ -796,6 +929,8 @@ struct LikelihoodGraphs {
     ms.functors = emit_functor("mf_f_lh", raw(lh.acc), cols, pn, ms.fp32, &ms.statements, &lh.form, sh) +
                   emit_functor("mf_f_grad", raw(grad.acc), cols, pn, ms.fp32, &ms.statements, &grad.form, sh);
     if (with_hessian) ms.functors += emit_functor("mf_f_hess", raw(hess.acc), cols, pn, ms.fp32, &ms.statements, &hess.form, sh);
+    ms.nmom = (int)mom_exprs.size();
+    if (ms.nmom) ms.functors += emit_functor("mf_f_mom", raw(mom_exprs), cols, pn, ms.fp32, &ms.statements, nullptr, sh);
     ms.nacc[EVAL_LH] = (int)lh.acc.size();
     ms.nacc[EVAL_GRAD] = (int)grad.acc.size();
     ms.nacc[EVAL_HESS] = with_hessian ? (int)hess.acc.size() : 0;
@@ -807,7 +942,9 @@ struct LikelihoodGraphs {
     const bool pf = o.hip_prefetch;
     ms.evals = {{"mf_eval_lh", "mf_f_lh", u, f, pf}, {"mf_eval_grad", "mf_f_grad", u, f, pf}};
     if (with_hessian) ms.evals.push_back({"mf_eval_hess", "mf_f_hess", u, f, pf});
-    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)", "MF_CHECK_KERNEL(mf_check_lh, mf_f_lh)"};
+    if (ms.nmom) ms.evals.push_back({"mf_eval_mom", "mf_f_mom", u, false, pf});
+    ms.kernels = {"MF_PREP_KERNEL", "MF_GENERATE_KERNEL(mf_f_gen)",
+                  ms.nmom ? "MF_CHECK_KERNEL_MOM(mf_check_lh, mf_f_lh, mf_f_mom)" : "MF_CHECK_KERNEL(mf_check_lh, mf_f_lh)"};
     if (!ev_exprs.empty()) ms.kernels.push_back("MF_PRECOMPUTE_KERNEL");
     return ms;
   }
@@ -830,6 +967,7 @@ class HipModel {
   int ncol = 0;  // data columns per event: observables, then precomputed event terms
   int nout[4] = {0, 0, 0, 0};
   int nacc[4] = {0, 0, 0, 0};  // per-event terms each kind's kernel accumulates (partials per chunk)
+  int nmom = 0;                // data moments per segment (mf_eval_mom; 0: none)
   bool fp32 = std::is_same<kernelT, float>::value;
   bool with_hessian = true;
   std::string source;
@@ -908,6 +1046,7 @@ class HipModel {
     nout[EVAL_GRAD] = 1 + ntheta;
     nout[EVAL_HESS] = hessian ? 1 + ntheta + ntheta * (ntheta + 1) / 2 : 0;
     for (int k = 1; k <= 3; k++) nacc[k] = nout[k] ? ms.nacc[k] : 0;
+    nmom = ms.nmom;
     statements = ms.statements;
     graph_nodes = g->nodes;
     source = ms.device_source();
@@ -1031,7 +1170,7 @@ class HipBatch {
   bool segs_stale_[4] = {true, true, true, true};  // per kind: device segment table differs from the mapped one
   bool dirty = true;
   std::vector<mf_segment> segs_h;
-  DevBuf params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d;
+  DevBuf params_d, hoist64_d, segs_d[4], chunks_d, partials_d, seg_done_d, moments_d;
   // generator scratch, kept between calls (grows as needed)
   DevBuf gen_powers_, gen_streams_, gen_states_, gen_base_, gen_maxprob_, gen_segs_, gen_props_, gen_slots_;
   int gen_levels_ = 0;
@@ -1089,8 +1228,12 @@ class HipBatch {
     // of nch0 >= resident/2 chunks to the nearest multiple of resident, at most doubling it
     chunk_cap = (int)(2 * (elems / m->ncol / chunk)) + 2 * S + 1 + m->resident;
     chunks_d.alloc(dev, (size_t)chunk_cap * sizeof(mf_chunk));
-    const int nmax = std::max(1, std::max(m->nacc[EVAL_LH], std::max(m->nacc[EVAL_GRAD], m->nacc[EVAL_HESS])));
+    const int nmax = std::max(std::max(1, m->nmom), std::max(m->nacc[EVAL_LH], std::max(m->nacc[EVAL_GRAD], m->nacc[EVAL_HESS])));
     partials_d.alloc(dev, (size_t)chunk_cap * nmax * sizeof(double));
+    if (m->nmom) {
+      moments_d.alloc(dev, (size_t)std::max(1, S) * m->nmom * sizeof(double));
+      hip_check(mf_fill_zero(dev, moments_d.ptr, moments_d.bytes), "HipBatch moments");
+    }
     seg_done_d.alloc(dev, (size_t)std::max(1, S) * sizeof(unsigned int));
     hip_check(mf_fill_zero(dev, seg_done_d.ptr, seg_done_d.bytes), "HipBatch counters");
     for (int k = 1; k <= 3; k++) {
@@ -1102,7 +1245,7 @@ class HipBatch {
     }
     std::memset(theta_h.ptr, 0, theta_h.bytes);
     done_h.alloc_mapped(dev, 64);
-    if (m->ncol > m->nobs) segs_pre_h.alloc_mapped(dev, std::max(1, S) * sizeof(mf_segment));
+    if (m->ncol > m->nobs || m->nmom) segs_pre_h.alloc_mapped(dev, std::max(1, S) * sizeof(mf_segment));
     std::memset(done_h.ptr, 0, done_h.bytes);
     request_.assign(S, 0);
     for (int k = 1; k <= 3; k++) fused_[k] = m->nout[k] > 0 && m->kernel_fused(kernel_name(k));
@@ -1209,16 +1352,19 @@ class HipBatch {
     precompute({s});
     hip_check(mf_sync(dev), "HipBatch upload");
   }
-  // event columns (precomputed event-only terms) of the listed slots from their observables;
-  // the segment table of this launch is read from mapped host memory
+  // after new data in the listed slots: their event columns (precomputed event-only terms) from
+  // their observables, and their data moments (mf_eval_mom: per-chunk sums and the final pass,
+  // totals into moments_d[slot]); the segment table of these launches is read from mapped host
+  // memory
   void precompute(const std::vector<int>& slots) {
-    if (m->ncol == m->nobs || slots.empty()) return;
+    if ((m->ncol == m->nobs && m->nmom == 0) || slots.empty()) return;
     rebuild();
     if (nchunks == 0) return;
     mf_segment* t = segs_pre_h.template as<mf_segment>();
     for (int s = 0; s < S; s++) {
       t[s] = segs_h[s];
       t[s].mode = 0;
+      t[s].out_off = s * m->nmom;
     }
     for (int s : slots) t[s].mode = active[s] ? 1 : 0;
     mf_eval_args a{};
@@ -1229,8 +1375,15 @@ class HipBatch {
     a.nseg = S;
     a.params = params_d.ptr;
     a.partials = partials_d.template as<double>();
-    a.flags = MF_EVAL_SKIP_FINAL;
-    hip_check(mf_launch_eval(m->mod, "mf_precompute", &a, grid(), m->block), "mf_precompute");
+    if (m->ncol > m->nobs) {
+      a.flags = MF_EVAL_SKIP_FINAL;
+      hip_check(mf_launch_eval(m->mod, "mf_precompute", &a, grid(), m->block), "mf_precompute");
+    }
+    if (m->nmom) {
+      a.flags = 0;
+      a.results = moments_d.template as<double>();
+      hip_check(mf_launch_eval(m->mod, "mf_eval_mom", &a, grid(), m->block), "mf_eval_mom");
+    }
     hip_check(mf_sync(dev), "HipBatch precompute");
   }
   // stored columns of segment s back as FP64 observables (centre added back)
@@ -1381,6 +1534,7 @@ class HipBatch {
     a.partials = partials_d.template as<double>();
     a.results = (double*)results_h[kind].dptr;
     a.hoist64 = hoist64_d.template as<double>();
+    a.moments = m->nmom ? moments_d.template as<double>() : nullptr;
     if (max_seg_chunks <= 1) a.flags |= MF_EVAL_SKIP_FINAL;
     else if (fused_[kind]) a.seg_done = seg_done_d.template as<unsigned int>();
     return a;

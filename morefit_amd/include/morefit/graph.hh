@@ -1498,6 +1498,137 @@ bool exp_pivot(std::vector<GraphPtr<kernelT, evalT>>& monos, const std::vector<s
   return true;
 }
 
+// ==================================================================== polynomials in one event variable
+// Coefficients a_0..a_d (parameter-only graphs; null: 0) of n as a polynomial in the variable
+// `var`, when n is one: sums, negations, products and non-negative integer powers of `var` and of
+// parameter-only subtrees.  false when n depends on any other event variable, on `var` in another
+// form, or exceeds degree max_deg.
+template <typename kernelT, typename evalT>
+bool poly_in(const ComputeGraphNode<kernelT, evalT>& n, const std::string& var, const std::vector<std::string>& params,
+             std::vector<GraphPtr<kernelT, evalT>>& a, int max_deg = 4) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  auto add = [](Ptr x, Ptr y) -> Ptr {
+    if (!x) return y;
+    if (!y) return x;
+    std::vector<Ptr> v;
+    v.emplace_back(std::move(x));
+    v.emplace_back(std::move(y));
+    return make_node<kernelT, evalT>(NodeKind::Sum, std::move(v));
+  };
+  a.clear();
+  std::vector<std::string> pv(params);
+  if (!n.contains_other_variables(pv)) {
+    a.emplace_back(n.copy());
+    return true;
+  }
+  switch (n.kind()) {
+    case NodeKind::Variable:
+      if (n.name() != var) return false;
+      a.emplace_back(nullptr);
+      a.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 1.0));
+      return true;
+    case NodeKind::Sum: {
+      for (const auto& c : n.children_) {
+        std::vector<Ptr> b;
+        if (!poly_in(*c, var, params, b, max_deg)) return false;
+        if (b.size() > a.size()) a.resize(b.size());
+        for (size_t j = 0; j < b.size(); j++)
+          if (b[j]) a[j] = add(std::move(a[j]), std::move(b[j]));
+      }
+      return true;
+    }
+    case NodeKind::Neg: {
+      if (!poly_in(*n.children_[0], var, params, a, max_deg)) return false;
+      for (auto& c : a)
+        if (c) {
+          std::vector<Ptr> v;
+          v.emplace_back(std::move(c));
+          c = make_node<kernelT, evalT>(NodeKind::Neg, std::move(v));
+        }
+      return true;
+    }
+    case NodeKind::Pow:
+    case NodeKind::Prod: {
+      std::vector<const ComputeGraphNode<kernelT, evalT>*> f;
+      int reps = 1;
+      if (n.kind() == NodeKind::Pow) {
+        const auto& e = *n.children_[1];
+        if (!e.is_constant() || e.value() != std::floor(e.value()) || e.value() < 0 || e.value() > max_deg) return false;
+        reps = (int)e.value();
+        f.push_back(n.children_[0].get());
+      } else {
+        for (const auto& c : n.children_) f.push_back(c.get());
+      }
+      a.clear();
+      a.emplace_back(make_node<kernelT, evalT>(NodeKind::Constant, {}, 1.0));
+      for (int r = 0; r < reps; r++)
+        for (auto* c : f) {
+          std::vector<Ptr> b;
+          if (!poly_in(*c, var, params, b, max_deg)) return false;
+          if (a.size() + b.size() - 1 > (size_t)max_deg + 1) return false;
+          std::vector<Ptr> p(a.size() + b.size() - 1);
+          for (size_t i = 0; i < a.size(); i++)
+            for (size_t j = 0; j < b.size(); j++) {
+              if (!a[i] || !b[j]) continue;
+              std::vector<Ptr> v;
+              v.emplace_back(a[i]->copy());
+              v.emplace_back(b[j]->copy());
+              p[i + j] = add(std::move(p[i + j]), make_node<kernelT, evalT>(NodeKind::Prod, std::move(v)));
+            }
+          a = std::move(p);
+        }
+      return true;
+    }
+    default: return false;
+  }
+}
+
+// Horner form of sum_j a_j x^j in t = x + s: the coefficients b_j of t^j (b_j = sum_{i >= j}
+// C(i, j) a_i (-s)^(i - j), parameter-only, simplified) applied as ((b_d t + b_{d-1}) t + ...) t + b_0.
+template <typename kernelT, typename evalT>
+GraphPtr<kernelT, evalT> horner_in_shift(const std::vector<GraphPtr<kernelT, evalT>>& a, const ComputeGraphNode<kernelT, evalT>& t,
+                                         const ComputeGraphNode<kernelT, evalT>* s) {
+  typedef GraphPtr<kernelT, evalT> Ptr;
+  const int d = (int)a.size() - 1;
+  auto C = [](double v) { return make_node<kernelT, evalT>(NodeKind::Constant, {}, v); };
+  std::vector<Ptr> b(d + 1);
+  for (int j = 0; j <= d; j++) {
+    std::vector<Ptr> terms;
+    for (int i = j; i <= d; i++) {
+      if (!a[i]) continue;
+      double binom = 1.0;
+      for (int k = 0; k < i - j; k++) binom *= (double)(i - k) / (double)(k + 1);
+      std::vector<Ptr> f;
+      f.emplace_back(C((i - j) % 2 ? -binom : binom));
+      f.emplace_back(a[i]->copy());
+      for (int k = 0; k < i - j && s; k++) f.emplace_back(s->copy());
+      if (i > j && !s) continue;  // no shift: x = t
+      terms.emplace_back(make_node<kernelT, evalT>(NodeKind::Prod, std::move(f)));
+    }
+    b[j] = terms.empty() ? Ptr() : make_node<kernelT, evalT>(NodeKind::Sum, std::move(terms))->simplify();
+  }
+  Ptr acc;
+  for (int j = d; j >= 0; j--) {
+    if (acc) {
+      std::vector<Ptr> v;
+      v.emplace_back(std::move(acc));
+      v.emplace_back(t.copy());
+      acc = make_node<kernelT, evalT>(NodeKind::Prod, std::move(v));
+    }
+    if (b[j] && !(b[j]->is_constant() && b[j]->value() == 0.0)) {
+      if (!acc) {
+        acc = std::move(b[j]);
+      } else {
+        std::vector<Ptr> v;
+        v.emplace_back(std::move(acc));
+        v.emplace_back(std::move(b[j]));
+        acc = make_node<kernelT, evalT>(NodeKind::Sum, std::move(v));
+      }
+    }
+  }
+  return acc ? std::move(acc) : C(0.0);
+}
+
 // ==================================================================== code emission with CSE
 // Emits the outputs of several graphs as one straight-line block of SSA statements
 //   const T t17 = ...;

@@ -38,7 +38,8 @@ class EvalArgs(ctypes.Structure):
     _fields_ = [("data", vp), ("segs", vp), ("chunks", vp), ("nchunks", ip), ("nseg", ip), ("params", vp),
                 ("partials", vp), ("results", vp), ("hoist64", vp), ("store", vp), ("store_stride", ctypes.c_longlong),
                 ("flags", ip), ("pad", ip), ("seg_done", vp), ("done_flag", vp), ("done_value", ip),
-                ("inline_params", ip), ("pinline", ctypes.c_uint32 * 128), ("hinline", ctypes.c_double * 64)]
+                ("inline_params", ip), ("pinline", ctypes.c_uint32 * 128), ("hinline", ctypes.c_double * 64),
+                ("moments", vp)]
 
 
 def _hip():
@@ -65,6 +66,7 @@ def test_noalloc_launch_matches_launch_with_partials():
     mdl = ctx.model(M.MASS, fp32=True)
     src = mdl.source()
     block = int(re.search(r"#define MF_BLOCK (\d+)", src).group(1))
+    nmom = int(re.search(r"#define MF_NMOM (\d+)", src).group(1))
     ntheta, nhoist, nout = mdl.ntheta, mdl.nhoist, mdl.nout[M.EVAL_GRAD]
     dev, mod = vp(), vp()
     assert hip.mf_device_open(0, ctypes.byref(dev)) == MF_OK, hip.mf_last_error()
@@ -87,6 +89,7 @@ def test_noalloc_launch_matches_launch_with_partials():
     chunks_d = alloc(ctypes.sizeof(Chunk) * (nmax // events_per_chunk + 1))
     partials_d = alloc(SCRATCH_BYTES)  # the caller's own buffer for mf_launch_eval
     res_d = alloc(8 * nout)
+    mom_d = alloc(8 * max(1, nmom))  # the segment's data moments (mf_eval_mom), read by the outputs' combination
     theta = np.array(M.MASS_START, dtype=np.float64)
     h2d(theta_d, theta)
     assert hip.mf_launch_prep(mod, theta_d, params_d, hoist_d, None, 1, None, None) == MF_OK, hip.mf_last_error()
@@ -100,9 +103,15 @@ def test_noalloc_launch_matches_launch_with_partials():
         seg = Segment(0, n, n, 0, 0, nch, 0, 1)
         assert hip.mf_memcpy_h2d(dev, segs_d, ctypes.addressof(seg), ctypes.sizeof(seg)) == MF_OK
         assert hip.mf_memcpy_h2d(dev, chunks_d, ctypes.addressof(chunks), ctypes.sizeof(chunks)) == MF_OK
+        if nmom:
+            m = EvalArgs()
+            m.data, m.segs, m.chunks, m.nchunks, m.nseg = data_d, segs_d, chunks_d, nch, 1
+            m.params, m.results, m.partials = params_d, mom_d, partials_d
+            assert hip.mf_launch_eval(mod, b"mf_eval_mom", ctypes.byref(m), nch, block) == MF_OK, hip.mf_last_error()
         a = EvalArgs()
         a.data, a.segs, a.chunks, a.nchunks, a.nseg = data_d, segs_d, chunks_d, nch, 1
         a.params, a.results, a.hoist64 = params_d, res_d, hoist_d
+        a.moments = mom_d if nmom else None
         a.partials = None if noalloc else partials_d
         say("eval", "noalloc" if noalloc else "with partials", "n", n, "chunks", nch)
         f = hip.mf_launch_eval_noalloc if noalloc else hip.mf_launch_eval
@@ -135,6 +144,11 @@ def test_noalloc_launch_matches_launch_with_partials():
     a.nchunks, a.seg_done = 1, res_d
     assert hip.mf_launch_eval_noalloc(mod, b"mf_eval_grad", ctypes.byref(a), 1, block) == MF_ERR_INVALID
     assert hip.mf_launch_eval_noalloc(mod, b"mf_prep", ctypes.byref(a), 1, block) != MF_OK
+    # and a launch whose outputs need the data moments without them
+    if nmom:
+        a.seg_done, a.moments, a.partials = None, None, partials_d
+        assert hip.mf_launch_eval(mod, b"mf_eval_grad", ctypes.byref(a), 1, block) == MF_ERR_INVALID
+        assert b"moments" in hip.mf_last_error()
 
     for p in bufs:
         hip.mf_free(dev, p)
