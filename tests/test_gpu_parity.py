@@ -108,6 +108,44 @@ def test_prefetch_off_matches_prefetch_on(model, fp32):
 
 
 @pytest.mark.parametrize("fp32", [False, True])
+def test_data_moments_follow_new_data(ctx, fp32):
+    """the mass density's pivot exponent (lambda m) is summed from per-segment data moments
+    (sum m, computed once per upload or generation): new data in a slot -- uploaded, or
+    generated on the device -- must give the sums of the new data, not the old ones"""
+    n = 200_001
+    mdl = ctx.model(M.MASS, fp32=fp32)
+    assert "#define MF_NMOM 1" in mdl.source()
+    b = mdl.batch([n, 5000])
+    theta = np.array(M.START[M.MASS])
+    tol = 2e-6 if fp32 else 1e-12
+
+    def check(data, seg):
+        stored = _stored(M.MASS, data, fp32)
+        ref = O.nll_sums(M.MASS, stored, theta, mode=2, nthreads=8)
+        absref = O.nll_sums(M.MASS, stored, theta, mode=12, nthreads=8)
+        for kind in (M.EVAL_LH, M.EVAL_GRAD, M.EVAL_HESS):
+            out = b.eval(kind, theta)[seg]
+            k = len(out)
+            assert np.all(np.abs(out - ref[:k]) <= tol * absref[:k]), (kind, seg, out - ref[:k])
+
+    small = O.generate(M.MASS, M.MASS_TRUTH, 5000, seed=8, nstreams=4)
+    b.upload(1, small)
+    for seed in (3, 4):
+        # a shifted sample the second time, so its sum of m differs clearly from the first
+        data = O.generate(M.MASS, M.MASS_TRUTH if seed == 3 else np.array([5.1, 0.03, 0.6, 2.0]), n, seed=seed, nstreams=16)
+        say("moments", "fp32", fp32, "upload seed", seed)
+        b.upload(0, data)
+        check(data, 0)
+        check(small, 1)
+    # generated on the device into both slots, read back: the sums are those of the downloaded data
+    b.generate(np.array([5.3, 0.05, 0.5, -3.0]), [99, 100])
+    check(b.download(0), 0)
+    check(b.download(1), 1)
+    b.close()
+    mdl.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
 def test_mass_at_lambda_zero_matches_oracle(ctx, fp32):
     """lambda = 0 takes the ConditionalEqual branch of the exponential's normalisation
     (reference pdf.hh:437-439): b - a, whose derivative rule contributes nothing at lambda = 0
