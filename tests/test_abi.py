@@ -137,6 +137,21 @@ def test_kernels_fit_register_budget():
     assert M.kernel_resources(M.KSTAR, False, fit=True, unroll=8)[M.EVAL_HESS]["unroll"] < 8
 
 
+def test_autotune_candidates_by_workload_size():
+    """The autotuner's launch shapes by workload: 2-16 events per thread below 4e6 events, the
+    default 8-64 from there, and the default plus 16 events' loads in flight per thread from 1e8
+    (where the kernel streams); every shape's kernels fit the register budget, an unroll of 16
+    included"""
+    small, mid, large = (M.autotune_candidates(n) for n in (10**6, 10**7, 5 * 10**8))
+    assert {e for _, e, _ in small} == {2, 4, 8, 16} and all(u <= e for _, e, u in small)
+    assert {e for _, e, _ in mid} == {8, 16, 32, 64} and {u for _, _, u in mid} == {4, 8}
+    assert set(mid) < set(large) and {s for s in large if s not in mid} == {(b, e, 16) for b in (256, 512) for e in (32, 64)}
+    for kind in (M.MASS, M.KSTAR):
+        for k, r in M.kernel_resources(kind, True, fit=True, unroll=16).items():
+            assert 1 <= r["unroll"] <= 16 and r["agpr"] == 0 and r["scratch"] == 0, (kind, k, r)
+    assert M.kernel_resources(M.MASS, True, fit=True, unroll=16)[M.EVAL_GRAD]["unroll"] == 16
+
+
 def test_fast_eval_extension_binds_and_checks_arguments():
     """morefit_amd._fast (Batch.eval through the CPython C API) is built, bound to the loaded
     library's mfh_batch_eval, and rejects a malformed call before reaching the library: wrong
