@@ -117,25 +117,37 @@ def issue_floor(kernel_ms):
 OTHERS_FILE = os.path.join(ROOT, "profiles", "mf_eval_grad_fp64_mass_angular_toys_counters.txt")
 
 
-def others_traffic():
-    """HBM bytes per launch of the secondary gradient kernels from their committed counter
-    summary, by workload label ({} without it)"""
+def others_counters(key):
+    """a per-workload figure of the secondary gradient kernels from their committed counter
+    summary (lines `key[workload]: value`), by workload label ({} without it)"""
     try:
         lines = open(OTHERS_FILE).read().splitlines()
     except OSError:
         return {}
     out = {}
     for l in lines:
-        if l.startswith("traffic_bytes_per_launch["):
+        if l.startswith(key + "["):
             k, v = l.split("]:", 1)
-            out[k[len("traffic_bytes_per_launch["):]] = float(v)
+            out[k[len(key) + 1:]] = float(v)
     return out
 
 
-def roofline_row(label, kernel_ms, algorithmic_bytes, traffic):
+def others_traffic():
+    """HBM bytes per launch of the secondary gradient kernels (FETCH_SIZE + WRITE_SIZE, scaled)"""
+    return others_counters("traffic_bytes_per_launch")
+
+
+def roofline_row(label, kernel_ms, algorithmic_bytes, traffic, floors=None):
+    """the kernel against the HBM roofline; with its VALU issue floor from the counter summary
+    (VALU wave instructions at the measured issue rates) the fraction of that floor beside it"""
     gbs = algorithmic_bytes / (kernel_ms * 1e-3) / 1e9
-    return {"workload": label, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "traffic": traffic.get(label), "kernel_ms": kernel_ms, "algorithmic_bytes": algorithmic_bytes}
+    row = {"workload": label, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "traffic": traffic.get(label), "kernel_ms": kernel_ms, "algorithmic_bytes": algorithmic_bytes}
+    floor = (floors or {}).get(label)
+    if floor:
+        row["valu_issue_floor_ms"] = floor
+        row["valu_issue_floor_frac"] = floor / kernel_ms
+    return row
 
 
 def plain_context(M, O):
@@ -789,13 +801,14 @@ def main():
             f"(1 thread), {cpu_more['toys_all_threads']['toy_fits_per_s']:.1f}/s ({cpu_more['toys_all_threads']['threads']} threads)")
 
     # the other gradient kernels against the same roofline (traffic from their counter summary)
-    ot = others_traffic()
-    other_rows = [roofline_row("mass FP64 N=1e7", k64_ms, n * 8, ot)]
+    ot, fl = others_traffic(), others_counters("valu_issue_floor_ms")
+    other_rows = [roofline_row("mass FP64 N=1e7", k64_ms, n * 8, ot, fl)]
     if angular:
-        other_rows.append(roofline_row("angular FP32 N=1e6", angular["fp32"]["kernel_us"] * 1e-3, 12_000_000, ot))
-        other_rows.append(roofline_row("angular FP64 N=1e6", angular["fp64"]["kernel_us"] * 1e-3, 24_000_000, ot))
+        other_rows.append(roofline_row("angular FP32 N=1e6", angular["fp32"]["kernel_us"] * 1e-3, 12_000_000, ot, fl))
+        other_rows.append(roofline_row("angular FP64 N=1e6", angular["fp64"]["kernel_us"] * 1e-3, 24_000_000, ot, fl))
     if toys:
-        other_rows.append(roofline_row("4096 toys FP32", toys["grad_kernel_ms"], int(round(toys["mean_events"] * toys["toys"])) * 4, ot))
+        other_rows.append(roofline_row("4096 toys FP32", toys["grad_kernel_ms"], int(round(toys["mean_events"] * toys["toys"])) * 4, ot,
+                                       fl))
         other_rows[-1]["launch_shape_bench"] = toys["launch_shape"]
 
     out = {
