@@ -516,13 +516,56 @@ __device__ __forceinline__ void mf_eval_impl(const mf_eval_args& a) {
         compute_group(x);
       }
     }
-    for (int i = ngroups * U * MF_BLOCK + (int)threadIdx.x; i < len; i += MF_BLOCK) {
-      MF_T x[MF_NCOL];
+    // the rest of the chunk (fewer than U * MF_BLOCK events) as one more group: its loads are
+    // issued together, so the tail waits on memory once, not once per event.  A lane whose event
+    // lies past the chunk's end loads the chunk's last event instead (every load stays in range)
+    // and its outputs are replaced by zeros; of the U slots only the first nu (uniform) hold
+    // events of the chunk and are computed
+    const int rest = len - ngroups * U * MF_BLOCK;
+    if constexpr (U == 1) {
+      // one event per lane at most: the plain loop (no selects, fewest registers)
+      for (int i = ngroups * MF_BLOCK + (int)threadIdx.x; i < len; i += MF_BLOCK) {
+        MF_T x[MF_NCOL];
 #pragma unroll
-      for (int d = 0; d < MF_NCOL; d++) x[d] = mf_load_nt(rs[d], i * (int)sizeof(MF_T), 0);
-      MF_T o[NO];
-      F::eval(x, P, o);
-      acc.add(o);
+        for (int d = 0; d < MF_NCOL; d++) x[d] = mf_load_nt(rs[d], i * (int)sizeof(MF_T), 0);
+        MF_T o[NO];
+        F::eval(x, P, o);
+        acc.add(o);
+      }
+    } else if (rest > 0) {
+      const int first = ngroups * U * MF_BLOCK + (int)threadIdx.x;
+      const int nu = (rest + MF_BLOCK - 1) / MF_BLOCK;
+      MF_T x[U][MF_NCOL];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = min(first + u * MF_BLOCK, len - 1);
+#pragma unroll
+        for (int d = 0; d < MF_NCOL; d++) x[u][d] = mf_load_nt(rs[d], i * (int)sizeof(MF_T), 0);
+      }
+#if MF_FP32 && MF_ACC == 2
+      MF_T g[NO];
+#pragma unroll
+      for (int k = 0; k < NO; k++) g[k] = 0.0f;
+#endif
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (u < nu) {
+          const bool in = first + u * MF_BLOCK < len;
+          MF_T o[NO];
+          F::eval(x[u], P, o);
+#if MF_FP32 && MF_ACC == 2
+#pragma unroll
+          for (int k = 0; k < NO; k++) g[k] += in ? o[k] : (MF_T)0;
+#else
+#pragma unroll
+          for (int k = 0; k < NO; k++) o[k] = in ? o[k] : (MF_T)0;
+          acc.add(o);
+#endif
+        }
+      }
+#if MF_FP32 && MF_ACC == 2
+      acc.add(g);
+#endif
     }
     double v[NO];
     acc.get(v);
