@@ -521,7 +521,20 @@ def profile_others():
         ctx.close()
 
 
+def single_thread_blas():
+    """numpy's BLAS on one thread in this process: its pool threads keep spinning for a while
+    after each call and take the cores that the batched fits' host threads run MIGRAD on (a
+    500 x 500 product before a 4096-toy study made the study's host steps 3-4x slower); the
+    bench's own BLAS work is small"""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return None
+    return threadpool_limits(limits=1, user_api="blas")
+
+
 def main():
+    single_thread_blas()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)

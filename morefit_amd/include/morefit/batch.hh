@@ -66,11 +66,13 @@ class FitScheduler {
     char* stack = nullptr;          // from FiberStacks, returned when the task leaves
     void* sp = nullptr;             // saved stack pointer of this fibre while it is suspended
     void** ret = nullptr;           // where the worker that resumed it saved its own
-    int pending = 0;
+    int pending = 0;                // kind of the evaluation it waits for (0: none)
+    unsigned long long posted = 0;  // launches issued before it posted that request
     bool done = false;
     BatchFitResult result;
     std::thread::id first_thread;   // OS thread the fibre first ran on
     std::vector<double> bad_theta;  // theta of the last evaluation that came back non-finite
+    typename std::list<Task>::iterator self;  // its node in tasks_
   };
 
   bool use_graph = false;  // direct launches: on this stack cheaper than a graph replay
@@ -98,11 +100,14 @@ class FitScheduler {
   Task* add(Batch* b, int seg, const double* start, long long id = 0) {
     tasks_.emplace_back();
     Task& t = tasks_.back();
+    t.self = std::prev(tasks_.end());
     t.owner = this;
     t.b = b;
     t.seg = seg;
     t.id = id;
     t.worker = (int)(added_++ % (unsigned long long)nthreads());
+    if ((int)owned_.size() <= t.worker) owned_.resize(t.worker + 1);
+    owned_[t.worker].push_back(&t);
     t.start.assign(start, start + b->m->ntheta);
     t.stack = FiberStacks::get().acquire();
     t.sp = fiber_prepare(t.stack, FiberStacks::kStackBytes, &FitScheduler::fiber_main, &t);
@@ -111,58 +116,59 @@ class FitScheduler {
   }
   size_t live() const { return tasks_.size(); }
 
-  // Runs every task that is not waiting until it posts a request or finishes (the host
-  // threads in parallel, each over its own tasks); finished tasks are handed to `on_done`
-  // and removed.  Returns the number still live.
+  // Runs every task whose request has been served (or that has none) until it posts its next
+  // request or finishes: each host thread over its own tasks, in parallel.  A thread places its
+  // tasks' new requests in their batches and collects its finished tasks itself; those are
+  // handed to `on_done` and removed here.  Returns the number still live.
   template <typename F>
   size_t step(F on_done) {
     const auto ts = std::chrono::steady_clock::now();
-    const int T = nthreads();
-    if (T > 1) {
+    const int W = (int)owned_.size();
+    // cleared here, not by the workers: a worker with no tasks left is not run again
+    finished_.resize(W);
+    for (auto& f : finished_) f.clear();
+    auto work = [this](int w) {
+      void* here = nullptr;
+      std::vector<Task*>& v = owned_[w];
+      std::vector<Task*>& fin = finished_[w];
+      size_t keep = 0;
+      for (Task* t : v) {
+        if (!t->done && (t->pending == 0 || t->posted < launches_)) {
+          t->pending = 0;
+          resume(*t, &here);
+          if (t->pending) t->b->request(t->seg, t->pending);  // its own segment's entry
+        }
+        if (t->done) fin.push_back(t);
+        else v[keep++] = t;
+      }
+      v.resize(keep);
+    };
+    if (nthreads() > 1) {
       // every fibre is resumed by the worker it was assigned at add(), however many are live:
       // code in a fibre may keep thread-local addresses (errno, the C++ exception globals)
       // across its switches, so it must never continue on another OS thread
-      groups_.assign(T, {});
       int used = 0;
-      for (auto& t : tasks_)
-        if (!t.pending && !t.done) {
-          groups_[t.worker].push_back(&t);
-          used = std::max(used, t.worker + 1);
-        }
-      if (used > 0)
-        HostThreads::get().run(used, [this](int w) {
-          void* here = nullptr;
-          for (Task* t : groups_[w]) resume(*t, &here);
-        });
+      for (int w = 0; w < W; w++)
+        if (!owned_[w].empty()) used = w + 1;
+      if (used > 0) HostThreads::get().run(used, work);
     } else {
-      void* here = nullptr;
-      for (auto& t : tasks_)
-        if (!t.pending && !t.done) resume(t, &here);
+      for (int w = 0; w < W; w++) work(w);
     }
-    for (auto it = tasks_.begin(); it != tasks_.end();) {
-      if (it->done) {
-        if (it->result.nonfinite_evals > 0) diagnose(*it);
-        on_done(*it);
-        FiberStacks::get().release(it->stack);
-        it = tasks_.erase(it);
-      } else {
-        ++it;
+    for (int w = 0; w < W; w++)
+      for (Task* t : finished_[w]) {
+        if (t->result.nonfinite_evals > 0) diagnose(*t);
+        on_done(*t);
+        FiberStacks::get().release(t->stack);
+        tasks_.erase(t->self);
       }
-    }
     stats.step_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
     return tasks_.size();
   }
-  // one shared launch per (batch, kind) over every pending request, then wakes the tasks
+  // one shared launch per (batch, kind) over every segment with a request, which serves every
+  // request posted so far
   void launch() {
     auto tw = std::chrono::steady_clock::now();
     for (Batch* b : batches_) {
-      bool any = false;
-      for (auto& t : tasks_)
-        if (t.b == b && t.pending) {
-          b->request(t.seg, t.pending);
-          any = true;
-        }
-      if (!any) continue;
       for (int k = 1; k <= 3; k++) {
         bool want = false;
         for (int s = 0; s < b->S && !want; s++) want = b->requested(s) == k;
@@ -172,7 +178,7 @@ class FitScheduler {
       }
       std::fill(b->request_.begin(), b->request_.end(), 0);
     }
-    for (auto& t : tasks_) t.pending = 0;
+    launches_++;
     stats.ticks++;
     stats.gpu_wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
   }
@@ -192,6 +198,7 @@ class FitScheduler {
       const auto& ti = t->b->m->theta_index;
       for (size_t k = 0; k < ti.size(); k++) th[k] = x[ti[k]];
       t->pending = kind;
+      t->posted = t->owner->launches_;
       mf_fiber_switch(&t->sp, *t->ret);
       note_thread(*t);
       const double* r = t->b->result(t->seg, kind);
@@ -240,8 +247,10 @@ class FitScheduler {
   std::set<Batch*> batches_;
   unsigned long long added_ = 0;
 
-  // per step: the tasks worker w resumes (its own, in order)
-  std::vector<std::vector<Task*>> groups_;
+  unsigned long long launches_ = 0;  // launch() calls so far
+  // the live tasks of worker w, in the order they were added, and those that finished in the
+  // last step
+  std::vector<std::vector<Task*>> owned_, finished_;
 
   // where the sums of a finished fit came back non-finite: count the offending events at the
   // last such theta on the device (the calling thread issues all device work)
