@@ -19,6 +19,8 @@
 #include <sys/mman.h>
 #include <unistd.h>
 
+#include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdint>
 #include <functional>
@@ -148,6 +150,11 @@ class FiberStacks {
 class HostThreads {
  public:
   static constexpr int kMaxThreads = 16;
+  // A worker that has done its part polls for the next run for kPollUs before it blocks, and
+  // the caller polls for the end of a run likewise: the ticks of a batched fit come every few
+  // hundred microseconds, and a blocked thread took tens of microseconds to wake (a run of 16
+  // threads over empty work: ~40-60 us through the condition variables alone).
+  static constexpr int kPollUs = 300;
   static HostThreads& get() {
     static HostThreads pool;
     return pool;
@@ -156,53 +163,81 @@ class HostThreads {
     if (T <= 0) return;
     if (T > kMaxThreads) throw std::invalid_argument("HostThreads::run: too many threads");
     std::lock_guard<std::mutex> serial(run_mu_);
-    std::unique_lock<std::mutex> lk(mu_);
-    while ((int)threads_.size() < T) {
-      const int w = (int)threads_.size();
-      threads_.emplace_back([this, w] { worker(w); });
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      while ((int)threads_.size() < T) {
+        const int w = (int)threads_.size();
+        threads_.emplace_back([this, w] { worker(w); });
+      }
+      fn_ = &fn;
+      active_ = T;
+      running_.store(T, std::memory_order_relaxed);
+      generation_.fetch_add(1, std::memory_order_release);
     }
-    fn_ = &fn;
-    active_ = T;
-    running_ = T;
-    ++generation_;
     cv_go_.notify_all();
-    cv_done_.wait(lk, [&] { return running_ == 0; });
+    auto finished = [this] { return running_.load(std::memory_order_acquire) == 0; };
+    if (!poll(finished)) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_done_.wait(lk, finished);
+    }
+    std::lock_guard<std::mutex> lk(mu_);
     fn_ = nullptr;
   }
   ~HostThreads() {
     {
       std::lock_guard<std::mutex> lk(mu_);
-      stopping_ = true;
-      cv_go_.notify_all();
+      stopping_.store(true, std::memory_order_release);
     }
+    cv_go_.notify_all();
     for (auto& t : threads_) t.join();
   }
 
  private:
   HostThreads() = default;
+  // polls `ready` for up to kPollUs (a pause between polls, a yield every 64); false when it
+  // never held
+  template <class P>
+  static bool poll(P ready) {
+    const auto end = std::chrono::steady_clock::now() + std::chrono::microseconds(kPollUs);
+    for (unsigned i = 1;; i++) {
+      if (ready()) return true;
+      if ((i & 63) == 0) {
+        if (std::chrono::steady_clock::now() > end) return false;
+        std::this_thread::yield();
+      } else {
+        __builtin_ia32_pause();
+      }
+    }
+  }
   void worker(int w) {
     unsigned long long seen = 0;
+    auto go = [&] { return stopping_.load(std::memory_order_acquire) || generation_.load(std::memory_order_acquire) != seen; };
     for (;;) {
       const std::function<void(int)>* fn = nullptr;
+      poll(go);
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_go_.wait(lk, [&] { return stopping_ || (generation_ != seen && w < active_); });
-        if (stopping_) return;
-        seen = generation_;
+        cv_go_.wait(lk, go);
+        if (stopping_.load(std::memory_order_acquire)) return;
+        seen = generation_.load(std::memory_order_acquire);
+        if (w >= active_) continue;  // not part of this run
         fn = fn_;
       }
       (*fn)(w);
-      std::lock_guard<std::mutex> lk(mu_);
-      if (--running_ == 0) cv_done_.notify_one();
+      if (running_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        std::lock_guard<std::mutex> lk(mu_);  // the caller checks under mu_ before it blocks
+        cv_done_.notify_one();
+      }
     }
   }
   std::mutex run_mu_, mu_;
   std::condition_variable cv_go_, cv_done_;
   std::vector<std::thread> threads_;
   const std::function<void(int)>* fn_ = nullptr;
-  unsigned long long generation_ = 0;
-  int active_ = 0, running_ = 0;
-  bool stopping_ = false;
+  std::atomic<unsigned long long> generation_{0};
+  std::atomic<int> running_{0};
+  std::atomic<bool> stopping_{false};
+  int active_ = 0;
 };
 
 }  // namespace morefit

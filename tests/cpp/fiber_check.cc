@@ -5,7 +5,9 @@
 // reused by a second round.  Prints "ok <sum>" or "FAIL <reason>" and exits non-zero on failure.
 #include <cfenv>
 #include <cmath>
+#include <chrono>
 #include <cstdio>
+#include <thread>
 #include <list>
 #include <stdexcept>
 #include <vector>
@@ -110,6 +112,37 @@ int main() {
       }
       total += t.acc;
       FiberStacks::get().release(t.stack);
+    }
+  }
+  // the thread pool alone: runs of 1-16 threads back to back (workers still polling) and after
+  // pauses longer than the polling window (workers blocked): every run calls fn(w) once for each
+  // w < T, worker w is always the same OS thread, and the caller sees every write of the run
+  {
+    const int R = 6000;
+    std::vector<int> last(HostThreads::kMaxThreads, -1), calls(HostThreads::kMaxThreads, 0);
+    std::vector<std::thread::id> ids(HostThreads::kMaxThreads);
+    for (int r = 0; r < R; r++) {
+      const int Tr = 1 + (r * 7) % HostThreads::kMaxThreads;
+      if (r % 1000 == 999) std::this_thread::sleep_for(std::chrono::milliseconds(2));
+      HostThreads::get().run(Tr, [&](int w) {
+        if (ids[w] == std::thread::id()) ids[w] = std::this_thread::get_id();
+        else if (ids[w] != std::this_thread::get_id()) last[w] = -2;  // moved: reported below
+        if (last[w] != -2) last[w] = r;
+        calls[w]++;
+      });
+      for (int w = 0; w < HostThreads::kMaxThreads; w++)
+        if ((w < Tr && last[w] != r) || last[w] == -2) {
+          std::printf("FAIL pool run %d (T=%d): worker %d saw %d\n", r, Tr, w, last[w]);
+          return 1;
+        }
+    }
+    int sum = 0;
+    for (int c : calls) sum += c;
+    int want = 0;
+    for (int r = 0; r < R; r++) want += 1 + (r * 7) % HostThreads::kMaxThreads;
+    if (sum != want) {
+      std::printf("FAIL pool calls %d, expected %d\n", sum, want);
+      return 1;
     }
   }
   // the second round ran on stacks of the first (the pool reuses them)
