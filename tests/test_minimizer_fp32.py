@@ -87,7 +87,9 @@ def test_fp32_fits_match_fp64_on_cpu_backend():
 def test_host_fibres_switch_and_reuse_stacks(tmp_path):
     """the batched fits' host fibres (morefit/fiber.hh): 3000 fibres on 8 pinned pool threads,
     ~22 switches each, rounding modes kept per fibre, exceptions and deep recursion inside
-    fibres, stacks reused by a second round (built at -O0 and -O2)"""
+    fibres, stacks reused by a second round; and the thread pool alone: 6000 runs of 1-16
+    threads, back to back (workers polling) and after pauses (workers blocked), each w called
+    once per run on its own OS thread (built at -O0 and -O2)"""
     for opt in ("-O0", "-O2"):
         exe = str(tmp_path / ("fiber_check" + opt))
         subprocess.check_call(["g++", opt, "-std=c++17", "-I", os.path.join(ROOT, "morefit_amd", "include"),
