@@ -33,6 +33,7 @@
  *   mf_launch_seed_streams          host seeding of per-work-item generator states (compute_opencl.hh:522-539), on the device
  *   mf_launch_expand_streams        no counterpart (the per-stream event ranges built on the device from per-slot counts)
  *   mf_sync                         Finish (clFinish)
+ *   mf_query                        no counterpart (a non-blocking stream check that lets the runtime retire finished launches)
  *   mf_graph_*                      no counterpart (per-iteration launch sequence as a hipGraph)
  *   mf_timer_*, mf_bench_copy       no counterpart (measurement)
  */
@@ -222,6 +223,10 @@ int mf_memcpy_d2h(mf_device* dev, void* dst, const void* src, size_t bytes);   /
 int mf_memcpy_d2d(mf_device* dev, void* dst, const void* src, size_t bytes);
 int mf_fill_zero(mf_device* dev, void* dptr, size_t bytes);                    /* kernel fill, graph-safe */
 int mf_sync(mf_device* dev);
+/* *idle = 1 when all work queued on the device's stream has finished, else 0; does not wait.
+ * Callers that learn of completion another way (a polled flag) call it now and then so that
+ * the runtime retires its finished launches */
+int mf_query(mf_device* dev, int* idle);
 
 /* modules: generated source (+ the kernel runtime) compiled by hipRTC for gfx950 and cached
  * in memory and on disk by a hash of source and options.

@@ -75,7 +75,7 @@ _capi = None
 # every symbol include/morefit_hip.h and include/morefit_capi.h declare
 HIP_SYMBOLS = [
     "mf_last_error", "mf_device_open", "mf_device_close", "mf_device_query", "mf_malloc", "mf_free", "mf_host_alloc",
-    "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_module_load",
+    "mf_host_free", "mf_memcpy_h2d", "mf_memcpy_d2h", "mf_memcpy_d2d", "mf_fill_zero", "mf_sync", "mf_query", "mf_module_load",
     "mf_module_compile_only", "mf_module_unload", "mf_module_has_kernel", "mf_module_compile_ms", "mf_launch_eval",
     "mf_launch_prep", "mf_launch_prep_inline", "mf_launch_generate", "mf_graph_begin", "mf_graph_end", "mf_graph_launch", "mf_graph_destroy",
     "mf_timer_begin", "mf_timer_end", "mf_bench_copy", "mf_debug_fail_next", "mf_module_resources", "mf_host_alloc_mapped", "mf_launch_seed_streams", "mf_module_occupancy",

@@ -603,6 +603,18 @@ int mf_sync(mf_device* dev) {
   return MF_OK;
 }
 
+int mf_query(mf_device* dev, int* idle) {
+  if (!dev || !idle) return set_error(MF_ERR_INVALID, "mf_query: null argument");
+  const hipError_t e = hipStreamQuery(dev->stream);
+  if (e == hipErrorNotReady) {
+    *idle = 0;
+    return MF_OK;
+  }
+  MF_HIP(e, "mf_query");
+  *idle = 1;
+  return MF_OK;
+}
+
 int mf_module_compile_only(const char* generated_src, const char* options, size_t* code_bytes) {
   MF_INJECT("mf_module_compile_only");
   if (!generated_src) return set_error(MF_ERR_INVALID, "mf_module_compile_only: null source");

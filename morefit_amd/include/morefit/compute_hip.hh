@@ -1595,13 +1595,26 @@ class HipBatch {
     const int* f = done_h.template as<int>();
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned spin = 1;; spin++) {
-      if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == done_value) return;
+      if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == done_value) {
+        // every kQueryEvery calls a non-blocking check of the stream: without one the runtime
+        // keeps every launch's bookkeeping and reclaims it all at once every ~1000 launches (two
+        // stalls of 100-250 us inside launch calls every ~500 calls).  Measured at N = 10^7 (one
+        // call ~20.6 us): every 64 calls the check costs ~20 us and no stall remains; every 8
+        // it costs ~15 us each time (+1.9 us per call), every 256 the stalls come back in pairs
+        if (++polled_ % kQueryEvery == 0) {
+          int idle = 0;
+          hip_check(mf_query(dev, &idle), "HipBatch run");
+        }
+        return;
+      }
       if ((spin & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kPollMicros)) break;
     }
     hip_check(mf_sync(dev), "HipBatch run");
     if (__atomic_load_n(f, __ATOMIC_ACQUIRE) != done_value) throw std::runtime_error("HipBatch: completion flag not set after the stream drained");
   }
   static constexpr int kPollMicros = 200;
+  static constexpr unsigned kQueryEvery = 64;
+  unsigned polled_ = 0;
   void set_modes(int kind, bool all) {
     mf_segment* t = segs_hk[kind].template as<mf_segment>();
     for (int s = 0; s < S; s++) {

@@ -64,6 +64,8 @@ def test_null_arguments_rejected():
     assert hip.mf_malloc(None, 16, None) == 1  # MF_ERR_INVALID
     hip.mf_launch_eval.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     assert hip.mf_launch_eval(None, b"k", None, 1, 256) == 1
+    hip.mf_query.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    assert hip.mf_query(None, None) == 1
 
 
 def test_generated_modules_compile_for_gfx950():
