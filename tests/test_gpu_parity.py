@@ -379,7 +379,8 @@ def test_polled_completion_returns_complete_totals(ctx, fp32):
     polls instead of waiting on the stream.  200 calls alternating two parameter points, all
     kinds: each result bitwise equal to the first polled result at that point and within 1e-12
     of the stream-synchronised result there (a flag published before the totals would show the
-    other point's totals)."""
+    other point's totals).  Every 64th polled call also checks the stream without waiting
+    (mf_query), so the 600 calls run that path nine times."""
     n = 1_000_003
     data = O.generate(M.MASS, M.MASS_TRUTH, n, seed=11, nstreams=16)
     mdl = ctx.model(M.MASS, fp32=fp32)
